@@ -1,0 +1,235 @@
+/*
+ * r2.h -- C ABI of the MI355X-native volumetric hot path of FrederikBoehm/Renderer2:
+ * the per-voxel mesh -> SGGX filter and the volume path integrator, as HIP kernels
+ * for gfx950 behind plain-pointer entry points.
+ *
+ * Conventions: every entry point returns R2_OK (0) or a negative R2_ERR_* code and
+ * never throws; r2_last_error() holds the message of the last failure on a context.
+ * A context is externally synchronised (one host thread at a time); the render
+ * server (r2_server_*) is internally synchronised.  Host buffers belong to the
+ * caller; device memory behind r2_grid / r2_env / r2_mesh handles belongs to the
+ * library until the matching *_free call.
+ *
+ * Reference citations are relative to the reference repository root.
+ */
+#ifndef R2_H
+#define R2_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define R2_OK 0
+#define R2_ERR_INVALID (-1)     /* bad argument / shape */
+#define R2_ERR_HIP (-2)         /* HIP runtime failure */
+#define R2_ERR_OOM (-3)         /* device or host allocation failed / budget exceeded */
+#define R2_ERR_UNSUPPORTED (-4) /* feature or size outside what the build supports */
+#define R2_ERR_IO (-5)          /* file missing or malformed */
+#define R2_ERR_CANCELLED (-6)   /* request cancelled / abort callback returned true */
+#define R2_ERR_BUSY (-7)        /* server queue full */
+
+#define R2_VERSION 1
+
+/* Grid record precision (r2_grid_desc.precision) */
+#define R2_PREC_FP32 0   /* 32-byte SFilteredDataCompact voxel records (filtered_data.hpp:33-61) */
+#define R2_PREC_PACKED 1 /* 16-byte records: FP16 density + 6 SGGX coefficients FP16/BF16 + rest packed */
+
+/* r2_render_params.flags */
+#define R2_FLAG_BRICK_GRID 1u   /* majorant DDA + brick density (config "UseBrickGrid": true) */
+#define R2_FLAG_FP64_ACCUM 2u   /* compensated FP64 framebuffer accumulation */
+
+typedef struct r2_ctx r2_ctx;
+typedef struct r2_grid r2_grid;
+typedef struct r2_env r2_env;
+typedef struct r2_mesh r2_mesh;
+typedef struct r2_server r2_server;
+
+/* ------------------------------------------------------------- context -- */
+/* Replaces the device setup of rt::Raytracer / filter::CFilter
+ * (raytracer/raytracer.cu:129-152, raytracer/filter.cu:20-57). */
+int r2_create(r2_ctx** out, int device);
+int r2_destroy(r2_ctx* ctx);
+const char* r2_last_error(const r2_ctx* ctx);
+int r2_version(void);
+/* The HIP stream the context launches on (a hipStream_t). */
+void* r2_stream(r2_ctx* ctx);
+
+/* ---------------------------------------------------------------- grid -- */
+/* A filtered volume: NanoVDB-layout record tree + brick majorant grid.
+ * Replaces CNVDBMedium(path, sigmaA, sigmaS) (raytracer/src/medium/nvdb_medium.cu:45-61,
+ * :130-173) with CAssetManager::loadMedium (raytracer/src/backend/asset_manager.cu:114-138)
+ * and the brick grid build CHostBrickGrid (raytracer/src/grid_brick/host_grid_brick.cpp:61-152),
+ * which here runs on the GPU. */
+typedef struct {
+  int32_t nx, ny, nz;          /* voxel counts; records are x fastest, then y, then z */
+  const void* records;         /* nx*ny*nz SFilteredDataCompact records, 32 bytes each */
+  int32_t records_on_device;   /* 1: `records` is a device pointer */
+  float map_mat[9];            /* NanoVDB Map mMatF, row-major 3x3: index -> model */
+  float map_vec[3];            /* NanoVDB Map mVecF */
+  float sigma_a[3], sigma_s[3];
+  float voxel_size_filtering;  /* filter voxel size of this LOD (nvdb_medium.cu:180-188) */
+  int32_t precision;           /* R2_PREC_* */
+} r2_grid_desc;
+
+typedef struct {
+  float index_to_model[12], model_to_index[12];  /* glm::mat4x3, column major */
+  float model_to_scaled[12], scaled_to_model[12];
+  int32_t ibb_min[3], size[3];
+  int32_t n_bricks[3];
+  int64_t n_leaves, n_nonempty_bricks;
+  float max_density, inv_max_density;
+  float bbox_min[3], bbox_max[3];                /* model-space bbox of the grid */
+  int64_t device_bytes;
+} r2_grid_info;
+
+int r2_grid_upload(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid** out);
+int r2_grid_free(r2_ctx* ctx, r2_grid* grid);
+int r2_grid_get_info(const r2_grid* grid, r2_grid_info* out);
+/* Parity probe: copy the brick grid back.  range: nb^3 u32 (half min | half max << 16),
+ * u8: dense (8*nb)^3 quantised densities (meaningful in non-empty bricks),
+ * mips: 3 levels of u32 ranges concatenated.  Any pointer may be NULL. */
+int r2_grid_download_bricks(r2_ctx* ctx, const r2_grid* grid, uint32_t* range, uint8_t* u8, uint32_t* mips);
+/* Parity probe: NanoVDB-layout voxel lookups. For each of n integer coordinates
+ * (xyz triples) write the 32-byte record the tree returns (background when inactive). */
+int r2_grid_lookup(r2_ctx* ctx, const r2_grid* grid, int32_t n, const int32_t* coords, uint8_t* out_records);
+
+/* ----------------------------------------------------------------- env -- */
+/* Lat-long HDR environment map with sin(theta)-weighted 2-D importance sampling.
+ * Replaces CEnvironmentMap(path) + CDistribution2D (raytracer/src/scene/environmentmap.cu:11-29,
+ * raytracer/src/sampling/distribution_2d.cu:7-14).  rgb: h rows of w RGB texels,
+ * row 0 = theta 0 (+y). */
+int r2_env_upload(r2_ctx* ctx, const float* rgb, int32_t w, int32_t h, r2_env** out);
+int r2_env_free(r2_ctx* ctx, r2_env* env);
+
+/* -------------------------------------------------------------- render -- */
+typedef struct {
+  const r2_grid* grid;
+  float pos[3], orientation[3], scaling[3];   /* CHostSceneobject(medium, pos, orientation, scaling) */
+} r2_instance;
+
+typedef struct {
+  int32_t width, height, spp, max_depth;
+  float cam_pos[3], cam_look_at[3], cam_up[3], fov_deg;  /* CCamera (raytracer/src/camera/camera.cu:20-32) */
+  uint32_t seed, stream;                                  /* RNG key */
+  int32_t n_instances;
+  const r2_instance* instances;
+  const r2_env* env;
+  uint32_t flags;                                         /* R2_FLAG_* */
+  int32_t sample_begin;                                   /* first sample index */
+} r2_render_params;
+
+/* Render one frame: linear HDR RGB, W*H*3 floats, row 0 = bottom (launch index y = 0).
+ * Replaces rt::Raytracer::renderFrame's spp loop of optixLaunch(W, H, 1) with
+ * __raygen__rg / CPathIntegrator::Li (raytracer/raytracer.cu:161-190,
+ * raytracer/src/backend/shaders.optix.cu:240-256,
+ * raytracer/include/integrators/path_integrator_impl.hpp:120-247).
+ * out_on_device: 1 if out_rgb is a device pointer. */
+int r2_render(r2_ctx* ctx, const r2_render_params* p, float* out_rgb, int32_t out_on_device);
+
+/* Parity probe: one path per (pixel, sample) pair; out: n*3 (L / spp);
+ * counters: n*8 u32 lookup counters (voxel records, majorants, brick densities,
+ * env lookups, DDA steps, free-flight steps, scatter events, unused) or NULL. */
+int r2_render_paths(r2_ctx* ctx, const r2_render_params* p, int32_t n, const int32_t* pixel,
+                    const int32_t* sample, float* out, uint32_t* counters);
+
+typedef struct {
+  double last_kernel_ms;        /* integrator kernel time of the last render (HIP events) */
+  double last_total_ms;         /* host wall time of the last r2_render call */
+  int64_t last_paths;
+  int32_t last_launches;
+} r2_stats;
+int r2_get_stats(const r2_ctx* ctx, r2_stats* out);
+
+/* ------------------------------------------------------------- filter -- */
+/* Triangle mesh for the mesh -> SGGX filter.  Replaces CMesh + OptiX GAS/IAS
+ * (raytracer/include/mesh/mesh.hpp:18-59, raytracer/src/mesh/mesh.cu:140-168)
+ * with a GPU-built BVH. */
+typedef struct {
+  int32_t n_vertices, n_triangles;
+  const float* positions;      /* n_vertices * 3, model space */
+  const float* normals;        /* n_vertices * 3 */
+  const int32_t* indices;      /* n_triangles * 3 */
+  const int32_t* material;     /* n_triangles material ids (NULL = all 0) */
+  int32_t n_materials;
+  const float* mat_diffuse;    /* n_materials * 3 */
+  const float* mat_specular;   /* n_materials * 3 */
+  const float* mat_roughness;  /* n_materials (specularRoughness, material.cu:146-148) */
+  const float* mat_ior;        /* n_materials */
+  float pos[3], orientation[3], scaling[3];  /* scene object transform */
+} r2_mesh_desc;
+int r2_mesh_upload(r2_ctx* ctx, const r2_mesh_desc* desc, r2_mesh** out);
+int r2_mesh_free(r2_ctx* ctx, r2_mesh* mesh);
+
+typedef struct {
+  float voxel_size;            /* this LOD's voxel size (world units) */
+  int32_t samples_per_voxel;   /* "SamplesPerVoxel" (<= 1000, mesh_filter.hpp:66) */
+  int32_t estimation_iterations;
+  float sigma_t, alpha;
+  int32_t clip_rays;
+  uint32_t seed;
+  int32_t lod;                 /* RNG stream tag */
+} r2_filter_params;
+
+typedef struct {
+  int32_t num_voxels[3];
+  float index_to_model[12], model_to_index[12], model_to_world[12], world_to_model[12];
+  float world_bb_min[3], world_bb_max[3];
+} r2_filter_grid;
+
+/* Grid set-up of one LOD (COpenvdbBackend::init + setupGrid,
+ * raytracer/src/filtering/openvdb_backend.cpp:26-106). */
+int r2_filter_setup(const r2_mesh* mesh, float voxel_size, r2_filter_grid* out);
+/* Filter one LOD: writes nx*ny*nz 32-byte SFilteredDataCompact records
+ * (CFilter::runFiltering's optixLaunch(nx, ny, nz) of __raygen__filtering /
+ * CMeshFilter::run, raytracer/filter.cu:74-123, raytracer/include/filtering/mesh_filter.hpp:51-122).
+ * out_on_device: 1 if out_records is a device pointer. */
+int r2_filter_lod(r2_ctx* ctx, const r2_mesh* mesh, const r2_filter_params* p, void* out_records,
+                  int32_t out_on_device, r2_filter_grid* grid_out);
+
+/* ------------------------------------------------------------- server -- */
+/* Device-resident render server with continuous batching (no reference
+ * counterpart beyond the interactive preview loop, raytracer/pipeline.cu:41-78). */
+typedef struct {
+  int64_t hbm_budget_bytes;    /* admission budget for per-request device memory */
+  int32_t max_inflight;        /* max requests resident on the device at once */
+  int32_t slice_paths;         /* paths per scheduling slice */
+} r2_server_config;
+
+typedef struct {
+  r2_render_params params;     /* instances / env must stay valid until the result is taken */
+  float* out_rgb;              /* host buffer W*H*3, filled when the request completes */
+} r2_request;
+
+typedef struct {
+  int64_t submitted, completed, rejected, cancelled;
+  double paths_per_sec;
+  double p50_ms, p99_ms;
+  int64_t peak_bytes;
+} r2_server_stats;
+
+int r2_server_start(r2_ctx* ctx, const r2_server_config* cfg, r2_server** out);
+/* Returns a request id >= 0, or R2_ERR_BUSY / R2_ERR_OOM (request larger than the budget). */
+int64_t r2_server_submit(r2_server* s, const r2_request* req);
+/* 1 = done (result written), 0 = pending, <0 = error / cancelled. */
+int r2_server_poll(r2_server* s, int64_t id, double* latency_ms);
+int r2_server_wait(r2_server* s, int64_t id, double* latency_ms);
+int r2_server_cancel(r2_server* s, int64_t id);
+int r2_server_get_stats(r2_server* s, r2_server_stats* out);
+int r2_server_stop(r2_server* s);
+
+/* ----------------------------------------------------- host utilities -- */
+/* Synthetic inputs of the benchmark configurations (seeded, deterministic):
+ * kind 1 = isotropic value-noise volume (S = I, normal +y), kind 2 = anisotropic
+ * fibre volume (S = R diag(1,1,0.05) R^T, von Mises-Fisher fibre field).
+ * Writes n^3 32-byte records.  density_scale: max density of the noise field. */
+int r2_synth_volume(int32_t kind, int32_t n, uint32_t seed, float density_scale, int32_t octaves,
+                    void* out_records);
+/* kind 0 = constant white (w x h of 1.0), kind 1 = procedural sky gradient + 0.5 degree sun. */
+int r2_synth_env(int32_t kind, int32_t w, int32_t h, float* out_rgb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

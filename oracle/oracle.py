@@ -1,0 +1,327 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module.  It is the checker and the CPU baseline, never the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libr2oracle.so"
+MAX_INST = 8
+NCNT = 7
+CNT_NAMES = ["vox", "maj", "brick", "env", "dda", "flight", "scatter"]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("records", C.c_void_p),
+        ("map_mat", C.c_float * 9), ("map_vec", C.c_float * 3),
+        ("sigma_a", C.c_float * 3), ("sigma_s", C.c_float * 3),
+        ("voxel_size_filtering", C.c_float),
+    ]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("grid", C.c_int), ("pos", C.c_float * 3), ("orientation", C.c_float * 3), ("scaling", C.c_float * 3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_grids", C.c_int), ("grids", GridDesc * MAX_INST),
+        ("n_instances", C.c_int), ("instances", InstanceDesc * MAX_INST),
+        ("env_rgb", C.c_void_p), ("env_w", C.c_int), ("env_h", C.c_int),
+    ]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_depth", C.c_int),
+        ("cam_pos", C.c_float * 3), ("cam_look_at", C.c_float * 3), ("cam_up", C.c_float * 3),
+        ("fov_deg", C.c_float),
+        ("seed", C.c_uint32), ("stream", C.c_uint32),
+        ("use_brick_grid", C.c_int), ("sample_begin", C.c_int), ("threads", C.c_int),
+    ]
+
+
+class Bricks(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nb", C.c_int * 3),
+        ("range", C.POINTER(C.c_uint32)), ("nonempty", C.POINTER(C.c_uint8)), ("u8", C.POINTER(C.c_uint8)),
+        ("mip", C.POINTER(C.c_uint32) * 3), ("n_nonempty", C.c_size_t),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        vp, u32p = C.c_void_p, C.POINTER(C.c_uint32)
+        L.r2o_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.r2o_half_from_float_glm.argtypes = [C.c_float]
+        L.r2o_half_from_float_glm.restype = C.c_uint16
+        L.r2o_half_from_float_rne.argtypes = [C.c_float]
+        L.r2o_half_from_float_rne.restype = C.c_uint16
+        L.r2o_half_to_float.argtypes = [C.c_uint16]
+        L.r2o_half_to_float.restype = C.c_float
+        L.r2o_brick_build.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(Bricks)]
+        L.r2o_brick_free.argtypes = [C.POINTER(Bricks)]
+        L.r2o_brick_majorant.argtypes = [C.POINTER(Bricks), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.r2o_brick_majorant.restype = C.c_float
+        L.r2o_brick_density.argtypes = [C.POINTER(Bricks), C.c_int, C.c_int, C.c_int]
+        L.r2o_brick_density.restype = C.c_float
+        for sfx in ("f32", "f64"):
+            R = C.c_float if sfx == "f32" else C.c_double
+            g = lambda n: getattr(L, f"{n}_{sfx}")  # noqa: E731
+            g("r2o_scene_new").argtypes = [C.POINTER(SceneDesc)]
+            g("r2o_scene_new").restype = vp
+            g("r2o_scene_free").argtypes = [vp]
+            g("r2o_render").argtypes = [vp, C.POINTER(RenderParams), vp, vp]
+            g("r2o_render_paths").argtypes = [vp, C.POINTER(RenderParams), C.c_int, vp, vp, vp, vp]
+            g("r2o_camera_view_to_world").argtypes = [C.POINTER(RenderParams), vp]
+            g("r2o_camera_ray").argtypes = [C.POINTER(RenderParams), C.c_int, C.c_int, C.c_uint32, vp]
+            g("r2o_scene_info").argtypes = [vp, C.c_int, vp]
+            g("r2o_filtered_data").argtypes = [vp, C.c_int, vp, vp]
+            g("r2o_decode_record").argtypes = [vp, vp]
+            g("r2o_dda_cells").argtypes = [vp, C.c_int, vp, R, C.c_int, vp]
+            g("r2o_dda_cells").restype = C.c_int
+            g("r2o_fresnel").argtypes = [R, R]
+            g("r2o_fresnel").restype = R
+            g("r2o_sggx_D").argtypes = [vp, vp]
+            g("r2o_sggx_D").restype = R
+            g("r2o_sggx_sigma").argtypes = [vp, vp]
+            g("r2o_sggx_sigma").restype = R
+            g("r2o_sggx_vndf").argtypes = [vp, vp, R, R, vp]
+            g("r2o_phase_sample").argtypes = [vp, vp, R, vp, C.c_uint32, C.c_uint32, vp]
+            g("r2o_phase_sample").restype = R
+            g("r2o_phase_p").argtypes = [vp, vp, R, vp, vp, C.c_uint32, C.c_uint32]
+            g("r2o_phase_p").restype = R
+            g("r2o_env_sample").argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+            g("r2o_env_le").argtypes = [vp, vp, vp]
+            g("r2o_env_tables").argtypes = [vp, vp, vp, vp]
+            g("r2o_tr_segment").argtypes = [vp, vp, C.c_int, C.c_uint32, C.c_uint32]
+            g("r2o_tr_segment").restype = R
+            g("r2o_sample_segment").argtypes = [vp, vp, C.c_int, C.c_uint32, C.c_uint32, vp]
+            g("r2o_sample_segment").restype = C.c_int
+            g("r2o_move_to_voxel_border").argtypes = [vp, vp, vp]
+            g("r2o_intersect").argtypes = [vp, vp, vp]
+            g("r2o_intersect").restype = C.c_int
+        if hasattr(L, "r2o_filter_voxels"):
+            pass
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.asarray(ctr, dtype=np.uint32).copy()
+    k = np.asarray(key, dtype=np.uint32).copy()
+    out = np.zeros(4, np.uint32)
+    u32p = C.POINTER(C.c_uint32)
+    lib().r2o_philox4x32_10(c.ctypes.data_as(u32p), k.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
+    return out
+
+
+class Brickgrid:
+    """CPU restatement of the brick majorant grid (host_grid_brick.cpp:61-152)."""
+
+    def __init__(self, dens: np.ndarray):
+        dens = np.ascontiguousarray(dens, dtype=np.float32)
+        nz, ny, nx = dens.shape
+        self.b = Bricks()
+        rc = lib().r2o_brick_build(_p(dens), nx, ny, nz, C.byref(self.b))
+        if rc != 0:
+            raise RuntimeError(f"r2o_brick_build rc={rc}")
+        self.nb = tuple(self.b.nb)
+        nbx, nby, nbz = self.nb
+        self.range = np.ctypeslib.as_array(self.b.range, shape=(nbz * nby * nbx,)).copy()
+        self.nonempty = np.ctypeslib.as_array(self.b.nonempty, shape=(nbz * nby * nbx,)).copy()
+        self.u8 = np.ctypeslib.as_array(self.b.u8, shape=(nbz * 8 * nby * 8 * nbx * 8,)).copy()
+        self.mips = []
+        for l in range(3):
+            n = (nbx >> (l + 1)) * (nby >> (l + 1)) * (nbz >> (l + 1))
+            self.mips.append(np.ctypeslib.as_array(self.b.mip[l], shape=(n,)).copy())
+
+    def majorant(self, bx, by, bz, mip):
+        return lib().r2o_brick_majorant(C.byref(self.b), bx, by, bz, mip)
+
+    def density(self, ix, iy, iz):
+        return lib().r2o_brick_density(C.byref(self.b), ix, iy, iz)
+
+    def __del__(self):
+        try:
+            lib().r2o_brick_free(C.byref(self.b))
+        except Exception:
+            pass
+
+
+class Scene:
+    """Oracle scene: grids (records) + instances + env map, FP32 or FP64."""
+
+    def __init__(self, grids, instances, env_rgb: np.ndarray, precision: str = "f64"):
+        assert precision in ("f32", "f64")
+        self.sfx = precision
+        self.R = np.float32 if precision == "f32" else np.float64
+        self._keep = []
+        d = SceneDesc()
+        d.n_grids = len(grids)
+        for i, g in enumerate(grids):
+            rec = np.ascontiguousarray(g["records"])
+            self._keep.append(rec)
+            gd = d.grids[i]
+            gd.nz, gd.ny, gd.nx = g["shape"]
+            gd.records = _p(rec)
+            gd.map_mat[:] = list(g["map_mat"])
+            gd.map_vec[:] = list(g["map_vec"])
+            gd.sigma_a[:] = list(g["sigma_a"])
+            gd.sigma_s[:] = list(g["sigma_s"])
+            gd.voxel_size_filtering = g["voxel_size_filtering"]
+        d.n_instances = len(instances)
+        for k, inst in enumerate(instances):
+            idd = d.instances[k]
+            idd.grid = inst.get("grid", 0)
+            idd.pos[:] = list(inst.get("pos", (0, 0, 0)))
+            idd.orientation[:] = list(inst.get("orientation", (0, 1, 0)))
+            idd.scaling[:] = list(inst.get("scaling", (1, 1, 1)))
+        env = np.ascontiguousarray(env_rgb, dtype=np.float32)
+        self._keep.append(env)
+        d.env_h, d.env_w = env.shape[0], env.shape[1]
+        d.env_rgb = _p(env)
+        self.desc = d
+        self.h = self._fn("r2o_scene_new")(C.byref(d))
+
+    def _fn(self, name):
+        return getattr(lib(), f"{name}_{self.sfx}")
+
+    def __del__(self):
+        try:
+            self._fn("r2o_scene_free")(self.h)
+        except Exception:
+            pass
+
+    @staticmethod
+    def params(cfg: dict, threads: int = 0) -> RenderParams:
+        p = RenderParams()
+        p.width, p.height = cfg["width"], cfg["height"]
+        p.spp, p.max_depth = cfg["spp"], cfg["max_depth"]
+        p.cam_pos[:] = list(cfg["cam_pos"])
+        p.cam_look_at[:] = list(cfg["cam_look_at"])
+        p.cam_up[:] = list(cfg["cam_up"])
+        p.fov_deg = cfg["fov_deg"]
+        p.seed, p.stream = cfg.get("seed", 1), cfg.get("stream", 0)
+        p.use_brick_grid = int(cfg.get("use_brick_grid", True))
+        p.sample_begin = cfg.get("sample_begin", 0)
+        p.threads = threads
+        return p
+
+    def render(self, cfg: dict, threads: int = 0, counters: bool = False):
+        p = self.params(cfg, threads)
+        out = np.zeros((p.height, p.width, 3), self.R)
+        cnt = np.zeros(8, np.uint64) if counters else None
+        self._fn("r2o_render")(self.h, C.byref(p), _p(out), _p(cnt) if counters else None)
+        return (out, cnt) if counters else out
+
+    def render_paths(self, cfg: dict, pix: np.ndarray, smp: np.ndarray, threads: int = 0, counters: bool = False):
+        p = self.params(cfg, threads)
+        pix = np.ascontiguousarray(pix, np.int32)
+        smp = np.ascontiguousarray(smp, np.int32)
+        n = len(pix)
+        out = np.zeros((n, 3), self.R)
+        cnt = np.zeros((n, 8), np.uint64) if counters else None
+        self._fn("r2o_render_paths")(self.h, C.byref(p), n, _p(pix), _p(smp), _p(out), _p(cnt) if counters else None)
+        return (out, cnt[:, :NCNT]) if counters else out
+
+    def info(self, medium: int = 0) -> dict:
+        out = np.zeros(64, self.R)
+        self._fn("r2o_scene_info")(self.h, medium, _p(out))
+        m = {}
+        names = ["index_to_model", "model_to_index", "model_to_scaled", "scaled_to_model"]
+        for q, nme in enumerate(names):
+            m[nme] = out[12 * q:12 * q + 12].copy()
+        m["ibb_min"] = out[48:51].copy()
+        m["size"] = out[51:54].copy()
+        m["inv_max_density"] = out[54]
+        m["bb_min"] = out[55:58].copy()
+        m["bb_max"] = out[58:61].copy()
+        m["ratio"] = out[61]
+        return m
+
+    def camera_view_to_world(self, cfg: dict) -> np.ndarray:
+        p = self.params(cfg)
+        out = np.zeros(12, self.R)
+        self._fn("r2o_camera_view_to_world")(C.byref(p), _p(out))
+        return out
+
+    def filtered_data(self, p3, medium: int = 0) -> np.ndarray:
+        p = np.ascontiguousarray(p3, self.R)
+        out = np.zeros(17, self.R)
+        self._fn("r2o_filtered_data")(self.h, medium, _p(p), _p(out))
+        return out
+
+    def env_sample(self, key: int, c0: int) -> np.ndarray:
+        out = np.zeros(7, self.R)
+        self._fn("r2o_env_sample")(self.h, key, c0, _p(out))
+        return out
+
+    def env_le(self, d) -> np.ndarray:
+        dd = np.ascontiguousarray(d, self.R)
+        out = np.zeros(4, self.R)
+        self._fn("r2o_env_le")(self.h, _p(dd), _p(out))
+        return out
+
+    def env_tables(self):
+        w, h = self.desc.env_w, self.desc.env_h
+        marg = np.zeros(h + 1, self.R)
+        rows = np.zeros((h, w + 1), self.R)
+        ints = np.zeros(h + 1, self.R)
+        self._fn("r2o_env_tables")(self.h, _p(marg), _p(rows), _p(ints))
+        return marg, rows, ints
+
+    def tr_segment(self, ray7, use_dda: bool, key: int, c0: int) -> float:
+        r = np.ascontiguousarray(ray7, self.R)
+        return self._fn("r2o_tr_segment")(self.h, _p(r), int(use_dda), key, c0)
+
+    def sample_segment(self, ray7, use_dda: bool, key: int, c0: int):
+        r = np.ascontiguousarray(ray7, self.R)
+        t = np.zeros(1, self.R)
+        hit = self._fn("r2o_sample_segment")(self.h, _p(r), int(use_dda), key, c0, _p(t))
+        return hit, float(t[0])
+
+    def move_to_voxel_border(self, ray7) -> np.ndarray:
+        r = np.ascontiguousarray(ray7, self.R)
+        out = np.zeros(7, self.R)
+        self._fn("r2o_move_to_voxel_border")(self.h, _p(r), _p(out))
+        return out
+
+    def intersect(self, ray7):
+        r = np.ascontiguousarray(ray7, self.R)
+        out = np.zeros(5, self.R)
+        hit = self._fn("r2o_intersect")(self.h, _p(r), _p(out))
+        return hit, out
+
+    def dda_cells(self, ray7, sigma_t: float = 1.0, max_cells: int = 4096, medium: int = 0) -> np.ndarray:
+        r = np.ascontiguousarray(ray7, self.R)
+        cells = np.zeros((max_cells, 4), np.int32)
+        n = self._fn("r2o_dda_cells")(self.h, medium, _p(r), sigma_t, max_cells, _p(cells))
+        return cells[:n].copy()
+
+
+def fn(name: str, precision: str = "f64"):
+    return getattr(lib(), f"{name}_{precision}")
+
+
+def as_arr(x, precision="f64"):
+    return np.ascontiguousarray(x, np.float32 if precision == "f32" else np.float64)

@@ -1,0 +1,100 @@
+/*
+ * Oracle (test infrastructure only): CPU restatement of the volumetric hot path of
+ * FrederikBoehm/Renderer2.  Never linked by the product library.
+ */
+#ifndef R2O_H
+#define R2O_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- precision-independent helpers (r2o_common.c) ---- */
+typedef struct {
+  uint32_t key[2];
+  uint32_t ctr[4];
+  uint32_t buf[4];
+  uint32_t n;
+  uint32_t block;
+} r2o_rng;
+
+void r2o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void r2o_rng_init(r2o_rng* s, uint32_t key0, uint32_t key1, uint32_t c0, uint32_t c1, uint32_t c3);
+uint32_t r2o_rng_next_u32(r2o_rng* s);
+
+uint16_t r2o_half_from_float_glm(float f);
+uint16_t r2o_half_from_float_rne(float f);
+float r2o_half_to_float(uint16_t h);
+uint32_t r2o_encode_range(float lo, float hi);
+void r2o_decode_range(uint32_t d, float* lo, float* hi);
+uint8_t r2o_encode_voxel(float value, float lo, float hi);
+float r2o_decode_voxel(uint8_t v, float lo, float hi);
+int r2o_voxel_active(const float* dens, int nx, int ny, int nz, int x, int y, int z);
+
+typedef struct {
+  int nx, ny, nz;   /* voxel counts of the source grid */
+  int nb[3];        /* padded brick counts per axis */
+  uint32_t* range;  /* nb^3 encoded half (min,max) */
+  uint8_t* nonempty;
+  uint8_t* u8;      /* dense (8*nb)^3 quantised densities */
+  uint32_t* mip[3]; /* min/max mips, level l+1 has nb >> (l+1) per axis */
+  size_t n_nonempty;
+} r2o_bricks;
+
+int r2o_brick_build(const float* dens, int nx, int ny, int nz, r2o_bricks* b);
+void r2o_brick_free(r2o_bricks* b);
+float r2o_brick_majorant(const r2o_bricks* b, int bx, int by, int bz, int mip);
+float r2o_brick_density(const r2o_bricks* b, int ix, int iy, int iz);
+
+/* ---- scene description shared by the FP32 / FP64 instantiations ---- */
+#define R2O_MAX_INSTANCES 8
+
+typedef struct {
+  int nx, ny, nz;
+  const uint8_t* records;   /* nx*ny*nz 32-byte SFilteredDataCompact, x fastest */
+  float map_mat[9];         /* NanoVDB Map mMatF (row-major 3x3): index -> model */
+  float map_vec[3];         /* NanoVDB Map mVecF */
+  float sigma_a[3], sigma_s[3];
+  float voxel_size_filtering;
+} r2o_grid_desc;
+
+typedef struct {
+  int grid;                 /* index into the grid list */
+  float pos[3], orientation[3], scaling[3];
+} r2o_instance_desc;
+
+typedef struct {
+  int n_grids;
+  r2o_grid_desc grids[R2O_MAX_INSTANCES];
+  int n_instances;
+  r2o_instance_desc instances[R2O_MAX_INSTANCES];
+  const float* env_rgb;     /* env_h rows of env_w RGB float texels (row 0 = theta 0) */
+  int env_w, env_h;
+} r2o_scene_desc;
+
+typedef struct {
+  int width, height, spp, max_depth;
+  float cam_pos[3], cam_look_at[3], cam_up[3], fov_deg;
+  uint32_t seed, stream;
+  int use_brick_grid;
+  int sample_begin;         /* first sample index (for split renders) */
+  int threads;              /* OpenMP threads (0 = default) */
+} r2o_render_params;
+
+/* per-path lookup counters (the reference's numLookups split by kind) */
+enum { R2O_CNT_VOX = 0, R2O_CNT_MAJ, R2O_CNT_BRICK, R2O_CNT_ENV, R2O_CNT_DDA, R2O_CNT_FLIGHT, R2O_CNT_SCATTER, R2O_NCNT };
+
+typedef struct r2o_scene_f32 r2o_scene_f32;
+typedef struct r2o_scene_f64 r2o_scene_f64;
+
+r2o_scene_f32* r2o_scene_new_f32(const r2o_scene_desc* d);
+void r2o_scene_free_f32(r2o_scene_f32* s);
+r2o_scene_f64* r2o_scene_new_f64(const r2o_scene_desc* d);
+void r2o_scene_free_f64(r2o_scene_f64* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,183 @@
+"""MI355X-native volumetric hot path of FrederikBoehm/Renderer2.
+
+The product is the native library renderer2_amd/lib/librenderer2.so (HIP
+kernels for gfx950 behind the C ABI of include/r2.h).  This package is a thin
+Python binding over that ABI for tests and benchmarks; it has no compute of its
+own and no fallback when the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+from ._capi import R2Error
+
+__all__ = ["Context", "Grid", "Env", "R2Error", "capi"]
+
+
+class Context:
+    """Owns an r2_ctx (one device, one HIP stream)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        capi.check(capi.lib().r2_create(C.byref(self.h), device))
+        self._grids = []
+
+    def close(self):
+        if self.h:
+            capi.lib().r2_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        return capi.check(rc, self.h)
+
+    # ---------------------------------------------------------------- grid
+    def upload_grid(self, g: dict) -> "Grid":
+        rec = np.ascontiguousarray(g["records"])
+        d = capi.GridDesc()
+        d.nz, d.ny, d.nx = g["shape"]
+        d.records = rec.ctypes.data
+        d.records_on_device = 0
+        d.map_mat[:] = list(g["map_mat"])
+        d.map_vec[:] = list(g["map_vec"])
+        d.sigma_a[:] = list(g["sigma_a"])
+        d.sigma_s[:] = list(g["sigma_s"])
+        d.voxel_size_filtering = g["voxel_size_filtering"]
+        d.precision = capi.R2_PREC_PACKED if g.get("precision") == "packed" else capi.R2_PREC_FP32
+        h = C.c_void_p()
+        self._check(capi.lib().r2_grid_upload(self.h, C.byref(d), C.byref(h)))
+        return Grid(self, h)
+
+    def upload_env(self, rgb: np.ndarray) -> "Env":
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h = C.c_void_p()
+        self._check(capi.lib().r2_env_upload(self.h, rgb.ctypes.data, rgb.shape[1], rgb.shape[0], C.byref(h)))
+        return Env(self, h)
+
+    # -------------------------------------------------------------- render
+    @staticmethod
+    def render_params(cfg: dict, grids, instances, env, flags: int = 0):
+        p = capi.RenderParams()
+        p.width, p.height = cfg["width"], cfg["height"]
+        p.spp, p.max_depth = cfg["spp"], cfg["max_depth"]
+        p.cam_pos[:] = list(cfg["cam_pos"])
+        p.cam_look_at[:] = list(cfg["cam_look_at"])
+        p.cam_up[:] = list(cfg["cam_up"])
+        p.fov_deg = cfg["fov_deg"]
+        p.seed, p.stream = cfg.get("seed", 1), cfg.get("stream", 0)
+        arr = (capi.Instance * max(1, len(instances)))()
+        for k, inst in enumerate(instances):
+            arr[k].grid = grids[inst.get("grid", 0)].h
+            arr[k].pos[:] = list(inst.get("pos", (0, 0, 0)))
+            arr[k].orientation[:] = list(inst.get("orientation", (0, 1, 0)))
+            arr[k].scaling[:] = list(inst.get("scaling", (1, 1, 1)))
+        p.n_instances = len(instances)
+        p.instances = arr
+        p.env = env.h
+        f = flags
+        if cfg.get("use_brick_grid", True):
+            f |= capi.R2_FLAG_BRICK_GRID
+        if cfg.get("fp64_accum", False):
+            f |= capi.R2_FLAG_FP64_ACCUM
+        p.flags = f
+        p.sample_begin = cfg.get("sample_begin", 0)
+        return p, arr
+
+    def render(self, cfg: dict, grids, instances, env) -> np.ndarray:
+        p, keep = self.render_params(cfg, grids, instances, env)
+        out = np.zeros((p.height, p.width, 3), np.float32)
+        self._check(capi.lib().r2_render(self.h, C.byref(p), out.ctypes.data, 0))
+        return out
+
+    def render_device(self, cfg: dict, grids, instances, env, out_dev_ptr: int):
+        """Render into a caller-owned device buffer (inputs and output stay in HBM)."""
+        p, keep = self.render_params(cfg, grids, instances, env)
+        self._check(capi.lib().r2_render(self.h, C.byref(p), out_dev_ptr, 1))
+
+    def render_paths(self, cfg: dict, grids, instances, env, pix, smp, counters: bool = False):
+        p, keep = self.render_params(cfg, grids, instances, env)
+        pix = np.ascontiguousarray(pix, np.int32)
+        smp = np.ascontiguousarray(smp, np.int32)
+        n = len(pix)
+        out = np.zeros((n, 3), np.float32)
+        cnt = np.zeros((n, 8), np.uint32) if counters else None
+        self._check(capi.lib().r2_render_paths(self.h, C.byref(p), n, pix.ctypes.data, smp.ctypes.data,
+                                               out.ctypes.data, cnt.ctypes.data if counters else None))
+        return (out, cnt[:, :7]) if counters else out
+
+    def stats(self) -> dict:
+        s = capi.Stats()
+        self._check(capi.lib().r2_get_stats(self.h, C.byref(s)))
+        return dict(kernel_ms=s.last_kernel_ms, total_ms=s.last_total_ms, paths=s.last_paths)
+
+    def stream(self) -> int:
+        return capi.lib().r2_stream(self.h) or 0
+
+
+class Grid:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def free(self):
+        if self.h:
+            capi.lib().r2_grid_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = capi.GridInfo()
+        capi.check(capi.lib().r2_grid_get_info(self.h, C.byref(i)))
+        return {k: (np.array(getattr(i, k)) if not isinstance(getattr(i, k), (int, float)) else getattr(i, k))
+                for k, _ in i._fields_}
+
+    def download_bricks(self):
+        inf = self.info()
+        nbx, nby, nbz = [int(x) for x in inf["n_bricks"]]
+        rng = np.zeros(nbx * nby * nbz, np.uint32)
+        u8 = np.zeros(nbx * nby * nbz * 512, np.uint8)
+        nm = sum((nbx >> (l + 1)) * (nby >> (l + 1)) * (nbz >> (l + 1)) for l in range(3))
+        mips = np.zeros(nm, np.uint32)
+        capi.check(capi.lib().r2_grid_download_bricks(self.ctx.h, self.h, rng.ctypes.data, u8.ctypes.data,
+                                                      mips.ctypes.data), self.ctx.h)
+        out_m, off = [], 0
+        for l in range(3):
+            c = (nbx >> (l + 1)) * (nby >> (l + 1)) * (nbz >> (l + 1))
+            out_m.append(mips[off:off + c])
+            off += c
+        return rng, u8, out_m
+
+    def lookup(self, coords: np.ndarray) -> np.ndarray:
+        coords = np.ascontiguousarray(coords, np.int32)
+        n = coords.shape[0]
+        out = np.zeros((n, 32), np.uint8)
+        capi.check(capi.lib().r2_grid_lookup(self.ctx.h, self.h, n, coords.ctypes.data, out.ctypes.data), self.ctx.h)
+        return out
+
+
+class Env:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def free(self):
+        if self.h:
+            capi.lib().r2_env_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

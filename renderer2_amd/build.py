@@ -1,0 +1,106 @@
+"""Build the native library (HIP for gfx950 + host C++) in-tree.
+
+    python -m renderer2_amd.build            # librenderer2.so
+    python -m renderer2_amd.build --oracle   # also the CPU oracle (test infrastructure)
+
+Output: renderer2_amd/lib/librenderer2.so (git-ignored, travels with the tree).
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "lib"
+
+SOURCES = [
+    "r2_api.cpp",
+    "r2_synth.cpp",
+    "r2_grid.hip",
+    "r2_render.hip",
+    "r2_filter.hip",
+    "r2_server.cpp",
+    "r2_frontend.cpp",
+]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("R2_OFFLOAD_ARCH", "gfx950")
+
+COMMON_FLAGS = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    "-I", str(ROOT / "include"),
+]
+
+
+def _compile(src: Path, obj: Path) -> None:
+    cmd = [HIPCC, *COMMON_FLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr[-6000:]}")
+
+
+def build_library(jobs: int | None = None, verbose: bool = True) -> Path:
+    LIB.mkdir(exist_ok=True)
+    srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
+    objs = [LIB / (s.stem + ".o") for s in srcs]
+    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(_compile, s, o): s for s, o in zip(srcs, objs)}
+        for f in cf.as_completed(futs):
+            f.result()
+            if verbose:
+                print(f"[build] compiled {futs[f].name}", flush=True)
+    out = LIB / "librenderer2.so"
+    tmp = LIB / "librenderer2.so.tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-lpthread"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{res.stderr[-6000:]}")
+    os.replace(tmp, out)
+    for o in objs:
+        o.unlink(missing_ok=True)
+    if verbose:
+        print(f"[build] linked {out}", flush=True)
+    return out
+
+
+def build_oracle(verbose: bool = True) -> Path:
+    """Compile the CPU oracle (test infrastructure) with its own Makefile."""
+    odir = ROOT / "oracle"
+    res = subprocess.run(["make", "-C", str(odir), "-j8"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout[-3000:]}\n{res.stderr[-3000:]}")
+    if verbose:
+        print(f"[build] oracle {odir / 'lib' / 'libr2oracle.so'}", flush=True)
+    return odir / "lib" / "libr2oracle.so"
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args(argv)
+    if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
+        print("hipcc not found", file=sys.stderr)
+        return 2
+    build_library(a.j)
+    if a.oracle:
+        build_oracle()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

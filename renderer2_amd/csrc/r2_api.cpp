@@ -1,0 +1,373 @@
+// C ABI glue: contexts, grid / env upload, scene assembly and the render entry
+// points of include/r2.h.  Host-side transforms follow the reference's host code
+// (camera.cu:20-37, sceneobject.cu:108-114,375-377, aabb.hpp:9-13,
+// environmentmap.cu:16-29, distribution_1d.cu:6-39, distribution_2d.cu:7-14).
+#include "r2_internal.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+using namespace r2;
+
+namespace r2host {
+
+int fail(r2_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+int hip_fail(r2_ctx* ctx, hipError_t e, const char* what) {
+  if (ctx) ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? R2_ERR_OOM : R2_ERR_HIP;
+}
+int ensure_scratch(r2_ctx* ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return R2_OK;
+  if (ctx->scratch) {
+    R2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  R2_HIP(ctx, hipMalloc(&ctx->scratch, bytes));
+  ctx->scratch_bytes = bytes;
+  return R2_OK;
+}
+
+// getRotation (functions.hpp:128-142)
+static m44 get_rotation(v3 n) {
+  const float cs = dot(n, mk(0.f, 1.f, 0.f));
+  if (cs == 1.f) return identity44();
+  if (cs == -1.f) {
+    m44 r = identity44();
+    for (int i = 0; i < 3; ++i) r.c[i][i] = -1.f;
+    return r;
+  }
+  const float angle = acosf(dot(n, mk(0.f, 1.f, 0.f)));
+  const v3 ax = normalize(cross(n, mk(0.f, 1.f, 0.f)));
+  const float c = cosf(angle), s = sinf(angle);
+  const v3 tmp = ax * (1.f - c);
+  m44 r = identity44();
+  r.c[0][0] = c + tmp.x * ax.x;
+  r.c[0][1] = tmp.x * ax.y + s * ax.z;
+  r.c[0][2] = tmp.x * ax.z - s * ax.y;
+  r.c[1][0] = tmp.y * ax.x - s * ax.z;
+  r.c[1][1] = c + tmp.y * ax.y;
+  r.c[1][2] = tmp.y * ax.z + s * ax.x;
+  r.c[2][0] = tmp.z * ax.x + s * ax.y;
+  r.c[2][1] = tmp.z * ax.y - s * ax.x;
+  r.c[2][2] = c + tmp.z * ax.z;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) r.c[i][j] = roundf(r.c[i][j]);
+  return r;
+}
+
+int build_scene(r2_ctx* ctx, const r2_render_params* p, DScene* out) {
+  if (!p || p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth < 0)
+    return fail(ctx, R2_ERR_INVALID, "render: bad frame size / spp / max_depth");
+  if ((int64_t)p->width * p->height > (1ll << 28)) return fail(ctx, R2_ERR_UNSUPPORTED, "render: frame too large");
+  if (p->n_instances < 0 || p->n_instances > kMaxInstances)
+    return fail(ctx, R2_ERR_UNSUPPORTED, "render: at most 4 medium instances");
+  if (!p->env) return fail(ctx, R2_ERR_INVALID, "render: an environment map is required (config Scene.Envmap)");
+  DScene& s = *out;
+  memset(&s, 0, sizeof(s));
+  s.W = p->width; s.H = p->height; s.spp = p->spp; s.max_depth = p->max_depth;
+  s.seed = p->seed; s.stream = p->stream; s.sample_begin = p->sample_begin; s.flags = p->flags;
+  s.env = p->env->dev;
+  // CCamera (camera.cu:20-37): glm::lookAt, then glm::inverse of the mat4x3
+  s.pixel_size = 1e-5f;
+  const float fov = p->fov_deg * 0.01745329251994329576923690768489f;
+  s.near_dist = ((float)p->width * s.pixel_size) / (2.f * tanf(fov / 2.f));
+  const v3 eye = mk(p->cam_pos[0], p->cam_pos[1], p->cam_pos[2]);
+  const v3 ctr = mk(p->cam_look_at[0], p->cam_look_at[1], p->cam_look_at[2]);
+  const v3 up = mk(p->cam_up[0], p->cam_up[1], p->cam_up[2]);
+  const v3 f = normalize(ctr - eye);
+  const v3 sv = normalize(cross(f, up));
+  const v3 u = cross(sv, f);
+  m44 M = identity44();
+  M.c[0][0] = sv.x; M.c[1][0] = sv.y; M.c[2][0] = sv.z;
+  M.c[0][1] = u.x; M.c[1][1] = u.y; M.c[2][1] = u.z;
+  M.c[0][2] = -f.x; M.c[1][2] = -f.y; M.c[2][2] = -f.z;
+  M.c[3][0] = -dot(sv, eye); M.c[3][1] = -dot(u, eye); M.c[3][2] = dot(f, eye);
+  s.view_to_world = to43(inverse44(to44(to43(M))));
+  // instances (CHostSceneobject(medium, pos, orientation, scaling), sceneobject.cu:50-65)
+  s.n_inst = p->n_instances;
+  for (int k = 0; k < p->n_instances; ++k) {
+    const r2_instance& id = p->instances[k];
+    if (!id.grid) return fail(ctx, R2_ERR_INVALID, "render: instance without grid");
+    s.grid[k] = id.grid->dev;
+    DInstance& in = s.inst[k];
+    in.grid = k;
+    m44 T = identity44();
+    T.c[3][0] = id.pos[0]; T.c[3][1] = id.pos[1]; T.c[3][2] = id.pos[2];
+    const m44 R = get_rotation(mk(id.orientation[0], id.orientation[1], id.orientation[2]));
+    m44 S = identity44();
+    S.c[0][0] = id.scaling[0]; S.c[1][1] = id.scaling[1]; S.c[2][2] = id.scaling[2];
+    in.m2w = to43(mul44(mul44(T, R), S));
+    in.w2m = inverse43(in.m2w);
+    const r2_grid_info& gi = id.grid->info;
+    const v3 a = mul_point(in.m2w, mk(gi.bbox_min[0], gi.bbox_min[1], gi.bbox_min[2]));
+    const v3 b = mul_point(in.m2w, mk(gi.bbox_max[0], gi.bbox_max[1], gi.bbox_max[2]));
+    in.bb_min[0] = fminf(a.x, b.x); in.bb_min[1] = fminf(a.y, b.y); in.bb_min[2] = fminf(a.z, b.z);
+    in.bb_max[0] = fmaxf(a.x, b.x); in.bb_max[1] = fmaxf(a.y, b.y); in.bb_max[2] = fmaxf(a.z, b.z);
+    // getFilterRenderRatio (sceneobject.cu:108-114)
+    const float vsr = (in.bb_max[0] - in.bb_min[0]) / (float)gi.size[0];
+    in.ratio = id.grid->voxel_size_filtering / vsr;
+  }
+  return R2_OK;
+}
+
+}  // namespace r2host
+
+using namespace r2host;
+
+extern "C" {
+
+int r2_version(void) { return R2_VERSION; }
+
+int r2_create(r2_ctx** out, int device) {
+  if (!out) return R2_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return R2_ERR_HIP;
+  if (device < 0 || device >= n) return R2_ERR_INVALID;
+  r2_ctx* c = new (std::nothrow) r2_ctx();
+  if (!c) return R2_ERR_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipMalloc((void**)&c->d_scene, sizeof(DScene)) != hipSuccess || hipMalloc((void**)&c->d_counter, 64) != hipSuccess) {
+    r2_destroy(c);
+    return R2_ERR_HIP;
+  }
+  *out = c;
+  return R2_OK;
+}
+
+int r2_destroy(r2_ctx* c) {
+  if (!c) return R2_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->scratch) (void)hipFree(c->scratch);
+  if (c->d_scene) (void)hipFree(c->d_scene);
+  if (c->d_counter) (void)hipFree(c->d_counter);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return R2_OK;
+}
+
+const char* r2_last_error(const r2_ctx* c) { return c ? c->err.c_str() : "null context"; }
+void* r2_stream(r2_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int r2_grid_upload(r2_ctx* ctx, const r2_grid_desc* d, r2_grid** out) {
+  if (!ctx || !d || !out) return R2_ERR_INVALID;
+  *out = nullptr;
+  if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !d->records) return fail(ctx, R2_ERR_INVALID, "grid: bad size / records");
+  if (d->nx > 8192 || d->ny > 8192 || d->nz > 8192) return fail(ctx, R2_ERR_UNSUPPORTED, "grid: at most 8192 voxels per axis");
+  if (d->precision != R2_PREC_FP32 && d->precision != R2_PREC_PACKED) return fail(ctx, R2_ERR_INVALID, "grid: bad precision");
+  (void)hipSetDevice(ctx->device);
+  r2_grid* g = new (std::nothrow) r2_grid();
+  if (!g) return R2_ERR_OOM;
+  const int rc = grid_build(ctx, d, g);
+  if (rc) {
+    r2_grid_free(ctx, g);
+    return rc;
+  }
+  *out = g;
+  return R2_OK;
+}
+
+int r2_grid_free(r2_ctx* ctx, r2_grid* g) {
+  if (!g) return R2_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : g->allocs)
+    if (p) (void)hipFree(p);
+  delete g;
+  return R2_OK;
+}
+
+int r2_grid_get_info(const r2_grid* g, r2_grid_info* out) {
+  if (!g || !out) return R2_ERR_INVALID;
+  *out = g->info;
+  return R2_OK;
+}
+
+int r2_grid_download_bricks(r2_ctx* ctx, const r2_grid* g, uint32_t* range, uint8_t* u8, uint32_t* mips) {
+  if (!ctx || !g) return R2_ERR_INVALID;
+  const int* nb = g->info.n_bricks;
+  const size_t nbt = (size_t)nb[0] * nb[1] * nb[2];
+  hipStream_t st = ctx->stream;
+  if (range) R2_HIP(ctx, hipMemcpyAsync(range, g->range0, nbt * 4, hipMemcpyDeviceToHost, st));
+  if (mips) {
+    size_t off = 0;
+    for (int l = 0; l < 3; ++l) {
+      const size_t cnt = (size_t)(nb[0] >> (l + 1)) * (nb[1] >> (l + 1)) * (nb[2] >> (l + 1));
+      R2_HIP(ctx, hipMemcpyAsync(mips + off, g->mip[l], cnt * 4, hipMemcpyDeviceToHost, st));
+      off += cnt;
+    }
+  }
+  if (u8) {
+    std::vector<uint2> bricks(nbt);
+    const size_t nne = (size_t)std::max<int64_t>(g->info.n_nonempty_bricks, 1);
+    std::vector<uint8_t> atlas(nne * 512);
+    R2_HIP(ctx, hipMemcpyAsync(bricks.data(), g->dev.bricks, nbt * sizeof(uint2), hipMemcpyDeviceToHost, st));
+    R2_HIP(ctx, hipMemcpyAsync(atlas.data(), g->dev.atlas, nne * 512, hipMemcpyDeviceToHost, st));
+    R2_HIP(ctx, hipStreamSynchronize(st));
+    const size_t ex = (size_t)nb[0] * 8, ey = (size_t)nb[1] * 8, ez = (size_t)nb[2] * 8;
+    memset(u8, 0, ex * ey * ez);
+    for (size_t bi = 0; bi < nbt; ++bi) {
+      if (bricks[bi].y == 0xffffffffu) continue;
+      const size_t bx = bi % nb[0], by = (bi / nb[0]) % nb[1], bz = bi / ((size_t)nb[0] * nb[1]);
+      const uint8_t* src = atlas.data() + (size_t)bricks[bi].y * 512;
+      for (int v = 0; v < 512; ++v) {
+        const size_t x = bx * 8 + (v & 7), y = by * 8 + ((v >> 3) & 7), z = bz * 8 + (v >> 6);
+        u8[(z * ey + y) * ex + x] = src[v];
+      }
+    }
+  }
+  R2_HIP(ctx, hipStreamSynchronize(st));
+  return R2_OK;
+}
+
+int r2_grid_lookup(r2_ctx* ctx, const r2_grid* g, int32_t n, const int32_t* coords, uint8_t* out) {
+  if (!ctx || !g || n < 0 || (n > 0 && (!coords || !out))) return R2_ERR_INVALID;
+  if (n == 0) return R2_OK;
+  if (g->dev.precision != R2_PREC_FP32) return fail(ctx, R2_ERR_UNSUPPORTED, "grid_lookup: FP32 records only");
+  return grid_lookup(ctx, g, n, coords, out);
+}
+
+int r2_env_upload(r2_ctx* ctx, const float* rgb, int32_t W, int32_t H, r2_env** out) {
+  if (!ctx || !rgb || !out || W < 2 || H < 2) return fail(ctx, R2_ERR_INVALID, "env: need at least 2x2 texels");
+  *out = nullptr;
+  const size_t n = (size_t)W * H;
+  std::vector<float> row_func(n), row_cdf((size_t)(W + 1) * H), row_int(H), marg_func(H), marg_cdf(H + 1);
+  // radiancePerPixel (environmentmap.cu:16-29)
+  for (int j = 0; j < H; ++j) {
+    const double st = sin(3.14159265358979323846 * (double)((float)j + 0.5f) / (double)(float)H);
+    for (int i = 0; i < W; ++i) {
+      const size_t p = (size_t)j * W + i;
+      const float rad = rgb[3 * p] + rgb[3 * p + 1] + rgb[3 * p + 2];
+      row_func[p] = (float)((double)rad * st);
+    }
+  }
+  // CDistribution1D ctor (distribution_1d.cu:6-39)
+  auto build_1d = [](const float* f, int nn, float* cdf) -> float {
+    cdf[0] = 0.f;
+    for (int i = 1; i <= nn; ++i) cdf[i] = cdf[i - 1] + f[i - 1] / (float)nn;
+    const float fi = cdf[nn];
+    if (fi == 0.f) {
+      for (int i = 1; i <= nn; ++i) cdf[i] = (float)i / (float)nn;
+    } else {
+      for (int i = 1; i <= nn; ++i) cdf[i] /= fi;
+    }
+    return fi;
+  };
+  // CDistribution2D::sumRows (distribution_2d.hpp:59-71): pushes at i % W == 0
+  {
+    float sum = 0.f;
+    int k = 0;
+    for (size_t i = 0; i < n; ++i) {
+      sum += row_func[i];
+      if (i % (size_t)W == 0) {
+        marg_func[k++] = sum;
+        sum = 0.f;
+      }
+    }
+  }
+  const float marg_int = build_1d(marg_func.data(), H, marg_cdf.data());
+  for (int j = 0; j < H; ++j) row_int[j] = build_1d(row_func.data() + (size_t)j * W, W, row_cdf.data() + (size_t)j * (W + 1));
+  r2_env* e = new (std::nothrow) r2_env();
+  if (!e) return R2_ERR_OOM;
+  auto up = [&](const void* src, size_t bytes) -> void* {
+    void* d = nullptr;
+    if (hipMalloc(&d, bytes) != hipSuccess) return nullptr;
+    e->allocs.push_back(d);
+    if (hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+  };
+  e->dev.w = W;
+  e->dev.h = H;
+  e->dev.tex = (const float*)up(rgb, n * 12);
+  e->dev.row_func = (const float*)up(row_func.data(), n * 4);
+  e->dev.row_cdf = (const float*)up(row_cdf.data(), row_cdf.size() * 4);
+  e->dev.row_int = (const float*)up(row_int.data(), H * 4);
+  e->dev.marg_func = (const float*)up(marg_func.data(), H * 4);
+  e->dev.marg_cdf = (const float*)up(marg_cdf.data(), (H + 1) * 4);
+  e->dev.marg_int = marg_int;
+  if (!e->dev.tex || !e->dev.row_func || !e->dev.row_cdf || !e->dev.row_int || !e->dev.marg_func || !e->dev.marg_cdf) {
+    r2_env_free(ctx, e);
+    return fail(ctx, R2_ERR_HIP, "env: device upload failed");
+  }
+  *out = e;
+  return R2_OK;
+}
+
+int r2_env_free(r2_ctx* ctx, r2_env* e) {
+  if (!e) return R2_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : e->allocs) (void)hipFree(p);
+  delete e;
+  return R2_OK;
+}
+
+int r2_render(r2_ctx* ctx, const r2_render_params* p, float* out, int32_t out_on_device) {
+  if (!ctx || !p || !out) return R2_ERR_INVALID;
+  const auto t0 = std::chrono::steady_clock::now();
+  (void)hipSetDevice(ctx->device);
+  DScene s;
+  int rc = build_scene(ctx, p, &s);
+  if (rc) return rc;
+  const size_t bytes = (size_t)p->width * p->height * 3 * sizeof(float);
+  float* d_out = out;
+  void* tmp = nullptr;
+  if (!out_on_device) {
+    R2_HIP(ctx, hipMalloc(&tmp, bytes));
+    d_out = (float*)tmp;
+  }
+  rc = launch_render(ctx, s, d_out, (p->flags & R2_FLAG_FP64_ACCUM) != 0);
+  if (rc == R2_OK && !out_on_device) {
+    hipError_t e = hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "hipMemcpy(framebuffer)");
+  }
+  if (tmp) (void)hipFree(tmp);
+  ctx->stats.last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int r2_render_paths(r2_ctx* ctx, const r2_render_params* p, int32_t n, const int32_t* pix, const int32_t* smp,
+                    float* out, uint32_t* counters) {
+  if (!ctx || !p || n < 0 || !pix || !smp || !out) return R2_ERR_INVALID;
+  if (n == 0) return R2_OK;
+  (void)hipSetDevice(ctx->device);
+  DScene s;
+  int rc = build_scene(ctx, p, &s);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i)
+    if (pix[i] < 0 || pix[i] >= p->width * p->height) return fail(ctx, R2_ERR_INVALID, "render_paths: pixel out of range");
+  void *dp = nullptr, *ds = nullptr, *dout = nullptr, *dc = nullptr;
+  struct F {
+    void** q[4];
+    ~F() { for (auto x : q) if (*x) (void)hipFree(*x); }
+  } f{{&dp, &ds, &dout, &dc}};
+  R2_HIP(ctx, hipMalloc(&dp, (size_t)n * 4));
+  R2_HIP(ctx, hipMalloc(&ds, (size_t)n * 4));
+  R2_HIP(ctx, hipMalloc(&dout, (size_t)n * 12));
+  if (counters) R2_HIP(ctx, hipMalloc(&dc, (size_t)n * kNumCnt * 4));
+  R2_HIP(ctx, hipMemcpy(dp, pix, (size_t)n * 4, hipMemcpyHostToDevice));
+  R2_HIP(ctx, hipMemcpy(ds, smp, (size_t)n * 4, hipMemcpyHostToDevice));
+  rc = launch_paths(ctx, s, n, (const int32_t*)dp, (const int32_t*)ds, (float*)dout, (uint32_t*)dc);
+  if (rc) return rc;
+  R2_HIP(ctx, hipMemcpy(out, dout, (size_t)n * 12, hipMemcpyDeviceToHost));
+  if (counters) R2_HIP(ctx, hipMemcpy(counters, dc, (size_t)n * kNumCnt * 4, hipMemcpyDeviceToHost));
+  return R2_OK;
+}
+
+int r2_get_stats(const r2_ctx* ctx, r2_stats* out) {
+  if (!ctx || !out) return R2_ERR_INVALID;
+  *out = ctx->stats;
+  return R2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,408 @@
+// Grid upload on the GPU: applies the reference's active-voxel rule, lays the
+// records out as a NanoVDB-style leaf pool (8^3 leaves) behind a dense leaf
+// table, finds the active index bbox and the max density, and builds the brick
+// majorant grid (dilated half ranges, u8 atlas, 3 min/max mips).
+//
+// Restates: COpenvdbBackend::setValues (raytracer/src/filtering/openvdb_backend.cpp:108-132),
+// CNVDBMedium::init / getMaxValue (raytracer/src/medium/nvdb_medium.cu:130-173,308-323),
+// CHostBrickGrid::CHostBrickGrid (raytracer/src/grid_brick/host_grid_brick.cpp:61-152).
+#include "r2_internal.hpp"
+#include <algorithm>
+#include <cstring>
+
+namespace {
+
+using namespace r2;
+
+__device__ __forceinline__ bool is_corner(int x, int y, int z, int nx, int ny, int nz) {
+  return (x == 0 || x == nx - 1) && (y == 0 || y == ny - 1) && (z == 0 || z == nz - 1);
+}
+
+// effective density (active rule) + leaf occupancy + bbox + max density
+__global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, float* __restrict__ dens,
+                         uint8_t* __restrict__ active, uint32_t* __restrict__ leaf_flag, int nlx, int nly,
+                         int* __restrict__ bbox, uint32_t* __restrict__ max_bits) {
+  const size_t n = (size_t)nx * ny * nz;
+  float lmax = 0.f;
+  int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    const float d = __uint_as_float(rec[2 * i].x);
+    const bool a = (d > 0.f) || is_corner(x, y, z, nx, ny, nz);
+    active[i] = a ? 1 : 0;
+    dens[i] = a ? d : 0.f;
+    if (a) {
+      leaf_flag[((size_t)(z >> 3) * nly + (y >> 3)) * nlx + (x >> 3)] = 1u;
+      lmax = fmaxf(lmax, d);
+      lo[0] = min(lo[0], x); lo[1] = min(lo[1], y); lo[2] = min(lo[2], z);
+      hi[0] = max(hi[0], x); hi[1] = max(hi[1], y); hi[2] = max(hi[2], z);
+    }
+  }
+  // wave-level reduction, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    lmax = fmaxf(lmax, __shfl_xor(lmax, off));
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = min(lo[a], __shfl_xor(lo[a], off));
+      hi[a] = max(hi[a], __shfl_xor(hi[a], off));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(max_bits, __float_as_uint(fmaxf(lmax, 0.f)));
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&bbox[a], lo[a]);
+      atomicMax(&bbox[3 + a], hi[a]);
+    }
+  }
+}
+
+// scatter records into the leaf pool; inactive voxels of an allocated leaf hold the background record
+__global__ void k_leaves(const uint4* __restrict__ rec, const uint8_t* __restrict__ active, int nx, int ny, int nz,
+                         const int32_t* __restrict__ leaf_table, int nlx, int nly, int nlz, uint4* __restrict__ leaves) {
+  const size_t nleaf_total = (size_t)nlx * nly * nlz;
+  for (size_t li = blockIdx.x; li < nleaf_total; li += gridDim.x) {
+    const int32_t slot = leaf_table[li];
+    if (slot < 0) continue;
+    const int lx = (int)(li % nlx), ly = (int)((li / nlx) % nly), lz = (int)(li / ((size_t)nlx * nly));
+    for (int v = threadIdx.x; v < kLeafVoxels; v += blockDim.x) {
+      const int x = lx * 8 + (v & 7), y = ly * 8 + ((v >> 3) & 7), z = lz * 8 + (v >> 6);
+      uint4 a = make_uint4(0u, 0u, 0u, 0u), b = make_uint4(0u, 0u, 0u, 0x3C00u);
+      if (x < nx && y < ny && z < nz) {
+        const size_t i = ((size_t)z * ny + y) * nx + x;
+        if (active[i]) {
+          a = rec[2 * i];
+          b = rec[2 * i + 1];
+        }
+      }
+      uint4* dst = leaves + ((size_t)slot * kLeafVoxels + v) * 2;
+      dst[0] = a;
+      dst[1] = b;
+    }
+  }
+}
+
+__device__ __forceinline__ float dens_at(const float* __restrict__ dens, int nx, int ny, int nz, int x, int y, int z) {
+  if ((unsigned)x >= (unsigned)nx || (unsigned)y >= (unsigned)ny || (unsigned)z >= (unsigned)nz) return 0.f;
+  return dens[((size_t)z * ny + y) * nx + x];
+}
+
+// one wavefront per brick: min/max over the dilated 10^3 neighbourhood
+__global__ void k_brick_range(const float* __restrict__ dens, int nx, int ny, int nz, int nbx, int nby, int nbz,
+                              uint32_t* __restrict__ range, uint32_t* __restrict__ nonempty) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nbt = nbx * nby * nbz;
+  if (wave >= nbt) return;
+  const int bx = wave % nbx, by = (wave / nbx) % nby, bz = wave / (nbx * nby);
+  float lo = FLT_MAX, hi = -FLT_MAX;
+  for (int k = lane; k < 1000; k += 64) {
+    const int x = k % 10 - 1, y = (k / 10) % 10 - 1, z = k / 100 - 1;
+    const float d = dens_at(dens, nx, ny, nz, bx * 8 + x, by * 8 + y, bz * 8 + z);
+    lo = fminf(lo, d);
+    hi = fmaxf(hi, d);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fminf(lo, __shfl_xor(lo, off));
+    hi = fmaxf(hi, __shfl_xor(hi, off));
+  }
+  if (lane == 0) {
+    range[wave] = encode_range(lo, hi);
+    nonempty[wave] = (hi == lo) ? 0u : 1u;
+  }
+}
+
+// one wavefront per brick: quantise the 512 voxels of every non-empty brick
+__global__ void k_brick_atlas(const float* __restrict__ dens, int nx, int ny, int nz, int nbx, int nby, int nbz,
+                              const uint32_t* __restrict__ range, const uint32_t* __restrict__ slot,
+                              const uint32_t* __restrict__ nonempty, uint2* __restrict__ bricks,
+                              uint8_t* __restrict__ atlas, float* __restrict__ maj0) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nbt = nbx * nby * nbz;
+  if (wave >= nbt) return;
+  const int bx = wave % nbx, by = (wave / nbx) % nby, bz = wave / (nbx * nby);
+  const uint32_t r = range[wave];
+  const bool ne = nonempty[wave] != 0u;
+  if (lane == 0) {
+    bricks[wave] = make_uint2(r, ne ? slot[wave] : 0xffffffffu);
+    maj0[wave] = range_hi(r);
+  }
+  if (!ne) return;
+  const float lo = range_lo(r), hi = range_hi(r);
+  uint8_t* dst = atlas + (size_t)slot[wave] * 512;
+  for (int v = lane; v < 512; v += 64) {
+    const int x = v & 7, y = (v >> 3) & 7, z = v >> 6;
+    dst[v] = encode_voxel(dens_at(dens, nx, ny, nz, bx * 8 + x, by * 8 + y, bz * 8 + z), lo, hi);
+  }
+}
+
+// min/max mip level from the level below (range encoded), plus its decoded max
+__global__ void k_brick_mip(const uint32_t* __restrict__ src, int px, int py, uint32_t* __restrict__ dst, int sx,
+                            int sy, int sz, float* __restrict__ maj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= sx * sy * sz) return;
+  const int x = i % sx, y = (i / sx) % sy, z = i / (sx * sy);
+  float lo = FLT_MAX, hi = -FLT_MAX;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const uint32_t c = src[((size_t)(2 * z + dz) * py + (2 * y + dy)) * px + (2 * x + dx)];
+        lo = fminf(lo, range_lo(c));
+        hi = fmaxf(hi, range_hi(c));
+      }
+  const uint32_t e = encode_range(lo, hi);
+  dst[i] = e;
+  maj[i] = range_hi(e);
+}
+
+// parity probe: the record the collapsed tree returns for integer voxel coordinates
+__global__ void k_lookup(DGrid g, int n, const int32_t* __restrict__ xyz, uint4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  int32_t leaf = -1;
+  if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny && (unsigned)z < (unsigned)g.nz)
+    leaf = g.leaf_table[((size_t)(z >> 3) * g.nly + (y >> 3)) * g.nlx + (x >> 3)];
+  uint4 a = make_uint4(0u, 0u, 0u, 0u), b = make_uint4(0u, 0u, 0u, 0x3C00u);
+  if (leaf >= 0) {
+    const uint4* p = g.leaves + ((size_t)leaf * kLeafVoxels + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7))) * 2;
+    a = p[0];
+    b = p[1];
+  }
+  out[2 * i] = a;
+  out[2 * i + 1] = b;
+}
+
+template <typename T>
+int dmalloc(r2_ctx* ctx, r2_grid* g, T** p, size_t n) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) return r2host::hip_fail(ctx, e, "hipMalloc(grid)");
+  g->allocs.push_back(q);
+  *p = (T*)q;
+  return R2_OK;
+}
+
+}  // namespace
+
+namespace r2host {
+
+int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
+  using namespace r2;
+  const int nx = desc->nx, ny = desc->ny, nz = desc->nz;
+  const size_t nv = (size_t)nx * ny * nz;
+  hipStream_t st = ctx->stream;
+  int rc;
+  // records staged in device memory
+  const uint4* d_rec = nullptr;
+  void* staged = nullptr;
+  if (desc->records_on_device) {
+    d_rec = (const uint4*)desc->records;
+  } else {
+    R2_HIP(ctx, hipMalloc(&staged, nv * 32));
+    hipError_t e = hipMemcpyAsync(staged, desc->records, nv * 32, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) { (void)hipFree(staged); return hip_fail(ctx, e, "hipMemcpyAsync(records)"); }
+    d_rec = (const uint4*)staged;
+  }
+  struct Tmp {
+    std::vector<void*> p;
+    ~Tmp() { for (void* q : p) (void)hipFree(q); }
+  } tmp;
+  if (staged) tmp.p.push_back(staged);
+  auto tmalloc = [&](void** q, size_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 1));
+    if (e == hipSuccess) tmp.p.push_back(*q);
+    return e;
+  };
+  const int nlx = (nx + 7) / 8, nly = (ny + 7) / 8, nlz = (nz + 7) / 8;
+  const size_t nleaf_total = (size_t)nlx * nly * nlz;
+  float* d_dens = nullptr;
+  uint8_t* d_active = nullptr;
+  uint32_t* d_leaf_flag = nullptr;
+  int* d_bbox = nullptr;
+  uint32_t* d_max = nullptr;
+  R2_HIP(ctx, tmalloc((void**)&d_dens, nv * 4));
+  R2_HIP(ctx, tmalloc((void**)&d_active, nv));
+  R2_HIP(ctx, tmalloc((void**)&d_leaf_flag, nleaf_total * 4));
+  R2_HIP(ctx, tmalloc((void**)&d_bbox, 6 * 4));
+  R2_HIP(ctx, tmalloc((void**)&d_max, 4));
+  const int bbox_init[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+  R2_HIP(ctx, hipMemcpyAsync(d_bbox, bbox_init, sizeof(bbox_init), hipMemcpyHostToDevice, st));
+  R2_HIP(ctx, hipMemsetAsync(d_max, 0, 4, st));
+  R2_HIP(ctx, hipMemsetAsync(d_leaf_flag, 0, nleaf_total * 4, st));
+  {
+    const int blocks = (int)std::min<size_t>((nv + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_active, dim3(blocks), dim3(256), 0, st, d_rec, nx, ny, nz, d_dens, d_active, d_leaf_flag, nlx,
+                       nly, d_bbox, d_max);
+    R2_HIP(ctx, hipGetLastError());
+  }
+  // leaf allocation: deterministic prefix order over the leaf table
+  std::vector<uint32_t> flags(nleaf_total);
+  int bbox[6];
+  uint32_t maxbits = 0;
+  R2_HIP(ctx, hipMemcpyAsync(flags.data(), d_leaf_flag, nleaf_total * 4, hipMemcpyDeviceToHost, st));
+  R2_HIP(ctx, hipMemcpyAsync(bbox, d_bbox, sizeof(bbox), hipMemcpyDeviceToHost, st));
+  R2_HIP(ctx, hipMemcpyAsync(&maxbits, d_max, 4, hipMemcpyDeviceToHost, st));
+  R2_HIP(ctx, hipStreamSynchronize(st));
+  std::vector<int32_t> table(nleaf_total);
+  int64_t nleaves = 0;
+  for (size_t i = 0; i < nleaf_total; ++i) table[i] = flags[i] ? (int32_t)(nleaves++) : -1;
+  if (nleaves == 0) return fail(ctx, R2_ERR_INVALID, "grid has no active voxel");
+
+  DGrid& d = g->dev;
+  d.nx = nx; d.ny = ny; d.nz = nz;
+  d.nlx = nlx; d.nly = nly; d.nlz = nlz;
+  d.precision = desc->precision;
+  int32_t* d_table = nullptr;
+  if ((rc = dmalloc(ctx, g, &d_table, nleaf_total))) return rc;
+  R2_HIP(ctx, hipMemcpyAsync(d_table, table.data(), nleaf_total * 4, hipMemcpyHostToDevice, st));
+  d.leaf_table = d_table;
+  uint4* d_leaves = nullptr;
+  if ((rc = dmalloc(ctx, g, &d_leaves, (size_t)nleaves * kLeafVoxels * 2))) return rc;
+  hipLaunchKernelGGL(k_leaves, dim3((unsigned)std::min<size_t>(nleaf_total, 65536)), dim3(256), 0, st, d_rec, d_active,
+                     nx, ny, nz, d_table, nlx, nly, nlz, d_leaves);
+  R2_HIP(ctx, hipGetLastError());
+  d.leaves = d_leaves;
+  d.leaves_packed = nullptr;
+
+  // brick grid
+  int nb[3];
+  const int n3[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    const int b = (int)ceilf((float)n3[a] / 8.f);
+    nb[a] = (int)ceilf((float)b / 8.f) * 8;
+    if (nb[a] >= 1024) return fail(ctx, R2_ERR_UNSUPPORTED, "exceeded max brick count of 1024 per axis");
+  }
+  const size_t nbt = (size_t)nb[0] * nb[1] * nb[2];
+  uint32_t *d_range = nullptr, *d_nonempty = nullptr, *d_slot = nullptr;
+  R2_HIP(ctx, tmalloc((void**)&d_range, nbt * 4));
+  R2_HIP(ctx, tmalloc((void**)&d_nonempty, nbt * 4));
+  R2_HIP(ctx, tmalloc((void**)&d_slot, nbt * 4));
+  {
+    const size_t threads = nbt * 64;
+    hipLaunchKernelGGL(k_brick_range, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, d_dens, nx, ny, nz,
+                       nb[0], nb[1], nb[2], d_range, d_nonempty);
+    R2_HIP(ctx, hipGetLastError());
+  }
+  std::vector<uint32_t> ne(nbt), slot(nbt);
+  R2_HIP(ctx, hipMemcpyAsync(ne.data(), d_nonempty, nbt * 4, hipMemcpyDeviceToHost, st));
+  R2_HIP(ctx, hipStreamSynchronize(st));
+  uint32_t nne = 0;
+  for (size_t i = 0; i < nbt; ++i) slot[i] = ne[i] ? nne++ : 0u;
+  R2_HIP(ctx, hipMemcpyAsync(d_slot, slot.data(), nbt * 4, hipMemcpyHostToDevice, st));
+  uint2* d_bricks = nullptr;
+  uint8_t* d_atlas = nullptr;
+  float* d_maj[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t* d_mip[3] = {nullptr, nullptr, nullptr};
+  if ((rc = dmalloc(ctx, g, &d_bricks, nbt))) return rc;
+  if ((rc = dmalloc(ctx, g, &d_atlas, (size_t)std::max<uint32_t>(nne, 1) * 512))) return rc;
+  if ((rc = dmalloc(ctx, g, &d_maj[0], nbt))) return rc;
+  {
+    const size_t threads = nbt * 64;
+    hipLaunchKernelGGL(k_brick_atlas, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, d_dens, nx, ny, nz,
+                       nb[0], nb[1], nb[2], d_range, d_slot, d_nonempty, d_bricks, d_atlas, d_maj[0]);
+    R2_HIP(ctx, hipGetLastError());
+  }
+  for (int l = 0; l < 3; ++l) {
+    const int sx = nb[0] >> (l + 1), sy = nb[1] >> (l + 1), sz = nb[2] >> (l + 1);
+    const int px = nb[0] >> l, py = nb[1] >> l;
+    const size_t cnt = (size_t)sx * sy * sz;
+    if ((rc = dmalloc(ctx, g, &d_mip[l], cnt))) return rc;
+    if ((rc = dmalloc(ctx, g, &d_maj[l + 1], cnt))) return rc;
+    const uint32_t* src = l == 0 ? d_range : d_mip[l - 1];
+    hipLaunchKernelGGL(k_brick_mip, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, src, px, py, d_mip[l], sx, sy,
+                       sz, d_maj[l + 1]);
+    R2_HIP(ctx, hipGetLastError());
+  }
+  // keep the level-0 ranges for the parity probe (d_range is temporary)
+  uint32_t* d_range_keep = nullptr;
+  if ((rc = dmalloc(ctx, g, &d_range_keep, nbt))) return rc;
+  R2_HIP(ctx, hipMemcpyAsync(d_range_keep, d_range, nbt * 4, hipMemcpyDeviceToDevice, st));
+  d.bricks = d_bricks;
+  d.atlas = d_atlas;
+  for (int l = 0; l < 4; ++l) d.maj[l] = d_maj[l];
+  for (int a = 0; a < 3; ++a) d.nb[a] = nb[a];
+  R2_HIP(ctx, hipStreamSynchronize(st));
+
+  // transforms (CNVDBMedium::init, nvdb_medium.cu:152-170)
+  r2_grid_info& info = g->info;
+  for (int a = 0; a < 3; ++a) {
+    info.ibb_min[a] = bbox[a];
+    info.size[a] = bbox[3 + a] - bbox[a] + 1;
+    info.n_bricks[a] = nb[a];
+    d.ibb_min[a] = info.ibb_min[a];
+    d.size[a] = info.size[a];
+  }
+  m44 nano = identity44();
+  const float* mf = desc->map_mat;
+  nano.c[0][0] = mf[0]; nano.c[0][1] = mf[3]; nano.c[0][2] = mf[6];
+  nano.c[1][0] = mf[1]; nano.c[1][1] = mf[4]; nano.c[1][2] = mf[7];
+  nano.c[2][0] = mf[2]; nano.c[2][1] = mf[5]; nano.c[2][2] = mf[8];
+  nano.c[3][0] = desc->map_vec[0]; nano.c[3][1] = desc->map_vec[1]; nano.c[3][2] = desc->map_vec[2];
+  m44 i2n = identity44();
+  i2n.c[0][0] = (float)info.size[0]; i2n.c[1][1] = (float)info.size[1]; i2n.c[2][2] = (float)info.size[2];
+  i2n.c[3][0] = (float)info.ibb_min[0]; i2n.c[3][1] = (float)info.ibb_min[1]; i2n.c[3][2] = (float)info.ibb_min[2];
+  const m43 i2m = to43(mul44(nano, i2n));
+  const m43 m2i = inverse43(i2m);
+  m43 i2s;
+  for (int k = 0; k < 12; ++k) i2s.m[k] = 0.f;
+  i2s.m[0] = (float)info.size[0]; i2s.m[4] = (float)info.size[1]; i2s.m[8] = (float)info.size[2];
+  i2s.m[9] = (float)info.ibb_min[0]; i2s.m[10] = (float)info.ibb_min[1]; i2s.m[11] = (float)info.ibb_min[2];
+  const m43 s2i = inverse43(i2s);
+  const m43 m2s = to43(mul44(to44(i2s), to44(m2i)));
+  const m43 s2m = to43(mul44(to44(i2m), to44(s2i)));
+  d.i2m = i2m; d.m2i = m2i; d.m2s = m2s; d.s2m = s2m;
+  memcpy(info.index_to_model, i2m.m, 48);
+  memcpy(info.model_to_index, m2i.m, 48);
+  memcpy(info.model_to_scaled, m2s.m, 48);
+  memcpy(info.scaled_to_model, s2m.m, 48);
+  const v3 c0 = mul_point(i2m, mk(0.f, 0.f, 0.f)), c1 = mul_point(i2m, mk(1.f, 1.f, 1.f));
+  info.bbox_min[0] = fminf(c0.x, c1.x); info.bbox_min[1] = fminf(c0.y, c1.y); info.bbox_min[2] = fminf(c0.z, c1.z);
+  info.bbox_max[0] = fmaxf(c0.x, c1.x); info.bbox_max[1] = fmaxf(c0.y, c1.y); info.bbox_max[2] = fmaxf(c0.z, c1.z);
+  float maxd;
+  memcpy(&maxd, &maxbits, 4);
+  info.max_density = maxd;
+  info.inv_max_density = 1.f / maxd;
+  info.n_leaves = nleaves;
+  info.n_nonempty_bricks = nne;
+  int64_t bytes = 0;
+  bytes += (int64_t)nleaf_total * 4 + nleaves * kLeafVoxels * 32 + (int64_t)nbt * (8 + 4 + 4) + (int64_t)nne * 512;
+  for (int l = 0; l < 3; ++l) bytes += (int64_t)(nb[0] >> (l + 1)) * (nb[1] >> (l + 1)) * (nb[2] >> (l + 1)) * 8;
+  info.device_bytes = bytes;
+  d.inv_max_density = info.inv_max_density;
+  // glm::max(a, glm::max(b, c)) = (x < y) ? y : x
+  {
+    const float a = desc->sigma_a[0] + desc->sigma_s[0], b = desc->sigma_a[1] + desc->sigma_s[1],
+                c = desc->sigma_a[2] + desc->sigma_s[2];
+    const float bc = (b < c) ? c : b;
+    d.sigma_t = (a < bc) ? bc : a;
+  }
+  for (int a = 0; a < 3; ++a) {
+    d.sigma_s[a] = desc->sigma_s[a];
+    g->sigma_a[a] = desc->sigma_a[a];
+    g->sigma_s[a] = desc->sigma_s[a];
+  }
+  g->voxel_size_filtering = desc->voxel_size_filtering;
+  // level-0 ranges and mip buffers stay reachable for r2_grid_download_bricks
+  g->range0 = d_range_keep;
+  for (int l = 0; l < 3; ++l) g->mip[l] = d_mip[l];
+  return R2_OK;
+}
+
+int grid_lookup(r2_ctx* ctx, const r2_grid* g, int n, const int32_t* coords, uint8_t* out) {
+  void *dc = nullptr, *dout = nullptr;
+  struct F {
+    void** q[2];
+    ~F() { for (auto x : q) if (*x) (void)hipFree(*x); }
+  } f{{&dc, &dout}};
+  R2_HIP(ctx, hipMalloc(&dc, (size_t)n * 12));
+  R2_HIP(ctx, hipMalloc(&dout, (size_t)n * 32));
+  R2_HIP(ctx, hipMemcpy(dc, coords, (size_t)n * 12, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_lookup, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, g->dev, n, (const int32_t*)dc,
+                     (uint4*)dout);
+  R2_HIP(ctx, hipGetLastError());
+  R2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  R2_HIP(ctx, hipMemcpy(out, dout, (size_t)n * 32, hipMemcpyDeviceToHost));
+  return R2_OK;
+}
+
+}  // namespace r2host

@@ -1,0 +1,57 @@
+// Internal host-side declarations shared by the translation units of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "../../include/r2.h"
+#include "r2_types.hpp"
+
+struct r2_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  r2_stats stats{};
+  // reusable device scratch for renders
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  r2::DScene* d_scene = nullptr;
+  uint32_t* d_counter = nullptr;
+};
+
+struct r2_grid {
+  r2::DGrid dev{};            // device view (pointers into `allocs`)
+  r2_grid_info info{};
+  std::vector<void*> allocs;  // owned device allocations
+  uint32_t* range0 = nullptr;  // level-0 brick ranges (also in allocs)
+  uint32_t* mip[3] = {nullptr, nullptr, nullptr};
+  float voxel_size_filtering = 0.f;
+  float sigma_a[3] = {0, 0, 0};
+  float sigma_s[3] = {0, 0, 0};
+};
+
+struct r2_env {
+  r2::DEnv dev{};
+  std::vector<void*> allocs;
+};
+
+namespace r2host {
+int fail(r2_ctx* ctx, int code, const std::string& msg);
+int hip_fail(r2_ctx* ctx, hipError_t e, const char* what);
+// grid build (r2_grid.hip)
+int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g);
+int grid_lookup(r2_ctx* ctx, const r2_grid* g, int n, const int32_t* coords, uint8_t* out);
+// render kernels (r2_render.hip)
+int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum);
+int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
+                 uint32_t* d_cnt);
+// scene assembly (r2_api.cpp)
+int build_scene(r2_ctx* ctx, const r2_render_params* p, r2::DScene* out);
+int ensure_scratch(r2_ctx* ctx, size_t bytes);
+}  // namespace r2host
+
+#define R2_HIP(ctx, call)                                   \
+  do {                                                      \
+    hipError_t _e = (call);                                 \
+    if (_e != hipSuccess) return r2host::hip_fail((ctx), _e, #call); \
+  } while (0)

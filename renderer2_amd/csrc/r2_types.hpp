@@ -1,0 +1,73 @@
+// Device-side scene description shared by the render kernels and the host glue.
+#pragma once
+#include "r2_math.hpp"
+
+namespace r2 {
+
+constexpr int kMaxInstances = 4;
+constexpr int kLeafDim = 8;            // NanoVDB leaf node: 8^3 voxels
+constexpr int kLeafVoxels = 512;
+constexpr float kDefaultTmax = 1e12f;  // CRay::DEFAULT_TMAX (ray.hpp:12)
+constexpr float kRayOffset = 1e-4f;    // CRay::OFFSET (ray.hpp:11)
+constexpr int kLoopCap = 65536;        // safety cap on every traversal loop
+
+// One filtered volume resident in HBM.
+//  * records: NanoVDB-layout sparse tree collapsed to one dense leaf table
+//    (leaf index per 8^3 node, -1 = background tile) + a pool of leaves, each
+//    512 voxel records contiguous (x fastest) so a wavefront's trilinear
+//    footprints fall into a few 32-byte-aligned lines.
+//  * brick majorant grid (8^3 bricks = the same nodes): per brick an encoded half
+//    (min, max) range and an atlas slot; a u8 atlas of 512 bytes per non-empty
+//    brick; and the max of the range per mip level 0..3 as float.
+struct DGrid {
+  const int32_t* leaf_table;
+  const uint4* leaves;          // 2 x uint4 (32 B) per voxel record (FP32 form)
+  const uint4* leaves_packed;   // 1 x uint4 (16 B) per voxel record (packed form)
+  int32_t nx, ny, nz;
+  int32_t nlx, nly, nlz;
+  const uint2* bricks;          // x = encoded range, y = atlas slot (0xffffffff: empty)
+  const uint8_t* atlas;
+  const float* maj[4];
+  int32_t nb[3];
+  int32_t precision;
+  m43 m2i, i2m, m2s, s2m;
+  int32_t ibb_min[3], size[3];
+  float sigma_s[3];
+  float sigma_t;
+  float inv_max_density;
+};
+
+struct DInstance {
+  int32_t grid;
+  m43 m2w, w2m;
+  float bb_min[3], bb_max[3];
+  float ratio;  // filterRenderRatio
+};
+
+struct DEnv {
+  int32_t w, h;
+  const float* tex;        // h * w * 3
+  const float* row_func;   // h * w
+  const float* row_cdf;    // h * (w + 1)
+  const float* row_int;    // h
+  const float* marg_func;  // h
+  const float* marg_cdf;   // h + 1
+  float marg_int;
+};
+
+struct DScene {
+  int32_t n_inst;
+  DInstance inst[kMaxInstances];
+  DGrid grid[kMaxInstances];
+  DEnv env;
+  m43 view_to_world;
+  float pixel_size, near_dist;
+  int32_t W, H, spp, max_depth;
+  uint32_t seed, stream;
+  int32_t sample_begin;
+  uint32_t flags;
+};
+
+enum { kCntVox = 0, kCntMaj, kCntBrick, kCntEnv, kCntDda, kCntFlight, kCntScatter, kNumCnt = 8 };
+
+}  // namespace r2

@@ -1,0 +1,879 @@
+// Device algorithms of the volume integrator: NanoVDB-layout voxel lookups,
+// brick majorant DDA, delta / ratio tracking, SGGX microflake phase, environment
+// map importance sampling and the path integrator.  Each function restates the
+// reference routine it cites; arithmetic keeps the reference's evaluation order.
+#pragma once
+#include "r2_types.hpp"
+
+namespace r2 {
+
+struct Cnt {
+  uint32_t c[kNumCnt];
+};
+
+template <bool kCount>
+__device__ __forceinline__ void count(Cnt& c, int k, uint32_t v = 1) {
+  if constexpr (kCount) c.c[k] += v;
+}
+
+struct Ray {
+  v3 o, d;
+  float tmax;
+  int medium;  // instance index, -1 = none
+};
+__device__ __forceinline__ Ray mkray(v3 o, v3 d, float tmax, int medium) {
+  Ray r; r.o = o; r.d = d; r.tmax = tmax; r.medium = medium; return r;
+}
+// CRay::transform (ray.hpp:41-47)
+__device__ __forceinline__ Ray ray_transform(const Ray& r, const m43& M) {
+  const v3 o = mul_point(M, r.o);
+  const v3 e = mul_point(M, r.o + r.tmax * r.d);
+  const v3 d = normalize(mul_dir(M, r.d));
+  return mkray(o, d, length(e - o), r.medium);
+}
+// CRay::transform2 (ray.hpp:49-55)
+__device__ __forceinline__ Ray ray_transform2(const Ray& r, const m43& M) {
+  const v3 o = mul_point(M, r.o);
+  const v3 e = mul_point(M, r.o + r.tmax * r.d);
+  const v3 d = mul_dir(M, r.d);
+  return mkray(o, d, length(e - o) / length(d), r.medium);
+}
+// nextFloatUp / nextFloatDown (functions.hpp:94-126): zero is not bumped
+__device__ __forceinline__ float next_up(float v) {
+  if (isinf(v) && v > 0.f) return v;
+  if (v == 0.f) return 0.f;
+  uint32_t u = f2u(v);
+  u = (v >= 0.f) ? u + 1u : u - 1u;
+  return u2f(u);
+}
+__device__ __forceinline__ float next_dn(float v) {
+  if (isinf(v) && v < 0.f) return v;
+  if (v == 0.f) return -0.f;
+  uint32_t u = f2u(v);
+  u = (v > 0.f) ? u - 1u : u + 1u;
+  return u2f(u);
+}
+// CRay::offsetRayOrigin (ray.hpp:71-84)
+__device__ __forceinline__ v3 offset_point(v3 p, v3 dir) {
+  const float d = fabsf(dir.x) * kRayOffset + fabsf(dir.y) * kRayOffset + fabsf(dir.z) * kRayOffset;
+  const v3 off = dir * d;
+  v3 po = p + off;
+  po.x = off.x > 0.f ? next_up(po.x) : (off.x < 0.f ? next_dn(po.x) : po.x);
+  po.y = off.y > 0.f ? next_up(po.y) : (off.y < 0.f ? next_dn(po.y) : po.y);
+  po.z = off.z > 0.f ? next_up(po.z) : (off.z < 0.f ? next_dn(po.z) : po.z);
+  return po;
+}
+__device__ __forceinline__ void ray_offset(Ray& r, v3 dir) {
+  const v3 end = r.o + r.tmax * r.d;
+  r.o = offset_point(r.o, dir);
+  r.tmax = length(end - r.o);
+}
+// CRay::spawnRay (ray.hpp:62-69)
+__device__ __forceinline__ Ray spawn_ray(v3 start, v3 end, int medium) {
+  v3 dir = end - start;
+  const v3 off = offset_point(start, normalize(end - start));
+  const float t = length(end - off);
+  dir = dir / t;
+  return mkray(off, dir, t, medium);
+}
+
+// ------------------------------------------------------------ records --
+struct Rec {
+  float density;
+  float S[6];  // xx yy zz xy xz yz
+  v3 diffuse, specular, normal;
+  float ior;
+};
+
+// SFilteredData(const SFilteredDataCompact&) (filtered_data.hpp:63-89), from the
+// eight 32-bit words of one 32-byte record.
+__device__ __forceinline__ Rec decode_record(uint4 a, uint4 b) {
+  Rec r;
+  const float inv16 = 1.f / 65535.f;
+  const float inv8 = 1.f / 255.f;
+  r.density = u2f(a.x);
+  const float sx = (float)(a.y & 0xffffu) * inv16, sy = (float)(a.y >> 16) * inv16;
+  const float sz = (float)(a.z & 0xffffu) * inv16;
+  const float rxy = 2.f * ((float)(a.z >> 16) * inv16) - 1.f;
+  const float rxz = 2.f * ((float)(a.w & 0xffffu) * inv16) - 1.f;
+  const float ryz = 2.f * ((float)(a.w >> 16) * inv16) - 1.f;
+  r.S[0] = sx * sx; r.S[1] = sy * sy; r.S[2] = sz * sz;
+  r.S[3] = rxy * sx * sy; r.S[4] = rxz * sx * sz; r.S[5] = ryz * sy * sz;
+  r.diffuse = mk((float)(b.x & 0xffu) * inv8, (float)((b.x >> 8) & 0xffu) * inv8, (float)((b.x >> 16) & 0xffu) * inv8);
+  r.specular = mk((float)(b.x >> 24) * inv8, (float)(b.y & 0xffu) * inv8, (float)((b.y >> 8) & 0xffu) * inv8);
+  const float nx = (float)(b.y >> 16), ny = (float)(b.z & 0xffffu), nz = (float)(b.z >> 16);
+  r.normal = normalize(mk(2.f * nx * inv16 - 1.f, 2.f * ny * inv16 - 1.f, 2.f * nz * inv16 - 1.f));
+  r.ior = half_to_float((uint16_t)(b.w & 0xffffu));
+  return r;
+}
+
+// Packed 16-byte record: FP16 density | 6 SGGX coefficients (FP16 diagonal,
+// BF16 off-diagonal) | FP16 ior | RGB565 diffuse + RGB565 specular | oct-16 normal.
+__device__ __forceinline__ Rec decode_packed(uint4 p) {
+  Rec r;
+  r.density = half_to_float((uint16_t)(p.x & 0xffffu));
+  r.S[0] = half_to_float((uint16_t)(p.x >> 16));
+  r.S[1] = half_to_float((uint16_t)(p.y & 0xffffu));
+  r.S[2] = half_to_float((uint16_t)(p.y >> 16));
+  r.S[3] = bf16_to_float((uint16_t)(p.z & 0xffffu));
+  r.S[4] = bf16_to_float((uint16_t)(p.z >> 16));
+  r.S[5] = bf16_to_float((uint16_t)(p.w & 0xffffu));
+  const uint32_t q = p.w >> 16;
+  // q: 5-bit diffuse grey | 5-bit specular grey | ... see pack_record (r2_grid.hip)
+  r.ior = 1.f + (float)(q & 0x3fu) * (1.f / 63.f);
+  const float dg = (float)((q >> 6) & 0x1fu) * (1.f / 31.f);
+  const float sg = (float)((q >> 11) & 0x1fu) * (1.f / 31.f);
+  r.diffuse = mk(dg, dg, dg);
+  r.specular = mk(sg, sg, sg);
+  r.normal = mk(0.f, 1.f, 0.f);
+  return r;
+}
+
+// operator*(float, SFilteredData) and operator+ (filtered_data.hpp:115-151);
+// interpolate(t, a, b) = (1 - t) * a + t * b (functions.hpp:66-72)
+__device__ __forceinline__ Rec rec_lerp(float t, const Rec& a, const Rec& b) {
+  const float wa = 1.f - t;
+  Rec o;
+  o.density = wa * a.density + t * b.density;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) o.S[i] = wa * a.S[i] + t * b.S[i];
+  o.diffuse = wa * a.diffuse + t * b.diffuse;
+  o.specular = wa * a.specular + t * b.specular;
+  o.normal = normalize(wa * normalize(a.normal) + t * normalize(b.normal));
+  o.ior = wa * a.ior + t * b.ior;
+  return o;
+}
+
+// NanoVDB ReadAccessor<Vec4d>::getValue over the collapsed tree.
+template <bool kCount>
+__device__ __forceinline__ Rec get_value(const DGrid& g, int x, int y, int z, Cnt& cnt) {
+  count<kCount>(cnt, kCntVox);
+  int32_t leaf = -1;
+  if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny && (unsigned)z < (unsigned)g.nz)
+    leaf = g.leaf_table[((size_t)(z >> 3) * g.nly + (y >> 3)) * g.nlx + (x >> 3)];
+  if (g.precision == 1) {
+    uint4 p = make_uint4(0u, 0u, 0u, 0u);
+    if (leaf >= 0) p = g.leaves_packed[(size_t)leaf * kLeafVoxels + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7))];
+    else p.w = 0u;  // background: ior code 0 -> 1.0
+    return decode_packed(p);
+  }
+  uint4 a = make_uint4(0u, 0u, 0u, 0u), b = make_uint4(0u, 0u, 0u, 0x3C00u);  // background: ior = 1
+  if (leaf >= 0) {
+    const uint4* p = g.leaves + ((size_t)leaf * kLeafVoxels + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7))) * 2;
+    a = p[0];
+    b = p[1];
+  }
+  return decode_record(a, b);
+}
+
+// CNVDBMedium::filteredData (nvdb_medium_impl.hpp:47-64), p normalised in [0,1]^3
+template <bool kCount>
+__device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
+  const float psx = p.x * (float)g.size[0] + (float)g.ibb_min[0] - 0.5f;
+  const float psy = p.y * (float)g.size[1] + (float)g.ibb_min[1] - 0.5f;
+  const float psz = p.z * (float)g.size[2] + (float)g.ibb_min[2] - 0.5f;
+  const int x = (int)floorf(psx), y = (int)floorf(psy), z = (int)floorf(psz);
+  const float dx = psx - (float)x, dy = psy - (float)y, dz = psz - (float)z;
+  Rec a = get_value<kCount>(g, x, y, z, cnt), b = get_value<kCount>(g, x + 1, y, z, cnt);
+  const Rec d00 = rec_lerp(dx, a, b);
+  a = get_value<kCount>(g, x, y + 1, z, cnt); b = get_value<kCount>(g, x + 1, y + 1, z, cnt);
+  const Rec d10 = rec_lerp(dx, a, b);
+  a = get_value<kCount>(g, x, y, z + 1, cnt); b = get_value<kCount>(g, x + 1, y, z + 1, cnt);
+  const Rec d01 = rec_lerp(dx, a, b);
+  a = get_value<kCount>(g, x, y + 1, z + 1, cnt); b = get_value<kCount>(g, x + 1, y + 1, z + 1, cnt);
+  const Rec d11 = rec_lerp(dx, a, b);
+  const Rec d0 = rec_lerp(dy, d00, d10);
+  const Rec d1 = rec_lerp(dy, d01, d11);
+  return rec_lerp(dz, d0, d1);
+}
+
+// CDeviceBrickGrid::lookupMajorant (device_grid_brick_impl.hpp:29-33)
+template <bool kCount>
+__device__ __forceinline__ float lookup_majorant(const DGrid& g, v3 ipos, int mip, Cnt& cnt) {
+  count<kCount>(cnt, kCntMaj);
+  const int bx = ((int)floorf(ipos.x)) >> 3, by = ((int)floorf(ipos.y)) >> 3, bz = ((int)floorf(ipos.z)) >> 3;
+  if ((unsigned)bx >= (unsigned)g.nb[0] || (unsigned)by >= (unsigned)g.nb[1] || (unsigned)bz >= (unsigned)g.nb[2])
+    return 0.f;
+  const int sx = g.nb[0] >> mip, sy = g.nb[1] >> mip;
+  return g.maj[mip][((size_t)(bz >> mip) * sy + (by >> mip)) * sx + (bx >> mip)];
+}
+// CDeviceBrickGrid::lookupDensity (device_grid_brick_impl.hpp:9-27)
+template <bool kCount>
+__device__ __forceinline__ float lookup_density(const DGrid& g, v3 ipos, v3 u, Cnt& cnt) {
+  count<kCount>(cnt, kCntBrick);
+  const int ix = (int)floorf(ipos.x + u.x - 0.5f), iy = (int)floorf(ipos.y + u.y - 0.5f),
+            iz = (int)floorf(ipos.z + u.z - 0.5f);
+  const int bx = ix >> 3, by = iy >> 3, bz = iz >> 3;
+  if ((unsigned)bx >= (unsigned)g.nb[0] || (unsigned)by >= (unsigned)g.nb[1] || (unsigned)bz >= (unsigned)g.nb[2])
+    return 0.f;
+  const uint2 br = g.bricks[((size_t)bz * g.nb[1] + by) * g.nb[0] + bx];
+  const float lo = range_lo(br.x), hi = range_hi(br.x);
+  uint8_t v = 0;
+  if (br.y != 0xffffffffu) v = g.atlas[(size_t)br.y * 512 + (((iz & 7) << 6) | ((iy & 7) << 3) | (ix & 7))];
+  return decode_voxel(v, lo, hi);
+}
+
+// CNVDBMedium::stepDDA (nvdb_medium_impl.hpp:241-246)
+__device__ __forceinline__ float step_dda(v3 pos, v3 inv_dir, int mip) {
+  const float dim = (float)(8 << mip);
+  const float ox = inv_dir.x >= 0.f ? dim + 0.5f : -0.5f;
+  const float oy = inv_dir.y >= 0.f ? dim + 0.5f : -0.5f;
+  const float oz = inv_dir.z >= 0.f ? dim + 0.5f : -0.5f;
+  const float inv_dim = 1.f / dim;
+  const float tx = (floorf(pos.x * inv_dim) * dim + ox - pos.x) * inv_dir.x;
+  const float ty = (floorf(pos.y * inv_dim) * dim + oy - pos.y) * inv_dir.y;
+  const float tz = (floorf(pos.z * inv_dim) * dim + oz - pos.z) * inv_dir.z;
+  const float myz = (tz < ty) ? tz : ty;
+  return (myz < tx) ? myz : tx;
+}
+
+// CFresnel::evaluate (fresnel.hpp:28-54), eta_i = 1, eta_t = ior
+__device__ __forceinline__ float fresnel(float cos_i, float ior) {
+  cos_i = cos_i < -1.f ? -1.f : (cos_i > 1.f ? 1.f : cos_i);
+  float eta_i = 1.f, eta_t = ior;
+  if (!(cos_i > 0.f)) {
+    eta_i = ior; eta_t = 1.f;
+    cos_i = fabsf(cos_i);
+  }
+  const float sin_i = sqrtf(gmax(0.f, 1.f - cos_i * cos_i));
+  const float sin_t = eta_i / eta_t * sin_i;
+  if (sin_t >= 1.f) return 1.f;
+  const float cos_t = sqrtf(gmax(0.f, 1.f - sin_t * sin_t));
+  const float rpar = ((eta_t * cos_i) - (eta_i * cos_t)) / ((eta_t * cos_i) + (eta_i * cos_t));
+  const float rper = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
+  return (rpar * rpar + rper * rper) / 2.f;
+}
+
+struct MHit {
+  int valid;
+  int inst;
+  v3 pos, normal;
+  float S[6];
+  float ior;
+};
+
+// tail of sampleInternal / sampleDDA at a real collision (nvdb_medium_impl.hpp:120-131,169-180)
+__device__ __forceinline__ v3 collision_albedo(const DGrid& g, const Rec& fd, const Ray& rm, MHit& mi) {
+  mi.valid = 1;
+  mi.pos = rm.o + rm.tmax * rm.d;
+  const v3 n = normalize(fd.normal);
+  mi.normal = n;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) mi.S[i] = fd.S[i];
+  mi.ior = fd.ior;
+  const float w = fresnel(fabsf(dot(normalize(-rm.d), n)), fd.ior);
+  const v3 b = fd.diffuse * (1.f - w) + fd.specular * w;
+  return (b * mk(g.sigma_s[0], g.sigma_s[1], g.sigma_s[2])) / g.sigma_t;
+}
+
+// CNVDBMedium::sampleInternal (nvdb_medium_impl.hpp:94-137)
+template <bool kCount>
+__device__ v3 sample_internal(const DGrid& g, Ray& rmodel, Rng& rng, float ratio, MHit& mi, Cnt& cnt) {
+  Ray ray = ray_transform2(rmodel, g.m2i);
+  float t = 0.f;
+  const float inv_max = 1.f / (ratio / g.inv_max_density);
+  for (int it = 0; it < kLoopCap; ++it) {
+    count<kCount>(cnt, kCntFlight);
+    t -= logf(1.f - rng.uni()) * inv_max / g.sigma_t;
+    if (t >= ray.tmax) break;
+    const Rec fd = filtered_data<kCount>(g, ray.o + t * ray.d, cnt);
+    if (ratio * fd.density * inv_max > rng.uni()) {
+      ray.tmax = t;
+      const Ray rw = ray_transform2(ray, g.i2m);
+      rmodel.tmax = rw.tmax;
+      return collision_albedo(g, fd, rw, mi);
+    }
+  }
+  return mk(1.f, 1.f, 1.f);
+}
+
+// CNVDBMedium::sampleDDA (nvdb_medium_impl.hpp:139-187)
+template <bool kCount>
+__device__ v3 sample_dda(const DGrid& g, Ray& rmodel, Rng& rng, float ratio, MHit& mi, Cnt& cnt) {
+  Ray iray = ray_transform2(rmodel, g.m2s);
+  const v3 ri = mk(1.f / iray.d.x, 1.f / iray.d.y, 1.f / iray.d.z);
+  float t = 0.f;
+  float tau = -logf(1.f - rng.uni());
+  float mip = 3.f;
+  for (int it = 0; it < kLoopCap && t < iray.tmax; ++it) {
+    count<kCount>(cnt, kCntDda);
+    const v3 cur = iray.o + t * iray.d;
+    const int imip = (int)roundf(mip);
+    const float maj = ratio * lookup_majorant<kCount>(g, cur, imip, cnt);
+    const float dt = step_dda(cur, ri, imip);
+    const float prev = t;
+    t += dt / g.sigma_t;
+    if (t == prev) break;
+    tau -= maj * dt;
+    mip = gmin(mip + 0.25f, 3.f);
+    if (tau > 0.f) continue;
+    t += tau / (maj * g.sigma_t);
+    if (t >= iray.tmax) break;
+    const float u0 = rng.uni(), u1 = rng.uni(), u2 = rng.uni();
+    const float d = lookup_density<kCount>(g, iray.o + t * iray.d, mk(u0, u1, u2), cnt);
+    if (rng.uni() * maj < d * ratio) {
+      iray.tmax = t;
+      const v3 sp = iray.o + t * iray.d;
+      const v3 pos = mk((sp.x - (float)g.ibb_min[0]) / (float)g.size[0], (sp.y - (float)g.ibb_min[1]) / (float)g.size[1],
+                        (sp.z - (float)g.ibb_min[2]) / (float)g.size[2]);
+      const Rec fd = filtered_data<kCount>(g, pos, cnt);
+      const Ray rw = ray_transform2(iray, g.s2m);
+      rmodel.tmax = rw.tmax;
+      return collision_albedo(g, fd, rw, mi);
+    }
+    tau = -logf(1.f - rng.uni());
+    mip = gmax(0.f, mip - 2.f);
+  }
+  return mk(1.f, 1.f, 1.f);
+}
+
+// CNVDBMedium::trInternal (nvdb_medium_impl.hpp:203-239)
+template <bool kCount>
+__device__ float tr_internal(const DGrid& g, const Ray& rmodel, Rng& rng, float ratio, Cnt& cnt) {
+  const Ray ray = ray_transform2(rmodel, g.m2i);
+  float tr = 1.f, t = 0.f;
+  const float inv_max = 1.f / (ratio / g.inv_max_density);
+  for (int it = 0; it < kLoopCap; ++it) {
+    count<kCount>(cnt, kCntFlight);
+    t -= logf(1.f - rng.uni()) * inv_max / g.sigma_t;
+    if (t >= ray.tmax) break;
+    const float d = filtered_data<kCount>(g, ray.o + t * ray.d, cnt).density;
+    tr *= 1.f - gmax(0.f, ratio * d * inv_max);
+    if (tr < 1.f) {
+      const float p = 1.f - tr;
+      if (rng.uni() < p) return 0.f;
+      tr /= 1.f - p;
+    }
+  }
+  return tr;
+}
+
+// CNVDBMedium::trDDA (nvdb_medium_impl.hpp:248-294)
+template <bool kCount>
+__device__ float tr_dda(const DGrid& g, const Ray& rmodel, Rng& rng, float ratio, Cnt& cnt) {
+  const Ray iray = ray_transform2(rmodel, g.m2s);
+  const v3 ri = mk(1.f / iray.d.x, 1.f / iray.d.y, 1.f / iray.d.z);
+  float tr = 1.f, t = 0.f;
+  const float inv_max = 1.f / (ratio / g.inv_max_density);
+  float tau = -logf(1.f - rng.uni());
+  float mip = 3.f;
+  const float vol_maj = 1.f / inv_max;
+  for (int it = 0; it < kLoopCap && t < iray.tmax; ++it) {
+    count<kCount>(cnt, kCntDda);
+    const v3 cur = iray.o + t * iray.d;
+    const int imip = (int)roundf(mip);
+    const float maj = ratio * lookup_majorant<kCount>(g, cur, imip, cnt);
+    const float dt = step_dda(cur, ri, imip);
+    const float prev = t;
+    t += dt / g.sigma_t;
+    if (t == prev) break;
+    tau -= maj * dt;
+    mip = gmin(mip + 0.25f, 3.f);
+    if (tau > 0.f) continue;
+    t += tau / (maj * g.sigma_t);
+    if (t >= iray.tmax) break;
+    const float u0 = rng.uni(), u1 = rng.uni(), u2 = rng.uni();
+    const float d = lookup_density<kCount>(g, iray.o + t * iray.d, mk(u0, u1, u2), cnt);
+    if (rng.uni() * maj < d * ratio) {
+      tr *= gmax(0.f, 1.f - vol_maj / maj);
+      if (tr < 0.1f) {
+        const float prob = 1.f - tr;
+        if (rng.uni() < prob) return 0.f;
+        tr /= 1.f - prob;
+      }
+    }
+    tau = -logf(1.f - rng.uni());
+    mip = gmax(0.f, mip - 2.f);
+  }
+  return tr;
+}
+
+// CNVDBMedium::moveToVoxelBorder (nvdb_medium_impl.hpp:296-320), ray in model space
+__device__ __forceinline__ Ray medium_move_to_voxel_border(const DGrid& g, const Ray& ray) {
+  const Ray ri = ray_transform(ray, g.m2i);
+  const float gamma = (3.f * FLT_EPSILON) / (1.f - 3.f * FLT_EPSILON);
+  float t1 = ri.tmax;
+  const float o[3] = {ri.o.x, ri.o.y, ri.o.z};
+  const float d[3] = {ri.d.x, ri.d.y, ri.d.z};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float size = (float)g.size[i], ibb = (float)g.ibb_min[i];
+    const float ip = o[i] * size + ibb;
+    const float vmin = (floorf(ip) - ibb) / size;
+    const float vmax = (ceilf(ip) - ibb) / size;
+    const float inv = 1.f / d[i];
+    float tn = (vmin - o[i]) * inv;
+    float tf = (vmax - o[i]) * inv;
+    if (tn > tf) { const float tmp = tn; tn = tf; tf = tmp; }
+    tf *= 1.f + 2.f * gamma;
+    t1 = tf < t1 ? tf : t1;
+  }
+  t1 *= (1.f - kRayOffset);
+  const Ray moved = mkray(ri.o + t1 * ri.d, ri.d, ri.tmax - t1, ri.medium);
+  Ray r = ray_transform(moved, g.i2m);
+  ray_offset(r, -ray.d);
+  return r;
+}
+
+// ----------------------------------------------------------------- SGGX --
+struct Sggx {
+  float S[3][3];
+  float invS[3][3];
+  v3 n;
+};
+__device__ __forceinline__ float det3(const float m[3][3]) {
+  return m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+         m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+}
+__device__ __forceinline__ v3 mul3(const float m[3][3], v3 v) {
+  return mk(m[0][0] * v.x + m[1][0] * v.y + m[2][0] * v.z, m[0][1] * v.x + m[1][1] * v.y + m[2][1] * v.z,
+            m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z);
+}
+// CSGGXMicroflakeDistribution ctor (sggx_phase_function.hpp:89-93): S and glm::inverse(S)
+__device__ __forceinline__ Sggx sggx_make(const float S6[6], v3 n) {
+  Sggx g;
+  g.S[0][0] = S6[0]; g.S[0][1] = S6[3]; g.S[0][2] = S6[4];
+  g.S[1][0] = S6[3]; g.S[1][1] = S6[1]; g.S[1][2] = S6[5];
+  g.S[2][0] = S6[4]; g.S[2][1] = S6[5]; g.S[2][2] = S6[2];
+  const float(*m)[3] = g.S;
+  const float o = 1.f / det3(m);
+  g.invS[0][0] = +(m[1][1] * m[2][2] - m[2][1] * m[1][2]) * o;
+  g.invS[1][0] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) * o;
+  g.invS[2][0] = +(m[1][0] * m[2][1] - m[2][0] * m[1][1]) * o;
+  g.invS[0][1] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) * o;
+  g.invS[1][1] = +(m[0][0] * m[2][2] - m[2][0] * m[0][2]) * o;
+  g.invS[2][1] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) * o;
+  g.invS[0][2] = +(m[0][1] * m[1][2] - m[1][1] * m[0][2]) * o;
+  g.invS[1][2] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) * o;
+  g.invS[2][2] = +(m[0][0] * m[1][1] - m[1][0] * m[0][1]) * o;
+  g.n = n;
+  return g;
+}
+constexpr float kPi = 3.14159265358979323846f;
+// D (sggx_phase_function.hpp:107-110)
+__device__ __forceinline__ float sggx_D(const Sggx& g, v3 w) {
+  const float v = dot(w, mul3(g.invS, w));
+  return 1.f / (kPi * sqrtf(fabsf(det3(g.S))) * v * v);
+}
+// sigma (sggx_phase_function.hpp:112-114)
+__device__ __forceinline__ float sggx_sigma(const Sggx& g, v3 w) { return sqrtf(gmax(dot(w, mul3(g.S, w)), 0.f)); }
+// CCoordinateFrame::fromNormal2 (objects.hpp:42-58)
+__device__ __forceinline__ void frame_frisvad(v3 N, v3& T, v3& B) {
+  if (N.z < -0.9999999f) {
+    T = mk(0.f, -1.f, 0.f);
+    B = mk(-1.f, 0.f, 0.f);
+  } else {
+    const float a = 1.f / (1.f + N.z);
+    const float b = -N.x * N.y * a;
+    T = normalize(mk(1.f - N.x * N.x * a, b, -N.x));
+    B = normalize(mk(b, 1.f - N.y * N.y * a, -N.y));
+  }
+}
+// CCoordinateFrame::fromNormal (objects.hpp:28-40)
+__device__ __forceinline__ void frame_basic(v3 N, v3& T, v3& B) {
+  if (fabsf(N.x) > fabsf(N.y))
+    T = mk(-N.z, 0.f, N.x) / sqrtf(N.x * N.x + N.z * N.z);
+  else
+    T = mk(0.f, N.z, -N.y) / sqrtf(N.y * N.y + N.z * N.z);
+  B = normalize(cross(N, T));
+}
+// sampleVNDF (sggx_phase_function.hpp:116-145), about the voxel normal
+__device__ __forceinline__ v3 sggx_vndf(const Sggx& g, float ux, float uy) {
+  v3 wj, wk;
+  frame_frisvad(g.n, wj, wk);
+  const v3 wi = g.n;
+  const v3 Swk = mul3(g.S, wk), Swj = mul3(g.S, wj), Swi = mul3(g.S, wi);
+  const float Skk = dot(wk, Swk), Skj = dot(wk, Swj), Ski = dot(wk, Swi);
+  const float Sjj = dot(wj, Swj), Sji = dot(wj, Swi), Sii = dot(wi, Swi);
+  float P[3][3];
+  P[0][0] = Skk; P[0][1] = Skj; P[0][2] = Ski;
+  P[1][0] = Skj; P[1][1] = Sjj; P[1][2] = Sji;
+  P[2][0] = Ski; P[2][1] = Sji; P[2][2] = Sii;
+  const float tmp = sqrtf(Sjj * Sii - Sji * Sji);
+  const v3 Mk = mk(sqrtf(fabsf(det3(P))) / tmp, 0.f, 0.f);
+  const float sc = 1.f / sqrtf(Sii);
+  const v3 Mj = mk(-(Ski * Sji - Skj * Sii) / tmp, tmp, 0.f) * sc;
+  const v3 Mi = mk(Ski, Sji, Sii) * sc;
+  const float sux = sqrtf(ux);
+  const float u = sux * cosf(2.f * kPi * uy);
+  const float v = sux * sinf(2.f * kPi * uy);
+  const float w = sqrtf(1.f - u * u - v * v);
+  v3 wm = (u * Mk + v * Mj) + w * Mi;
+  wm = normalize(wm);
+  const v3 r = mk(wk.x * wm.x + wj.x * wm.y + wi.x * wm.z, wk.y * wm.x + wj.y * wm.y + wi.y * wm.z,
+                  wk.z * wm.x + wj.z * wm.y + wi.z * wm.z);
+  return normalize(r);
+}
+// CSGGXSpecularPhaseFunction::p (sggx_phase_function.hpp:168-171)
+__device__ __forceinline__ float sggx_spec_p(const Sggx& g, v3 wo, v3 wi) {
+  const v3 wh = normalize(wi + wo);
+  return sggx_D(g, wh) / (4.f * sggx_sigma(g, wo));
+}
+// concentricSampleDisk + cosineSampleHemisphere (sampler.hpp:52-80)
+__device__ __forceinline__ v3 cosine_hemisphere(Rng& rng) {
+  const float r1 = rng.uni(), r2 = rng.uni();
+  const float ox = 2.f * r1 - 1.f, oy = 2.f * r2 - 1.f;
+  v3 d = mk(0.f, 0.f, 0.f);
+  if (!(ox == 0.f && oy == 0.f)) {
+    float theta, r;
+    if (fabsf(ox) > fabsf(oy)) {
+      r = ox;
+      theta = (kPi / 4.f) * (oy / ox);
+    } else {
+      r = oy;
+      theta = kPi / 2.f - (kPi / 4.f) * (ox / oy);
+    }
+    d = r * mk(cosf(theta), sinf(theta), 0.f);
+  }
+  const float z = sqrtf(gmax(0.f, 1.f - d.x * d.x - d.y * d.y));
+  return mk(d.x, d.y, z);
+}
+// CSGGXPhaseFunction::p (sggx_phase_function.hpp:214-218)
+__device__ __forceinline__ float phase_p(const float S6[6], v3 n, float ior, v3 wo, v3 wi, Rng& rng) {
+  const Sggx g = sggx_make(S6, n);
+  const float w = fresnel(fabsf(dot(normalize(wo), n)), ior);
+  const float ux = rng.uni(), uy = rng.uni();
+  const v3 wm = sggx_vndf(g, ux, uy);
+  const float pd = gmax(dot(wi, wm), 0.f) / kPi;
+  return pd * (1.f - w) + sggx_spec_p(g, wo, wi) * w;
+}
+// CSGGXPhaseFunction::sampleP (sggx_phase_function.hpp:220-229)
+__device__ __forceinline__ float phase_sample(const float S6[6], v3 n, float ior, v3 wo, v3& wi, Rng& rng) {
+  const Sggx g = sggx_make(S6, n);
+  const float pf = fresnel(fabsf(dot(normalize(wo), n)), ior);
+  if (rng.uni() < pf) {
+    const float ux = rng.uni(), uy = rng.uni();
+    const v3 wm = sggx_vndf(g, ux, uy);
+    const v3 I = -wo;
+    wi = I - (wm * dot(wm, I)) * 2.f;
+    return sggx_spec_p(g, wo, wi);
+  }
+  const float ux = rng.uni(), uy = rng.uni();
+  const v3 wn = sggx_vndf(g, ux, uy);
+  v3 T, B;
+  frame_basic(wn, T, B);
+  const v3 wm = cosine_hemisphere(rng);
+  const v3 ww = mk(T.x * wm.x + B.x * wm.y + wn.x * wm.z, T.y * wm.x + B.y * wm.y + wn.y * wm.z,
+                   T.z * wm.x + B.z * wm.y + wn.z * wm.z);
+  wi = normalize(ww);
+  return gmax(dot(wi, wn), 0.f) / kPi;
+}
+
+// ------------------------------------------------------------------ env --
+__device__ __forceinline__ int find_interval(const float* cdf, int size, float u) {
+  int first = 0, len = size;
+  while (len > 0) {
+    const int half = len >> 1, mid = first + half;
+    if (cdf[mid] <= u) {
+      first = mid + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  const int r = first - 1;
+  return r < 0 ? 0 : (r > size - 2 ? size - 2 : r);
+}
+// CDistribution1D::sampleContinuous (distribution_1d.hpp:54-75)
+__device__ __forceinline__ float sample_1d(const float* f, const float* cdf, float fint, int n, float u, float& pdf,
+                                           int& off) {
+  const int o = find_interval(cdf, n + 1, u);
+  off = o;
+  const float c0 = cdf[o], c1 = cdf[o + 1];
+  float du = u - c0;
+  if ((c1 - c0) > 0.f) du /= (c1 - c0);
+  pdf = f[o] / fint;
+  return ((float)o + du) / (float)n;
+}
+// bilinear, wrap-addressed, normalised-coordinate texel fetch (texture.cu:166-189)
+__device__ __forceinline__ v3 env_tex(const DEnv& e, float x, float y) {
+  const int W = e.w, H = e.h;
+  x = x - floorf(x);
+  y = y - floorf(y);
+  const float tx = x * (float)W - 0.5f, ty = y * (float)H - 0.5f;
+  const float fx = floorf(tx), fy = floorf(ty);
+  const float a = tx - fx, b = ty - fy;
+  int i0 = (int)fx, j0 = (int)fy;
+  int i1 = i0 + 1, j1 = j0 + 1;
+  i0 = ((i0 % W) + W) % W; i1 = ((i1 % W) + W) % W;
+  j0 = ((j0 % H) + H) % H; j1 = ((j1 % H) + H) % H;
+  const float* t = e.tex;
+  const float* p00 = t + ((size_t)j0 * W + i0) * 3;
+  const float* p10 = t + ((size_t)j0 * W + i1) * 3;
+  const float* p01 = t + ((size_t)j1 * W + i0) * 3;
+  const float* p11 = t + ((size_t)j1 * W + i1) * 3;
+  float r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    r[c] = (1.f - a) * (1.f - b) * p00[c] + a * (1.f - b) * p10[c] + (1.f - a) * b * p01[c] + a * b * p11[c];
+  return mk(r[0], r[1], r[2]);
+}
+// CEnvironmentMap::sample (environmentmap.hpp:28-42)
+template <bool kCount>
+__device__ __forceinline__ v3 env_sample(const DEnv& e, Rng& rng, v3& dir, float& pdf, Cnt& cnt) {
+  count<kCount>(cnt, kCntEnv);
+  float p1, p0;
+  int v, dummy;
+  const float y = sample_1d(e.marg_func, e.marg_cdf, e.marg_int, e.h, rng.uni(), p1, v);
+  const float x = sample_1d(e.row_func + (size_t)v * e.w, e.row_cdf + (size_t)v * (e.w + 1), e.row_int[v], e.w,
+                            rng.uni(), p0, dummy);
+  pdf = p0 * p1;
+  const float phi = x * 2.f * kPi;
+  const float theta = y * kPi;
+  const float st = sinf(theta);
+  if (st == 0.f) pdf = 0.f;
+  else pdf /= (st * 2.f * kPi * kPi);
+  dir = mk(sinf(phi) * st, cosf(theta), cosf(phi) * st);
+  return env_tex(e, x, y);
+}
+// CDistribution2D::pdf (distribution_2d.hpp:73-84)
+__device__ __forceinline__ float dist2d_pdf(const DEnv& e, float px, float py) {
+  const int W = e.w, H = e.h;
+  int lr = (int)(py * (float)(H - 2));
+  lr = lr < 0 ? 0 : (lr > H - 2 ? H - 2 : lr);
+  const int ur = lr + 1;
+  int lc = (int)(px * (float)(W - 2));
+  lc = lc < 0 ? 0 : (lc > W - 2 ? W - 2 : lc);
+  const int uc = lc + 1;
+  const float* f = e.row_func;
+  const float sx = px * ((float)W - 2.f), sy = py * ((float)H - 2.f);
+  const float lo = f[(size_t)lr * W + lc] * ((float)uc - sx) + f[(size_t)lr * W + uc] * (sx - (float)lc);
+  const float up = f[(size_t)ur * W + lc] * ((float)uc - sx) + f[(size_t)ur * W + uc] * (sx - (float)lc);
+  return lo * ((float)ur - sy) + up * (sy - (float)lr);
+}
+// CEnvironmentMap::le (environmentmap.hpp:44-55)
+template <bool kCount>
+__device__ __forceinline__ v3 env_le(const DEnv& e, v3 d, float& pdf, Cnt& cnt) {
+  count<kCount>(cnt, kCntEnv);
+  const float yc = d.y < -1.f ? -1.f : (d.y > 1.f ? 1.f : d.y);
+  const float theta = acosf(yc);
+  const float p = atan2f(d.x, d.z);
+  const float phi = p < 0.f ? (p + 2.f * kPi) : p;
+  const float x = phi / (2.f * kPi);
+  const float y = theta / kPi;
+  const float st = sinf(theta);
+  pdf = st == 0.f ? 0.f : dist2d_pdf(e, x, y) / (2.f * kPi * kPi * st);
+  return env_tex(e, x, y);
+}
+
+// ---------------------------------------------------------------- scene --
+struct Hit {
+  int hit;
+  float t;
+  v3 pos;
+  int medium;
+};
+// __intersection__volume (shaders.optix.cu:192-236) with nanovdb::Ray::intersects
+// (Williams et al. 2005 slab test) and BBox::isInside; closest accepted hit wins.
+__device__ __forceinline__ Hit scene_intersect(const DScene& s, Ray& ray) {
+  Hit h;
+  h.hit = 0; h.t = 0.f; h.medium = -1; h.pos = mk(0.f, 0.f, 0.f);
+  const float best = ray.tmax;
+  const float o[3] = {ray.o.x, ray.o.y, ray.o.z};
+  const float inv[3] = {1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z};
+  const int sg0 = inv[0] < 0.f, sg1 = inv[1] < 0.f, sg2 = inv[2] < 0.f;
+  for (int k = 0; k < s.n_inst; ++k) {
+    const DInstance& in = s.inst[k];
+    const float* bmin = in.bb_min;
+    const float* bmax = in.bb_max;
+    const float span1 = kDefaultTmax;
+    float t0 = ((sg0 ? bmax : bmin)[0] - o[0]) * inv[0];
+    float t2 = ((sg1 ? bmin : bmax)[1] - o[1]) * inv[1];
+    if (t0 > t2) continue;
+    float t1 = ((sg0 ? bmin : bmax)[0] - o[0]) * inv[0];
+    float t3 = ((sg1 ? bmax : bmin)[1] - o[1]) * inv[1];
+    if (t3 > t1) continue;
+    if (t3 > t0) t0 = t3;
+    if (t2 < t1) t1 = t2;
+    t3 = ((sg2 ? bmax : bmin)[2] - o[2]) * inv[2];
+    if (t3 > t1) continue;
+    t2 = ((sg2 ? bmin : bmax)[2] - o[2]) * inv[2];
+    if (t0 > t2) continue;
+    if (t3 > t0) t0 = t3;
+    if (span1 < t0) continue;
+    if (t2 < t1) t1 = t2;
+    if (0.f > t1) continue;
+    if (0.f > t0) t0 = 0.f;
+    if (span1 < t1) t1 = span1;
+    if (!(t1 < kDefaultTmax)) continue;
+    const bool inside = o[0] > bmin[0] && o[1] > bmin[1] && o[2] > bmin[2] && o[0] < bmax[0] && o[1] < bmax[1] &&
+                        o[2] < bmax[2];
+    const float t = inside ? t1 : t0;
+    if (!(t >= 0.f && t <= best)) continue;
+    if (h.hit && !(t < h.t)) continue;
+    h.hit = 1; h.t = t; h.medium = k;
+  }
+  if (h.hit) {
+    h.pos = mk(ray.o.x + h.t * ray.d.x, ray.o.y + h.t * ray.d.y, ray.o.z + h.t * ray.d.z);
+    ray.tmax = h.t;
+  }
+  return h;
+}
+
+// CMediumInstance::sample (medium_instance_impl.hpp:7-18)
+template <bool kCount>
+__device__ v3 inst_sample(const DScene& s, int k, Ray& rw, Rng& rng, bool dda, MHit& mi, Cnt& cnt) {
+  const DInstance& in = s.inst[k];
+  const DGrid& g = s.grid[in.grid];
+  Ray rm = ray_transform2(rw, in.w2m);
+  MHit h;
+  h.valid = 0;
+  const v3 alb = dda ? sample_dda<kCount>(g, rm, rng, in.ratio, h, cnt) : sample_internal<kCount>(g, rm, rng, in.ratio, h, cnt);
+  const Ray rwn = ray_transform2(rm, in.m2w);
+  rw.tmax = rwn.tmax;
+  mi.valid = h.valid;
+  if (h.valid) {
+    mi.inst = k;
+    mi.pos = mul_point(in.m2w, h.pos);
+    mi.normal = normalize(mul_dir(in.m2w, h.normal));
+#pragma unroll
+    for (int i = 0; i < 6; ++i) mi.S[i] = h.S[i];
+    mi.ior = h.ior;
+    count<kCount>(cnt, kCntScatter);
+  }
+  return alb;
+}
+template <bool kCount>
+__device__ float inst_tr(const DScene& s, int k, const Ray& rw, Rng& rng, bool dda, Cnt& cnt) {
+  const DInstance& in = s.inst[k];
+  const DGrid& g = s.grid[in.grid];
+  const Ray rm = ray_transform2(rw, in.w2m);
+  return dda ? tr_dda<kCount>(g, rm, rng, in.ratio, cnt) : tr_internal<kCount>(g, rm, rng, in.ratio, cnt);
+}
+// CMediumInstance::moveToVoxelBorder (medium_instance_impl.hpp:38-41)
+__device__ __forceinline__ Ray inst_move_to_voxel_border(const DScene& s, int k, const Ray& ray) {
+  const DInstance& in = s.inst[k];
+  const Ray rm = ray_transform(ray, in.w2m);
+  const Ray moved = medium_move_to_voxel_border(s.grid[in.grid], rm);
+  Ray r = ray_transform(moved, in.m2w);
+  ray_offset(r, -ray.d);
+  return r;
+}
+// CDeviceScene::intersectTr (device_scene_impl.hpp:93-114): 1 = blocked by a surface hit
+template <bool kCount>
+__device__ int intersect_tr(const DScene& s, const Ray& ray, Rng& rng, bool dda, float& Tr, Cnt& cnt) {
+  Tr = 1.f;
+  v3 p0 = ray.o;
+  const v3 p1 = p0 + ray.tmax * ray.d;
+  int cur = ray.medium;
+  for (int it = 0; it < 64; ++it) {
+    Ray r = spawn_ray(p0, p1, cur);
+    const Hit h = scene_intersect(s, r);
+    if (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur)) Tr *= inst_tr<kCount>(s, r.medium, r, rng, dda, cnt);
+    if (!h.hit) return 0;
+    p0 = h.pos;
+    cur = r.medium < 0 ? h.medium : -1;
+  }
+  return 1;
+}
+
+// direct (path_integrator_impl.hpp:23-118), medium interactions
+template <bool kCount>
+__device__ v3 direct(const DScene& s, const MHit& mi, v3 wo, Rng& rng, bool dda, Cnt& cnt) {
+  v3 L = mk(0.f, 0.f, 0.f);
+  {
+    v3 ldir;
+    float light_pdf = 0.f;
+    const v3 Le = env_sample<kCount>(s.env, rng, ldir, light_pdf, cnt);
+    if (light_pdf > 0.f) {
+      ldir = normalize(ldir);
+      Ray rl = mkray(mi.pos, ldir, kDefaultTmax, mi.inst);
+      ray_offset(rl, mi.normal);
+      rl = inst_move_to_voxel_border(s, mi.inst, rl);
+      float trs;
+      const int hit = intersect_tr<kCount>(s, rl, rng, dda, trs, cnt);
+      const float p = phase_p(mi.S, mi.normal, mi.ior, wo, ldir, rng);
+      const float mis = (1.f * light_pdf) / (1.f * light_pdf + 1.f * p);
+      const float tr = hit ? 0.f : trs;
+      L = L + ((Le * (mis * p)) * tr) / light_pdf;
+    }
+  }
+  {
+    v3 wi;
+    const float p = phase_sample(mi.S, mi.normal, mi.ior, wo, wi, rng);
+    if (p > 0.f) {
+      wi = normalize(wi);
+      Ray rb = mkray(mi.pos, wi, kDefaultTmax, mi.inst);
+      ray_offset(rb, mi.normal);
+      rb = inst_move_to_voxel_border(s, mi.inst, rb);
+      float trs;
+      const int hit = intersect_tr<kCount>(s, rb, rng, dda, trs, cnt);
+      float light_pdf;
+      const v3 Le = env_le<kCount>(s.env, wi, light_pdf, cnt);
+      const float mis = (1.f * p) / (1.f * p + 1.f * light_pdf);
+      const float tr = hit ? 0.f : trs;
+      L = L + ((Le * (mis * p)) * tr) / p;
+    }
+  }
+  return L;
+}
+
+// CPixelSampler::samplePixel (pixel_sampler.hpp:36-47)
+__device__ __forceinline__ Ray camera_ray(const DScene& s, int px, int py, Rng& rng) {
+  const float rh = rng.uni();
+  const float h = ((float)(px - s.W / 2) + rh) * s.pixel_size;
+  (void)rng.uni();  // randomVertical: drawn, unused
+  const float v = ((float)(py - s.H / 2) + rh) * s.pixel_size;
+  const v3 d = normalize(mk(h, v, -s.near_dist));
+  const Ray vr = mkray(mk(0.f, 0.f, 0.f), d, kDefaultTmax, -1);
+  return ray_transform(vr, s.view_to_world);
+}
+
+// CPathIntegrator::Li (path_integrator_impl.hpp:120-247) for media-only scenes
+template <bool kCount>
+__device__ v3 Li(const DScene& s, int px, int py, Rng& rng, Cnt& cnt) {
+  const bool dda = (s.flags & 1u) != 0u;
+  v3 L = mk(0.f, 0.f, 0.f), thr = mk(1.f, 1.f, 1.f);
+  Ray ray = camera_ray(s, px, py, rng);
+  int iters = 0;
+  for (int b = 0; b < s.max_depth; ++b) {
+    if (++iters > 4 * s.max_depth + 64) break;
+    const Hit si = scene_intersect(s, ray);
+    MHit mi;
+    mi.valid = 0;
+    if (si.medium >= 0) {
+      if (ray.medium >= 0) {
+        if (si.medium != ray.medium) break;
+        thr = thr * inst_sample<kCount>(s, ray.medium, ray, rng, dda, mi, cnt);
+      } else {
+        Ray mr = mkray(si.pos, normalize(ray.d), kDefaultTmax, si.medium);
+        ray_offset(mr, ray.d);
+        const Hit se = scene_intersect(s, mr);
+        if (se.hit) {
+          if (se.medium >= 0 && si.medium != se.medium) break;
+          thr = thr * inst_sample<kCount>(s, mr.medium, mr, rng, dda, mi, cnt);
+        }
+      }
+      if (thr.x == 0.f || thr.y == 0.f || thr.z == 0.f) break;
+    }
+    if (mi.valid) {
+      const v3 wo = -ray.d;
+      L = L + thr * direct<kCount>(s, mi, wo, rng, dda, cnt);
+      v3 wi;
+      (void)phase_sample(mi.S, mi.normal, mi.ior, wo, wi, rng);
+      Ray nr = mkray(mi.pos, normalize(wi), kDefaultTmax, mi.inst);
+      ray_offset(nr, wi);
+      ray = inst_move_to_voxel_border(s, mi.inst, nr);
+    } else {
+      if (b == 0 && !si.hit) {
+        float p;
+        L = L + env_le<kCount>(s.env, ray.d, p, cnt) * thr;
+      }
+      if (!si.hit) break;
+      Ray nr = mkray(si.pos, normalize(ray.d), kDefaultTmax, ray.medium < 0 ? si.medium : -1);
+      ray_offset(nr, ray.d);
+      ray = nr;
+      --b;
+      continue;
+    }
+    if (b > 3) {
+      const float p = 1.f - (thr.x + thr.y + thr.z) / 3.f;
+      if (rng.uni() <= p) break;
+      thr = thr / (1.f - p);
+    }
+  }
+  if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z)) return mk(0.f, 0.f, 0.f);
+  return L / (float)s.spp;
+}
+
+}  // namespace r2
