@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the volume integrator on one MI355X.
+
+Workload (default "c2"): 256^3 anisotropic SGGX fibre volume (seeded 4-octave
+value noise in [0,8], von Mises-Fisher fibre field, albedo 0.9) under a
+procedural 512x256 sky + sun environment (1e4:1), 1024x1024, 64 spp per step,
+max depth 16, FP32.  One step = one r2_render call: the next 64 samples of the
+frame (progressive accumulation), so the K timed steps together form a
+K*64-spp frame whose relMSE against the FP64 4096-spp oracle (1/64 strided
+pixel subset, committed under tests/golden/) is reported.
+
+Prints ONE JSON line.  Inputs (grid, env) and outputs (per-step frames) stay in
+HBM during the timed region.  The roofline entry uses the integrator kernel's
+HIP-event duration measured inside the library on its own stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "M paths/sec at 1024² on 256³ SGGX grid @relMSE≤1e-3; p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per lookup kind: 32-byte voxel record,
+# 4-byte majorant, brick density = 4 B indirection + 4 B range + 1 B atlas,
+# env = 2 CDF reads (8 B) + 4 RGB texels (48 B); +12 B framebuffer per path
+B_VOX_FP32, B_VOX_PACKED, B_MAJ, B_BRICK, B_ENV, B_FB = 32, 16, 4, 9, 56, 12
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def relmse(x, r):
+    x = np.asarray(x, np.float64)
+    r = np.asarray(r, np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def cpu_baseline(w, threads: int, stride: int, spp: int):
+    """Time the FP32 oracle (CPU restatement, OpenMP) on a strided pixel subset."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O  # the CPU oracle: checker and baseline only
+
+    cfg = dict(w["render"])
+    W, H = cfg["width"], cfg["height"]
+    ys, xs = np.meshgrid(np.arange(0, H, stride), np.arange(0, W, stride), indexing="ij")
+    pix = (ys * W + xs).reshape(-1).astype(np.int32)
+    P = np.repeat(pix, spp)
+    S = np.tile(np.arange(spp, dtype=np.int32), len(pix))
+    sc = O.Scene(w["grids"], w["instances"], w["env"], "f32")
+    t0 = time.perf_counter()
+    _, cnt = sc.render_paths(cfg, P, S, threads=threads, counters=True)
+    dt = time.perf_counter() - t0
+    mean_cnt = cnt.astype(np.float64).mean(axis=0)
+    return len(P) / dt / 1e6, dt, len(P), dict(zip(O.CNT_NAMES, mean_cnt.tolist()))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=2, help="pixel stride of the CPU baseline sample")
+    ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel of the CPU baseline sample")
+    ap.add_argument("--no-brick-grid", action="store_true")
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(rendering shards by pixel tiles / sample ranges with one framebuffer reduce, see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import renderer2_amd as r2
+    from renderer2_amd import _hip, scenes
+
+    w = scenes.make_workload(a.workload, seed=1)
+    cfg = dict(w["render"], use_brick_grid=not a.no_brick_grid)
+    W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
+    ctx = r2.Context(0)
+    grids = [ctx.upload_grid(g) for g in w["grids"]]
+    env = ctx.upload_env(w["env"])
+    frame_bytes = W * H * 3 * 4
+    bufs = [_hip.DeviceBuffer(frame_bytes) for _ in range(a.steps)]
+    warm = _hip.DeviceBuffer(frame_bytes)
+
+    for k in range(a.warmup):
+        ctx.render_device(dict(cfg, sample_begin=1_000_000 + k * spp), grids, w["instances"], env, warm.addr)
+    _hip.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        ctx.render_device(dict(cfg, sample_begin=k * spp), grids, w["instances"], env, bufs[k].addr)
+        kernel_ms.append(ctx.stats()["kernel_ms"])
+    _hip.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    paths_per_step = W * H * spp
+    value = paths_per_step * a.steps / elapsed / 1e6
+    avg_kernel_ms = float(np.mean(kernel_ms))
+
+    # accumulated K*spp frame (each step is already divided by its own spp)
+    acc = np.zeros((H, W, 3), np.float64)
+    for b in bufs:
+        acc += b.to_host(np.float32, (H, W, 3))
+    acc /= a.steps
+    quality = None
+    gold = GOLDEN / f"{a.workload}_oracle_fp64.npz"
+    if gold.exists():
+        z = np.load(gold, allow_pickle=False)
+        st = int(z["stride"])
+        sub = acc[::st, ::st]
+        quality = dict(relmse=relmse(sub, z["image"]), spp=a.steps * spp, oracle_spp=int(z["spp"]),
+                       oracle="FP64 CPU oracle, 1/%d strided pixel subset" % (st * st),
+                       relmse_1step=relmse(bufs[0].to_host(np.float32, (H, W, 3))[::st, ::st], z["image"]))
+
+    cpu = None
+    counts = None
+    if not a.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        v, dt, n, counts = cpu_baseline(w, threads, a.cpu_stride, a.cpu_spp)
+        cpu = {"value": round(v, 4), "unit": "M paths/s", "cores": threads, "kind": "port",
+               "sample": f"FP32 oracle, 1/{a.cpu_stride ** 2} strided pixel subset x {a.cpu_spp} spp "
+                         f"= {n} paths in {dt:.1f} s"}
+    roof = None
+    if counts is not None:
+        b_vox = B_VOX_PACKED if w["grids"][0].get("precision") == "packed" else B_VOX_FP32
+        b_path = (counts["vox"] * b_vox + counts["maj"] * B_MAJ + counts["brick"] * B_BRICK + counts["env"] * B_ENV
+                  + B_FB)
+        algo_bytes = b_path * paths_per_step
+        achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "bytes_per_path": round(b_path, 1), "kernel": "k_render", "kernel_ms": round(avg_kernel_ms, 3)}
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "M paths/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": {"c1": "64^3 isotropic SGGX value-noise volume, 256x256, 16 spp, depth 8",
+                                "c2": "256^3 anisotropic SGGX fibre volume, procedural sky+sun env, "
+                                      "1024x1024, 64 spp per step, depth 16",
+                                "c4": "512^3 isotropic volume, packed FP16/BF16 records, 1024x1024, 256 spp"}
+                               [a.workload],
+                   "grid": w["params"]["n"], "width": W, "height": H, "spp_per_step": spp,
+                   "max_depth": cfg["max_depth"], "brick_grid": cfg["use_brick_grid"],
+                   "paths_per_step": paths_per_step},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "quality": quality,
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -240,9 +240,9 @@ class Scene:
         smp = np.ascontiguousarray(smp, np.int32)
         n = len(pix)
         out = np.zeros((n, 3), self.R)
-        cnt = np.zeros((n, 8), np.uint64) if counters else None
+        cnt = np.zeros((n, NCNT), np.uint64) if counters else None
         self._fn("r2o_render_paths")(self.h, C.byref(p), n, _p(pix), _p(smp), _p(out), _p(cnt) if counters else None)
-        return (out, cnt[:, :NCNT]) if counters else out
+        return (out, cnt) if counters else out
 
     def info(self, medium: int = 0) -> dict:
         out = np.zeros(64, self.R)

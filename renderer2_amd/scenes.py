@@ -14,6 +14,9 @@ import numpy as np
 from . import _capi as capi
 
 CAMERA = dict(cam_pos=(0.0, 0.0, 2.5), cam_look_at=(0.0, 0.0, 0.0), cam_up=(0.0, 1.0, 0.0), fov_deg=40.0)
+# The anisotropic workload's camera is not fixed by its definition; at distance
+# 2.0 the unit cube fills the 40 degree frame, so every path meets the volume.
+CAMERA_CLOSE = dict(CAMERA, cam_pos=(0.0, 0.0, 2.0))
 
 # name -> generator parameters and render settings
 WORKLOADS = {
@@ -22,7 +25,7 @@ WORKLOADS = {
                width=256, height=256, spp=16, max_depth=8),
     # 256^3 anisotropic fibre SGGX volume in [0,8], albedo 0.9, procedural sky + sun, 1024^2, 64 spp, depth 16
     "c2": dict(kind=2, n=256, scale=8.0, octaves=4, albedo=0.9, env="sky", env_w=512, env_h=256,
-               width=1024, height=1024, spp=64, max_depth=16),
+               width=1024, height=1024, spp=64, max_depth=16, camera="close"),
     # the config-1 field at 512^3 (packed records), 1024^2, 256 spp
     "c4": dict(kind=1, n=512, scale=4.0, octaves=3, albedo=0.8, env="white", env_w=32, env_h=16,
                width=1024, height=1024, spp=256, max_depth=8, precision="packed"),
@@ -61,8 +64,9 @@ def make_workload(name: str, seed: int = 1, **overrides) -> dict:
         precision=w.get("precision", "fp32"),
     )
     env = synth_env(w["env"], w["env_w"], w["env_h"])
+    cam = CAMERA_CLOSE if w.get("camera") == "close" else CAMERA
     render = dict(width=w["width"], height=w["height"], spp=w["spp"], max_depth=w["max_depth"], seed=seed,
-                  stream=0, use_brick_grid=True, **CAMERA)
+                  stream=0, use_brick_grid=True, **cam)
     return dict(name=name, grids=[grid], instances=[dict(grid=0, pos=(0, 0, 0), orientation=(0, 1, 0),
                                                          scaling=(1, 1, 1))],
                 env=env, render=render, params=w)
