@@ -10,7 +10,9 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "librenderer2.so"
+import os
+
+LIB_PATH = Path(os.environ.get("R2_LIB_PATH", Path(__file__).resolve().parent / "lib" / "librenderer2.so"))
 
 R2_OK = 0
 R2_ERR = {

@@ -18,7 +18,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
-LIB = PKG / "lib"
+LIB = Path(os.environ.get("R2_LIB_DIR", PKG / "lib"))
 
 SOURCES = [
     "r2_api.cpp",
@@ -42,6 +42,7 @@ COMMON_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     "-I", str(ROOT / "include"),
+    *os.environ.get("R2_EXTRA_FLAGS", "").split(),
 ]
 
 
@@ -53,7 +54,7 @@ def _compile(src: Path, obj: Path) -> None:
 
 
 def build_library(jobs: int | None = None, verbose: bool = True) -> Path:
-    LIB.mkdir(exist_ok=True)
+    LIB.mkdir(parents=True, exist_ok=True)
     srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
     objs = [LIB / (s.stem + ".o") for s in srcs]
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))

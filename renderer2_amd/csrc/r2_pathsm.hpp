@@ -1,0 +1,426 @@
+// Persistent path state machine for the volume integrator (gfx950).
+//
+// CPathIntegrator::Li (path_integrator_impl.hpp:120-247) with direct()
+// (:23-118) and CDeviceScene::intersectTr (device_scene_impl.hpp:93-114) is
+// re-expressed as explicit per-lane states.  Every free-flight (sampleDDA /
+// sampleInternal) and transmittance (trDDA / trInternal) traversal of every
+// path runs through ONE shared step loop, so the lanes of a wavefront advance
+// their traversals together whatever shading stage each path is in; shading
+// events run in batches between traversal rounds.  A lane whose path ends takes
+// the next sample of its work item, or a new item: the wavefront ballots the
+// lanes that need work, takes that many items with one atomic, and hands them
+// out by prefix rank.  Random numbers are drawn in exactly the order of the
+// recursive formulation, so each path's result is independent of scheduling.
+#pragma once
+#include "r2_volume.hpp"
+
+namespace r2 {
+
+enum : int {
+  ST_IDLE = 0,    // no more work
+  ST_FETCH,       // needs a work item (handled wave-wide)
+  ST_TRAV,        // inside a traversal
+  ST_START,       // new sample of the current item
+  ST_BOUNCE,      // top of the bounce loop
+  ST_AFTER_SAMPLE,
+  ST_NOHIT,
+  ST_SCATTER,
+  ST_LIGHT_DONE,
+  ST_BRDF_SAMPLE,
+  ST_BRDF_DONE,
+  ST_DIRECT_DONE,
+  ST_SH_SEG,
+  ST_SH_NEXT,
+  ST_FINISH,
+};
+
+enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO = 4 };
+
+struct PathLane {
+  // work item
+  uint32_t pix;
+  int32_t px, py;
+  int32_t s_cur, s_end;
+  uint64_t item;
+  float ax, ay, az;
+  // path
+  Rng rng;
+  int stage, b, iters;
+  v3 L, thr, Ld;
+  Ray ray;
+  v3 si_pos;
+  int si_medium, si_hit;
+  // scatter event
+  v3 mi_pos, mi_n;
+  float S[6];
+  float mi_ior;
+  int mi_inst;
+  // pending NEE sample (light direction / phase direction, radiance, pdf)
+  v3 pdir, pLe;
+  float ppdf;
+  // shadow ray walk (intersectTr)
+  v3 sh_p0, sh_p1;
+  int sh_cur, sh_next, sh_hit, sh_seg, sh_ret;
+  float sh_Tr;
+  // traversal
+  int tk, tinst, tsteps, tres;
+  v3 to, td, tri;
+  float tt, ttau, tmip, ttmax, ttr;
+};
+
+template <bool kSingle>
+__device__ __forceinline__ int lane_grid(const DScene& s, int inst) {
+  return kSingle ? s.inst[0].grid : s.inst[inst].grid;
+}
+
+// --------------------------------------------------------------- traversal --
+// Prologue of sampleDDA / sampleInternal / trDDA / trInternal on instance k.
+template <bool kSingle>
+__device__ __forceinline__ void start_traversal(const DScene& s, PathLane& p, int k, const Ray& rw, int kind) {
+  const int ki = kSingle ? 0 : k;
+  const DInstance& in = s.inst[ki];
+  const DGrid& g = s.grid[in.grid];
+  const Ray rm = ray_transform2(rw, in.w2m);
+  p.tinst = k;
+  p.tk = kind;
+  p.tsteps = 0;
+  p.tres = 0;
+  p.tt = 0.f;
+  p.ttr = 1.f;
+  if (kind == TK_FLIGHT_DDA || kind == TK_TR_DDA) {
+    const Ray ir = ray_transform2(rm, g.m2s);
+    p.to = ir.o;
+    p.td = ir.d;
+    p.ttmax = ir.tmax;
+    p.tri = mk(1.f / ir.d.x, 1.f / ir.d.y, 1.f / ir.d.z);
+    p.ttau = -logf(1.f - p.rng.uni());
+    p.tmip = 3.f;
+  } else {
+    const Ray ir = ray_transform2(rm, g.m2i);
+    p.to = ir.o;
+    p.td = ir.d;
+    p.ttmax = ir.tmax;
+  }
+  p.stage = ST_TRAV;
+}
+
+// One step of the traversal loop body.  Returns true when the traversal ended;
+// p.tres = 1 for a real collision of a free flight.
+template <bool kSingle, bool kCount>
+__device__ __forceinline__ bool traversal_step(const DScene& s, PathLane& p, Cnt& cnt) {
+  const int ki = kSingle ? 0 : p.tinst;
+  const DInstance& in = s.inst[ki];
+  const DGrid& g = s.grid[in.grid];
+  const float ratio = in.ratio;
+  if (p.tk == TK_FLIGHT_DDA || p.tk == TK_TR_DDA) {
+    if (!(p.tsteps < kLoopCap && p.tt < p.ttmax)) return true;
+    ++p.tsteps;
+    count<kCount>(cnt, kCntDda);
+    const v3 cur = p.to + p.tt * p.td;
+    const int imip = (int)roundf(p.tmip);
+    const float maj = ratio * lookup_majorant<kCount>(g, cur, imip, cnt);
+    const float dt = step_dda(cur, p.tri, imip);
+    const float prev = p.tt;
+    p.tt += dt / g.sigma_t;
+    if (p.tt == prev) return true;
+    p.ttau -= maj * dt;
+    p.tmip = gmin(p.tmip + 0.25f, 3.f);
+    if (p.ttau > 0.f) return false;
+    p.tt += p.ttau / (maj * g.sigma_t);
+    if (p.tt >= p.ttmax) return true;
+    const float u0 = p.rng.uni(), u1 = p.rng.uni(), u2 = p.rng.uni();
+    const float d = lookup_density<kCount>(g, p.to + p.tt * p.td, mk(u0, u1, u2), cnt);
+    if (p.rng.uni() * maj < d * ratio) {
+      if (p.tk == TK_FLIGHT_DDA) {
+        p.tres = 1;
+        return true;
+      }
+      const float inv_max = 1.f / (ratio / g.inv_max_density);
+      const float vol_maj = 1.f / inv_max;
+      p.ttr *= gmax(0.f, 1.f - vol_maj / maj);
+      if (p.ttr < 0.1f) {
+        const float prob = 1.f - p.ttr;
+        if (p.rng.uni() < prob) {
+          p.ttr = 0.f;
+          return true;
+        }
+        p.ttr /= 1.f - prob;
+      }
+    }
+    p.ttau = -logf(1.f - p.rng.uni());
+    p.tmip = gmax(0.f, p.tmip - 2.f);
+    return false;
+  }
+  // delta / ratio tracking against the global majorant (sampleInternal / trInternal)
+  if (!(p.tsteps < kLoopCap)) return true;
+  ++p.tsteps;
+  count<kCount>(cnt, kCntFlight);
+  const float inv_max = 1.f / (ratio / g.inv_max_density);
+  p.tt -= logf(1.f - p.rng.uni()) * inv_max / g.sigma_t;
+  if (p.tt >= p.ttmax) return true;
+  const Rec fd = filtered_data<kCount>(g, p.to + p.tt * p.td, cnt);
+  if (p.tk == TK_FLIGHT_DELTA) {
+    if (ratio * fd.density * inv_max > p.rng.uni()) {
+      p.tres = 1;
+      return true;
+    }
+    return false;
+  }
+  p.ttr *= 1.f - gmax(0.f, ratio * fd.density * inv_max);
+  if (p.ttr < 1.f) {
+    const float pr = 1.f - p.ttr;
+    if (p.rng.uni() < pr) {
+      p.ttr = 0.f;
+      return true;
+    }
+    p.ttr /= 1.f - pr;
+  }
+  return false;
+}
+
+// Collision tail of sampleDDA / sampleInternal + CMediumInstance::sample's
+// transform back to world space; returns the albedo, fills the scatter record.
+template <bool kSingle, bool kCount>
+__device__ __forceinline__ v3 flight_collision(const DScene& s, PathLane& p, Cnt& cnt) {
+  const int ki = kSingle ? 0 : p.tinst;
+  const DInstance& in = s.inst[ki];
+  const DGrid& g = s.grid[in.grid];
+  Rec fd;
+  Ray rw;
+  const Ray ir = mkray(p.to, p.td, p.tt, -1);
+  if (p.tk == TK_FLIGHT_DDA) {
+    const v3 sp = p.to + p.tt * p.td;
+    const v3 pos = mk((sp.x - (float)g.ibb_min[0]) / (float)g.size[0], (sp.y - (float)g.ibb_min[1]) / (float)g.size[1],
+                      (sp.z - (float)g.ibb_min[2]) / (float)g.size[2]);
+    fd = filtered_data<kCount>(g, pos, cnt);
+    rw = ray_transform2(ir, g.s2m);
+  } else {
+    Cnt dummy;
+    fd = filtered_data<false>(g, p.to + p.tt * p.td, dummy);  // same lookup as the accepted step
+    rw = ray_transform2(ir, g.i2m);
+  }
+  MHit h;
+  const v3 alb = collision_albedo(g, fd, rw, h);
+  p.mi_inst = p.tinst;
+  p.mi_pos = mul_point(in.m2w, h.pos);
+  p.mi_n = normalize(mul_dir(in.m2w, h.normal));
+#pragma unroll
+  for (int i = 0; i < 6; ++i) p.S[i] = h.S[i];
+  p.mi_ior = h.ior;
+  count<kCount>(cnt, kCntScatter);
+  return alb;
+}
+
+template <bool kSingle>
+__device__ __forceinline__ Ray lane_move_to_voxel_border(const DScene& s, int k, const Ray& r) {
+  return inst_move_to_voxel_border(s, kSingle ? 0 : k, r);
+}
+
+__device__ __forceinline__ void start_shadow(PathLane& p, const Ray& r, int ret) {
+  p.sh_Tr = 1.f;
+  p.sh_p0 = r.o;
+  p.sh_p1 = r.o + r.tmax * r.d;
+  p.sh_cur = r.medium;
+  p.sh_seg = 0;
+  p.sh_ret = ret;
+  p.stage = ST_SH_SEG;
+}
+
+// One transition of a lane that is not traversing, not fetching and not idle.
+template <bool kSingle, bool kCount>
+__device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cnt) {
+  const bool dda = (s.flags & 1u) != 0u;
+  switch (p.stage) {
+    case ST_START: {
+      p.rng.init(s.seed, s.stream, p.pix, (uint32_t)(s.sample_begin + p.s_cur), 0u);
+      p.ray = camera_ray(s, p.px, p.py, p.rng);
+      p.L = mk(0.f, 0.f, 0.f);
+      p.thr = mk(1.f, 1.f, 1.f);
+      p.b = 0;
+      p.iters = 0;
+      p.stage = ST_BOUNCE;
+      break;
+    }
+    case ST_BOUNCE: {
+      if (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64) {
+        p.stage = ST_FINISH;
+        break;
+      }
+      const Hit si = scene_intersect(s, p.ray);
+      p.si_pos = si.pos;
+      p.si_medium = si.medium;
+      p.si_hit = si.hit;
+      p.tres = 0;
+      p.tk = 0;
+      if (si.medium >= 0) {
+        if (p.ray.medium >= 0) {
+          if (si.medium != p.ray.medium) {
+            p.stage = ST_FINISH;
+            break;
+          }
+          start_traversal<kSingle>(s, p, p.ray.medium, p.ray, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
+        } else {
+          Ray mr = mkray(si.pos, normalize(p.ray.d), kDefaultTmax, si.medium);
+          ray_offset(mr, p.ray.d);
+          const Hit se = scene_intersect(s, mr);
+          if (se.hit) {
+            if (se.medium >= 0 && si.medium != se.medium) {
+              p.stage = ST_FINISH;
+              break;
+            }
+            start_traversal<kSingle>(s, p, mr.medium, mr, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
+          } else {
+            p.stage = ST_AFTER_SAMPLE;
+          }
+        }
+      } else {
+        p.stage = ST_NOHIT;
+      }
+      break;
+    }
+    case ST_AFTER_SAMPLE: {
+      const bool collided = p.tres != 0;
+      if (collided) p.thr = p.thr * flight_collision<kSingle, kCount>(s, p, cnt);
+      p.tres = 0;
+      if (p.thr.x == 0.f || p.thr.y == 0.f || p.thr.z == 0.f) {
+        p.stage = ST_FINISH;
+        break;
+      }
+      p.stage = collided ? ST_SCATTER : ST_NOHIT;
+      break;
+    }
+    case ST_NOHIT: {
+      if (p.b == 0 && !p.si_hit) {
+        float pdf;
+        p.L = p.L + env_le<kCount>(s.env, p.ray.d, pdf, cnt) * p.thr;
+      }
+      if (!p.si_hit) {
+        p.stage = ST_FINISH;
+        break;
+      }
+      Ray nr = mkray(p.si_pos, normalize(p.ray.d), kDefaultTmax, p.ray.medium < 0 ? p.si_medium : -1);
+      ray_offset(nr, p.ray.d);
+      p.ray = nr;
+      p.stage = ST_BOUNCE;
+      break;
+    }
+    case ST_SCATTER: {
+      p.Ld = mk(0.f, 0.f, 0.f);
+      v3 ldir;
+      float lpdf = 0.f;
+      const v3 Le = env_sample<kCount>(s.env, p.rng, ldir, lpdf, cnt);
+      if (lpdf > 0.f) {
+        ldir = normalize(ldir);
+        Ray rl = mkray(p.mi_pos, ldir, kDefaultTmax, p.mi_inst);
+        ray_offset(rl, p.mi_n);
+        rl = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, rl);
+        p.pdir = ldir;
+        p.pLe = Le;
+        p.ppdf = lpdf;
+        start_shadow(p, rl, ST_LIGHT_DONE);
+      } else {
+        p.stage = ST_BRDF_SAMPLE;
+      }
+      break;
+    }
+    case ST_LIGHT_DONE: {
+      const v3 wo = -p.ray.d;
+      const float ph = phase_p(p.S, p.mi_n, p.mi_ior, wo, p.pdir, p.rng);
+      const float mis = (1.f * p.ppdf) / (1.f * p.ppdf + 1.f * ph);
+      const float tr = p.sh_hit ? 0.f : p.sh_Tr;
+      p.Ld = p.Ld + ((p.pLe * (mis * ph)) * tr) / p.ppdf;
+      p.stage = ST_BRDF_SAMPLE;
+      break;
+    }
+    case ST_BRDF_SAMPLE: {
+      const v3 wo = -p.ray.d;
+      v3 wi;
+      const float ph = phase_sample(p.S, p.mi_n, p.mi_ior, wo, wi, p.rng);
+      if (ph > 0.f) {
+        wi = normalize(wi);
+        Ray rb = mkray(p.mi_pos, wi, kDefaultTmax, p.mi_inst);
+        ray_offset(rb, p.mi_n);
+        rb = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, rb);
+        p.pdir = wi;
+        p.ppdf = ph;
+        start_shadow(p, rb, ST_BRDF_DONE);
+      } else {
+        p.stage = ST_DIRECT_DONE;
+      }
+      break;
+    }
+    case ST_BRDF_DONE: {
+      float lpdf;
+      const v3 Le = env_le<kCount>(s.env, p.pdir, lpdf, cnt);
+      const float ph = p.ppdf;
+      const float mis = (1.f * ph) / (1.f * ph + 1.f * lpdf);
+      const float tr = p.sh_hit ? 0.f : p.sh_Tr;
+      p.Ld = p.Ld + ((Le * (mis * ph)) * tr) / ph;
+      p.stage = ST_DIRECT_DONE;
+      break;
+    }
+    case ST_DIRECT_DONE: {
+      p.L = p.L + p.thr * p.Ld;
+      const v3 wo = -p.ray.d;
+      v3 wi;
+      (void)phase_sample(p.S, p.mi_n, p.mi_ior, wo, wi, p.rng);
+      Ray nr = mkray(p.mi_pos, normalize(wi), kDefaultTmax, p.mi_inst);
+      ray_offset(nr, wi);
+      p.ray = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, nr);
+      if (p.b > 3) {
+        const float q = 1.f - (p.thr.x + p.thr.y + p.thr.z) / 3.f;
+        if (p.rng.uni() <= q) {
+          p.stage = ST_FINISH;
+          break;
+        }
+        p.thr = p.thr / (1.f - q);
+      }
+      ++p.b;
+      p.stage = ST_BOUNCE;
+      break;
+    }
+    case ST_SH_SEG: {
+      if (++p.sh_seg > 64) {
+        p.sh_hit = 1;
+        p.stage = p.sh_ret;
+        break;
+      }
+      Ray rr = spawn_ray(p.sh_p0, p.sh_p1, p.sh_cur);
+      const Hit h = scene_intersect(s, rr);
+      p.sh_hit = h.hit;
+      p.sh_next = rr.medium < 0 ? h.medium : -1;
+      const bool trav = h.hit && rr.medium >= 0 && (h.medium < 0 || h.medium == p.sh_cur);
+      p.sh_p0 = h.pos;
+      if (trav) {
+        start_traversal<kSingle>(s, p, rr.medium, rr, dda ? TK_TR_DDA : TK_TR_RATIO);
+      } else {
+        p.stage = ST_SH_NEXT;
+      }
+      break;
+    }
+    case ST_SH_NEXT: {
+      if (!p.sh_hit) {
+        p.stage = p.sh_ret;
+        break;
+      }
+      p.sh_cur = p.sh_next;
+      p.stage = ST_SH_SEG;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Called when a traversal ended: hand the result to the owning stage.
+__device__ __forceinline__ void end_traversal(PathLane& p) {
+  if (p.tk == TK_TR_DDA || p.tk == TK_TR_RATIO) {
+    p.sh_Tr *= p.ttr;
+    p.stage = ST_SH_NEXT;
+  } else {
+    p.stage = ST_AFTER_SAMPLE;
+  }
+}
+
+
+}  // namespace r2

@@ -1,15 +1,14 @@
 // Volume integrator kernels for gfx950.
 //
-// Work item = (sample chunk, pixel).  A persistent grid of wavefronts pulls
-// 64-item batches from one global counter (one atomic per wavefront); the 64
-// lanes of a batch are one 8x8 pixel tile of one chunk, so primary rays of a
-// wavefront are coherent.  Each lane runs the K samples of its chunk in order
-// (CPathIntegrator::Li per sample, path_integrator_impl.hpp:120-247) and
-// writes the chunk sum; a second kernel adds the chunk sums in chunk order.
-// Results are independent of scheduling: every path's random stream is keyed by
-// (seed, stream, pixel, sample) and the reduction order is fixed.
+// Work item = (sample chunk, pixel), items numbered so that 64 consecutive items
+// are one 8x8 pixel tile of one chunk (coherent primary rays).  A persistent
+// grid of wavefronts runs the path state machine of r2_pathsm.hpp; each lane
+// runs the K samples of its item in order and writes the chunk sum; a second
+// kernel adds the chunk sums in chunk order.  Results are independent of
+// scheduling: every path's random stream is keyed by (seed, stream, pixel,
+// sample) and the reduction order is fixed.
 #include "r2_internal.hpp"
-#include "r2_volume.hpp"
+#include "r2_pathsm.hpp"
 #include <algorithm>
 
 namespace {
@@ -27,38 +26,112 @@ __device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int
   }
 }
 
-template <typename Acc>
-__global__ void __launch_bounds__(256) k_render(const DScene* __restrict__ sp, uint32_t* __restrict__ counter,
-                                                uint64_t total_items, int chunk, Acc* __restrict__ partial) {
+// Persistent path state machine (see r2_pathsm.hpp).  Render mode: item =
+// (sample chunk, pixel), output = chunk sums.  Probe mode (probe_pix != null):
+// item = one (pixel, sample) path, output = its radiance and lookup counters.
+#ifndef R2_SM_WAVES
+#define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for (measured best)
+#endif
+template <bool kSingle, bool kCount, typename Acc>
+__global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __restrict__ sp, uint64_t* __restrict__ counter,
+                                                  uint64_t total, int chunk, Acc* __restrict__ partial,
+                                                  const int32_t* __restrict__ probe_pix,
+                                                  const int32_t* __restrict__ probe_smp, float* __restrict__ probe_out,
+                                                  uint32_t* __restrict__ probe_cnt) {
   const DScene& s = *sp;
-  const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
   const int lane = threadIdx.x & 63;
+  const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
+  PathLane p;
+  p.stage = ST_FETCH;
+  p.item = 0;
   Cnt cnt;
-  while (true) {
-    uint64_t base = 0;
-    if (lane == 0) base = atomicAdd((unsigned long long*)counter, 64ull);
-    base = __shfl(base, 0);
-    if (base >= total_items) break;
-    const uint64_t item = base + (uint64_t)lane;
-    if (item < total_items) {
-      const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
-      int x, y;
-      item_pixel(q, s.W, s.H, x, y);
-      const uint32_t pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-      Acc ax = 0, ay = 0, az = 0;
-      const int s0 = (int)c * chunk, s1 = min(s.spp, s0 + chunk);
-      for (int smp = s0; smp < s1; ++smp) {
-        Rng rng;
-        rng.init(s.seed, s.stream, pix, (uint32_t)(s.sample_begin + smp), 0u);
-        const v3 L = Li<false>(s, x, y, rng, cnt);
-        ax += (Acc)L.x;
-        ay += (Acc)L.y;
-        az += (Acc)L.z;
+#pragma unroll
+  for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
+  for (int guard = 0; guard < (1 << 24); ++guard) {
+    // ---- work fetch: one atomic per wavefront, items handed out by prefix rank
+    const uint64_t need = __ballot(p.stage == ST_FETCH);
+    if (need) {
+      const int n = __popcll(need);
+      const int leader = __ffsll((unsigned long long)need) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)n);
+      base = __shfl(base, leader);
+      if (p.stage == ST_FETCH) {
+        const uint64_t item = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (item < total) {
+          p.item = item;
+          p.ax = p.ay = p.az = 0.f;
+          if (probe_pix) {
+            p.pix = (uint32_t)probe_pix[item];
+            p.s_cur = probe_smp[item];
+            p.s_end = p.s_cur + 1;
+          } else {
+            const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
+            int x, y;
+            item_pixel(q, s.W, s.H, x, y);
+            p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+            p.s_cur = (int)c * chunk;
+            p.s_end = min(s.spp, p.s_cur + chunk);
+          }
+          p.px = (int)(p.pix % (uint32_t)s.W);
+          p.py = (int)(p.pix / (uint32_t)s.W);
+          p.stage = ST_START;
+        } else {
+          p.stage = ST_IDLE;
+        }
       }
-      Acc* o = partial + ((size_t)c * npix + pix) * 3;
-      o[0] = ax;
-      o[1] = ay;
-      o[2] = az;
+    }
+    if (__ballot(p.stage != ST_IDLE) == 0) break;
+    // ---- shading events, batched across the wavefront
+    for (int k = 0; k < 12; ++k) {
+      const bool ev = p.stage >= ST_START;
+      if (!__any(ev)) break;
+      if (ev) {
+        if (p.stage == ST_FINISH) {
+          v3 L = p.L;
+          if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))
+            L = mk(0.f, 0.f, 0.f);
+          else
+            L = L / (float)s.spp;
+          if (probe_pix) {
+            probe_out[3 * p.item] = L.x;
+            probe_out[3 * p.item + 1] = L.y;
+            probe_out[3 * p.item + 2] = L.z;
+            if (kCount && probe_cnt) {
+              for (int c = 0; c < kNumCnt; ++c) probe_cnt[p.item * kNumCnt + c] = cnt.c[c];
+            }
+#pragma unroll
+            for (int c = 0; c < kNumCnt; ++c) cnt.c[c] = 0;
+            p.stage = ST_FETCH;
+          } else {
+            p.ax += (Acc)L.x;
+            p.ay += (Acc)L.y;
+            p.az += (Acc)L.z;
+            ++p.s_cur;
+            if (p.s_cur < p.s_end) {
+              p.stage = ST_START;
+            } else {
+              const uint32_t c = (uint32_t)(p.item / npix);
+              Acc* o = partial + ((size_t)c * npix + p.pix) * 3;
+              o[0] = p.ax;
+              o[1] = p.ay;
+              o[2] = p.az;
+              p.stage = ST_FETCH;
+            }
+          }
+        } else {
+          event_step<kSingle, kCount>(s, p, cnt);
+        }
+      }
+    }
+    // ---- traversal rounds: every traversing lane advances one DDA / tracking step
+    for (int k = 0; k < 256; ++k) {
+      const uint64_t tm = __ballot(p.stage == ST_TRAV);
+      if (tm == 0) break;
+      if (k >= 8 && __popcll(tm) < 16) break;
+      if (p.stage == ST_TRAV) {
+        if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
+      }
     }
   }
 }
@@ -85,37 +158,34 @@ __global__ void k_reduce(const Acc* __restrict__ partial, int nchunks, uint32_t 
   }
 }
 
-__global__ void __launch_bounds__(256) k_paths(const DScene* __restrict__ sp, int n, const int32_t* __restrict__ pix,
-                                               const int32_t* __restrict__ smp, float* __restrict__ out,
-                                               uint32_t* __restrict__ cnt_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const DScene& s = *sp;
-  const int p = pix[i];
-  const int x = p % s.W, y = p / s.W;
-  Rng rng;
-  rng.init(s.seed, s.stream, (uint32_t)p, (uint32_t)smp[i], 0u);
-  Cnt cnt;
-  for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
-  const v3 L = Li<true>(s, x, y, rng, cnt);
-  out[3 * i] = L.x;
-  out[3 * i + 1] = L.y;
-  out[3 * i + 2] = L.z;
-  if (cnt_out)
-    for (int k = 0; k < kNumCnt; ++k) cnt_out[(size_t)i * kNumCnt + k] = cnt.c[k];
-}
-
 int g_num_cus = 0;
 
 }  // namespace
 
 namespace r2host {
 
+static int num_cus(r2_ctx* ctx) {
+  if (!g_num_cus) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) v = 0;
+    g_num_cus = v > 0 ? v : 256;
+  }
+  return g_num_cus;
+}
+
+template <bool kSingle, bool kCount, typename Acc>
+static void launch_sm(r2_ctx* ctx, int blocks, uint64_t total, int chunk, Acc* partial, const int32_t* pp,
+                      const int32_t* ps, float* po, uint32_t* pc) {
+  hipLaunchKernelGGL((k_paths_sm<kSingle, kCount, Acc>), dim3(blocks), dim3(256), 0, ctx->stream, ctx->d_scene,
+                     (uint64_t*)ctx->d_counter, total, chunk, partial, pp, ps, po, pc);
+}
+
 int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64) {
   using namespace r2;
   const uint64_t npix = (uint64_t)scene.W * scene.H;
   const uint64_t target = 1ull << 21;
-  int chunk = (int)std::min<uint64_t>((uint64_t)scene.spp, std::max<uint64_t>(1, (npix * scene.spp + target - 1) / target));
+  const int chunk =
+      (int)std::min<uint64_t>((uint64_t)scene.spp, std::max<uint64_t>(1, (npix * scene.spp + target - 1) / target));
   const int nchunks = (scene.spp + chunk - 1) / chunk;
   const uint64_t total = (uint64_t)nchunks * npix;
   const size_t acc_bytes = fp64 ? 8 : 4;
@@ -125,19 +195,18 @@ int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64)
   hipStream_t st = ctx->stream;
   R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(DScene), hipMemcpyHostToDevice, st));
   R2_HIP(ctx, hipMemsetAsync(ctx->d_counter, 0, 8, st));
-  if (!g_num_cus) {
-    int v = 0;
-    R2_HIP(ctx, hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    g_num_cus = v > 0 ? v : 256;
-  }
-  const int blocks = (int)std::min<uint64_t>((uint64_t)g_num_cus * 16, (total + 255) / 256);
+  const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total + 255) / 256);
+  const bool single = scene.n_inst == 1;
   R2_HIP(ctx, hipEventRecord(ctx->ev0, st));
-  if (fp64)
-    hipLaunchKernelGGL(k_render<double>, dim3(blocks), dim3(256), 0, st, ctx->d_scene, ctx->d_counter, total, chunk,
-                       (double*)ctx->scratch);
-  else
-    hipLaunchKernelGGL(k_render<float>, dim3(blocks), dim3(256), 0, st, ctx->d_scene, ctx->d_counter, total, chunk,
-                       (float*)ctx->scratch);
+  if (fp64) {
+    double* part = (double*)ctx->scratch;
+    if (single) launch_sm<true, false, double>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
+    else launch_sm<false, false, double>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
+  } else {
+    float* part = (float*)ctx->scratch;
+    if (single) launch_sm<true, false, float>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
+    else launch_sm<false, false, float>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
+  }
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipEventRecord(ctx->ev1, st));
   const uint32_t n3 = (uint32_t)(npix * 3);
@@ -161,7 +230,12 @@ int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_p
                  uint32_t* d_cnt) {
   hipStream_t st = ctx->stream;
   R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(r2::DScene), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_paths, dim3((n + 255) / 256), dim3(256), 0, st, ctx->d_scene, n, d_pix, d_smp, d_out, d_cnt);
+  R2_HIP(ctx, hipMemsetAsync(ctx->d_counter, 0, 8, st));
+  const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, ((uint64_t)n + 255) / 256);
+  if (scene.n_inst == 1)
+    launch_sm<true, true, float>(ctx, blocks, (uint64_t)n, 1, nullptr, d_pix, d_smp, d_out, d_cnt);
+  else
+    launch_sm<false, true, float>(ctx, blocks, (uint64_t)n, 1, nullptr, d_pix, d_smp, d_out, d_cnt);
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipStreamSynchronize(st));
   return R2_OK;

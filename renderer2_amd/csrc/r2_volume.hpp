@@ -174,16 +174,19 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
   const float psz = p.z * (float)g.size[2] + (float)g.ibb_min[2] - 0.5f;
   const int x = (int)floorf(psx), y = (int)floorf(psy), z = (int)floorf(psz);
   const float dx = psx - (float)x, dy = psy - (float)y, dz = psz - (float)z;
+  // same arithmetic as d00, d10 -> d0, d01, d11 -> d1, then z; evaluated so that
+  // at most three decoded records are live at a time
   Rec a = get_value<kCount>(g, x, y, z, cnt), b = get_value<kCount>(g, x + 1, y, z, cnt);
-  const Rec d00 = rec_lerp(dx, a, b);
-  a = get_value<kCount>(g, x, y + 1, z, cnt); b = get_value<kCount>(g, x + 1, y + 1, z, cnt);
-  const Rec d10 = rec_lerp(dx, a, b);
-  a = get_value<kCount>(g, x, y, z + 1, cnt); b = get_value<kCount>(g, x + 1, y, z + 1, cnt);
-  const Rec d01 = rec_lerp(dx, a, b);
-  a = get_value<kCount>(g, x, y + 1, z + 1, cnt); b = get_value<kCount>(g, x + 1, y + 1, z + 1, cnt);
-  const Rec d11 = rec_lerp(dx, a, b);
-  const Rec d0 = rec_lerp(dy, d00, d10);
-  const Rec d1 = rec_lerp(dy, d01, d11);
+  Rec e = rec_lerp(dx, a, b);
+  a = get_value<kCount>(g, x, y + 1, z, cnt);
+  b = get_value<kCount>(g, x + 1, y + 1, z, cnt);
+  const Rec d0 = rec_lerp(dy, e, rec_lerp(dx, a, b));
+  a = get_value<kCount>(g, x, y, z + 1, cnt);
+  b = get_value<kCount>(g, x + 1, y, z + 1, cnt);
+  e = rec_lerp(dx, a, b);
+  a = get_value<kCount>(g, x, y + 1, z + 1, cnt);
+  b = get_value<kCount>(g, x + 1, y + 1, z + 1, cnt);
+  const Rec d1 = rec_lerp(dy, e, rec_lerp(dx, a, b));
   return rec_lerp(dz, d0, d1);
 }
 
