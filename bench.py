@@ -71,6 +71,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-stride", type=int, default=2, help="pixel stride of the CPU baseline sample")
     ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-brick-grid", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="MiB copied device-to-device before rendering (reference kernel for PMC byte counters)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -92,6 +94,13 @@ def main(argv=None) -> int:
     bufs = [_hip.DeviceBuffer(frame_bytes) for _ in range(a.steps)]
     warm = _hip.DeviceBuffer(frame_bytes)
 
+    if a.copy_calib > 0:
+        nb = a.copy_calib << 20
+        src, dst = _hip.DeviceBuffer(nb), _hip.DeviceBuffer(nb)
+        _hip.check(_hip.hip().hipMemset(src.ptr, 1, nb), "hipMemset")
+        _hip.check(_hip.hip().hipMemcpy(dst.ptr, src.ptr, nb, _hip.D2D), "hipMemcpy D2D")
+        _hip.synchronize()
+        del src, dst
     for k in range(a.warmup):
         ctx.render_device(dict(cfg, sample_begin=1_000_000 + k * spp), grids, w["instances"], env, warm.addr)
     _hip.synchronize()

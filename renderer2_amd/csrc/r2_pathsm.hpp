@@ -42,7 +42,8 @@ struct PathLane {
   int32_t px, py;
   int32_t s_cur, s_end;
   uint64_t item;
-  float ax, ay, az;
+  float ax, ay, az;     // FP32 item sum (same order as the reference's framebuffer +=)
+  double dax, day, daz;  // FP64 item sum (compensated framebuffer form)
   // path
   Rng rng;
   int stage, b, iters;
@@ -227,10 +228,12 @@ __device__ __forceinline__ void start_shadow(PathLane& p, const Ray& r, int ret)
 }
 
 // One transition of a lane that is not traversing, not fetching and not idle.
+// `st` is the state to run (the lane's stage); passing a wave-uniform value lets
+// the switch compile to one uniform branch.
 template <bool kSingle, bool kCount>
-__device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cnt) {
+__device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cnt, int st) {
   const bool dda = (s.flags & 1u) != 0u;
-  switch (p.stage) {
+  switch (st) {
     case ST_START: {
       p.rng.init(s.seed, s.stream, p.pix, (uint32_t)(s.sample_begin + p.s_cur), 0u);
       p.ray = camera_ray(s, p.px, p.py, p.rng);

@@ -10,6 +10,7 @@
 #include "r2_internal.hpp"
 #include "r2_pathsm.hpp"
 #include <algorithm>
+#include <cstdio>
 
 namespace {
 using namespace r2;
@@ -29,6 +30,76 @@ __device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int
 // Persistent path state machine (see r2_pathsm.hpp).  Render mode: item =
 // (sample chunk, pixel), output = chunk sums.  Probe mode (probe_pix != null):
 // item = one (pixel, sample) path, output = its radiance and lookup counters.
+#ifndef R2_EV_MODE
+#define R2_EV_MODE 0      // 0: every event lane advances once per iteration; 1: most populated state first
+#endif
+#ifndef R2_EV_ITERS
+#define R2_EV_ITERS 12    // event iterations per scheduling round
+#endif
+#ifndef R2_TR_MIN_LANES
+#define R2_TR_MIN_LANES 16  // leave the traversal round when fewer lanes are still traversing
+#endif
+#ifndef R2_TR_MIN_ROUNDS
+#define R2_TR_MIN_ROUNDS 8
+#endif
+#ifdef R2_SM_STATS
+// debug-build scheduling statistics: [0] outer iterations, [1] event iterations,
+// [2] traversal rounds, [3] lanes active in traversal rounds, [16+2s] executions
+// of state s, [17+2s] lanes in those executions
+__device__ unsigned long long g_sm_stats[64];
+#define SM_STAT(i, v) do { if (lane == 0) atomicAdd(&g_sm_stats[(i)], (unsigned long long)(v)); } while (0)
+#else
+#define SM_STAT(i, v) do {} while (0)
+#endif
+// end of one path: radiance, NaN guard, accumulation into the item or probe output
+template <typename Acc>
+__device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, int chunk,
+                                            Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
+                                            float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
+  v3 L = p.L;
+  if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))
+    L = mk(0.f, 0.f, 0.f);
+  else
+    L = L / (float)s.spp;
+  if (probe_pix) {
+    probe_out[3 * p.item] = L.x;
+    probe_out[3 * p.item + 1] = L.y;
+    probe_out[3 * p.item + 2] = L.z;
+    if (probe_cnt)
+      for (int c = 0; c < kNumCnt; ++c) probe_cnt[p.item * kNumCnt + c] = cnt.c[c];
+#pragma unroll
+    for (int c = 0; c < kNumCnt; ++c) cnt.c[c] = 0;
+    p.stage = ST_FETCH;
+    return;
+  }
+  if constexpr (sizeof(Acc) == 8) {
+    p.dax += (double)L.x;
+    p.day += (double)L.y;
+    p.daz += (double)L.z;
+  } else {
+    p.ax += L.x;
+    p.ay += L.y;
+    p.az += L.z;
+  }
+  ++p.s_cur;
+  if (p.s_cur < p.s_end) {
+    p.stage = ST_START;
+  } else {
+    const uint32_t c = (uint32_t)(p.item / npix);
+    Acc* o = partial + ((size_t)c * npix + p.pix) * 3;
+    if constexpr (sizeof(Acc) == 8) {
+      o[0] = p.dax;
+      o[1] = p.day;
+      o[2] = p.daz;
+    } else {
+      o[0] = p.ax;
+      o[1] = p.ay;
+      o[2] = p.az;
+    }
+    p.stage = ST_FETCH;
+  }
+}
+
 #ifndef R2_SM_WAVES
 #define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for (measured best)
 #endif
@@ -61,6 +132,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
         if (item < total) {
           p.item = item;
           p.ax = p.ay = p.az = 0.f;
+          p.dax = p.day = p.daz = 0.0;
           if (probe_pix) {
             p.pix = (uint32_t)probe_pix[item];
             p.s_cur = probe_smp[item];
@@ -82,53 +154,49 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
       }
     }
     if (__ballot(p.stage != ST_IDLE) == 0) break;
+    SM_STAT(0, 1);
     // ---- shading events, batched across the wavefront
-    for (int k = 0; k < 12; ++k) {
+#if R2_EV_MODE == 0
+    for (int k = 0; k < R2_EV_ITERS; ++k) {
       const bool ev = p.stage >= ST_START;
       if (!__any(ev)) break;
+      SM_STAT(1, 1);
+#ifdef R2_SM_STATS
+      for (int st = ST_START; st <= ST_FINISH; ++st) {
+        const int c = __popcll(__ballot(p.stage == st));
+        if (c) { SM_STAT(16 + 2 * st, 1); SM_STAT(17 + 2 * st, c); }
+      }
+#endif
       if (ev) {
-        if (p.stage == ST_FINISH) {
-          v3 L = p.L;
-          if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))
-            L = mk(0.f, 0.f, 0.f);
-          else
-            L = L / (float)s.spp;
-          if (probe_pix) {
-            probe_out[3 * p.item] = L.x;
-            probe_out[3 * p.item + 1] = L.y;
-            probe_out[3 * p.item + 2] = L.z;
-            if (kCount && probe_cnt) {
-              for (int c = 0; c < kNumCnt; ++c) probe_cnt[p.item * kNumCnt + c] = cnt.c[c];
-            }
-#pragma unroll
-            for (int c = 0; c < kNumCnt; ++c) cnt.c[c] = 0;
-            p.stage = ST_FETCH;
-          } else {
-            p.ax += (Acc)L.x;
-            p.ay += (Acc)L.y;
-            p.az += (Acc)L.z;
-            ++p.s_cur;
-            if (p.s_cur < p.s_end) {
-              p.stage = ST_START;
-            } else {
-              const uint32_t c = (uint32_t)(p.item / npix);
-              Acc* o = partial + ((size_t)c * npix + p.pix) * 3;
-              o[0] = p.ax;
-              o[1] = p.ay;
-              o[2] = p.az;
-              p.stage = ST_FETCH;
-            }
-          }
-        } else {
-          event_step<kSingle, kCount>(s, p, cnt);
-        }
+        if (p.stage == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+        else event_step<kSingle, kCount>(s, p, cnt, p.stage);
       }
     }
+#else
+    // run the most populated event state each iteration (wave-uniform branch)
+    for (int k = 0; k < R2_EV_ITERS; ++k) {
+      int best = -1, bestc = 0;
+      for (int st = ST_START; st <= ST_FINISH; ++st) {
+        const int c = __popcll(__ballot(p.stage == st));
+        if (c > bestc) {
+          bestc = c;
+          best = st;
+        }
+      }
+      if (best < 0) break;
+      if (p.stage == best) {
+        if (best == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+        else event_step<kSingle, kCount>(s, p, cnt, best);
+      }
+    }
+#endif
     // ---- traversal rounds: every traversing lane advances one DDA / tracking step
     for (int k = 0; k < 256; ++k) {
       const uint64_t tm = __ballot(p.stage == ST_TRAV);
       if (tm == 0) break;
-      if (k >= 8 && __popcll(tm) < 16) break;
+      if (k >= R2_TR_MIN_ROUNDS && __popcll(tm) < R2_TR_MIN_LANES) break;
+      SM_STAT(2, 1);
+      SM_STAT(3, __popcll(tm));
       if (p.stage == ST_TRAV) {
         if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
       }
@@ -221,6 +289,18 @@ int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64)
   float ms = 0.f;
   R2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->stats.last_kernel_ms = ms;
+#ifdef R2_SM_STATS
+  {
+    unsigned long long st_host[64];
+    R2_HIP(ctx, hipMemcpyFromSymbol(st_host, HIP_SYMBOL(g_sm_stats), sizeof(st_host)));
+    fprintf(stderr, "[sm-stats] paths %llu outer %llu ev_iters %llu trav_rounds %llu trav_lanes %llu\n",
+            (unsigned long long)(npix * scene.spp), st_host[0], st_host[1], st_host[2], st_host[3]);
+    for (int k = ST_START; k <= ST_FINISH; ++k)
+      fprintf(stderr, "[sm-stats] state %d: execs %llu lanes %llu\n", k, st_host[16 + 2 * k], st_host[17 + 2 * k]);
+    const unsigned long long z[64] = {0};
+    R2_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(g_sm_stats), z, sizeof(z)));
+  }
+#endif
   ctx->stats.last_paths = (int64_t)(npix * scene.spp);
   ctx->stats.last_launches = 2;
   return R2_OK;

@@ -1,0 +1,462 @@
+"""Pins of the CPU oracle: known-answer vectors, closed forms, invariants and
+brute force, chosen so that a plausible mistake (a dropped term, a wrong sign
+or index, a transposed operand) fails one of them.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from renderer2_amd import scenes
+
+# ------------------------------------------------------------------ RNG --
+
+
+def test_philox_known_answers():
+    """Random123 known-answer vectors of philox4x32-10."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in kat:
+        assert tuple(int(x) for x in O.philox(ctr, key)) == want
+
+
+# ----------------------------------------------------------------- half --
+
+
+def test_half_rne_matches_numpy():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(20000).astype(np.float32) * 10,
+                        rng.uniform(-70000, 70000, 2000).astype(np.float32),
+                        rng.uniform(-1e-4, 1e-4, 2000).astype(np.float32),
+                        np.array([0, -0.0, 65504, 65519, 65520, 6e-8, 3e-8, np.inf, -np.inf], np.float32)])
+    L = O.lib()
+    got = np.array([L.r2o_half_from_float_rne(float(v)) for v in x], np.uint16)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).view(np.uint16)
+    assert np.array_equal(got, want)
+
+
+def test_half_to_float_exhaustive():
+    L = O.lib()
+    h = np.arange(0, 1 << 16, dtype=np.uint32)
+    finite = (h & 0x7C00) != 0x7C00
+    got = np.array([L.r2o_half_to_float(int(v)) for v in h[finite]], np.float32)
+    want = h[finite].astype(np.uint16).view(np.float16).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_half_tie_away():
+    """glm's conversion rounds half-ulp ties away from zero; RNE rounds them to even."""
+    L = O.lib()
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0.01, 1000, 5000).astype(np.float32)
+    ties = ((x.view(np.uint32) & 0x1FFF) == 0x1000)
+    x = x[~ties]
+    glm = np.array([L.r2o_half_from_float_glm(float(v)) for v in x], np.uint16)
+    assert np.array_equal(glm, x.astype(np.float16).view(np.uint16))  # identical off ties
+    # 1 + 2^-11 is halfway between 1 and 1 + 2^-10: RNE -> 1.0 (even), away -> 1 + 2^-10
+    t = np.float32(1.0 + 2.0 ** -11)
+    assert L.r2o_half_from_float_rne(float(t)) == 0x3C00
+    assert L.r2o_half_from_float_glm(float(t)) == 0x3C01
+    # 1 + 3*2^-11 is halfway between 1+2^-10 (odd) and 1+2^-9 (even): both round up
+    t = np.float32(1.0 + 3 * 2.0 ** -11)
+    assert L.r2o_half_from_float_rne(float(t)) == 0x3C02
+    assert L.r2o_half_from_float_glm(float(t)) == 0x3C02
+
+
+# -------------------------------------------------------------- bricks --
+
+
+def _h2f(u16):
+    return np.asarray(u16, np.uint16).view(np.float16).astype(np.float32)
+
+
+def test_brick_grid_invariants():
+    rng = np.random.default_rng(2)
+    n = 37  # not a multiple of 8: exercises the padded bricks
+    dens = rng.uniform(0.1, 5.0, (n, n, n)).astype(np.float32)
+    dens[5:20, 7:30, 2:9] = 0.0          # inactive block -> background 0
+    dens[20:28, 0:8, 0:8] = 2.5          # constant region
+    bg = O.Brickgrid(dens)
+    nbx, nby, nbz = bg.nb
+    assert bg.nb == (8, 8, 8)  # ceil(ceil(37/8)/8)*8
+    lo = _h2f(bg.range & 0xFFFF).reshape(nbz, nby, nbx)
+    hi = _h2f(bg.range >> 16).reshape(nbz, nby, nbx)
+    # effective (active-rule) densities, padded by the background
+    act = dens.copy()
+    corner = np.zeros_like(act, bool)
+    for z in (0, n - 1):
+        for y in (0, n - 1):
+            for x in (0, n - 1):
+                corner[z, y, x] = True
+    act[(dens <= 0) & ~corner] = 0
+    pad = np.zeros((nbz * 8 + 2, nby * 8 + 2, nbx * 8 + 2), np.float32)
+    pad[1:n + 1, 1:n + 1, 1:n + 1] = act
+    hulp = lambda v: np.maximum(np.abs(v) * 2.0 ** -11, 2.0 ** -25)  # half ulp bound  # noqa: E731
+    for bz in range(nbz):
+        for by in range(nby):
+            for bx in range(nbx):
+                blk = pad[bz * 8:bz * 8 + 10, by * 8:by * 8 + 10, bx * 8:bx * 8 + 10]
+                tmin, tmax = blk.min(), blk.max()
+                assert abs(lo[bz, by, bx] - tmin) <= hulp(tmin)
+                assert abs(hi[bz, by, bx] - tmax) <= hulp(tmax)
+                ne = bg.nonempty[(bz * nby + by) * nbx + bx]
+                assert ne == (tmin != tmax)
+    # quantised densities within 1/510 of the range (+ half rounding) inside non-empty bricks
+    for _ in range(3000):
+        x, y, z = rng.integers(0, n, 3)
+        b = (z // 8, y // 8, x // 8)
+        v = bg.density(int(x), int(y), int(z))
+        rl, rh = lo[b], hi[b]
+        assert abs(v - act[z, y, x]) <= (rh - rl) / 510 * 1.001 + hulp(rh) + 1e-6
+    # mips are the min / max of their 2x2x2 children
+    prev_lo, prev_hi = lo, hi
+    for l in range(3):
+        m = bg.mips[l]
+        s = (nbz >> (l + 1), nby >> (l + 1), nbx >> (l + 1))
+        mlo = _h2f(m & 0xFFFF).reshape(s)
+        mhi = _h2f(m >> 16).reshape(s)
+        clo = prev_lo.reshape(s[0], 2, s[1], 2, s[2], 2).min(axis=(1, 3, 5))
+        chi = prev_hi.reshape(s[0], 2, s[1], 2, s[2], 2).max(axis=(1, 3, 5))
+        assert np.array_equal(mlo, clo) and np.array_equal(mhi, chi)
+        prev_lo, prev_hi = mlo, mhi
+    # majorant lookup: border outside the brick grid, range.y inside
+    assert bg.majorant(-1, 0, 0, 0) == 0.0 and bg.majorant(nbx, 0, 0, 0) == 0.0
+    assert bg.majorant(3, 4, 5, 0) == hi[5, 4, 3]
+    assert bg.majorant(3, 4, 5, 2) == mhi_at(bg, 3, 4, 5, 2)
+
+
+def mhi_at(bg, bx, by, bz, mip):
+    nbx, nby, nbz = bg.nb
+    s = (nbz >> mip, nby >> mip, nbx >> mip)
+    m = bg.mips[mip - 1].reshape(s)
+    return float(_h2f(m[bz >> mip, by >> mip, bx >> mip] >> 16))
+
+
+# ---------------------------------------------------------- SGGX, Fresnel --
+
+
+def _sphere_grid(n=400):
+    """Midpoint quadrature over the unit sphere (theta, phi)."""
+    th = (np.arange(n) + 0.5) * math.pi / n
+    ph = (np.arange(2 * n) + 0.5) * math.pi / n
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    w = np.sin(T) * (math.pi / n) ** 2
+    d = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1)
+    return d.reshape(-1, 3), w.reshape(-1)
+
+
+def _rot_S(axis, eig):
+    a = np.asarray(axis, float)
+    a /= np.linalg.norm(a)
+    e = np.array([1.0, 0, 0]) if abs(a[0]) < 0.9 else np.array([0, 1.0, 0])
+    b1 = np.cross(a, e)
+    b1 /= np.linalg.norm(b1)
+    b2 = np.cross(a, b1)
+    R = np.stack([b1, b2, a], 1)
+    return R @ np.diag(eig) @ R.T
+
+
+@pytest.mark.parametrize("S", [np.eye(3), _rot_S((0.3, 0.8, 0.2), (1, 1, 0.05)), _rot_S((1, 0, 1), (0.2, 0.5, 1.0))])
+def test_sggx_projected_area(S):
+    """int D(m) <w,m>+ dm = sigma(w) = sqrt(w^T S w) (Heitz et al. 2015, SGGX projected area)."""
+    S6 = np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    Dfn = O.fn("r2o_sggx_D")
+    sfn = O.fn("r2o_sggx_sigma")
+    d, w = _sphere_grid(200)
+    s6 = O.as_arr(S6)
+    D = np.array([Dfn(s6.ctypes.data, O.as_arr(m).ctypes.data) for m in d])
+    for wdir in ([0, 0, 1], [0.6, 0.0, 0.8], [0.36, -0.48, 0.8]):
+        wv = np.asarray(wdir, float)
+        proj = np.sum(D * np.maximum(d @ wv, 0) * w)
+        sig = sfn(s6.ctypes.data, O.as_arr(wv).ctypes.data)
+        assert abs(proj - sig) < 3e-3 * max(sig, 1e-3) + 1e-4
+        assert abs(sig - math.sqrt(wv @ S @ wv)) < 1e-12
+
+
+@pytest.mark.parametrize("S,n,ior", [(np.eye(3), (0.0, 1.0, 0.0), 1.5),
+                                     (_rot_S((0.3, 0.8, 0.2), (1, 1, 0.05)), (0.0, 0.6, 0.8), 1.5),
+                                     (_rot_S((1, 0, 1), (0.2, 0.5, 1.0)), (1.0, 0.0, 0.0), 1.33)])
+def test_phase_function_normalised(S, n, ior):
+    """The Fresnel mix of the specular lobe D(wh) / (4 sigma(wo)) and the diffuse lobe
+    max(wi . m, 0) / pi integrates to 1 over wi (same VNDF sample m for every wi)."""
+    S6 = O.as_arr([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    nn = O.as_arr(n)
+    pfn = O.fn("r2o_phase_p")
+    d, w = _sphere_grid(300)
+    for wo in ([0.0, 0.0, 1.0], [0.48, 0.6, 0.64]):
+        wo = O.as_arr(wo)
+        p = np.array([pfn(S6.ctypes.data, nn.ctypes.data, ior, wo.ctypes.data, O.as_arr(wi).ctypes.data, 5, 1)
+                      for wi in d])
+        assert abs(np.sum(p * w) - 1.0) < 5e-3
+
+
+def test_sggx_vndf_distribution():
+    """VNDF samples about the voxel normal n follow D(m)<n,m>+ / sigma(n)."""
+    S = _rot_S((0.2, 0.9, 0.3), (1, 1, 0.1))
+    S6 = O.as_arr([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    n = np.array([0.0, 0.6, 0.8])
+    vfn = O.fn("r2o_sggx_vndf")
+    rng = np.random.default_rng(3)
+    N = 40000
+    u = rng.uniform(0, 1, (N, 2))
+    out = np.zeros(3)
+    ms = np.empty((N, 3))
+    nn = O.as_arr(n)
+    for i in range(N):
+        vfn(S6.ctypes.data, nn.ctypes.data, u[i, 0], u[i, 1], out.ctypes.data)
+        ms[i] = out
+    assert np.all(ms @ n >= -1e-9)  # visible normals face n
+    # compare E[f(m)] with the quadrature of f * D<n,m>/sigma for a few test functions
+    Dfn = O.fn("r2o_sggx_D")
+    d, w = _sphere_grid(200)
+    D = np.array([Dfn(S6.ctypes.data, O.as_arr(m).ctypes.data) for m in d])
+    sig = math.sqrt(n @ S @ n)
+    pdf = D * np.maximum(d @ n, 0) / sig
+    for f in (lambda m: m[:, 0], lambda m: m[:, 1] ** 2, lambda m: m[:, 2] * m[:, 0]):
+        est = f(ms).mean()
+        ref = np.sum(f(d) * pdf * w)
+        assert abs(est - ref) < 4 * f(ms).std() / math.sqrt(N) + 2e-3
+
+
+def test_fresnel_textbook():
+    fr = O.fn("r2o_fresnel")
+    assert abs(fr(1.0, 1.5) - 0.04) < 1e-12            # ((n-1)/(n+1))^2
+    assert abs(fr(1e-9, 1.5) - 1.0) < 1e-6             # grazing
+    assert fr(-0.1, 1.5) == 1.0                        # inside, beyond the critical angle
+    c = math.cos(math.radians(45))
+    st = math.sin(math.radians(45)) / 1.5
+    ct = math.sqrt(1 - st * st)
+    rs = ((c - 1.5 * ct) / (c + 1.5 * ct)) ** 2
+    rp = ((1.5 * c - ct) / (1.5 * c + ct)) ** 2
+    assert abs(fr(c, 1.5) - 0.5 * (rs + rp)) < 1e-12
+
+
+# ------------------------------------------------------------------ env --
+
+
+def _scene(env, n=16, dens=1.0, S=np.eye(3), sigma_s=0.8, sigma_a=0.2, fill=None, precision="f64"):
+    rec = scenes.synth_volume(1, n, 1, 1.0, 1)
+    d = rec[..., 0:4].copy().view(np.float32)
+    d[...] = dens if fill is None else np.asarray(fill, np.float32)[..., None]
+    rec[..., 0:4] = d.view(np.uint8)
+    vs = np.float32(1.0) / np.float32(n)
+    grid = dict(records=rec, shape=(n, n, n), map_mat=(vs, 0, 0, 0, vs, 0, 0, 0, vs), map_vec=(-0.5,) * 3,
+                sigma_a=(sigma_a,) * 3, sigma_s=(sigma_s,) * 3, voxel_size_filtering=float(vs))
+    return O.Scene([grid], [dict(grid=0)], env, precision), grid
+
+
+def test_env_sampling_matches_pdf():
+    """Sampled directions are distributed as the returned pdf (histogram over texels)."""
+    rng = np.random.default_rng(4)
+    H, W = 8, 16
+    env = rng.uniform(0.1, 2.0, (H, W, 3)).astype(np.float32)
+    sc, _ = _scene(env)
+    N = 60000
+    th = np.empty(N)
+    ph = np.empty(N)
+    pdfs = np.empty(N)
+    for i in range(N):
+        r = sc.env_sample(9, i)
+        d = r[3:6]
+        th[i] = math.acos(max(-1, min(1, d[1])))
+        ph[i] = math.atan2(d[0], d[2]) % (2 * math.pi)
+        pdfs[i] = r[6]
+    # the pdf of a direction inside texel (j, i) is f_ji / integral / (2 pi^2 sin theta): check
+    # that E[1 / pdf] = 4 pi (an unbiased estimate of the sphere's solid angle)
+    assert abs(np.mean(1.0 / pdfs) / (4 * math.pi) - 1.0) < 0.02
+
+
+def test_env_marginal_row_shift():
+    """CDistribution2D::sumRows pushes at i % W == 0, so texel (r, W-1) is counted in
+    marginal row r+1: with one bright texel at the end of row 3, samples land in
+    row 4 (a flat row) far more often than in row 3."""
+    H, W = 8, 16
+    env = np.full((H, W, 3), 1e-3, np.float32)
+    env[3, W - 1] = 1e3
+    sc, _ = _scene(env)
+    rows = np.zeros(H)
+    for i in range(4000):
+        r = sc.env_sample(5, i)
+        th = math.acos(max(-1, min(1, r[4])))
+        rows[min(H - 1, int(th / math.pi * H))] += 1
+    assert rows[4] > 0.9 * rows.sum()
+    marg, _, _ = sc.env_tables()
+    # the shifted marginal jumps at row 4, not row 3
+    assert marg[5] - marg[4] > 0.9 and marg[4] - marg[3] < 0.01
+
+
+# ----------------------------------------------------- media / tracking --
+
+
+def _ray7(o, d, tmax=1e12):
+    return np.array(list(o) + list(d) + [tmax], np.float64)
+
+
+@pytest.mark.parametrize("use_dda", [True, False])
+def test_homogeneous_transmittance(use_dda):
+    """E[Tr] = exp(-rho sigma_t L) for a homogeneous cube (both trackers are unbiased)."""
+    rho = 1.7
+    env = np.ones((4, 8, 3), np.float32)
+    sc, _ = _scene(env, n=16, dens=rho)
+    L = 0.6
+    ray = _ray7((-0.3, 0.05, 0.02), (1.0, 0.0, 0.0), L)
+    N = 20000
+    tr = np.array([sc.tr_segment(ray, use_dda, 7, i) for i in range(N)])
+    want = math.exp(-rho * 1.0 * L)
+    assert abs(tr.mean() - want) < 4 * tr.std() / math.sqrt(N) + 1e-3
+
+
+@pytest.mark.parametrize("use_dda", [True, False])
+def test_homogeneous_free_flight(use_dda):
+    """P(collision before L) = 1 - exp(-rho sigma_t L) for the free-flight samplers."""
+    rho = 2.3
+    env = np.ones((4, 8, 3), np.float32)
+    sc, _ = _scene(env, n=16, dens=rho)
+    L = 0.4
+    ray = _ray7((-0.2, -0.1, 0.13), (0.0, 0.6, 0.8), L)
+    N = 20000
+    hits = np.array([sc.sample_segment(ray, use_dda, 11, i)[0] for i in range(N)])
+    want = 1 - math.exp(-rho * L)
+    assert abs(hits.mean() - want) < 4 * math.sqrt(want * (1 - want) / N)
+
+
+def test_trilinear_density_brute_force():
+    rng = np.random.default_rng(6)
+    n = 12
+    fill = rng.uniform(0.2, 3.0, (n, n, n)).astype(np.float32)
+    env = np.ones((4, 8, 3), np.float32)
+    sc, _ = _scene(env, n=n, fill=fill)
+    pad = np.zeros((n + 2, n + 2, n + 2))
+    pad[1:-1, 1:-1, 1:-1] = fill
+    for _ in range(400):
+        p = rng.uniform(-0.05, 1.05, 3)
+        out = sc.filtered_data(p)
+        q = p * n - 0.5  # voxel i's value sits at continuous coordinate i + 0.5
+        i0 = np.floor(q).astype(int)
+        f = q - i0
+        acc = 0.0
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    x, y, z = i0 + [dx, dy, dz]
+                    v = pad[z + 1, y + 1, x + 1] if -1 <= x <= n and -1 <= y <= n and -1 <= z <= n else 0.0
+                    acc += v * (f[0] if dx else 1 - f[0]) * (f[1] if dy else 1 - f[1]) * (f[2] if dz else 1 - f[2])
+        assert abs(out[0] - acc) < 1e-9 * max(1, abs(acc))
+
+
+def test_record_decode_known_answers():
+    """S = I encodes as sigma = 65535 and r = 32767 (truncation), decoding to
+    r = 2*32767/65535 - 1 = -1/65535; the background decodes to density 0, ior 1."""
+    rec = scenes.synth_volume(1, 8, 1, 1.0, 1)[0, 0, 0].copy()
+    out = np.zeros(17)
+    O.fn("r2o_decode_record")(rec.ctypes.data, out.ctypes.data)
+    w = rec.view(np.uint16)
+    assert list(w[2:5]) == [65535] * 3 and list(w[5:8]) == [32767] * 3
+    assert out[1] == 1.0 and abs(out[4] - (-1.0 / 65535)) < 1e-15
+    assert abs(out[16] - 1.5) == 0 and np.allclose(out[7:13], 1.0)
+
+
+def test_intersect_slab_brute_force():
+    env = np.ones((4, 8, 3), np.float32)
+    sc, _ = _scene(env, n=8)
+    rng = np.random.default_rng(8)
+    for _ in range(500):
+        o = rng.uniform(-1.5, 1.5, 3)
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        hit, out = sc.intersect(_ray7(o, d))
+        ts = np.linspace(0, 5, 20001)
+        pts = o[None] + ts[:, None] * d[None]
+        inside = np.all(np.abs(pts) <= 0.5, axis=1)
+        if np.all(np.abs(o) < 0.5):
+            assert hit and abs(out[0] - ts[inside][-1]) < 1e-3  # exit distance from inside
+        elif inside.any():
+            assert hit and abs(out[0] - ts[inside][0]) < 1e-3
+        else:
+            assert not hit or out[0] == 0.0
+
+
+def test_move_to_voxel_border_stays_in_voxel():
+    env = np.ones((4, 8, 3), np.float32)
+    n = 16
+    sc, _ = _scene(env, n=n)
+    rng = np.random.default_rng(9)
+    for _ in range(300):
+        o = rng.uniform(-0.45, 0.45, 3)
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        m = sc.move_to_voxel_border(_ray7(o, d))
+        vi = np.floor((o + 0.5) * n)
+        vo = np.floor((m[:3] + 0.5) * n)
+        assert np.array_equal(vi, vo)              # same voxel
+        adv = (m[:3] + 0.5) * n + d * 2e-3 * n      # a step past the face leaves it
+        assert not np.array_equal(np.floor(adv), vi)
+        assert np.allclose(m[3:6], d, atol=1e-12)
+
+
+def test_camera_fov_and_shared_jitter():
+    """Horizontal half-angle of the outermost ray = fov/2; the x and y jitter are the
+    same sample (CPixelSampler uses randomHorizontal for both axes)."""
+    cfg = dict(scenes.make_workload("c1", n=8, width=64, height=48, spp=1)["render"])
+    sc, _ = _scene(np.ones((4, 8, 3), np.float32))
+    L = O.lib()
+    out = np.zeros(7)
+    p = sc.params(cfg)
+    fov = math.radians(cfg["fov_deg"])
+    near = 64 * 1e-5 / (2 * math.tan(fov / 2))
+    for px, py in [(0, 0), (63, 47), (10, 30), (40, 5)]:
+        fn = getattr(L, "r2o_camera_ray_f64")
+        import ctypes as C
+        fn(C.byref(p), px, py, 3, out.ctypes.data)
+        d = out[3:6]
+        # camera at +z looking at the origin: view space x = world x, y = world y, -z = world -z
+        h = d[0] / -d[2] * near / 1e-5
+        v = d[1] / -d[2] * near / 1e-5
+        jx = h - (px - 32)
+        jy = v - (py - 24)
+        assert 0 <= jx < 1 and abs(jx - jy) < 1e-6
+        assert np.allclose(out[:3], cfg["cam_pos"])
+
+
+def test_absorbing_cube_double_tracking():
+    """Pure absorber (sigma_s = 0): a camera path survives only if its free flights
+    see no collision.  The reference tracks the entry segment twice
+    (path_integrator_impl.hpp:144-191), so the centre pixel converges to
+    exp(-2 rho sigma_t L) instead of exp(-rho sigma_t L)."""
+    rho = 0.8
+    env = np.ones((4, 8, 3), np.float32)
+    sc, _ = _scene(env, n=16, dens=rho, sigma_s=0.0, sigma_a=1.0)
+    cfg = dict(width=8, height=8, spp=4000, max_depth=8, cam_pos=(0, 0, 3.0), cam_look_at=(0, 0, 0), cam_up=(0, 1, 0),
+               fov_deg=1.0, seed=3, use_brick_grid=True)
+    img = sc.render(cfg)
+    centre = img[3:5, 3:5].mean()
+    want = math.exp(-2 * rho * 1.0)
+    assert abs(centre - want) < 0.02
+    assert abs(centre - math.exp(-rho)) > 0.1
+
+
+def test_empty_volume_shows_env():
+    """Zero density: every camera path crosses the box without collision and the
+    pixel equals the env radiance in its direction (bounce-0 miss)."""
+    rng = np.random.default_rng(10)
+    env = rng.uniform(0.5, 2.0, (8, 16, 3)).astype(np.float32)
+    fill = np.zeros((8, 8, 8), np.float32)
+    sc, _ = _scene(env, n=8, fill=fill)
+    cfg = dict(width=16, height=16, spp=1, max_depth=8, cam_pos=(0, 0, 2.0), cam_look_at=(0, 0, 0), cam_up=(0, 1, 0),
+               fov_deg=40.0, seed=3, use_brick_grid=True)
+    img = sc.render(cfg)
+    pix = np.arange(16 * 16, dtype=np.int32)
+    out = sc.render_paths(cfg, pix, np.zeros_like(pix))
+    assert np.allclose(img.reshape(-1, 3), out)
+    import ctypes as C
+    r = np.zeros(7)
+    p = sc.params(cfg)
+    for k in (0, 37, 255):
+        O.lib().r2o_camera_ray_f64(C.byref(p), int(k % 16), int(k // 16), 0, r.ctypes.data)
+        le = sc.env_le(r[3:6])
+        assert np.allclose(out[k], le[:3], rtol=1e-9)
