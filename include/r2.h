@@ -145,7 +145,8 @@ int r2_get_stats(const r2_ctx* ctx, r2_stats* out);
 /* ------------------------------------------------------------- filter -- */
 /* Triangle mesh for the mesh -> SGGX filter.  Replaces CMesh + OptiX GAS/IAS
  * (raytracer/include/mesh/mesh.hpp:18-59, raytracer/src/mesh/mesh.cu:140-168)
- * with a GPU-built BVH. */
+ * with per-LOD candidate lists: each voxel is tested against the triangles
+ * within reach of its rays. */
 typedef struct {
   int32_t n_vertices, n_triangles;
   const float* positions;      /* n_vertices * 3, model space */
@@ -181,12 +182,28 @@ typedef struct {
 /* Grid set-up of one LOD (COpenvdbBackend::init + setupGrid,
  * raytracer/src/filtering/openvdb_backend.cpp:26-106). */
 int r2_filter_setup(const r2_mesh* mesh, float voxel_size, r2_filter_grid* out);
-/* Filter one LOD: writes nx*ny*nz 32-byte SFilteredDataCompact records
- * (CFilter::runFiltering's optixLaunch(nx, ny, nz) of __raygen__filtering /
- * CMeshFilter::run, raytracer/filter.cu:74-123, raytracer/include/filtering/mesh_filter.hpp:51-122).
+/* Filter one LOD: writes nx*ny*nz 32-byte SFilteredDataCompact records, voxel id
+ * x + nx*(y + ny*z) (CFilter::runFiltering's optixLaunch(nx, ny, nz) of
+ * __raygen__filtering / CMeshFilter::run, raytracer/filter.cu:74-123,
+ * raytracer/src/backend/shaders.optix.cu:258-284,
+ * raytracer/include/filtering/mesh_filter.hpp:51-122).
  * out_on_device: 1 if out_records is a device pointer. */
 int r2_filter_lod(r2_ctx* ctx, const r2_mesh* mesh, const r2_filter_params* p, void* out_records,
                   int32_t out_on_device, r2_filter_grid* grid_out);
+/* Filter only the listed voxels of the same LOD grid (host ids and host output,
+ * n records in list order).  Probe entry point with the per-voxel results of
+ * r2_filter_lod, for spot checks of large grids. */
+int r2_filter_voxels(r2_ctx* ctx, const r2_mesh* mesh, const r2_filter_params* p, int32_t n,
+                     const uint32_t* voxel_ids, void* out_records);
+
+/* Counters of the last r2_filter_lod: grid voxels, voxels with candidate
+ * triangles, (voxel, triangle) candidate pairs, rays cast, filter kernel time
+ * and wall time of the call. */
+typedef struct {
+  int64_t voxels, active_voxels, pairs, rays;
+  double filter_kernel_ms, total_ms;
+} r2_filter_stats;
+int r2_filter_get_stats(const r2_ctx* ctx, r2_filter_stats* out);
 
 /* ------------------------------------------------------------- server -- */
 /* Device-resident render server with continuous batching (no reference
@@ -228,6 +245,11 @@ int r2_synth_volume(int32_t kind, int32_t n, uint32_t seed, float density_scale,
                     void* out_records);
 /* kind 0 = constant white (w x h of 1.0), kind 1 = procedural sky gradient + 0.5 degree sun. */
 int r2_synth_env(int32_t kind, int32_t w, int32_t h, float* out_rgb);
+/* Procedural mesh: noise-displaced cube-sphere (12*res^2 triangles, material 0) plus
+ * n_quads random thin square quads (2 triangles each, material 1) in a shell around it.
+ * With positions == NULL only the counts are written. */
+int r2_synth_mesh(int32_t res, int32_t n_quads, float quad_size, uint32_t seed, int32_t* n_vertices,
+                  int32_t* n_triangles, float* positions, float* normals, int32_t* indices, int32_t* material);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,34 @@ class Bricks(C.Structure):
     ]
 
 
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("n_vertices", C.c_int), ("n_triangles", C.c_int),
+        ("positions", C.c_void_p), ("normals", C.c_void_p), ("indices", C.c_void_p),
+        ("material", C.c_void_p), ("n_materials", C.c_int),
+        ("mat_diffuse", C.c_void_p), ("mat_specular", C.c_void_p),
+        ("mat_roughness", C.c_void_p), ("mat_ior", C.c_void_p),
+        ("pos", C.c_float * 3), ("orientation", C.c_float * 3), ("scaling", C.c_float * 3),
+    ]
+
+
+class FilterGrid(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int * 3),
+        ("index_to_model", C.c_float * 12), ("model_to_index", C.c_float * 12),
+        ("model_to_world", C.c_float * 12), ("world_to_model", C.c_float * 12),
+        ("world_bb_min", C.c_float * 3), ("world_bb_max", C.c_float * 3),
+    ]
+
+
+class FilterParams(C.Structure):
+    _fields_ = [
+        ("samples_per_voxel", C.c_int), ("estimation_iterations", C.c_int), ("clip_rays", C.c_int),
+        ("lod", C.c_int), ("threads", C.c_int),
+        ("sigma_t", C.c_float), ("alpha", C.c_float), ("seed", C.c_uint32),
+    ]
+
+
 _lib = None
 
 
@@ -115,8 +143,19 @@ def lib() -> C.CDLL:
             g("r2o_move_to_voxel_border").argtypes = [vp, vp, vp]
             g("r2o_intersect").argtypes = [vp, vp, vp]
             g("r2o_intersect").restype = C.c_int
-        if hasattr(L, "r2o_filter_voxels"):
-            pass
+        L.r2o_filter_setup.argtypes = [C.POINTER(MeshDesc), C.c_float, C.POINTER(FilterGrid)]
+        L.r2o_filter_new.argtypes = [C.POINTER(MeshDesc)]
+        L.r2o_filter_new.restype = vp
+        L.r2o_filter_free.argtypes = [vp]
+        L.r2o_filter_run.argtypes = [vp, C.POINTER(FilterGrid), C.POINTER(FilterParams), C.c_int64, vp, vp]
+        L.r2o_filter_probe.argtypes = [vp, C.POINTER(FilterGrid), C.POINTER(FilterParams), C.c_int64, vp, vp, vp, vp,
+                                       vp, vp]
+        L.r2o_filter_closest_hit.argtypes = [vp, vp, vp, C.c_float, vp]
+        L.r2o_det_expf.argtypes = [C.c_float]
+        L.r2o_det_expf.restype = C.c_float
+        L.r2o_det_logf.argtypes = [C.c_float]
+        L.r2o_det_logf.restype = C.c_float
+        L.r2o_det_sincosf.argtypes = [C.c_float, vp, vp]
         _lib = L
     return _lib
 
@@ -325,3 +364,111 @@ def fn(name: str, precision: str = "f64"):
 
 def as_arr(x, precision="f64"):
     return np.ascontiguousarray(x, np.float32 if precision == "f32" else np.float64)
+
+
+class MeshFilter:
+    """CPU restatement of CMeshFilter over one mesh (BVH built once).
+
+    mesh: dict with positions (V,3) f32, normals (V,3) f32, indices (T,3) i32,
+    material (T,) i32 or None, mat_diffuse (M,3), mat_specular (M,3),
+    mat_roughness (M,), mat_ior (M,), pos/orientation/scaling (3,).
+    """
+
+    def __init__(self, mesh: dict):
+        self._keep = {}
+        for k in ("positions", "normals", "mat_diffuse", "mat_specular", "mat_roughness", "mat_ior"):
+            self._keep[k] = np.ascontiguousarray(mesh[k], np.float32)
+        self._keep["indices"] = np.ascontiguousarray(mesh["indices"], np.int32)
+        mat = mesh.get("material")
+        self._keep["material"] = None if mat is None else np.ascontiguousarray(mat, np.int32)
+        d = MeshDesc()
+        d.n_vertices = self._keep["positions"].shape[0]
+        d.n_triangles = self._keep["indices"].shape[0]
+        for k in ("positions", "normals", "indices", "mat_diffuse", "mat_specular", "mat_roughness", "mat_ior"):
+            setattr(d, k, _p(self._keep[k]))
+        d.material = None if self._keep["material"] is None else _p(self._keep["material"])
+        d.n_materials = self._keep["mat_ior"].shape[0]
+        for k, dflt in (("pos", (0, 0, 0)), ("orientation", (0, 1, 0)), ("scaling", (1, 1, 1))):
+            setattr(d, k, (C.c_float * 3)(*[float(v) for v in mesh.get(k, dflt)]))
+        self.desc = d
+        self._h = lib().r2o_filter_new(C.byref(d))
+        if not self._h:
+            raise ValueError("empty mesh")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().r2o_filter_free(self._h)
+            self._h = None
+
+    def setup(self, voxel_size: float) -> FilterGrid:
+        g = FilterGrid()
+        if lib().r2o_filter_setup(C.byref(self.desc), C.c_float(voxel_size), C.byref(g)) != 0:
+            raise ValueError("r2o_filter_setup failed")
+        return g
+
+    def run(self, grid: FilterGrid, samples: int = 64, iterations: int = 8, sigma_t: float = 1.0,
+            alpha: float = 1.0, clip_rays: bool = True, seed: int = 0, lod: int = 0, ids=None,
+            threads: int = 0) -> np.ndarray:
+        """32-byte records (n, 32) u8 for `ids` (voxel ids x + y*nx + z*nx*ny) or every voxel."""
+        p = FilterParams(samples, iterations, int(bool(clip_rays)), lod, threads, sigma_t, alpha, seed)
+        if ids is None:
+            n = int(grid.nv[0]) * int(grid.nv[1]) * int(grid.nv[2])
+            idp = None
+        else:
+            ids = np.ascontiguousarray(ids, np.int64)
+            n = ids.shape[0]
+            idp = _p(ids)
+        out = np.zeros((n, 32), np.uint8)
+        if lib().r2o_filter_run(self._h, C.byref(grid), C.byref(p), n, idp, _p(out)) != 0:
+            raise ValueError("r2o_filter_run failed")
+        return out
+
+    def probe(self, grid: FilterGrid, vid: int, samples: int = 64, iterations: int = 8, sigma_t: float = 1.0,
+              alpha: float = 1.0, clip_rays: bool = True, seed: int = 0, lod: int = 0) -> dict:
+        """Per-ray view of one voxel: origins, directions, ray tmax, hit triangle (-1: miss), hit t."""
+        p = FilterParams(samples, iterations, int(bool(clip_rays)), lod, 1, sigma_t, alpha, seed)
+        rec = np.zeros(32, np.uint8)
+        o = np.zeros((samples, 3), np.float32)
+        d = np.zeros((samples, 3), np.float32)
+        tm = np.zeros(samples, np.float32)
+        tri = np.zeros(samples, np.int32)
+        t = np.zeros(samples, np.float32)
+        hits = lib().r2o_filter_probe(self._h, C.byref(grid), C.byref(p), int(vid), _p(rec), _p(o), _p(d), _p(tm),
+                                      _p(tri), _p(t))
+        if hits < 0:
+            raise ValueError("r2o_filter_probe failed")
+        return {"hits": hits, "record": rec, "o": o, "d": d, "tmax": tm, "tri": tri, "t": t}
+
+    def closest_hit(self, o, d, tmax: float):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        out = np.zeros(3, np.float32)
+        tri = lib().r2o_filter_closest_hit(self._h, _p(o), _p(d), C.c_float(tmax), _p(out))
+        return tri, out
+
+
+def grid_dict(g: FilterGrid) -> dict:
+    return {
+        "nv": tuple(g.nv),
+        "index_to_model": np.array(g.index_to_model, np.float32),
+        "model_to_index": np.array(g.model_to_index, np.float32),
+        "model_to_world": np.array(g.model_to_world, np.float32),
+        "world_to_model": np.array(g.world_to_model, np.float32),
+        "world_bb_min": np.array(g.world_bb_min, np.float32),
+        "world_bb_max": np.array(g.world_bb_max, np.float32),
+    }
+
+
+def decode_records(rec: np.ndarray) -> dict:
+    """Field view of (n, 32) SFilteredDataCompact records."""
+    rec = np.ascontiguousarray(rec).reshape(-1, 32)
+    u16 = rec.view(np.uint16)
+    return {
+        "density": rec[:, 0:4].copy().view(np.float32)[:, 0],
+        "sigma": u16[:, 2:5],
+        "r": u16[:, 5:8],
+        "diffuse": rec[:, 16:19],
+        "specular": rec[:, 19:22],
+        "normal": u16[:, 11:14],
+        "ior": u16[:, 14].view(np.float16).astype(np.float32),
+    }

@@ -94,6 +94,47 @@ void r2o_scene_free_f32(r2o_scene_f32* s);
 r2o_scene_f64* r2o_scene_new_f64(const r2o_scene_desc* d);
 void r2o_scene_free_f64(r2o_scene_f64* s);
 
+/* ---- mesh -> SGGX filter (r2o_filter.c) ---- */
+typedef struct {
+  int n_vertices, n_triangles;
+  const float* positions;   /* 3 per vertex, model space */
+  const float* normals;     /* 3 per vertex, model space */
+  const int32_t* indices;   /* 3 per triangle */
+  const int32_t* material;  /* material id per triangle, or NULL (all 0) */
+  int n_materials;
+  const float* mat_diffuse;   /* 3 per material */
+  const float* mat_specular;  /* 3 per material */
+  const float* mat_roughness; /* 1 per material: SGGX sigma of buildS */
+  const float* mat_ior;       /* 1 per material */
+  float pos[3], orientation[3], scaling[3];
+} r2o_mesh_desc;
+
+typedef struct {
+  int nv[3];
+  float index_to_model[12], model_to_index[12], model_to_world[12], world_to_model[12]; /* glm mat4x3, column major */
+  float world_bb_min[3], world_bb_max[3];
+} r2o_filter_grid;
+
+typedef struct {
+  int samples_per_voxel, estimation_iterations, clip_rays, lod, threads;
+  float sigma_t, alpha;
+  uint32_t seed;
+} r2o_filter_params;
+
+typedef struct r2o_filter_ctx r2o_filter_ctx;
+
+int r2o_filter_setup(const r2o_mesh_desc* d, float voxel_size, r2o_filter_grid* g);
+r2o_filter_ctx* r2o_filter_new(const r2o_mesh_desc* d);
+void r2o_filter_free(r2o_filter_ctx* c);
+int r2o_filter_run(const r2o_filter_ctx* c, const r2o_filter_grid* g, const r2o_filter_params* p, int64_t n_ids,
+                   const int64_t* ids, uint8_t* out);
+int r2o_filter_probe(const r2o_filter_ctx* c, const r2o_filter_grid* g, const r2o_filter_params* p, int64_t vid,
+                     uint8_t* rec, float* o, float* d, float* tmax, int32_t* tri, float* t);
+int r2o_filter_closest_hit(const r2o_filter_ctx* c, const float o[3], const float d[3], float tmax, float out[3]);
+float r2o_det_expf(float x);
+float r2o_det_logf(float x);
+void r2o_det_sincosf(float x, float* s, float* c);
+
 #ifdef __cplusplus
 }
 #endif

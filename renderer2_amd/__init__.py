@@ -14,7 +14,12 @@ import numpy as np
 from . import _capi as capi
 from ._capi import R2Error
 
-__all__ = ["Context", "Grid", "Env", "R2Error", "capi"]
+__all__ = ["Context", "Grid", "Env", "Mesh", "R2Error", "capi", "filter_params"]
+
+
+def filter_params(voxel_size: float, samples: int = 64, iterations: int = 8, sigma_t: float = 1.0,
+                  alpha: float = 1.0, clip_rays: bool = True, seed: int = 0, lod: int = 0) -> "capi.FilterParams":
+    return capi.FilterParams(voxel_size, samples, iterations, sigma_t, alpha, int(bool(clip_rays)), seed, lod)
 
 
 class Context:
@@ -113,6 +118,54 @@ class Context:
                                                out.ctypes.data, cnt.ctypes.data if counters else None))
         return (out, cnt[:, :7]) if counters else out
 
+    # -------------------------------------------------------------- filter
+    def upload_mesh(self, mesh: dict) -> "Mesh":
+        keep = {}
+        for k in ("positions", "normals", "mat_diffuse", "mat_specular", "mat_roughness", "mat_ior"):
+            keep[k] = np.ascontiguousarray(mesh[k], np.float32)
+        keep["indices"] = np.ascontiguousarray(mesh["indices"], np.int32)
+        mat = mesh.get("material")
+        keep["material"] = None if mat is None else np.ascontiguousarray(mat, np.int32)
+        d = capi.MeshDesc()
+        d.n_vertices = keep["positions"].shape[0]
+        d.n_triangles = keep["indices"].shape[0]
+        for k in ("positions", "normals", "indices", "mat_diffuse", "mat_specular", "mat_roughness", "mat_ior"):
+            setattr(d, k, keep[k].ctypes.data)
+        d.material = None if keep["material"] is None else keep["material"].ctypes.data
+        d.n_materials = keep["mat_ior"].shape[0]
+        d.pos[:] = [float(v) for v in mesh.get("pos", (0, 0, 0))]
+        d.orientation[:] = [float(v) for v in mesh.get("orientation", (0, 1, 0))]
+        d.scaling[:] = [float(v) for v in mesh.get("scaling", (1, 1, 1))]
+        h = C.c_void_p()
+        self._check(capi.lib().r2_mesh_upload(self.h, C.byref(d), C.byref(h)))
+        return Mesh(self, h)
+
+    def filter_lod(self, mesh: "Mesh", params, out_dev_ptr: int | None = None):
+        """Filter one LOD.  Returns (records (nz, ny, nx, 32) u8 or None when written
+        to device memory, r2_filter_grid)."""
+        g = capi.FilterGrid()
+        capi.check(capi.lib().r2_filter_setup(mesh.h, C.c_float(params.voxel_size), C.byref(g)), self.h)
+        nx, ny, nz = list(g.num_voxels)
+        if out_dev_ptr is None:
+            out = np.zeros((nz, ny, nx, 32), np.uint8)
+            self._check(capi.lib().r2_filter_lod(self.h, mesh.h, C.byref(params), out.ctypes.data, 0, C.byref(g)))
+            return out, g
+        self._check(capi.lib().r2_filter_lod(self.h, mesh.h, C.byref(params), C.c_void_p(out_dev_ptr), 1,
+                                             C.byref(g)))
+        return None, g
+
+    def filter_voxels(self, mesh: "Mesh", params, ids) -> np.ndarray:
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.zeros((ids.shape[0], 32), np.uint8)
+        self._check(capi.lib().r2_filter_voxels(self.h, mesh.h, C.byref(params), ids.shape[0], ids.ctypes.data,
+                                                out.ctypes.data))
+        return out
+
+    def filter_stats(self) -> dict:
+        s = capi.FilterStats()
+        self._check(capi.lib().r2_filter_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
     def stats(self) -> dict:
         s = capi.Stats()
         self._check(capi.lib().r2_get_stats(self.h, C.byref(s)))
@@ -181,3 +234,24 @@ class Env:
             self.free()
         except Exception:
             pass
+
+
+class Mesh:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def free(self):
+        if self.h:
+            capi.lib().r2_mesh_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def setup(self, voxel_size: float) -> "capi.FilterGrid":
+        g = capi.FilterGrid()
+        capi.check(capi.lib().r2_filter_setup(self.h, C.c_float(voxel_size), C.byref(g)))
+        return g

@@ -109,6 +109,13 @@ class FilterGrid(C.Structure):
     ]
 
 
+class FilterStats(C.Structure):
+    _fields_ = [
+        ("voxels", C.c_int64), ("active_voxels", C.c_int64), ("pairs", C.c_int64), ("rays", C.c_int64),
+        ("filter_kernel_ms", C.c_double), ("total_ms", C.c_double),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_inflight", C.c_int32), ("slice_paths", C.c_int32)]
 
@@ -148,6 +155,9 @@ SIGNATURES = [
     ("r2_filter_setup", C.c_int, [C.c_void_p, C.c_float, C.POINTER(FilterGrid)]),
     ("r2_filter_lod", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FilterParams), C.c_void_p, C.c_int32,
                                 C.POINTER(FilterGrid)]),
+    ("r2_filter_voxels", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FilterParams), C.c_int32, C.c_void_p,
+                                   C.c_void_p]),
+    ("r2_filter_get_stats", C.c_int, [C.c_void_p, C.POINTER(FilterStats)]),
     ("r2_server_start", C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     ("r2_server_submit", C.c_int64, [C.c_void_p, C.POINTER(Request)]),
     ("r2_server_poll", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
@@ -157,6 +167,8 @@ SIGNATURES = [
     ("r2_server_stop", C.c_int, [C.c_void_p]),
     ("r2_synth_volume", C.c_int, [C.c_int32, C.c_int32, C.c_uint32, C.c_float, C.c_int32, C.c_void_p]),
     ("r2_synth_env", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    ("r2_synth_mesh", C.c_int, [C.c_int32, C.c_int32, C.c_float, C.c_uint32, C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 

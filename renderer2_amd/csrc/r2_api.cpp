@@ -35,7 +35,7 @@ int ensure_scratch(r2_ctx* ctx, size_t bytes) {
 }
 
 // getRotation (functions.hpp:128-142)
-static m44 get_rotation(v3 n) {
+m44 get_rotation(v3 n) {
   const float cs = dot(n, mk(0.f, 1.f, 0.f));
   if (cs == 1.f) return identity44();
   if (cs == -1.f) {

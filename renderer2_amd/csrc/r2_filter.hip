@@ -1,0 +1,920 @@
+// Mesh -> SGGX voxel filter on gfx950 (CMeshFilter, raytracer/include/filtering/mesh_filter.hpp:21-266,
+// launched per voxel by __raygen__filtering, raytracer/src/backend/shaders.optix.cu:258-284).
+//
+// Per LOD grid:
+//   1. k_tri_prep   per triangle: world AABB -> conservative range of voxels whose
+//                   ray region (the clip sphere, or the voxel plus one ray length)
+//                   can reach it, split into 64-voxel tasks.
+//   2. k_pairs      one wave per task, one voxel per lane: exact point/triangle
+//                   distance against the voxel's reach radius; passing (voxel, tri)
+//                   pairs are counted, then emitted as 64-bit keys voxel<<32 | tri.
+//   3. radix sort + run-length encode: sorted candidate list per active voxel.
+//   4. k_filter     one wave per active voxel, the 64 lanes are the voxel's rays.
+//                   Every lane tests every candidate triangle (uniform loop; the
+//                   triangle is a wave-uniform load), keeps the lexicographic
+//                   minimum of (t, triangle index), and the per-ray sums are
+//                   reduced in ray order through LDS, so the record equals the
+//                   sequential loop of the reference bit for bit.
+// A voxel with no candidate cannot be hit by any of its rays: its record is the
+// no-hit record, written for all voxels up front by k_fill_empty.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "r2_internal.hpp"
+#include "r2_codec.hpp"
+#include "r2_detmath.hpp"
+
+using namespace r2;
+
+struct r2_mesh {
+  int32_t n_vertices = 0, n_triangles = 0, n_materials = 0;
+  float model_min[3] = {0, 0, 0}, model_max[3] = {0, 0, 0};
+  m43 m2w{};               // scene object modelToWorld = T(pos) * getRotation(orientation) * S(scaling)
+  float4* tri = nullptr;   // 3 per triangle: world-space vertices (w unused)
+  int32_t* idx = nullptr;  // 3 per triangle
+  float* nrm = nullptr;    // 3 per vertex, model space
+  int32_t* mat = nullptr;  // material id per triangle
+  float* matf = nullptr;   // 8 per material: diffuse rgb, specular rgb, roughness, ior
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+constexpr int kWaves = 4;  // waves per 256-thread block
+constexpr int kVals = 17;  // per-ray sums: S6, diffuse3, specular3, normal3, ior, distance
+constexpr uint32_t kTag = 0xF17Eu;
+
+struct FConst {
+  m43 i2m, m2w, w2m, i2w;  // indexToModel, modelToWorld, worldToModel, indexToWorld (mat4x3)
+  m43 mesh_m2w;            // mesh object modelToWorld (hit normals)
+  m44 rs;                  // getRotation((1,0,0)) of the clip sphere
+  int nx, ny, nz;
+  float diagonal, tmax_fixed, radius;
+  float reach;             // conservative distance from the voxel centre a ray can reach
+  float bbmin[3], inv_vs[3], range_margin;
+  int n_samples, iterations, clip;
+  float sigma_t, alpha;
+  uint32_t seed, lod;
+  int n_materials;
+};
+
+__device__ __forceinline__ float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+__device__ __forceinline__ v3 f4v(float4 a) { return mk(a.x, a.y, a.z); }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// voxel centre: modelToWorld * (indexToModel * (v + 0.5))  (mesh_filter.hpp:47)
+__device__ __forceinline__ v3 voxel_center(const FConst& c, int ix, int iy, int iz) {
+  const v3 vm = mul_point(c.i2m, mk((float)ix + 0.5f, (float)iy + 0.5f, (float)iz + 0.5f));
+  return mul_point(c.m2w, vm);
+}
+
+// ---------------------------------------------------------------- mesh ----
+__global__ void k_world_tris(const float* __restrict__ pos, const int32_t* __restrict__ idx, m43 m2w, int n,
+                             float4* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  for (int k = 0; k < 3; ++k) {
+    const int v = idx[3 * t + k];
+    const v3 p = mul_point(m2w, mk(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]));
+    out[3 * t + k] = make_float4(p.x, p.y, p.z, 0.f);
+  }
+}
+
+// -------------------------------------------------------------- binning ---
+// squared distance from p to triangle (a, b, c): closest point by Voronoi
+// regions (Ericson, Real-Time Collision Detection, 5.1.5), relative to a
+__device__ float dist2_point_triangle(v3 p, v3 a, v3 b, v3 c) {
+  const v3 ab = b - a, ac = c - a, ap = p - a;
+  const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.f && d2 <= 0.f) return dot(ap, ap);
+  const v3 bp = p - b;
+  const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.f && d4 <= d3) return dot(bp, bp);
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+    const float v = d1 / (d1 - d3);
+    const v3 q = ap - ab * v;
+    return dot(q, q);
+  }
+  const v3 cp = p - c;
+  const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.f && d5 <= d6) return dot(cp, cp);
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+    const float w = d2 / (d2 - d6);
+    const v3 q = ap - ac * w;
+    return dot(q, q);
+  }
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const v3 q = bp - (c - b) * w;
+    return dot(q, q);
+  }
+  const float den = 1.f / (va + vb + vc);
+  const float v = vb * den, w = vc * den;
+  const v3 q = ap - ab * v - ac * w;
+  return dot(q, q);
+}
+
+struct TriRange {
+  int lo[3];
+  int dim[3];
+};
+
+__global__ void k_tri_prep(const float4* __restrict__ tri, int n, FConst c, TriRange* __restrict__ rng,
+                           unsigned long long* __restrict__ tasks) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float4 a = tri[3 * t], b = tri[3 * t + 1], d = tri[3 * t + 2];
+  const float mn[3] = {fminf(a.x, fminf(b.x, d.x)), fminf(a.y, fminf(b.y, d.y)), fminf(a.z, fminf(b.z, d.z))};
+  const float mx[3] = {fmaxf(a.x, fmaxf(b.x, d.x)), fmaxf(a.y, fmaxf(b.y, d.y)), fmaxf(a.z, fmaxf(b.z, d.z))};
+  const int nv[3] = {c.nx, c.ny, c.nz};
+  TriRange r;
+  unsigned long long vol = 1;
+  for (int k = 0; k < 3; ++k) {
+    // voxel centres lie at bbmin + (i + 0.5) * vs; keep those within reach of the AABB
+    const float flo = (mn[k] - c.reach - c.bbmin[k]) * c.inv_vs[k] - 0.5f - c.range_margin;
+    const float fhi = (mx[k] + c.reach - c.bbmin[k]) * c.inv_vs[k] - 0.5f + c.range_margin;
+    int lo = (flo > -1.f) ? (int)floorf(flo) : -1;
+    int hi = (fhi < (float)nv[k]) ? (int)ceilf(fhi) : nv[k];
+    lo = max(lo, 0);
+    hi = min(hi, nv[k] - 1);
+    if (!(flo == flo) || !(fhi == fhi) || hi < lo) {  // NaN vertex or out of the grid
+      lo = 0;
+      hi = -1;
+    }
+    r.lo[k] = lo;
+    r.dim[k] = hi - lo + 1;
+    vol *= (unsigned long long)max(r.dim[k], 0);
+  }
+  rng[t] = r;
+  tasks[t] = (vol + 63ull) >> 6;
+}
+
+// one wave per 64-voxel task; kEmit = false counts, true writes keys
+template <bool kEmit>
+__global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, int n_tris, FConst c,
+                                               const TriRange* __restrict__ rng,
+                                               const unsigned long long* __restrict__ task_begin,
+                                               unsigned long long n_tasks, unsigned long long* __restrict__ counter,
+                                               unsigned long long* __restrict__ keys, unsigned long long cap) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long task =
+      (unsigned long long)blockIdx.x * kWaves + (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (task >= n_tasks) return;
+  // last triangle whose first task is <= task (wave-uniform binary search)
+  int lo = 0, hi = n_tris - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (task_begin[mid] <= task) lo = mid; else hi = mid - 1;
+  }
+  const int t = lo;
+  const TriRange r = rng[t];
+  const unsigned long long local = (task - task_begin[t]) * 64ull + (unsigned long long)lane;
+  const unsigned long long vol = (unsigned long long)r.dim[0] * r.dim[1] * r.dim[2];
+  bool pass = false;
+  uint32_t vid = 0;
+  if (local < vol) {
+    const unsigned long long dxy = (unsigned long long)r.dim[0] * r.dim[1];
+    const int iz = r.lo[2] + (int)(local / dxy);
+    const unsigned long long rem = local % dxy;
+    const int iy = r.lo[1] + (int)(rem / r.dim[0]);
+    const int ix = r.lo[0] + (int)(rem % r.dim[0]);
+    const v3 ctr = voxel_center(c, ix, iy, iz);
+    const v3 a = f4v(tri[3 * t]), b = f4v(tri[3 * t + 1]), d = f4v(tri[3 * t + 2]);
+    pass = dist2_point_triangle(ctr, a, b, d) <= c.reach * c.reach;
+    vid = (uint32_t)ix + (uint32_t)c.nx * ((uint32_t)iy + (uint32_t)c.ny * (uint32_t)iz);
+  }
+  const unsigned long long ball = __ballot(pass);
+  const int cnt = __popcll(ball);
+  if (cnt == 0) return;
+  if (!kEmit) {
+    if (lane == 0) atomicAdd(counter, (unsigned long long)cnt);
+    return;
+  }
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(counter, (unsigned long long)cnt);
+  base = __shfl(base, 0);
+  const unsigned long long slot = base + (unsigned long long)__popcll(ball & ((1ull << lane) - 1ull));
+  if (pass && slot < cap) {
+    keys[slot] = ((unsigned long long)vid << 32) | (unsigned long long)(uint32_t)t;
+  }
+}
+
+__global__ void k_split(const unsigned long long* __restrict__ keys, unsigned long long n, uint32_t* __restrict__ vox,
+                        uint32_t* __restrict__ tri) {
+  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long k = keys[i];
+  vox[i] = (uint32_t)(k >> 32);
+  tri[i] = (uint32_t)k;
+}
+
+// no-hit record: density 0, S 0, colours 0, normal 0 (NaN -> 0), ior half(1)
+__global__ void k_fill_empty(uint4* __restrict__ rec, unsigned long long nvox) {
+  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvox) return;
+  rec[2 * i] = make_uint4(0u, 0u, 0u, 0u);
+  rec[2 * i + 1] = make_uint4(0u, 0u, 0u, 0x3C00u);
+}
+
+// -------------------------------------------------------------- filter ----
+// uniforms 5i .. 5i+4 of the voxel's Philox stream (ray i draws 3 for the
+// origin, 2 for the direction, generateRay mesh_filter.hpp:260-266)
+__device__ __forceinline__ void ray_uniforms(uint32_t seed, uint32_t lod, uint32_t vid, int i, float u[5]) {
+  const uint32_t k0 = 5u * (uint32_t)i;
+  const uint32_t b0 = k0 >> 2;
+  uint32_t p0[4] = {vid, 0u, b0, kTag};
+  uint32_t p1[4] = {vid, 0u, b0 + 1u, kTag};
+  philox4x32_10(p0, seed, lod);
+  philox4x32_10(p1, seed, lod);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const uint32_t k = k0 + (uint32_t)j;
+    const uint32_t* src = ((k >> 2) == b0) ? p0 : p1;
+    const uint32_t s = k & 3u;
+    const uint32_t x = s == 0 ? src[0] : (s == 1 ? src[1] : (s == 2 ? src[2] : src[3]));
+    u[j] = (float)(x >> 8) * 5.9604644775390625e-08f;
+  }
+}
+
+// Sphere::intersect of a world-space ray (sphere.hpp:33-90); 0 on a miss
+__device__ float sphere_exit(const m43& w2m_s, float radius, v3 o, v3 d, float ray_tmax) {
+  const v3 mo = mul_point(w2m_s, o);
+  const v3 md = normalize(mul_dir(w2m_s, d));
+  const float a = dot(md, md);
+  const float b = 2.0f * dot(md, mo);
+  const float cc = dot(mo, mo) - radius * radius;
+  const float disc = b * b - 4.f * a * cc;
+  if (disc == 0.f) {
+    const float t = (-b + sqrtf(disc)) / (2.f * a);
+    if (t > 0.f && t <= ray_tmax) return t;
+  } else if (disc > 0.f) {
+    const float sq = sqrtf(disc);
+    const float den = 1.f / (2.0f * a);
+    const float t1 = (-b + sq) * den, t2 = (-b - sq) * den;
+    const float lo = gmin(t1, t2), hi = gmax(t1, t2);
+    if (hi > 0.f) {
+      if (lo > 0.f && lo <= ray_tmax) return lo;
+      if (lo <= 0.f && hi <= ray_tmax) return hi;
+    }
+  }
+  return 0.f;
+}
+
+// watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013)
+struct WRay {
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+__device__ __forceinline__ WRay wt_setup(v3 d) {
+  WRay w;
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  w.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  w.kx = w.kz + 1 == 3 ? 0 : w.kz + 1;
+  w.ky = w.kx + 1 == 3 ? 0 : w.kx + 1;
+  if (comp(d, w.kz) < 0.f) {
+    const int t = w.kx;
+    w.kx = w.ky;
+    w.ky = t;
+  }
+  w.Sx = comp(d, w.kx) / comp(d, w.kz);
+  w.Sy = comp(d, w.ky) / comp(d, w.kz);
+  w.Sz = 1.f / comp(d, w.kz);
+  return w;
+}
+__device__ __forceinline__ bool wt_hit(const WRay& w, v3 o, v3 v0, v3 v1, v3 v2, float tmax, float* t, float* b1,
+                                       float* b2) {
+  const v3 A = v0 - o, B = v1 - o, Cc = v2 - o;
+  const float Ax = comp(A, w.kx) - w.Sx * comp(A, w.kz), Ay = comp(A, w.ky) - w.Sy * comp(A, w.kz);
+  const float Bx = comp(B, w.kx) - w.Sx * comp(B, w.kz), By = comp(B, w.ky) - w.Sy * comp(B, w.kz);
+  const float Cx = comp(Cc, w.kx) - w.Sx * comp(Cc, w.kz), Cy = comp(Cc, w.ky) - w.Sy * comp(Cc, w.kz);
+  float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  if (U == 0.f || V == 0.f || W == 0.f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+  const float det = U + V + W;
+  if (det == 0.f) return false;
+  const float Az = w.Sz * comp(A, w.kz), Bz = w.Sz * comp(B, w.kz), Cz = w.Sz * comp(Cc, w.kz);
+  const float T = U * Az + V * Bz + W * Cz;
+  if (det < 0.f && (T >= 0.f || T < tmax * det)) return false;
+  if (det > 0.f && (T <= 0.f || T > tmax * det)) return false;
+  const float rd = 1.f / det;
+  *t = T * rd;
+  *b1 = V * rd;
+  *b2 = W * rd;
+  return true;
+}
+
+// one wave per work voxel.  direct: work[w] indexes act/off/cnt; otherwise work
+// holds voxel ids, looked up in the sorted active list, and records go to out[w].
+__global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __restrict__ work, uint32_t n_work,
+                                                int direct, const uint32_t* __restrict__ act,
+                                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt,
+                                                uint32_t n_act, const uint32_t* __restrict__ cand,
+                                                const float4* __restrict__ tri, const int32_t* __restrict__ tidx,
+                                                const float* __restrict__ nrm, const int32_t* __restrict__ tmat,
+                                                const float* __restrict__ matf, uint4* __restrict__ out,
+                                                int ts_cap) {
+  extern __shared__ float lds[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + (uint32_t)wave));
+  if (w >= n_work) return;
+  float* V = lds + wave * (kVals * 64 + 64 + ts_cap);  // [64][kVals] per-ray values (later: exp terms)
+  int* F = (int*)(V + kVals * 64);                      // [64] hit flags
+  float* TS = V + kVals * 64 + 64;                      // [ts_cap] clip distances of the hits, in ray order
+
+  uint32_t vid, beg = 0, nc = 0;
+  size_t orec;
+  if (direct) {
+    vid = act[w];
+    beg = off[w];
+    nc = cnt[w];
+    orec = vid;
+  } else {
+    vid = work[w];
+    orec = w;
+    uint32_t lo = 0, hi = n_act;  // first active voxel >= vid
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (act[mid] < vid) lo = mid + 1; else hi = mid;
+    }
+    if (lo < n_act && act[lo] == vid) {
+      beg = off[lo];
+      nc = cnt[lo];
+    }
+  }
+  vid = (uint32_t)__builtin_amdgcn_readfirstlane((int)vid);
+  beg = (uint32_t)__builtin_amdgcn_readfirstlane((int)beg);
+  nc = (uint32_t)__builtin_amdgcn_readfirstlane((int)nc);
+  const int ix = (int)(vid % (uint32_t)c.nx), iy = (int)((vid / (uint32_t)c.nx) % (uint32_t)c.ny),
+            iz = (int)(vid / ((uint32_t)c.nx * (uint32_t)c.ny));
+  const v3 ctr = voxel_center(c, ix, iy, iz);
+  // clip sphere: model = translate(ctr) * getRotation((1,0,0))
+  m44 Ts = identity44();
+  Ts.c[3][0] = ctr.x;
+  Ts.c[3][1] = ctr.y;
+  Ts.c[3][2] = ctr.z;
+  const m43 m2w_s = to43(mul44(Ts, c.rs));
+  const m43 w2m_s = to43(inverse44(to44(m2w_s)));
+
+  float acc = 0.f;  // lane q < kVals: running sum of quantity q over the hit rays, in ray order
+  uint32_t hits = 0;
+  for (int r0 = 0; r0 < c.n_samples; r0 += 64) {
+    const int i = r0 + lane;
+    const bool live = i < c.n_samples;
+    float u[5];
+    ray_uniforms(c.seed, c.lod, vid, i, u);
+    const v3 o = mul_point(c.i2w, mk((float)ix + u[0], (float)iy + u[1], (float)iz + u[2]));
+    const float z = 1.f - 2.f * u[3];
+    const float rr = sqrtf(gmax(0.f, 1.f - z * z));
+    const float phi = (float)(2.0 * 3.14159265358979323846 * (double)u[4]);
+    float sphi, cphi;
+    det_sincosf(phi, &sphi, &cphi);
+    const v3 d = mk(rr * cphi, rr * sphi, z);
+    const float cur_tmax = c.clip ? sphere_exit(w2m_s, c.radius, o, d, 1e12f) : c.tmax_fixed;
+    const WRay wr = wt_setup(d);
+    int best = -1;
+    float bt = 0.f, bb1 = 0.f, bb2 = 0.f;
+    for (uint32_t k = 0; k < nc; ++k) {
+      const uint32_t t = cand[beg + k];
+      const v3 p0 = f4v(tri[3 * t]), p1 = f4v(tri[3 * t + 1]), p2 = f4v(tri[3 * t + 2]);
+      float th, b1, b2;
+      if (live && wt_hit(wr, o, p0, p1, p2, cur_tmax, &th, &b1, &b2)) {
+        if (best < 0 || th < bt || (th == bt && (int)t < best)) {
+          best = (int)t;
+          bt = th;
+          bb1 = b1;
+          bb2 = b2;
+        }
+      }
+    }
+    const bool hit = best >= 0;
+    float val[kVals];
+#pragma unroll
+    for (int q = 0; q < kVals; ++q) val[q] = 0.f;
+    if (hit) {
+      // closest-hit: barycentric smooth normal, modelToWorld, normalize (shaders.optix.cu:48-58,106-114)
+      const int i0 = tidx[3 * best], i1 = tidx[3 * best + 1], i2 = tidx[3 * best + 2];
+      const v3 N0 = mk(nrm[3 * i0], nrm[3 * i0 + 1], nrm[3 * i0 + 2]);
+      const v3 N1 = mk(nrm[3 * i1], nrm[3 * i1 + 1], nrm[3 * i1 + 2]);
+      const v3 N2 = mk(nrm[3 * i2], nrm[3 * i2 + 1], nrm[3 * i2 + 2]);
+      const float w0 = 1.f - bb1 - bb2;
+      v3 n = normalize((w0 * N0 + bb1 * N1) + bb2 * N2);
+      n = normalize(mul_dir(c.mesh_m2w, n));
+      int m = tmat ? tmat[best] : 0;
+      m = (m >= 0 && m < c.n_materials) ? m : 0;
+      const float* mf = matf + 8 * m;
+      const float sg = mf[6];
+      // buildS(n, sigma) = n n^T + sigma (I - n n^T)  (sggx_phase_function.hpp:95-105)
+      const float xx = n.x * n.x, xy = n.x * n.y, xz = n.x * n.z, yy = n.y * n.y, yz = n.y * n.z, zz = n.z * n.z;
+      val[0] = xx + sg * (yy + zz);
+      val[1] = yy + sg * (xx + zz);
+      val[2] = zz + sg * (xx + yy);
+      val[3] = xy + sg * (-xy);
+      val[4] = xz + sg * (-xz);
+      val[5] = yz + sg * (-yz);
+      val[6] = mf[0];
+      val[7] = mf[1];
+      val[8] = mf[2];
+      val[9] = mf[3];
+      val[10] = mf[4];
+      val[11] = mf[5];
+      val[12] = n.x;
+      val[13] = n.y;
+      val[14] = n.z;
+      val[15] = mf[7];
+      val[16] = cur_tmax;
+    }
+#pragma unroll
+    for (int q = 0; q < kVals; ++q) V[lane * kVals + q] = val[q];
+    F[lane] = hit ? 1 : 0;
+    const unsigned long long hb = __ballot(hit);
+    if (hit) TS[hits + (uint32_t)__popcll(hb & ((1ull << lane) - 1ull))] = cur_tmax;
+    wave_sync();
+    const int nr = min(64, c.n_samples - r0);
+    if (lane < kVals) {
+      for (int r = 0; r < nr; ++r)
+        if (F[r]) acc += V[r * kVals + lane];
+    }
+    hits += (uint32_t)__popcll(hb);
+    wave_sync();
+  }
+  float sum[kVals];
+#pragma unroll
+  for (int q = 0; q < kVals; ++q) sum[q] = __shfl(acc, q);
+
+  uint32_t wout[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0x3C00u};
+  if (hits > 0) {
+    // estimateDensity (mesh_filter.hpp:182-204, 216-226)
+    const float avg = sum[16] / (float)hits;
+    const float P = (float)hits / (float)c.n_samples;
+    float dens = -det_logf(1.f - P) / (avg * c.sigma_t);
+    const float na = c.alpha / c.sigma_t;
+    for (int it = 0; it < c.iterations; ++it) {
+      float Pv;
+      if (c.clip) {
+        for (uint32_t h = (uint32_t)lane; h < hits; h += 64) V[h] = det_expf(-dens * c.sigma_t * TS[h]);
+        wave_sync();
+        float e = 0.f;
+        for (uint32_t h = 0; h < hits; ++h) e += V[h];
+        wave_sync();
+        Pv = 1.f - e / (float)hits;
+      } else {
+        Pv = 1.f - det_expf(-dens * c.sigma_t * c.tmax_fixed);
+      }
+      dens = dens - na * (Pv - P);
+    }
+    const float ih = 1.f / (float)hits;
+    const float S6[6] = {sum[0] * ih, sum[1] * ih, sum[2] * ih, sum[3] * ih, sum[4] * ih, sum[5] * ih};
+    const float dif[3] = {sum[6] * ih, sum[7] * ih, sum[8] * ih};
+    const float spec[3] = {sum[9] * ih, sum[10] * ih, sum[11] * ih};
+    const v3 nm = normalize(mul_dir(c.w2m, mk(sum[12], sum[13], sum[14]) * ih));
+    const float nv[3] = {nm.x, nm.y, nm.z};
+    encode_record(dens, S6, dif, spec, nv, sum[15] * ih, wout);
+  }
+  if (lane == 0) {
+    out[2 * orec] = make_uint4(wout[0], wout[1], wout[2], wout[3]);
+    out[2 * orec + 1] = make_uint4(wout[4], wout[5], wout[6], wout[7]);
+  }
+}
+
+// ----------------------------------------------------------------- host ---
+struct DevBufs {
+  std::vector<void*> p;
+  ~DevBufs() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <class T>
+  hipError_t alloc(T** out, size_t bytes) {
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+    if (e == hipSuccess) p.push_back(q);
+    *out = (T*)q;
+    return e;
+  }
+};
+
+#define R2F_HIP(expr, what)                           \
+  do {                                                \
+    const hipError_t e_ = (expr);                     \
+    if (e_ != hipSuccess) return r2host::hip_fail(ctx, e_, what); \
+  } while (0)
+
+// setupGrid (openvdb_backend.cpp:47-106) on the host, same arithmetic as glm
+int setup_grid(const r2_mesh* m, float voxel_size, r2_filter_grid* g) {
+  if (!(voxel_size > 0.f)) return R2_ERR_INVALID;
+  const m44 m2w = to44(m->m2w);
+  const m43 w2m = to43(inverse44(m2w));
+  const v3 a = mul_point(m->m2w, mk(m->model_min[0], m->model_min[1], m->model_min[2]));
+  const v3 b = mul_point(m->m2w, mk(m->model_max[0], m->model_max[1], m->model_max[2]));
+  const v3 wmin = mk(gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z));
+  const v3 wmax = mk(gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z));
+  const v3 ext = wmax - wmin;
+  const float fn[3] = {ext.x / voxel_size, ext.y / voxel_size, ext.z / voxel_size};
+  int nv[3];
+  for (int i = 0; i < 3; ++i) {
+    const float v = ceilf(fn[i] - 0.001f);
+    if (!(v >= 1.f) || v > 65536.f) return R2_ERR_INVALID;
+    nv[i] = (int)v;
+    g->num_voxels[i] = nv[i];
+  }
+  m44 T = identity44();
+  T.c[3][0] = wmin.x;
+  T.c[3][1] = wmin.y;
+  T.c[3][2] = wmin.z;
+  m44 S1 = identity44();
+  S1.c[0][0] = ext.x;
+  S1.c[1][1] = ext.y;
+  S1.c[2][2] = ext.z;
+  m44 S2 = identity44();
+  S2.c[0][0] = 1.f / (float)nv[0];
+  S2.c[1][1] = 1.f / (float)nv[1];
+  S2.c[2][2] = 1.f / (float)nv[2];
+  const m44 i2m = mul44(mul44(mul44(to44(w2m), T), S1), S2);
+  const m44 m2i = inverse44(i2m);
+  const m44 m2w4 = inverse44(to44(w2m));
+  const m44 i2w = mul44(m2w4, i2m);
+  // glm mat4 * vec4 adds the column products pairwise: (c0 x + c1 y) + (c2 z + c3 w)
+  const float n3[3] = {(float)nv[0], (float)nv[1], (float)nv[2]};
+  float c0[3], c1[3];
+  for (int r = 0; r < 3; ++r) {
+    c0[r] = (i2w.c[0][r] * 0.f + i2w.c[1][r] * 0.f) + (i2w.c[2][r] * 0.f + i2w.c[3][r] * 1.f);
+    c1[r] = (i2w.c[0][r] * n3[0] + i2w.c[1][r] * n3[1]) + (i2w.c[2][r] * n3[2] + i2w.c[3][r] * 1.f);
+  }
+  const m43 i2m3 = to43(i2m), m2i3 = to43(m2i), m2w3 = to43(m2w4);
+  memcpy(g->index_to_model, i2m3.m, sizeof(float) * 12);
+  memcpy(g->model_to_index, m2i3.m, sizeof(float) * 12);
+  memcpy(g->model_to_world, m2w3.m, sizeof(float) * 12);
+  memcpy(g->world_to_model, w2m.m, sizeof(float) * 12);
+  for (int r = 0; r < 3; ++r) {
+    g->world_bb_min[r] = gmin(c0[r], c1[r]);
+    g->world_bb_max[r] = gmax(c0[r], c1[r]);
+  }
+  return R2_OK;
+}
+
+FConst make_consts(const r2_mesh* m, const r2_filter_grid& g, const r2_filter_params& p) {
+  FConst c{};
+  memcpy(c.i2m.m, g.index_to_model, sizeof(float) * 12);
+  memcpy(c.m2w.m, g.model_to_world, sizeof(float) * 12);
+  memcpy(c.w2m.m, g.world_to_model, sizeof(float) * 12);
+  c.i2w = to43(mul44(to44(c.m2w), to44(c.i2m)));
+  c.mesh_m2w = m->m2w;
+  c.rs = r2host::get_rotation(mk(1.f, 0.f, 0.f));
+  c.nx = g.num_voxels[0];
+  c.ny = g.num_voxels[1];
+  c.nz = g.num_voxels[2];
+  // m_voxelSize = (worldBB.max - worldBB.min) / numVoxels (mesh_filter.hpp:48)
+  const v3 vs = mk((g.world_bb_max[0] - g.world_bb_min[0]) / (float)c.nx,
+                   (g.world_bb_max[1] - g.world_bb_min[1]) / (float)c.ny,
+                   (g.world_bb_max[2] - g.world_bb_min[2]) / (float)c.nz);
+  c.diagonal = length(vs);
+  c.tmax_fixed = vs.x;
+  c.radius = 1.1f * c.diagonal / 2.f;
+  // a ray starts inside the voxel (within diagonal/2 of the centre) and ends on
+  // the clip sphere, or after tmax_fixed when rays are not clipped
+  const float r = p.clip_rays ? c.radius : 0.5f * c.diagonal + c.tmax_fixed;
+  float scale = 0.f;
+  for (int k = 0; k < 3; ++k) scale = fmaxf(scale, fmaxf(fabsf(g.world_bb_min[k]), fabsf(g.world_bb_max[k])));
+  c.reach = r * 1.001f + 1e-5f * scale;
+  const float vsa[3] = {vs.x, vs.y, vs.z};
+  float minvs = vsa[0];
+  for (int k = 0; k < 3; ++k) {
+    c.bbmin[k] = g.world_bb_min[k];
+    c.inv_vs[k] = 1.f / vsa[k];
+    minvs = fminf(minvs, vsa[k]);
+  }
+  c.range_margin = 0.05f + 1e-5f * scale / minvs;
+  c.n_samples = p.samples_per_voxel;
+  c.iterations = p.estimation_iterations;
+  c.clip = p.clip_rays ? 1 : 0;
+  c.sigma_t = p.sigma_t;
+  c.alpha = p.alpha;
+  c.seed = p.seed;
+  c.lod = (uint32_t)p.lod;
+  c.n_materials = m->n_materials;
+  return c;
+}
+
+struct Binning {
+  DevBufs bufs;
+  uint32_t* act = nullptr;
+  uint32_t* off = nullptr;
+  uint32_t* cnt = nullptr;
+  uint32_t* cand = nullptr;
+  uint32_t n_act = 0;
+  unsigned long long n_pairs = 0, n_tasks = 0;
+};
+
+// candidate lists per active voxel (sorted by voxel id, then triangle index)
+int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
+  hipStream_t s = ctx->stream;
+  const int n = m->n_triangles;
+  TriRange* rng = nullptr;
+  unsigned long long *tasks = nullptr, *begin = nullptr, *counter = nullptr;
+  R2F_HIP(B->bufs.alloc(&rng, sizeof(TriRange) * n), "filter: alloc ranges");
+  R2F_HIP(B->bufs.alloc(&tasks, sizeof(unsigned long long) * n), "filter: alloc tasks");
+  R2F_HIP(B->bufs.alloc(&begin, sizeof(unsigned long long) * n), "filter: alloc task offsets");
+  R2F_HIP(B->bufs.alloc(&counter, sizeof(unsigned long long) * 2), "filter: alloc counter");
+  hipLaunchKernelGGL(k_tri_prep, dim3((n + 255) / 256), dim3(256), 0, s, m->tri, n, c, rng, tasks);
+  R2F_HIP(hipGetLastError(), "filter: k_tri_prep");
+  size_t tb = 0;
+  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, tasks, begin, n, s), "filter: scan size");
+  void* tmp = nullptr;
+  R2F_HIP(B->bufs.alloc(&tmp, tb), "filter: alloc scan temp");
+  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, tasks, begin, n, s), "filter: scan tasks");
+  unsigned long long hb[2] = {0, 0};
+  R2F_HIP(hipMemcpyAsync(&hb[0], begin + (n - 1), sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipMemcpyAsync(&hb[1], tasks + (n - 1), sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipStreamSynchronize(s), "filter: sync");
+  B->n_tasks = hb[0] + hb[1];
+  if (B->n_tasks == 0) return R2_OK;
+  if (B->n_tasks > (1ull << 40)) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: triangle/voxel overlap too large");
+  const unsigned long long nblk = (B->n_tasks + kWaves - 1) / kWaves;
+  const dim3 grid((unsigned)std::min<unsigned long long>(nblk, 0x7FFFFFFFull));
+  if (nblk > 0x7FFFFFFFull) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: too many binning tasks");
+  R2F_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s), "filter: clear counter");
+  hipLaunchKernelGGL(k_pairs<false>, grid, dim3(256), 0, s, m->tri, n, c, rng, begin, B->n_tasks, counter,
+                     (unsigned long long*)nullptr, 0ull);
+  R2F_HIP(hipGetLastError(), "filter: k_pairs count");
+  unsigned long long np = 0;
+  R2F_HIP(hipMemcpyAsync(&np, counter, sizeof(np), hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipStreamSynchronize(s), "filter: sync");
+  B->n_pairs = np;
+  if (np == 0) return R2_OK;
+  if (np > 0x7FFFFFFFull) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: more than 2^31 voxel/triangle pairs");
+  unsigned long long *keys = nullptr, *keys2 = nullptr;
+  R2F_HIP(B->bufs.alloc(&keys, sizeof(unsigned long long) * np), "filter: alloc keys");
+  R2F_HIP(B->bufs.alloc(&keys2, sizeof(unsigned long long) * np), "filter: alloc keys");
+  R2F_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s), "filter: clear counter");
+  hipLaunchKernelGGL(k_pairs<true>, grid, dim3(256), 0, s, m->tri, n, c, rng, begin, B->n_tasks, counter, keys, np);
+  R2F_HIP(hipGetLastError(), "filter: k_pairs emit");
+  unsigned long long np2 = 0;
+  R2F_HIP(hipMemcpyAsync(&np2, counter, sizeof(np2), hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipStreamSynchronize(s), "filter: sync");
+  if (np2 != np) return r2host::fail(ctx, R2_ERR_HIP, "filter: candidate emit count differs from the count pass");
+  const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
+  int vbits = 1;
+  while ((1ull << vbits) < nvox) ++vbits;
+  size_t sb = 0;
+  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sb, keys, keys2, (int)np, 0, 32 + vbits, s), "filter: sort size");
+  void* stmp = nullptr;
+  R2F_HIP(B->bufs.alloc(&stmp, sb), "filter: alloc sort temp");
+  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(stmp, sb, keys, keys2, (int)np, 0, 32 + vbits, s), "filter: sort");
+  uint32_t *vox = nullptr, *one = nullptr;
+  R2F_HIP(B->bufs.alloc(&vox, sizeof(uint32_t) * np), "filter: alloc voxels");
+  R2F_HIP(B->bufs.alloc(&B->cand, sizeof(uint32_t) * np), "filter: alloc candidates");
+  hipLaunchKernelGGL(k_split, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, keys2, np, vox, B->cand);
+  R2F_HIP(hipGetLastError(), "filter: k_split");
+  R2F_HIP(B->bufs.alloc(&B->act, sizeof(uint32_t) * np), "filter: alloc active");
+  R2F_HIP(B->bufs.alloc(&B->cnt, sizeof(uint32_t) * np), "filter: alloc counts");
+  R2F_HIP(B->bufs.alloc(&B->off, sizeof(uint32_t) * np), "filter: alloc offsets");
+  R2F_HIP(B->bufs.alloc(&one, sizeof(uint32_t) * 2), "filter: alloc run count");
+  size_t rb = 0;
+  R2F_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, rb, vox, B->act, B->cnt, one, (int)np, s), "filter: rle size");
+  void* rtmp = nullptr;
+  R2F_HIP(B->bufs.alloc(&rtmp, rb), "filter: alloc rle temp");
+  R2F_HIP(hipcub::DeviceRunLengthEncode::Encode(rtmp, rb, vox, B->act, B->cnt, one, (int)np, s), "filter: rle");
+  uint32_t nruns = 0;
+  R2F_HIP(hipMemcpyAsync(&nruns, one, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipStreamSynchronize(s), "filter: sync");
+  B->n_act = nruns;
+  size_t ob = 0;
+  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, ob, B->cnt, B->off, (int)nruns, s), "filter: scan size");
+  void* otmp = nullptr;
+  R2F_HIP(B->bufs.alloc(&otmp, ob), "filter: alloc scan temp");
+  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(otmp, ob, B->cnt, B->off, (int)nruns, s), "filter: scan counts");
+  return R2_OK;
+}
+
+int launch_filter(r2_ctx* ctx, const r2_mesh* m, const FConst& c, const Binning& B, const uint32_t* work,
+                  uint32_t n_work, bool direct, uint4* out, float* kernel_ms) {
+  if (n_work == 0) {
+    *kernel_ms = 0.f;
+    return R2_OK;
+  }
+  const int ts_cap = ((c.n_samples + 63) / 64) * 64;
+  const size_t lds = sizeof(float) * kWaves * (kVals * 64 + 64 + ts_cap);
+  const unsigned blocks = (n_work + kWaves - 1) / kWaves;
+  R2F_HIP(hipEventRecord(ctx->ev0, ctx->stream), "filter: event");
+  hipLaunchKernelGGL(k_filter, dim3(blocks), dim3(256), lds, ctx->stream, c, work, n_work, direct ? 1 : 0, B.act,
+                     B.off, B.cnt, B.n_act, B.cand, m->tri, m->idx, m->nrm, m->mat, m->matf, out, ts_cap);
+  R2F_HIP(hipGetLastError(), "filter: k_filter");
+  R2F_HIP(hipEventRecord(ctx->ev1, ctx->stream), "filter: event");
+  R2F_HIP(hipEventSynchronize(ctx->ev1), "filter: sync");
+  R2F_HIP(hipEventElapsedTime(kernel_ms, ctx->ev0, ctx->ev1), "filter: elapsed");
+  return R2_OK;
+}
+
+int check_params(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p) {
+  if (!m || !p) return r2host::fail(ctx, R2_ERR_INVALID, "filter: null mesh / params");
+  if (p->samples_per_voxel <= 0 || p->samples_per_voxel > 1000)
+    return r2host::fail(ctx, R2_ERR_INVALID, "filter: samples_per_voxel must be in [1, 1000]");
+  if (p->estimation_iterations < 0) return r2host::fail(ctx, R2_ERR_INVALID, "filter: negative iterations");
+  if (!(p->sigma_t > 0.f)) return r2host::fail(ctx, R2_ERR_INVALID, "filter: sigma_t must be > 0");
+  return R2_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" int r2_mesh_upload(r2_ctx* ctx, const r2_mesh_desc* d, r2_mesh** out) {
+  if (!ctx || !d || !out) return R2_ERR_INVALID;
+  *out = nullptr;
+  if (d->n_vertices <= 0 || d->n_triangles <= 0 || !d->positions || !d->normals || !d->indices)
+    return r2host::fail(ctx, R2_ERR_INVALID, "mesh: empty or missing arrays");
+  if (d->n_materials <= 0 || !d->mat_diffuse || !d->mat_specular || !d->mat_roughness || !d->mat_ior)
+    return r2host::fail(ctx, R2_ERR_INVALID, "mesh: missing material table");
+  for (int64_t i = 0; i < 3ll * d->n_triangles; ++i)
+    if (d->indices[i] < 0 || d->indices[i] >= d->n_vertices)
+      return r2host::fail(ctx, R2_ERR_INVALID, "mesh: vertex index out of range");
+  if (d->material)
+    for (int64_t t = 0; t < d->n_triangles; ++t)
+      if (d->material[t] < 0 || d->material[t] >= d->n_materials)
+        return r2host::fail(ctx, R2_ERR_INVALID, "mesh: material id out of range");
+  r2_mesh* m = new r2_mesh();
+  m->n_vertices = d->n_vertices;
+  m->n_triangles = d->n_triangles;
+  m->n_materials = d->n_materials;
+  // mesh AABB over the vertices (model space)
+  for (int k = 0; k < 3; ++k) {
+    m->model_min[k] = FLT_MAX;
+    m->model_max[k] = -FLT_MAX;
+  }
+  for (int64_t v = 0; v < d->n_vertices; ++v)
+    for (int k = 0; k < 3; ++k) {
+      m->model_min[k] = gmin(d->positions[3 * v + k], m->model_min[k]);
+      m->model_max[k] = gmax(d->positions[3 * v + k], m->model_max[k]);
+    }
+  m44 T = identity44();
+  T.c[3][0] = d->pos[0];
+  T.c[3][1] = d->pos[1];
+  T.c[3][2] = d->pos[2];
+  const m44 R = r2host::get_rotation(mk(d->orientation[0], d->orientation[1], d->orientation[2]));
+  m44 S = identity44();
+  S.c[0][0] = d->scaling[0];
+  S.c[1][1] = d->scaling[1];
+  S.c[2][2] = d->scaling[2];
+  m->m2w = to43(mul44(mul44(T, R), S));
+  std::vector<float> matf((size_t)8 * d->n_materials);
+  for (int i = 0; i < d->n_materials; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      matf[8 * i + k] = d->mat_diffuse[3 * i + k];
+      matf[8 * i + 3 + k] = d->mat_specular[3 * i + k];
+    }
+    matf[8 * i + 6] = d->mat_roughness[i];
+    matf[8 * i + 7] = d->mat_ior[i];
+  }
+  auto bail = [&](hipError_t e, const char* what) {
+    for (void* q : m->allocs) (void)hipFree(q);
+    delete m;
+    return r2host::hip_fail(ctx, e, what);
+  };
+  auto dalloc = [&](void** p, size_t bytes) {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) m->allocs.push_back(*p);
+    return e;
+  };
+  float* dpos = nullptr;
+  hipError_t e;
+  const size_t nt = (size_t)d->n_triangles, nv = (size_t)d->n_vertices;
+  if ((e = dalloc((void**)&dpos, sizeof(float) * 3 * nv)) != hipSuccess) return bail(e, "mesh: alloc");
+  if ((e = dalloc((void**)&m->nrm, sizeof(float) * 3 * nv)) != hipSuccess) return bail(e, "mesh: alloc");
+  if ((e = dalloc((void**)&m->idx, sizeof(int32_t) * 3 * nt)) != hipSuccess) return bail(e, "mesh: alloc");
+  if ((e = dalloc((void**)&m->mat, sizeof(int32_t) * nt)) != hipSuccess) return bail(e, "mesh: alloc");
+  if ((e = dalloc((void**)&m->matf, sizeof(float) * matf.size())) != hipSuccess) return bail(e, "mesh: alloc");
+  if ((e = dalloc((void**)&m->tri, sizeof(float4) * 3 * nt)) != hipSuccess) return bail(e, "mesh: alloc");
+  hipStream_t s = ctx->stream;
+  if ((e = hipMemcpyAsync(dpos, d->positions, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return bail(e, "mesh: copy");
+  if ((e = hipMemcpyAsync(m->nrm, d->normals, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return bail(e, "mesh: copy");
+  if ((e = hipMemcpyAsync(m->idx, d->indices, sizeof(int32_t) * 3 * nt, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return bail(e, "mesh: copy");
+  if (d->material) {
+    if ((e = hipMemcpyAsync(m->mat, d->material, sizeof(int32_t) * nt, hipMemcpyHostToDevice, s)) != hipSuccess)
+      return bail(e, "mesh: copy");
+  } else if ((e = hipMemsetAsync(m->mat, 0, sizeof(int32_t) * nt, s)) != hipSuccess) {
+    return bail(e, "mesh: clear");
+  }
+  if ((e = hipMemcpyAsync(m->matf, matf.data(), sizeof(float) * matf.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+    return bail(e, "mesh: copy");
+  hipLaunchKernelGGL(k_world_tris, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, dpos, m->idx, m->m2w,
+                     (int)nt, m->tri);
+  if ((e = hipGetLastError()) != hipSuccess) return bail(e, "mesh: k_world_tris");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return bail(e, "mesh: sync");
+  // positions are only needed for the world-space triangles
+  (void)hipFree(dpos);
+  m->allocs.erase(std::find(m->allocs.begin(), m->allocs.end(), (void*)dpos));
+  *out = m;
+  return R2_OK;
+}
+
+extern "C" int r2_mesh_free(r2_ctx* ctx, r2_mesh* m) {
+  (void)ctx;
+  if (!m) return R2_OK;
+  for (void* q : m->allocs) (void)hipFree(q);
+  delete m;
+  return R2_OK;
+}
+
+extern "C" int r2_filter_setup(const r2_mesh* m, float voxel_size, r2_filter_grid* out) {
+  if (!m || !out) return R2_ERR_INVALID;
+  return setup_grid(m, voxel_size, out);
+}
+
+static int filter_common(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p, r2_filter_grid* g, FConst* c,
+                         Binning* B) {
+  int rc = check_params(ctx, m, p);
+  if (rc) return rc;
+  rc = setup_grid(m, p->voxel_size, g);
+  if (rc) return r2host::fail(ctx, rc, "filter: bad voxel size (grid set-up failed)");
+  const unsigned long long nvox = (unsigned long long)g->num_voxels[0] * g->num_voxels[1] * g->num_voxels[2];
+  if (nvox >= (1ull << 32)) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: more than 2^32 voxels");
+  *c = make_consts(m, *g, *p);
+  return bin_triangles(ctx, m, *c, B);
+}
+
+extern "C" int r2_filter_lod(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p, void* out_records,
+                             int32_t out_on_device, r2_filter_grid* grid_out) {
+  if (!ctx || !out_records) return R2_ERR_INVALID;
+  const auto t0 = std::chrono::steady_clock::now();
+  r2_filter_grid g{};
+  FConst c{};
+  Binning B;
+  int rc = filter_common(ctx, m, p, &g, &c, &B);
+  if (rc) return rc;
+  const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
+  uint4* rec = nullptr;
+  DevBufs tmp;
+  if (out_on_device) {
+    rec = (uint4*)out_records;
+  } else {
+    R2F_HIP(tmp.alloc(&rec, 32ull * nvox), "filter: alloc records");
+  }
+  hipLaunchKernelGGL(k_fill_empty, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream, rec, nvox);
+  R2F_HIP(hipGetLastError(), "filter: k_fill_empty");
+  float kms = 0.f;
+  rc = launch_filter(ctx, m, c, B, nullptr, B.n_act, true, rec, &kms);
+  if (rc) return rc;
+  if (!out_on_device) {
+    R2F_HIP(hipMemcpyAsync(out_records, rec, 32ull * nvox, hipMemcpyDeviceToHost, ctx->stream), "filter: d2h records");
+  }
+  R2F_HIP(hipStreamSynchronize(ctx->stream), "filter: sync");
+  if (grid_out) *grid_out = g;
+  const auto t1 = std::chrono::steady_clock::now();
+  ctx->fstats.active_voxels = B.n_act;
+  ctx->fstats.pairs = (int64_t)B.n_pairs;
+  ctx->fstats.voxels = (int64_t)nvox;
+  ctx->fstats.rays = (int64_t)B.n_act * p->samples_per_voxel;
+  ctx->fstats.filter_kernel_ms = kms;
+  ctx->fstats.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return R2_OK;
+}
+
+extern "C" int r2_filter_voxels(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p, int32_t n,
+                                const uint32_t* voxel_ids, void* out_records) {
+  if (!ctx || n < 0 || (n > 0 && (!voxel_ids || !out_records))) return R2_ERR_INVALID;
+  r2_filter_grid g{};
+  FConst c{};
+  Binning B;
+  int rc = filter_common(ctx, m, p, &g, &c, &B);
+  if (rc) return rc;
+  if (n == 0) return R2_OK;
+  const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
+  for (int i = 0; i < n; ++i)
+    if (voxel_ids[i] >= nvox) return r2host::fail(ctx, R2_ERR_INVALID, "filter: voxel id out of range");
+  DevBufs tmp;
+  uint32_t* dids = nullptr;
+  uint4* drec = nullptr;
+  R2F_HIP(tmp.alloc(&dids, sizeof(uint32_t) * n), "filter: alloc ids");
+  R2F_HIP(tmp.alloc(&drec, 32ull * n), "filter: alloc records");
+  R2F_HIP(hipMemcpyAsync(dids, voxel_ids, sizeof(uint32_t) * n, hipMemcpyHostToDevice, ctx->stream), "filter: h2d");
+  float kms = 0.f;
+  rc = launch_filter(ctx, m, c, B, dids, (uint32_t)n, false, drec, &kms);
+  if (rc) return rc;
+  R2F_HIP(hipMemcpyAsync(out_records, drec, 32ull * n, hipMemcpyDeviceToHost, ctx->stream), "filter: d2h");
+  R2F_HIP(hipStreamSynchronize(ctx->stream), "filter: sync");
+  return R2_OK;
+}
+
+extern "C" int r2_filter_get_stats(const r2_ctx* ctx, r2_filter_stats* out) {
+  if (!ctx || !out) return R2_ERR_INVALID;
+  *out = ctx->fstats;
+  return R2_OK;
+}

@@ -17,6 +17,7 @@ struct r2_ctx {
   size_t scratch_bytes = 0;
   r2::DScene* d_scene = nullptr;
   uint32_t* d_counter = nullptr;
+  r2_filter_stats fstats{};
 };
 
 struct r2_grid {
@@ -48,6 +49,8 @@ int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_p
 // scene assembly (r2_api.cpp)
 int build_scene(r2_ctx* ctx, const r2_render_params* p, r2::DScene* out);
 int ensure_scratch(r2_ctx* ctx, size_t bytes);
+// getRotation (functions.hpp:128-142), host only
+r2::m44 get_rotation(r2::v3 n);
 }  // namespace r2host
 
 #define R2_HIP(ctx, call)                                   \

@@ -177,3 +177,116 @@ extern "C" int r2_synth_env(int32_t kind, int32_t W, int32_t H, float* out) {
   }
   return R2_OK;
 }
+
+// Procedural mesh of the mesh->volume workload: a cube-sphere of 6*res^2 quads
+// (2 triangles each), radius 1 displaced along the normal by 0.1*(fbm - 0.5),
+// with smooth (area-weighted) vertex normals, plus n_quads randomly oriented
+// square quads of side `quad_size` (2 triangles, flat normals) whose centres are
+// uniform in the shell 1.05 < r < 1.3.  Triangle material ids: 0 for the
+// sphere, 1 for the quads.  Returns the vertex / triangle counts when the output
+// pointers are NULL.
+extern "C" int r2_synth_mesh(int32_t res, int32_t n_quads, float quad_size, uint32_t seed, int32_t* n_vertices,
+                             int32_t* n_triangles, float* positions, float* normals, int32_t* indices,
+                             int32_t* material) {
+  if (res < 1 || res > 2048 || n_quads < 0 || !n_vertices || !n_triangles) return R2_ERR_INVALID;
+  const int64_t vs = (int64_t)6 * (res + 1) * (res + 1), ts = (int64_t)12 * res * res;
+  const int64_t nv = vs + (int64_t)4 * n_quads, nt = ts + (int64_t)2 * n_quads;
+  if (nv > INT32_MAX || nt > INT32_MAX) return R2_ERR_UNSUPPORTED;
+  *n_vertices = (int32_t)nv;
+  *n_triangles = (int32_t)nt;
+  if (!positions) return R2_OK;
+  if (!normals || !indices) return R2_ERR_INVALID;
+  // faces: (axis, sign) -> vertex grid on the cube face, projected to the sphere
+  const int R1 = res + 1;
+  parallel_for((int64_t)6 * R1, [&](int64_t fr) {
+    const int f = (int)(fr / R1), j = (int)(fr % R1);
+    const int ax = f >> 1;
+    const float sg = (f & 1) ? -1.f : 1.f;
+    for (int i = 0; i < R1; ++i) {
+      const float a = 2.f * (float)i / (float)res - 1.f, b = 2.f * (float)j / (float)res - 1.f;
+      float c[3];
+      c[ax] = sg;
+      c[(ax + 1) % 3] = a * sg;
+      c[(ax + 2) % 3] = b;
+      const v3 d = normalize(mk(c[0], c[1], c[2]));
+      const float r = 1.f + 0.1f * (fbm(seed + 77u, 4, 3.f, d.x + 2.f, d.y + 2.f, d.z + 2.f) - 0.5f);
+      const int64_t v = ((int64_t)f * R1 + j) * R1 + i;
+      positions[3 * v] = d.x * r;
+      positions[3 * v + 1] = d.y * r;
+      positions[3 * v + 2] = d.z * r;
+    }
+  });
+  parallel_for((int64_t)6 * res, [&](int64_t fr) {
+    const int f = (int)(fr / res), j = (int)(fr % res);
+    for (int i = 0; i < res; ++i) {
+      const int32_t v00 = (int32_t)(((int64_t)f * R1 + j) * R1 + i), v10 = v00 + 1, v01 = v00 + R1, v11 = v01 + 1;
+      const int64_t t = ((int64_t)f * res + j) * res * 2 + 2 * i;
+      indices[3 * t] = v00; indices[3 * t + 1] = v10; indices[3 * t + 2] = v11;
+      indices[3 * t + 3] = v00; indices[3 * t + 4] = v11; indices[3 * t + 5] = v01;
+      if (material) material[t] = material[t + 1] = 0;
+    }
+  });
+  // smooth normals: area-weighted face normals accumulated per vertex (vertices
+  // on cube edges are duplicated per face, so neighbouring faces share position
+  // but not normal; the displacement field makes those seams sub-voxel)
+  std::vector<float> acc((size_t)vs * 3, 0.f);
+  for (int64_t t = 0; t < ts; ++t) {
+    const int32_t* id = indices + 3 * t;
+    const v3 p0 = mk(positions[3 * id[0]], positions[3 * id[0] + 1], positions[3 * id[0] + 2]);
+    const v3 p1 = mk(positions[3 * id[1]], positions[3 * id[1] + 1], positions[3 * id[1] + 2]);
+    const v3 p2 = mk(positions[3 * id[2]], positions[3 * id[2] + 1], positions[3 * id[2] + 2]);
+    const v3 fn = cross(p1 - p0, p2 - p0);
+    for (int k = 0; k < 3; ++k) {
+      acc[3 * id[k]] += fn.x;
+      acc[3 * id[k] + 1] += fn.y;
+      acc[3 * id[k] + 2] += fn.z;
+    }
+  }
+  for (int64_t v = 0; v < vs; ++v) {
+    v3 n = mk(acc[3 * v], acc[3 * v + 1], acc[3 * v + 2]);
+    const v3 p = mk(positions[3 * v], positions[3 * v + 1], positions[3 * v + 2]);
+    if (dot(n, p) < 0.f) n = -n;  // outward
+    n = normalize(n);
+    normals[3 * v] = n.x;
+    normals[3 * v + 1] = n.y;
+    normals[3 * v + 2] = n.z;
+  }
+  // foliage quads
+  parallel_for(n_quads, [&](int64_t q) {
+    uint32_t c[4] = {(uint32_t)q, 0u, 0u, 0x0A1Du};
+    philox4x32_10(c, seed, 0x9u);
+    const float u[4] = {(float)(c[0] >> 8) * (1.f / 16777216.f), (float)(c[1] >> 8) * (1.f / 16777216.f),
+                        (float)(c[2] >> 8) * (1.f / 16777216.f), (float)(c[3] >> 8) * (1.f / 16777216.f)};
+    uint32_t c2[4] = {(uint32_t)q, 1u, 0u, 0x0A1Du};
+    philox4x32_10(c2, seed, 0x9u);
+    const float w[2] = {(float)(c2[0] >> 8) * (1.f / 16777216.f), (float)(c2[1] >> 8) * (1.f / 16777216.f)};
+    // centre: uniform direction, radius uniform in volume of the shell
+    const float z = 1.f - 2.f * u[0], rr = sqrtf(fmaxf(0.f, 1.f - z * z)), ph = 6.2831853f * u[1];
+    const float r3a = 1.05f * 1.05f * 1.05f, r3b = 1.3f * 1.3f * 1.3f;
+    const float rad = cbrtf(r3a + u[2] * (r3b - r3a));
+    const v3 ctr = mk(rr * cosf(ph), rr * sinf(ph), z) * rad;
+    // orientation: uniform normal, random spin
+    const float nz = 1.f - 2.f * u[3], nr = sqrtf(fmaxf(0.f, 1.f - nz * nz)), nph = 6.2831853f * w[0];
+    const v3 n = mk(nr * cosf(nph), nr * sinf(nph), nz);
+    const v3 e = fabsf(n.x) < 0.9f ? mk(1.f, 0.f, 0.f) : mk(0.f, 1.f, 0.f);
+    const v3 t0 = normalize(cross(n, e)), b0 = cross(n, t0);
+    const float sp = 6.2831853f * w[1];
+    const v3 tt = t0 * cosf(sp) + b0 * sinf(sp), bb = cross(n, tt);
+    const float h = 0.5f * quad_size;
+    const v3 corner[4] = {ctr - tt * h - bb * h, ctr + tt * h - bb * h, ctr + tt * h + bb * h, ctr - tt * h + bb * h};
+    const int64_t v0 = vs + 4 * q;
+    for (int k = 0; k < 4; ++k) {
+      positions[3 * (v0 + k)] = corner[k].x;
+      positions[3 * (v0 + k) + 1] = corner[k].y;
+      positions[3 * (v0 + k) + 2] = corner[k].z;
+      normals[3 * (v0 + k)] = n.x;
+      normals[3 * (v0 + k) + 1] = n.y;
+      normals[3 * (v0 + k) + 2] = n.z;
+    }
+    const int64_t t = ts + 2 * q;
+    indices[3 * t] = (int32_t)v0; indices[3 * t + 1] = (int32_t)(v0 + 1); indices[3 * t + 2] = (int32_t)(v0 + 2);
+    indices[3 * t + 3] = (int32_t)v0; indices[3 * t + 4] = (int32_t)(v0 + 2); indices[3 * t + 5] = (int32_t)(v0 + 3);
+    if (material) material[t] = material[t + 1] = 1;
+  });
+  return R2_OK;
+}

@@ -44,6 +44,49 @@ def synth_env(kind: str, w: int, h: int) -> np.ndarray:
     return out
 
 
+# materials of the synthetic mesh: 0 = displaced sphere, 1 = foliage quads
+# (diffuse rgb, specular rgb, SGGX roughness sigma, ior)
+MESH_MATERIALS = dict(
+    mat_diffuse=np.array([[0.60, 0.50, 0.40], [0.20, 0.50, 0.10]], np.float32),
+    mat_specular=np.array([[0.04, 0.04, 0.04], [0.10, 0.10, 0.10]], np.float32),
+    mat_roughness=np.array([0.30, 0.60], np.float32),
+    mat_ior=np.array([1.50, 1.33], np.float32),
+)
+
+# mesh -> SGGX filter workload: ~2 M triangle displaced sphere (12 res^2) plus
+# 200 k thin quads of side 0.01 in a shell around it, filtered to LOD grids of
+# 512^3, 256^3 and 128^3 (voxel size doubling from the finest), 64 rays per
+# voxel, 8 density iterations, rays clipped to the voxel's bounding sphere
+FILTER_WORKLOADS = {
+    "c3": dict(res=408, n_quads=200_000, quad_size=0.01, finest=512, lods=3, samples=64, iterations=8,
+               sigma_t=1.0, alpha=1.0, clip_rays=True),
+}
+
+
+def synth_mesh(res: int, n_quads: int = 0, quad_size: float = 0.01, seed: int = 1) -> dict:
+    """Seeded noise-displaced cube-sphere (12 res^2 triangles) plus thin quads."""
+    L = capi.lib()
+    nv, nt = C.c_int32(0), C.c_int32(0)
+    capi.check(L.r2_synth_mesh(res, n_quads, quad_size, seed, C.byref(nv), C.byref(nt), None, None, None, None))
+    pos = np.zeros((nv.value, 3), np.float32)
+    nrm = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.int32)
+    mat = np.zeros(nt.value, np.int32)
+    capi.check(L.r2_synth_mesh(res, n_quads, quad_size, seed, C.byref(nv), C.byref(nt), capi.ptr(pos), capi.ptr(nrm),
+                               capi.ptr(idx), capi.ptr(mat)))
+    mesh = dict(positions=pos, normals=nrm, indices=idx, material=mat, pos=(0.0, 0.0, 0.0),
+                orientation=(0.0, 1.0, 0.0), scaling=(1.0, 1.0, 1.0))
+    mesh.update({k: v.copy() for k, v in MESH_MATERIALS.items()})
+    return mesh
+
+
+def finest_voxel_size(mesh: dict, n: int) -> float:
+    """Voxel size that gives about n voxels along the mesh's longest world axis."""
+    p = mesh["positions"]
+    ext = (p.max(axis=0) - p.min(axis=0)) * np.abs(np.asarray(mesh.get("scaling", (1, 1, 1)), np.float32))
+    return float(np.float32(ext.max()) / np.float32(n))
+
+
 def make_workload(name: str, seed: int = 1, **overrides) -> dict:
     """Build the inputs of one workload.  `overrides` may change n, width,
     height, spp, max_depth (smaller parity-test sizes)."""
