@@ -34,6 +34,28 @@ int ensure_scratch(r2_ctx* ctx, size_t bytes) {
   return R2_OK;
 }
 
+hipError_t pool_get(r2_ctx* ctx, int slot, size_t bytes, void** out) {
+  *out = nullptr;
+  if (slot < 0 || slot >= r2_ctx::kPoolSlots) return hipErrorInvalidValue;
+  bytes = bytes ? bytes : 1;
+  if (ctx->pool_bytes[slot] < bytes) {
+    if (ctx->pool[slot]) {
+      const hipError_t e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return e;
+      (void)hipFree(ctx->pool[slot]);
+      ctx->pool[slot] = nullptr;
+      ctx->pool_bytes[slot] = 0;
+    }
+    // grow by 1/8 to absorb small size changes between calls
+    const size_t want = bytes + bytes / 8;
+    const hipError_t e = hipMalloc(&ctx->pool[slot], want);
+    if (e != hipSuccess) return e;
+    ctx->pool_bytes[slot] = want;
+  }
+  *out = ctx->pool[slot];
+  return hipSuccess;
+}
+
 // getRotation (functions.hpp:128-142)
 m44 get_rotation(v3 n) {
   const float cs = dot(n, mk(0.f, 1.f, 0.f));
@@ -151,6 +173,8 @@ int r2_destroy(r2_ctx* c) {
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->d_scene) (void)hipFree(c->d_scene);
   if (c->d_counter) (void)hipFree(c->d_counter);
+  for (int i = 0; i < r2_ctx::kPoolSlots; ++i)
+    if (c->pool[i]) (void)hipFree(c->pool[i]);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -162,24 +162,27 @@ __global__ void k_tri_prep(const float4* __restrict__ tri, int n, FConst c, TriR
   tasks[t] = (vol + 63ull) >> 6;
 }
 
+// task -> triangle map: mark each triangle's first task, then a max-scan fills the rest
+__global__ void k_task_mark(const unsigned long long* __restrict__ tasks, const unsigned long long* __restrict__ begin,
+                            int n, uint32_t* __restrict__ task_tri) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || tasks[t] == 0) return;
+  task_tri[begin[t]] = (uint32_t)t;
+}
+
 // one wave per 64-voxel task; kEmit = false counts, true writes keys
 template <bool kEmit>
-__global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, int n_tris, FConst c,
+__global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, FConst c,
                                                const TriRange* __restrict__ rng,
                                                const unsigned long long* __restrict__ task_begin,
-                                               unsigned long long n_tasks, unsigned long long* __restrict__ counter,
+                                               const uint32_t* __restrict__ task_tri, unsigned long long n_tasks,
+                                               unsigned long long* __restrict__ counter,
                                                unsigned long long* __restrict__ keys, unsigned long long cap) {
   const int lane = threadIdx.x & 63;
   const unsigned long long task =
       (unsigned long long)blockIdx.x * kWaves + (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (task >= n_tasks) return;
-  // last triangle whose first task is <= task (wave-uniform binary search)
-  int lo = 0, hi = n_tris - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (task_begin[mid] <= task) lo = mid; else hi = mid - 1;
-  }
-  const int t = lo;
+  const int t = __builtin_amdgcn_readfirstlane((int)task_tri[task]);
   const TriRange r = rng[t];
   const unsigned long long local = (task - task_begin[t]) * 64ull + (unsigned long long)lane;
   const unsigned long long vol = (unsigned long long)r.dim[0] * r.dim[1] * r.dim[2];
@@ -494,21 +497,6 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
 }
 
 // ----------------------------------------------------------------- host ---
-struct DevBufs {
-  std::vector<void*> p;
-  ~DevBufs() {
-    for (void* q : p) (void)hipFree(q);
-  }
-  template <class T>
-  hipError_t alloc(T** out, size_t bytes) {
-    void* q = nullptr;
-    const hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
-    if (e == hipSuccess) p.push_back(q);
-    *out = (T*)q;
-    return e;
-  }
-};
-
 #define R2F_HIP(expr, what)                           \
   do {                                                \
     const hipError_t e_ = (expr);                     \
@@ -612,7 +600,6 @@ FConst make_consts(const r2_mesh* m, const r2_filter_grid& g, const r2_filter_pa
 }
 
 struct Binning {
-  DevBufs bufs;
   uint32_t* act = nullptr;
   uint32_t* off = nullptr;
   uint32_t* cnt = nullptr;
@@ -621,22 +608,36 @@ struct Binning {
   unsigned long long n_pairs = 0, n_tasks = 0;
 };
 
+// context pool slots of the binning buffers
+enum {
+  kSlotRange = 0, kSlotTasks, kSlotBegin, kSlotCounter, kSlotTaskMark, kSlotTaskTri, kSlotKeys, kSlotKeys2,
+  kSlotVox, kSlotCand, kSlotAct, kSlotCnt, kSlotOff, kSlotTemp, kSlotRuns, kSlotRecords, kSlotIds
+};
+
+template <class T>
+hipError_t pooled(r2_ctx* ctx, int slot, size_t count, T** out) {
+  void* p = nullptr;
+  const hipError_t e = r2host::pool_get(ctx, slot, sizeof(T) * count, &p);
+  *out = (T*)p;
+  return e;
+}
+
 // candidate lists per active voxel (sorted by voxel id, then triangle index)
 int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   hipStream_t s = ctx->stream;
   const int n = m->n_triangles;
   TriRange* rng = nullptr;
   unsigned long long *tasks = nullptr, *begin = nullptr, *counter = nullptr;
-  R2F_HIP(B->bufs.alloc(&rng, sizeof(TriRange) * n), "filter: alloc ranges");
-  R2F_HIP(B->bufs.alloc(&tasks, sizeof(unsigned long long) * n), "filter: alloc tasks");
-  R2F_HIP(B->bufs.alloc(&begin, sizeof(unsigned long long) * n), "filter: alloc task offsets");
-  R2F_HIP(B->bufs.alloc(&counter, sizeof(unsigned long long) * 2), "filter: alloc counter");
+  R2F_HIP(pooled(ctx, kSlotRange, n, &rng), "filter: alloc ranges");
+  R2F_HIP(pooled(ctx, kSlotTasks, n, &tasks), "filter: alloc tasks");
+  R2F_HIP(pooled(ctx, kSlotBegin, n, &begin), "filter: alloc task offsets");
+  R2F_HIP(pooled(ctx, kSlotCounter, 2, &counter), "filter: alloc counter");
   hipLaunchKernelGGL(k_tri_prep, dim3((n + 255) / 256), dim3(256), 0, s, m->tri, n, c, rng, tasks);
   R2F_HIP(hipGetLastError(), "filter: k_tri_prep");
+  void* tmp = nullptr;
   size_t tb = 0;
   R2F_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, tasks, begin, n, s), "filter: scan size");
-  void* tmp = nullptr;
-  R2F_HIP(B->bufs.alloc(&tmp, tb), "filter: alloc scan temp");
+  R2F_HIP(r2host::pool_get(ctx, kSlotTemp, tb, &tmp), "filter: alloc scan temp");
   R2F_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, tasks, begin, n, s), "filter: scan tasks");
   unsigned long long hb[2] = {0, 0};
   R2F_HIP(hipMemcpyAsync(&hb[0], begin + (n - 1), sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "filter: d2h");
@@ -644,12 +645,20 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   R2F_HIP(hipStreamSynchronize(s), "filter: sync");
   B->n_tasks = hb[0] + hb[1];
   if (B->n_tasks == 0) return R2_OK;
-  if (B->n_tasks > (1ull << 40)) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: triangle/voxel overlap too large");
-  const unsigned long long nblk = (B->n_tasks + kWaves - 1) / kWaves;
-  const dim3 grid((unsigned)std::min<unsigned long long>(nblk, 0x7FFFFFFFull));
-  if (nblk > 0x7FFFFFFFull) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: too many binning tasks");
+  if (B->n_tasks > 0x7FFFFFFFull) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: triangle/voxel overlap too large");
+  const int nt = (int)B->n_tasks;
+  uint32_t *mark = nullptr, *task_tri = nullptr;
+  R2F_HIP(pooled(ctx, kSlotTaskMark, nt, &mark), "filter: alloc task map");
+  R2F_HIP(pooled(ctx, kSlotTaskTri, nt, &task_tri), "filter: alloc task map");
+  R2F_HIP(hipMemsetAsync(mark, 0, sizeof(uint32_t) * nt, s), "filter: clear task map");
+  hipLaunchKernelGGL(k_task_mark, dim3((n + 255) / 256), dim3(256), 0, s, tasks, begin, n, mark);
+  R2F_HIP(hipGetLastError(), "filter: k_task_mark");
+  R2F_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, tb, mark, task_tri, hipcub::Max(), nt, s), "filter: scan size");
+  R2F_HIP(r2host::pool_get(ctx, kSlotTemp, tb, &tmp), "filter: alloc scan temp");
+  R2F_HIP(hipcub::DeviceScan::InclusiveScan(tmp, tb, mark, task_tri, hipcub::Max(), nt, s), "filter: task map");
+  const dim3 grid((unsigned)((B->n_tasks + kWaves - 1) / kWaves));
   R2F_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s), "filter: clear counter");
-  hipLaunchKernelGGL(k_pairs<false>, grid, dim3(256), 0, s, m->tri, n, c, rng, begin, B->n_tasks, counter,
+  hipLaunchKernelGGL(k_pairs<false>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, counter,
                      (unsigned long long*)nullptr, 0ull);
   R2F_HIP(hipGetLastError(), "filter: k_pairs count");
   unsigned long long np = 0;
@@ -659,10 +668,11 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   if (np == 0) return R2_OK;
   if (np > 0x7FFFFFFFull) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: more than 2^31 voxel/triangle pairs");
   unsigned long long *keys = nullptr, *keys2 = nullptr;
-  R2F_HIP(B->bufs.alloc(&keys, sizeof(unsigned long long) * np), "filter: alloc keys");
-  R2F_HIP(B->bufs.alloc(&keys2, sizeof(unsigned long long) * np), "filter: alloc keys");
+  R2F_HIP(pooled(ctx, kSlotKeys, np, &keys), "filter: alloc keys");
+  R2F_HIP(pooled(ctx, kSlotKeys2, np, &keys2), "filter: alloc keys");
   R2F_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s), "filter: clear counter");
-  hipLaunchKernelGGL(k_pairs<true>, grid, dim3(256), 0, s, m->tri, n, c, rng, begin, B->n_tasks, counter, keys, np);
+  hipLaunchKernelGGL(k_pairs<true>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, counter, keys,
+                     np);
   R2F_HIP(hipGetLastError(), "filter: k_pairs emit");
   unsigned long long np2 = 0;
   R2F_HIP(hipMemcpyAsync(&np2, counter, sizeof(np2), hipMemcpyDeviceToHost, s), "filter: d2h");
@@ -671,34 +681,32 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
   int vbits = 1;
   while ((1ull << vbits) < nvox) ++vbits;
+  const int npi = (int)np;
   size_t sb = 0;
-  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sb, keys, keys2, (int)np, 0, 32 + vbits, s), "filter: sort size");
-  void* stmp = nullptr;
-  R2F_HIP(B->bufs.alloc(&stmp, sb), "filter: alloc sort temp");
-  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(stmp, sb, keys, keys2, (int)np, 0, 32 + vbits, s), "filter: sort");
-  uint32_t *vox = nullptr, *one = nullptr;
-  R2F_HIP(B->bufs.alloc(&vox, sizeof(uint32_t) * np), "filter: alloc voxels");
-  R2F_HIP(B->bufs.alloc(&B->cand, sizeof(uint32_t) * np), "filter: alloc candidates");
+  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sb, keys, keys2, npi, 0, 32 + vbits, s), "filter: sort size");
+  R2F_HIP(r2host::pool_get(ctx, kSlotTemp, sb, &tmp), "filter: alloc sort temp");
+  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, sb, keys, keys2, npi, 0, 32 + vbits, s), "filter: sort");
+  uint32_t *vox = nullptr, *runs = nullptr;
+  R2F_HIP(pooled(ctx, kSlotVox, np, &vox), "filter: alloc voxels");
+  R2F_HIP(pooled(ctx, kSlotCand, np, &B->cand), "filter: alloc candidates");
   hipLaunchKernelGGL(k_split, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, keys2, np, vox, B->cand);
   R2F_HIP(hipGetLastError(), "filter: k_split");
-  R2F_HIP(B->bufs.alloc(&B->act, sizeof(uint32_t) * np), "filter: alloc active");
-  R2F_HIP(B->bufs.alloc(&B->cnt, sizeof(uint32_t) * np), "filter: alloc counts");
-  R2F_HIP(B->bufs.alloc(&B->off, sizeof(uint32_t) * np), "filter: alloc offsets");
-  R2F_HIP(B->bufs.alloc(&one, sizeof(uint32_t) * 2), "filter: alloc run count");
+  R2F_HIP(pooled(ctx, kSlotAct, np, &B->act), "filter: alloc active");
+  R2F_HIP(pooled(ctx, kSlotCnt, np, &B->cnt), "filter: alloc counts");
+  R2F_HIP(pooled(ctx, kSlotOff, np, &B->off), "filter: alloc offsets");
+  R2F_HIP(pooled(ctx, kSlotRuns, 2, &runs), "filter: alloc run count");
   size_t rb = 0;
-  R2F_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, rb, vox, B->act, B->cnt, one, (int)np, s), "filter: rle size");
-  void* rtmp = nullptr;
-  R2F_HIP(B->bufs.alloc(&rtmp, rb), "filter: alloc rle temp");
-  R2F_HIP(hipcub::DeviceRunLengthEncode::Encode(rtmp, rb, vox, B->act, B->cnt, one, (int)np, s), "filter: rle");
+  R2F_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, rb, vox, B->act, B->cnt, runs, npi, s), "filter: rle size");
+  R2F_HIP(r2host::pool_get(ctx, kSlotTemp, rb, &tmp), "filter: alloc rle temp");
+  R2F_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, rb, vox, B->act, B->cnt, runs, npi, s), "filter: rle");
   uint32_t nruns = 0;
-  R2F_HIP(hipMemcpyAsync(&nruns, one, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipMemcpyAsync(&nruns, runs, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "filter: d2h");
   R2F_HIP(hipStreamSynchronize(s), "filter: sync");
   B->n_act = nruns;
   size_t ob = 0;
   R2F_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, ob, B->cnt, B->off, (int)nruns, s), "filter: scan size");
-  void* otmp = nullptr;
-  R2F_HIP(B->bufs.alloc(&otmp, ob), "filter: alloc scan temp");
-  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(otmp, ob, B->cnt, B->off, (int)nruns, s), "filter: scan counts");
+  R2F_HIP(r2host::pool_get(ctx, kSlotTemp, ob, &tmp), "filter: alloc scan temp");
+  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, ob, B->cnt, B->off, (int)nruns, s), "filter: scan counts");
   return R2_OK;
 }
 
@@ -861,11 +869,10 @@ extern "C" int r2_filter_lod(r2_ctx* ctx, const r2_mesh* m, const r2_filter_para
   if (rc) return rc;
   const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
   uint4* rec = nullptr;
-  DevBufs tmp;
   if (out_on_device) {
     rec = (uint4*)out_records;
   } else {
-    R2F_HIP(tmp.alloc(&rec, 32ull * nvox), "filter: alloc records");
+    R2F_HIP(pooled(ctx, kSlotRecords, 2ull * nvox, &rec), "filter: alloc records");
   }
   hipLaunchKernelGGL(k_fill_empty, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream, rec, nvox);
   R2F_HIP(hipGetLastError(), "filter: k_fill_empty");
@@ -899,11 +906,10 @@ extern "C" int r2_filter_voxels(r2_ctx* ctx, const r2_mesh* m, const r2_filter_p
   const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
   for (int i = 0; i < n; ++i)
     if (voxel_ids[i] >= nvox) return r2host::fail(ctx, R2_ERR_INVALID, "filter: voxel id out of range");
-  DevBufs tmp;
   uint32_t* dids = nullptr;
   uint4* drec = nullptr;
-  R2F_HIP(tmp.alloc(&dids, sizeof(uint32_t) * n), "filter: alloc ids");
-  R2F_HIP(tmp.alloc(&drec, 32ull * n), "filter: alloc records");
+  R2F_HIP(pooled(ctx, kSlotIds, (size_t)n, &dids), "filter: alloc ids");
+  R2F_HIP(pooled(ctx, kSlotRecords, 2ull * n, &drec), "filter: alloc records");
   R2F_HIP(hipMemcpyAsync(dids, voxel_ids, sizeof(uint32_t) * n, hipMemcpyHostToDevice, ctx->stream), "filter: h2d");
   float kms = 0.f;
   rc = launch_filter(ctx, m, c, B, dids, (uint32_t)n, false, drec, &kms);

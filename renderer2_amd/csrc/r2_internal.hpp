@@ -18,6 +18,10 @@ struct r2_ctx {
   r2::DScene* d_scene = nullptr;
   uint32_t* d_counter = nullptr;
   r2_filter_stats fstats{};
+  // grow-only device buffers reused across calls (filter binning scratch)
+  static constexpr int kPoolSlots = 24;
+  void* pool[kPoolSlots] = {};
+  size_t pool_bytes[kPoolSlots] = {};
 };
 
 struct r2_grid {
@@ -49,6 +53,8 @@ int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_p
 // scene assembly (r2_api.cpp)
 int build_scene(r2_ctx* ctx, const r2_render_params* p, r2::DScene* out);
 int ensure_scratch(r2_ctx* ctx, size_t bytes);
+// device buffer of at least `bytes` in pool slot `slot` (contents undefined)
+hipError_t pool_get(r2_ctx* ctx, int slot, size_t bytes, void** out);
 // getRotation (functions.hpp:128-142), host only
 r2::m44 get_rotation(r2::v3 n);
 }  // namespace r2host

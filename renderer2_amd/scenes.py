@@ -58,9 +58,27 @@ MESH_MATERIALS = dict(
 # 512^3, 256^3 and 128^3 (voxel size doubling from the finest), 64 rays per
 # voxel, 8 density iterations, rays clipped to the voxel's bounding sphere
 FILTER_WORKLOADS = {
+    # filter settings of the reference's example config (SigmaT 10, Alpha 0.1,
+    # ClipRays true); one constant material for sphere and quads: the quad
+    # generator's base colour (0.8, 0.443, 0.231), roughness 0.5, ior 1.5
     "c3": dict(res=408, n_quads=200_000, quad_size=0.01, finest=512, lods=3, samples=64, iterations=8,
-               sigma_t=1.0, alpha=1.0, clip_rays=True),
+               sigma_t=10.0, alpha=0.1, clip_rays=True,
+               material=dict(diffuse=(0.800000, 0.442748, 0.231366), specular=(0.04, 0.04, 0.04),
+                             roughness=0.5, ior=1.5)),
 }
+
+
+def make_filter_mesh(name: str = "c3", seed: int = 1, **overrides) -> dict:
+    """The mesh of a filter workload with its constant material."""
+    w = dict(FILTER_WORKLOADS[name])
+    w.update(overrides)
+    m = synth_mesh(w["res"], w["n_quads"], w["quad_size"], seed=seed)
+    mat = w["material"]
+    m["mat_diffuse"] = np.array([mat["diffuse"]] * 2, np.float32)
+    m["mat_specular"] = np.array([mat["specular"]] * 2, np.float32)
+    m["mat_roughness"] = np.array([mat["roughness"]] * 2, np.float32)
+    m["mat_ior"] = np.array([mat["ior"]] * 2, np.float32)
+    return m
 
 
 def synth_mesh(res: int, n_quads: int = 0, quad_size: float = 0.01, seed: int = 1) -> dict:
