@@ -14,7 +14,7 @@ import numpy as np
 from . import _capi as capi
 from ._capi import R2Error
 
-__all__ = ["Context", "Grid", "Env", "Mesh", "R2Error", "capi", "filter_params"]
+__all__ = ["Context", "Grid", "Env", "Mesh", "Server", "R2Error", "capi", "filter_params"]
 
 
 def filter_params(voxel_size: float, samples: int = 64, iterations: int = 8, sigma_t: float = 1.0,
@@ -166,6 +166,13 @@ class Context:
         self._check(capi.lib().r2_filter_get_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    # -------------------------------------------------------------- server
+    def server(self, hbm_budget_bytes: int = 8 << 30, max_inflight: int = 16, slice_paths: int = 1 << 24) -> "Server":
+        cfg = capi.ServerConfig(hbm_budget_bytes, max_inflight, slice_paths)
+        h = C.c_void_p()
+        self._check(capi.lib().r2_server_start(self.h, C.byref(cfg), C.byref(h)))
+        return Server(self, h)
+
     def stats(self) -> dict:
         s = capi.Stats()
         self._check(capi.lib().r2_get_stats(self.h, C.byref(s)))
@@ -255,3 +262,54 @@ class Mesh:
         g = capi.FilterGrid()
         capi.check(capi.lib().r2_filter_setup(self.h, C.c_float(voxel_size), C.byref(g)))
         return g
+
+
+class Server:
+    """Render server on a context: submit() returns a request id, wait() the frame."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+        self._req = {}
+
+    def submit(self, cfg: dict, grids, instances, env) -> int:
+        p, keep = Context.render_params(cfg, grids, instances, env)
+        out = np.zeros((p.height, p.width, 3), np.float32)
+        r = capi.Request(p, out.ctypes.data)
+        rid = capi.lib().r2_server_submit(self.h, C.byref(r))
+        if rid < 0:
+            raise capi.R2Error(int(rid), "r2_server_submit")
+        self._req[rid] = (out, keep)
+        return int(rid)
+
+    def poll(self, rid: int):
+        lat = C.c_double(0.0)
+        rc = capi.lib().r2_server_poll(self.h, rid, C.byref(lat))
+        if rc < 0:
+            raise capi.R2Error(rc, f"request {rid}")
+        return (self._req[rid][0], lat.value) if rc == 1 else None
+
+    def wait(self, rid: int):
+        lat = C.c_double(0.0)
+        rc = capi.lib().r2_server_wait(self.h, rid, C.byref(lat))
+        if rc < 0:
+            raise capi.R2Error(rc, f"request {rid}")
+        return self._req[rid][0], lat.value
+
+    def cancel(self, rid: int):
+        capi.check(capi.lib().r2_server_cancel(self.h, rid))
+
+    def stats(self) -> dict:
+        s = capi.ServerStats()
+        capi.check(capi.lib().r2_server_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def stop(self):
+        if self.h:
+            capi.lib().r2_server_stop(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

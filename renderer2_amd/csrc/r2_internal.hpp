@@ -50,6 +50,20 @@ int grid_lookup(r2_ctx* ctx, const r2_grid* g, int n, const int32_t* coords, uin
 int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum);
 int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
                  uint32_t* d_cnt);
+// one request's share of a batched launch (render server)
+struct BatchSlice {
+  const r2::DScene* d_scene;
+  void* d_partial;
+  uint64_t first, count;
+  int chunk;
+  bool single, fp64;
+};
+constexpr int kMaxBatchEntries = 256;
+void render_chunking(const r2::DScene& scene, int* chunk, int* nchunks);
+int launch_reduce(hipStream_t st, const void* partial, bool fp64, int nchunks, uint32_t n3, float* out);
+// d_ents: kMaxBatchEntries entries of device scratch, d_counters: as many u64
+int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& items, void* d_ents, uint64_t* d_counters);
+size_t batch_entry_bytes();
 // scene assembly (r2_api.cpp)
 int build_scene(r2_ctx* ctx, const r2_render_params* p, r2::DScene* out);
 int ensure_scratch(r2_ctx* ctx, size_t bytes);

@@ -100,6 +100,61 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
   }
 }
 
+// One scheduling round of a wavefront: shading events, batched across the
+// wavefront, then traversal rounds in which every traversing lane advances one
+// DDA / tracking step.
+template <bool kSingle, bool kCount, typename Acc>
+__device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt, int lane, uint32_t npix, int chunk,
+                                          Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
+                                          float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
+  // ---- shading events, batched across the wavefront
+#if R2_EV_MODE == 0
+  for (int k = 0; k < R2_EV_ITERS; ++k) {
+    const bool ev = p.stage >= ST_START;
+    if (!__any(ev)) break;
+    SM_STAT(1, 1);
+#ifdef R2_SM_STATS
+    for (int st = ST_START; st <= ST_FINISH; ++st) {
+      const int c = __popcll(__ballot(p.stage == st));
+      if (c) { SM_STAT(16 + 2 * st, 1); SM_STAT(17 + 2 * st, c); }
+    }
+#endif
+    if (ev) {
+      if (p.stage == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+      else event_step<kSingle, kCount>(s, p, cnt, p.stage);
+    }
+  }
+#else
+  // run the most populated event state each iteration (wave-uniform branch)
+  for (int k = 0; k < R2_EV_ITERS; ++k) {
+    int best = -1, bestc = 0;
+    for (int st = ST_START; st <= ST_FINISH; ++st) {
+      const int c = __popcll(__ballot(p.stage == st));
+      if (c > bestc) {
+        bestc = c;
+        best = st;
+      }
+    }
+    if (best < 0) break;
+    if (p.stage == best) {
+      if (best == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+      else event_step<kSingle, kCount>(s, p, cnt, best);
+    }
+  }
+#endif
+  // ---- traversal rounds: every traversing lane advances one DDA / tracking step
+  for (int k = 0; k < 256; ++k) {
+    const uint64_t tm = __ballot(p.stage == ST_TRAV);
+    if (tm == 0) break;
+    if (k >= R2_TR_MIN_ROUNDS && __popcll(tm) < R2_TR_MIN_LANES) break;
+    SM_STAT(2, 1);
+    SM_STAT(3, __popcll(tm));
+    if (p.stage == ST_TRAV) {
+      if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
+    }
+  }
+}
+
 #ifndef R2_SM_WAVES
 #define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for (measured best)
 #endif
@@ -155,52 +210,78 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
     }
     if (__ballot(p.stage != ST_IDLE) == 0) break;
     SM_STAT(0, 1);
-    // ---- shading events, batched across the wavefront
-#if R2_EV_MODE == 0
-    for (int k = 0; k < R2_EV_ITERS; ++k) {
-      const bool ev = p.stage >= ST_START;
-      if (!__any(ev)) break;
-      SM_STAT(1, 1);
-#ifdef R2_SM_STATS
-      for (int st = ST_START; st <= ST_FINISH; ++st) {
-        const int c = __popcll(__ballot(p.stage == st));
-        if (c) { SM_STAT(16 + 2 * st, 1); SM_STAT(17 + 2 * st, c); }
-      }
-#endif
-      if (ev) {
-        if (p.stage == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-        else event_step<kSingle, kCount>(s, p, cnt, p.stage);
-      }
-    }
-#else
-    // run the most populated event state each iteration (wave-uniform branch)
-    for (int k = 0; k < R2_EV_ITERS; ++k) {
-      int best = -1, bestc = 0;
-      for (int st = ST_START; st <= ST_FINISH; ++st) {
-        const int c = __popcll(__ballot(p.stage == st));
-        if (c > bestc) {
-          bestc = c;
-          best = st;
+    run_round<kSingle, kCount, Acc>(s, p, cnt, lane, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+  }
+}
+
+// Batched form for the render server: one launch serves several requests, each
+// an entry with its own scene, item range and chunk-sum buffer.  A wavefront
+// serves one entry at a time (the scene stays wave-uniform); when the entry is
+// exhausted and all its lanes are idle it moves to the next entry.  Per item the
+// computation is the one of k_paths_sm, so results do not depend on the mix.
+struct BatchEntry {
+  const DScene* scene;  // device copy of the request's scene
+  void* partial;        // chunk sums of the request (Acc[nchunks * npix * 3])
+  uint64_t first;       // first item of this slice in the request's item space
+  uint64_t count;       // items of this slice
+  int chunk;            // samples per item
+  int pad;
+};
+
+template <bool kSingle, typename Acc>
+__global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEntry* __restrict__ ents, int n_ent,
+                                                                  uint64_t* __restrict__ counters) {
+  const int lane = threadIdx.x & 63;
+  int e = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % (unsigned)n_ent));
+  int misses = 0;
+  PathLane p;
+  p.stage = ST_FETCH;
+  p.item = 0;
+  Cnt cnt;
+#pragma unroll
+  for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
+  for (int guard = 0; guard < (1 << 24); ++guard) {
+    const BatchEntry& E = ents[e];
+    const DScene& s = *E.scene;
+    const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
+    const int chunk = E.chunk;
+    const uint64_t need = __ballot(p.stage == ST_FETCH);
+    if (need) {
+      const int n = __popcll(need);
+      const int leader = __ffsll((unsigned long long)need) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd((unsigned long long*)&counters[e], (unsigned long long)n);
+      base = __shfl(base, leader);
+      if (p.stage == ST_FETCH) {
+        const uint64_t local = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (local < E.count) {
+          const uint64_t item = E.first + local;
+          p.item = item;
+          p.ax = p.ay = p.az = 0.f;
+          p.dax = p.day = p.daz = 0.0;
+          const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
+          int x, y;
+          item_pixel(q, s.W, s.H, x, y);
+          p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+          p.s_cur = (int)c * chunk;
+          p.s_end = min(s.spp, p.s_cur + chunk);
+          p.px = (int)(p.pix % (uint32_t)s.W);
+          p.py = (int)(p.pix / (uint32_t)s.W);
+          p.stage = ST_START;
+        } else {
+          p.stage = ST_IDLE;
         }
       }
-      if (best < 0) break;
-      if (p.stage == best) {
-        if (best == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-        else event_step<kSingle, kCount>(s, p, cnt, best);
-      }
     }
-#endif
-    // ---- traversal rounds: every traversing lane advances one DDA / tracking step
-    for (int k = 0; k < 256; ++k) {
-      const uint64_t tm = __ballot(p.stage == ST_TRAV);
-      if (tm == 0) break;
-      if (k >= R2_TR_MIN_ROUNDS && __popcll(tm) < R2_TR_MIN_LANES) break;
-      SM_STAT(2, 1);
-      SM_STAT(3, __popcll(tm));
-      if (p.stage == ST_TRAV) {
-        if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
-      }
+    if (__ballot(p.stage != ST_IDLE) == 0) {
+      // this entry has no work left for the wavefront: try the next one
+      if (++misses >= n_ent) break;
+      e = (e + 1 == n_ent) ? 0 : e + 1;
+      p.stage = ST_FETCH;
+      continue;
     }
+    misses = 0;
+    run_round<kSingle, false, Acc>(s, p, cnt, lane, npix, chunk, (Acc*)E.partial, nullptr, nullptr, nullptr);
   }
 }
 
@@ -248,13 +329,73 @@ static void launch_sm(r2_ctx* ctx, int blocks, uint64_t total, int chunk, Acc* p
                      (uint64_t*)ctx->d_counter, total, chunk, partial, pp, ps, po, pc);
 }
 
-int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64) {
-  using namespace r2;
+size_t batch_entry_bytes() { return sizeof(BatchEntry); }
+
+// samples per work item: about 2^21 items per frame
+void render_chunking(const r2::DScene& scene, int* chunk_out, int* nchunks_out) {
   const uint64_t npix = (uint64_t)scene.W * scene.H;
   const uint64_t target = 1ull << 21;
   const int chunk =
       (int)std::min<uint64_t>((uint64_t)scene.spp, std::max<uint64_t>(1, (npix * scene.spp + target - 1) / target));
-  const int nchunks = (scene.spp + chunk - 1) / chunk;
+  *chunk_out = chunk;
+  *nchunks_out = (scene.spp + chunk - 1) / chunk;
+}
+
+int launch_reduce(hipStream_t st, const void* partial, bool fp64, int nchunks, uint32_t n3, float* out) {
+  if (fp64)
+    hipLaunchKernelGGL(k_reduce<double>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const double*)partial, nchunks, n3,
+                       out);
+  else
+    hipLaunchKernelGGL(k_reduce<float>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const float*)partial, nchunks, n3,
+                       out);
+  return hipGetLastError() == hipSuccess ? R2_OK : R2_ERR_HIP;
+}
+
+int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& items, void* d_ents, uint64_t* d_counters) {
+  using namespace r2;
+  if (items.empty()) return R2_OK;
+  // one launch per (single instance, accumulator type) group
+  std::vector<BatchEntry> host(items.size());
+  int off = 0;
+  for (int g = 0; g < 4; ++g) {
+    const bool single = (g & 1) != 0, fp64 = (g & 2) != 0;
+    const int first = off;
+    uint64_t total = 0;
+    for (const BatchSlice& b : items) {
+      if (b.single != single || b.fp64 != fp64 || b.count == 0) continue;
+      BatchEntry& e = host[off++];
+      e.scene = b.d_scene;
+      e.partial = b.d_partial;
+      e.first = b.first;
+      e.count = b.count;
+      e.chunk = b.chunk;
+      e.pad = 0;
+      total += b.count;
+    }
+    const int n = off - first;
+    if (n == 0) continue;
+    BatchEntry* de = (BatchEntry*)d_ents + first;
+    R2_HIP(ctx, hipMemcpyAsync(de, &host[first], sizeof(BatchEntry) * n, hipMemcpyHostToDevice, st));
+    R2_HIP(ctx, hipMemsetAsync(d_counters + first, 0, sizeof(uint64_t) * n, st));
+    const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total + 255) / 256);
+    if (single && fp64)
+      hipLaunchKernelGGL((k_paths_batch<true, double>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
+    else if (single)
+      hipLaunchKernelGGL((k_paths_batch<true, float>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
+    else if (fp64)
+      hipLaunchKernelGGL((k_paths_batch<false, double>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
+    else
+      hipLaunchKernelGGL((k_paths_batch<false, float>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
+    R2_HIP(ctx, hipGetLastError());
+  }
+  return R2_OK;
+}
+
+int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64) {
+  using namespace r2;
+  const uint64_t npix = (uint64_t)scene.W * scene.H;
+  int chunk = 1, nchunks = 1;
+  render_chunking(scene, &chunk, &nchunks);
   const uint64_t total = (uint64_t)nchunks * npix;
   const size_t acc_bytes = fp64 ? 8 : 4;
   const size_t part_bytes = (size_t)nchunks * npix * 3 * acc_bytes;

@@ -1,0 +1,352 @@
+// Device-resident render server with continuous batching.
+//
+// The reference has no server; its nearest relative is the interactive preview
+// loop (raytracer/pipeline.cu:41-78) that renders one frame at a time.  Here a
+// host scheduler thread owns a HIP stream and
+//   * admits queued requests in arrival order while the sum of their device
+//     memory (scene copy, chunk sums, frame) fits the HBM budget and fewer than
+//     max_inflight are resident;
+//   * runs slices: one batched persistent launch (k_paths_batch) whose work is
+//     an equal share of paths from every resident request, so requests that
+//     arrive while others run join at the next slice and their work overlaps
+//     on the device;
+//   * reduces a request's chunk sums when all its items are done, copies the
+//     frame to the caller's buffer and frees its memory, which admits the next.
+// A request is split into the same work items (sample chunk x pixel) and the
+// same chunk-ordered reduction as a standalone r2_render, so its result is the
+// same alone and in any mix.
+#include "r2_internal.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum ReqState { RQ_QUEUED, RQ_ACTIVE, RQ_DONE, RQ_CANCELLED, RQ_FAILED };
+
+struct SrvReq {
+  int64_t id = 0;
+  r2_render_params params{};
+  std::vector<r2_instance> inst;
+  float* out = nullptr;
+  r2::DScene scene{};
+  int chunk = 1, nchunks = 1;
+  bool fp64 = false, single = true;
+  uint64_t total_items = 0, next_item = 0;
+  size_t bytes = 0;
+  void* d_mem = nullptr;
+  r2::DScene* d_scene = nullptr;
+  void* d_partial = nullptr;
+  float* d_frame = nullptr;
+  ReqState state = RQ_QUEUED;
+  bool cancel = false;
+  int rc = R2_OK;
+  Clock::time_point t_submit, t_done;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct r2_server {
+  r2_ctx* ctx = nullptr;
+  r2_ctx shadow;  // error sink and stream of the scheduler thread
+  r2_server_config cfg{};
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<int64_t> queue;
+  std::map<int64_t, std::unique_ptr<SrvReq>> reqs;
+  std::vector<int64_t> active;
+  bool stopping = false;
+  int64_t next_id = 0;
+  int64_t inflight_bytes = 0, peak_bytes = 0;
+  int64_t n_submitted = 0, n_completed = 0, n_rejected = 0, n_cancelled = 0;
+  std::vector<double> latencies;
+  int64_t paths_done = 0;
+  bool have_first = false;
+  Clock::time_point t_first, t_last;
+  void* d_ents = nullptr;
+  uint64_t* d_counters = nullptr;
+};
+
+namespace {
+
+void finish_locked(r2_server* s, SrvReq& r, ReqState st, int rc) {
+  if (r.d_mem) {
+    (void)hipFree(r.d_mem);
+    r.d_mem = nullptr;
+    s->inflight_bytes -= (int64_t)r.bytes;
+  }
+  r.state = st;
+  r.rc = rc;
+  r.t_done = Clock::now();
+  s->active.erase(std::remove(s->active.begin(), s->active.end(), r.id), s->active.end());
+  if (st == RQ_DONE) {
+    ++s->n_completed;
+    s->latencies.push_back(std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count());
+    s->paths_done += (int64_t)r.scene.W * r.scene.H * r.scene.spp;
+    s->t_last = r.t_done;
+  } else if (st == RQ_CANCELLED) {
+    ++s->n_cancelled;
+  }
+  s->cv_done.notify_all();
+}
+
+// admit queued requests in order while memory and slots allow (lock held)
+void admit_locked(r2_server* s) {
+  while (!s->queue.empty() && (int)s->active.size() < s->cfg.max_inflight) {
+    SrvReq& r = *s->reqs[s->queue.front()];
+    if (r.cancel) {
+      s->queue.pop_front();
+      finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
+      continue;
+    }
+    if (s->inflight_bytes + (int64_t)r.bytes > s->cfg.hbm_budget_bytes) break;
+    void* mem = nullptr;
+    if (hipMalloc(&mem, r.bytes) != hipSuccess) {
+      if (s->active.empty()) {  // nothing to wait for: fail the request
+        s->queue.pop_front();
+        finish_locked(s, r, RQ_FAILED, R2_ERR_OOM);
+        continue;
+      }
+      break;
+    }
+    s->queue.pop_front();
+    r.d_mem = mem;
+    r.d_scene = (r2::DScene*)mem;
+    r.d_partial = (char*)mem + align256(sizeof(r2::DScene));
+    const size_t part = (size_t)r.nchunks * r.scene.W * r.scene.H * 3 * (r.fp64 ? 8 : 4);
+    r.d_frame = (float*)((char*)r.d_partial + align256(part));
+    if (hipMemcpyAsync(r.d_scene, &r.scene, sizeof(r2::DScene), hipMemcpyHostToDevice, s->shadow.stream) != hipSuccess) {
+      finish_locked(s, r, RQ_FAILED, R2_ERR_HIP);
+      continue;
+    }
+    s->inflight_bytes += (int64_t)r.bytes;
+    s->peak_bytes = std::max(s->peak_bytes, s->inflight_bytes);
+    r.state = RQ_ACTIVE;
+    s->active.push_back(r.id);
+    if (!s->have_first) {
+      s->have_first = true;
+      s->t_first = Clock::now();
+    }
+  }
+}
+
+void scheduler(r2_server* s) {
+  (void)hipSetDevice(s->ctx->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    admit_locked(s);
+    // drop cancelled active requests
+    for (int64_t id : std::vector<int64_t>(s->active)) {
+      SrvReq& r = *s->reqs[id];
+      if (r.cancel) finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
+    }
+    if (s->active.empty()) {
+      if (s->stopping) break;
+      s->cv_work.wait(lk);
+      continue;
+    }
+    // plan a slice: an equal share of paths per resident request, in whole
+    // 64-item tiles, at most kMaxBatchEntries requests
+    std::vector<r2host::BatchSlice> slice;
+    std::vector<std::pair<int64_t, uint64_t>> took;
+    const int n_act = std::min<int>((int)s->active.size(), r2host::kMaxBatchEntries);
+    const uint64_t share = std::max<uint64_t>(1, (uint64_t)s->cfg.slice_paths / (uint64_t)n_act);
+    for (int i = 0; i < n_act; ++i) {
+      SrvReq& r = *s->reqs[s->active[i]];
+      uint64_t items = std::max<uint64_t>(64 * 64, share / (uint64_t)r.chunk);
+      items = (items + 63) & ~(uint64_t)63;
+      items = std::min<uint64_t>(items, r.total_items - r.next_item);
+      r2host::BatchSlice b{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.fp64};
+      slice.push_back(b);
+      took.push_back({r.id, items});
+    }
+    lk.unlock();
+    int rc = r2host::launch_batch(&s->shadow, s->shadow.stream, slice, s->d_ents, s->d_counters);
+    if (rc == R2_OK && hipStreamSynchronize(s->shadow.stream) != hipSuccess) rc = R2_ERR_HIP;
+    lk.lock();
+    for (auto& t : took) {
+      SrvReq& r = *s->reqs[t.first];
+      if (rc != R2_OK) {
+        finish_locked(s, r, RQ_FAILED, rc);
+        continue;
+      }
+      r.next_item += t.second;
+      if (r.next_item < r.total_items || r.cancel) continue;
+      // all items done: reduce in chunk order and copy the frame out
+      const uint32_t n3 = (uint32_t)((uint64_t)r.scene.W * r.scene.H * 3);
+      int rr = r2host::launch_reduce(s->shadow.stream, r.d_partial, r.fp64, r.nchunks, n3, r.d_frame);
+      if (rr == R2_OK && hipMemcpyAsync(r.out, r.d_frame, (size_t)n3 * 4, hipMemcpyDeviceToHost, s->shadow.stream) !=
+                             hipSuccess)
+        rr = R2_ERR_HIP;
+      if (rr == R2_OK && hipStreamSynchronize(s->shadow.stream) != hipSuccess) rr = R2_ERR_HIP;
+      finish_locked(s, r, rr == R2_OK ? RQ_DONE : RQ_FAILED, rr);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int r2_server_start(r2_ctx* ctx, const r2_server_config* cfg, r2_server** out) {
+  if (!ctx || !out) return R2_ERR_INVALID;
+  *out = nullptr;
+  auto s = std::make_unique<r2_server>();
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg;
+  if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = 8ll << 30;
+  if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 16;
+  s->cfg.max_inflight = std::min(s->cfg.max_inflight, r2host::kMaxBatchEntries);
+  if (s->cfg.slice_paths <= 0) s->cfg.slice_paths = 1 << 24;
+  s->shadow.device = ctx->device;
+  if (hipSetDevice(ctx->device) != hipSuccess) return r2host::fail(ctx, R2_ERR_HIP, "server: hipSetDevice");
+  if (hipStreamCreate(&s->shadow.stream) != hipSuccess) return r2host::fail(ctx, R2_ERR_HIP, "server: stream");
+  if (hipMalloc(&s->d_ents, r2host::batch_entry_bytes() * r2host::kMaxBatchEntries) != hipSuccess ||
+      hipMalloc((void**)&s->d_counters, sizeof(uint64_t) * r2host::kMaxBatchEntries) != hipSuccess) {
+    if (s->d_ents) (void)hipFree(s->d_ents);
+    (void)hipStreamDestroy(s->shadow.stream);
+    return r2host::fail(ctx, R2_ERR_OOM, "server: device tables");
+  }
+  r2_server* raw = s.release();
+  raw->th = std::thread(scheduler, raw);
+  *out = raw;
+  return R2_OK;
+}
+
+extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
+  if (!s || !req || !req->out_rgb) return R2_ERR_INVALID;
+  auto r = std::make_unique<SrvReq>();
+  r->params = req->params;
+  if (r->params.n_instances < 0 || r->params.n_instances > r2::kMaxInstances ||
+      (r->params.n_instances > 0 && !r->params.instances))
+    return R2_ERR_INVALID;
+  r->inst.assign(r->params.instances, r->params.instances + r->params.n_instances);
+  r->params.instances = r->inst.data();
+  r->out = req->out_rgb;
+  {
+    // scene assembly on the caller's thread (validates the request)
+    r2_ctx probe;
+    probe.device = s->ctx->device;
+    const int rc = r2host::build_scene(&probe, &r->params, &r->scene);
+    if (rc != R2_OK) return rc;
+  }
+  r2host::render_chunking(r->scene, &r->chunk, &r->nchunks);
+  r->fp64 = (r->params.flags & R2_FLAG_FP64_ACCUM) != 0;
+  r->single = r->scene.n_inst == 1;
+  const uint64_t npix = (uint64_t)r->scene.W * r->scene.H;
+  r->total_items = (uint64_t)r->nchunks * npix;
+  r->bytes = align256(sizeof(r2::DScene)) + align256((size_t)r->nchunks * npix * 3 * (r->fp64 ? 8 : 4)) +
+             align256((size_t)npix * 3 * 4);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) return R2_ERR_CANCELLED;
+  ++s->n_submitted;
+  if ((int64_t)r->bytes > s->cfg.hbm_budget_bytes) {
+    ++s->n_rejected;
+    return R2_ERR_OOM;
+  }
+  const int64_t id = s->next_id++;
+  r->id = id;
+  r->t_submit = Clock::now();
+  s->queue.push_back(id);
+  s->reqs[id] = std::move(r);
+  s->cv_work.notify_one();
+  return id;
+}
+
+static int poll_locked(r2_server* s, int64_t id, double* lat) {
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return R2_ERR_INVALID;
+  SrvReq& r = *it->second;
+  if (r.state == RQ_QUEUED || r.state == RQ_ACTIVE) return 0;
+  if (lat) *lat = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+  return r.state == RQ_DONE ? 1 : (r.rc < 0 ? r.rc : R2_ERR_HIP);
+}
+
+extern "C" int r2_server_poll(r2_server* s, int64_t id, double* latency_ms) {
+  if (!s) return R2_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  return poll_locked(s, id, latency_ms);
+}
+
+extern "C" int r2_server_wait(r2_server* s, int64_t id, double* latency_ms) {
+  if (!s) return R2_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    const int rc = poll_locked(s, id, latency_ms);
+    if (rc != 0) return rc;
+    s->cv_done.wait(lk);
+  }
+}
+
+extern "C" int r2_server_cancel(r2_server* s, int64_t id) {
+  if (!s) return R2_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return R2_ERR_INVALID;
+  it->second->cancel = true;
+  s->cv_work.notify_one();
+  return R2_OK;
+}
+
+extern "C" int r2_server_get_stats(r2_server* s, r2_server_stats* out) {
+  if (!s || !out) return R2_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
+  memset(out, 0, sizeof(*out));
+  out->submitted = s->n_submitted;
+  out->completed = s->n_completed;
+  out->rejected = s->n_rejected;
+  out->cancelled = s->n_cancelled;
+  out->peak_bytes = s->peak_bytes;
+  if (s->have_first && s->n_completed > 0) {
+    const double span = std::chrono::duration<double>(s->t_last - s->t_first).count();
+    out->paths_per_sec = span > 0 ? (double)s->paths_done / span : 0.0;
+  }
+  if (!s->latencies.empty()) {
+    std::vector<double> l = s->latencies;
+    std::sort(l.begin(), l.end());
+    // nearest-rank percentiles
+    auto pct = [&](double q) {
+      size_t k = (size_t)std::ceil(q * (double)l.size());
+      k = std::min(std::max<size_t>(k, 1), l.size());
+      return l[k - 1];
+    };
+    out->p50_ms = pct(0.50);
+    out->p99_ms = pct(0.99);
+  }
+  return R2_OK;
+}
+
+extern "C" int r2_server_stop(r2_server* s) {
+  if (!s) return R2_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    for (auto& kv : s->reqs)
+      if (kv.second->state == RQ_QUEUED || kv.second->state == RQ_ACTIVE) kv.second->cancel = true;
+    s->cv_work.notify_all();
+  }
+  if (s->th.joinable()) s->th.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& kv : s->reqs) {
+      SrvReq& r = *kv.second;
+      if (r.state == RQ_QUEUED) finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
+    }
+  }
+  (void)hipFree(s->d_ents);
+  (void)hipFree(s->d_counters);
+  (void)hipStreamDestroy(s->shadow.stream);
+  delete s;
+  return R2_OK;
+}

@@ -1,0 +1,127 @@
+"""Render server: continuous batching must not change results.
+
+Bar: every request's frame from the server equals the frame of a standalone
+r2_render with the same parameters bit for bit (same work items, same
+chunk-ordered reduction), whatever else is resident; admission respects the HBM
+budget; cancellation and oversize rejection behave as documented.
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import renderer2_amd as r2
+from renderer2_amd import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def world(ctx):
+    w1 = scenes.make_workload("c1", n=48)
+    w2 = scenes.make_workload("c2", n=64)
+    grids = [ctx.upload_grid(w1["grids"][0]), ctx.upload_grid(w2["grids"][0])]
+    env = ctx.upload_env(w2["env"])
+    return dict(grids=grids, env=env, base1=w1["render"], base2=w2["render"])
+
+
+def _requests(world):
+    b1, b2 = world["base1"], world["base2"]
+    reqs = []
+    shapes = [(64, 64, 8), (96, 64, 4), (128, 128, 16), (32, 32, 64), (80, 48, 12), (128, 96, 6), (64, 128, 3),
+              (48, 48, 33)]
+    for k, (W, H, spp) in enumerate(shapes):
+        base = b1 if k % 2 == 0 else b2
+        ang = 0.4 * k
+        cam = (2.3 * np.sin(ang), 0.3 * (k % 3 - 1), 2.3 * np.cos(ang))
+        cfg = dict(base, width=W, height=H, spp=spp, seed=100 + k, cam_pos=cam, max_depth=4 + k % 5,
+                   fp64_accum=(k == 5))
+        inst = [dict(grid=k % 2)]
+        if k == 7:  # two instances of different grids
+            inst = [dict(grid=0, pos=(-0.3, 0, 0), scaling=(0.5, 0.5, 0.5)),
+                    dict(grid=1, pos=(0.3, 0, 0), scaling=(0.5, 0.5, 0.5))]
+        reqs.append((cfg, inst))
+    return reqs
+
+
+def test_server_matches_standalone_renders(ctx, world):
+    reqs = _requests(world)
+    alone = [ctx.render(cfg, world["grids"], inst, world["env"]) for cfg, inst in reqs]
+    srv = ctx.server(hbm_budget_bytes=1 << 30, max_inflight=8, slice_paths=1 << 16)
+    try:
+        ids = [srv.submit(cfg, world["grids"], inst, world["env"]) for cfg, inst in reqs[:5]]
+        time.sleep(0.01)  # the rest arrives while the first ones are on the GPU
+        ids += [srv.submit(cfg, world["grids"], inst, world["env"]) for cfg, inst in reqs[5:]]
+        for k, rid in enumerate(ids):
+            img, lat = srv.wait(rid)
+            assert lat > 0
+            assert np.array_equal(img, alone[k]), f"request {k}: max diff {np.abs(img - alone[k]).max()}"
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["rejected"] == 0
+        assert 0 < st["p50_ms"] <= st["p99_ms"]
+        assert st["paths_per_sec"] > 0
+    finally:
+        srv.stop()
+
+
+def test_server_budget_admission_and_rejection(ctx, world):
+    reqs = _requests(world)
+    cfg, inst = reqs[2]  # 128x128 x 16 spp
+    one = 256 * 3 + 128 * 128 * 3 * 4 * 2 + 128 * 128 * 3 * 4  # upper estimate of one request's bytes
+    srv = ctx.server(hbm_budget_bytes=one + 4096, max_inflight=4, slice_paths=1 << 15)
+    try:
+        ref = ctx.render(cfg, world["grids"], inst, world["env"])
+        ids = [srv.submit(dict(cfg), world["grids"], inst, world["env"]) for _ in range(4)]
+        for rid in ids:
+            img, _ = srv.wait(rid)
+            assert np.array_equal(img, ref)
+        st = srv.stats()
+        assert st["peak_bytes"] <= one + 4096
+        big = dict(cfg, width=1024, height=1024, spp=64)
+        with pytest.raises(r2.R2Error) as e:
+            srv.submit(big, world["grids"], inst, world["env"])
+        assert e.value.code == -3  # R2_ERR_OOM: larger than the whole budget
+        assert srv.stats()["rejected"] == 1
+    finally:
+        srv.stop()
+
+
+def test_server_cancel(ctx, world):
+    cfg, inst = _requests(world)[2]
+    srv = ctx.server(max_inflight=1, slice_paths=1 << 14)
+    try:
+        a = srv.submit(dict(cfg, spp=256), world["grids"], inst, world["env"])
+        b = srv.submit(dict(cfg, spp=8), world["grids"], inst, world["env"])  # queued behind a
+        srv.cancel(b)
+        with pytest.raises(r2.R2Error) as e:
+            srv.wait(b)
+        assert e.value.code == -6  # R2_ERR_CANCELLED
+        img, _ = srv.wait(a)
+        assert np.isfinite(img).all()
+        assert srv.stats()["cancelled"] == 1
+    finally:
+        srv.stop()
+
+
+def test_server_concurrent_submitters(ctx, world):
+    """Submissions from several host threads (the server API is internally synchronised)."""
+    reqs = _requests(world)[:6]
+    alone = [ctx.render(cfg, world["grids"], inst, world["env"]) for cfg, inst in reqs]
+    srv = ctx.server(max_inflight=3, slice_paths=1 << 16)
+    out = [None] * len(reqs)
+
+    def worker(k):
+        rid = srv.submit(reqs[k][0], world["grids"], reqs[k][1], world["env"])
+        out[k] = srv.wait(rid)[0]
+
+    try:
+        th = [threading.Thread(target=worker, args=(k,)) for k in range(len(reqs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join(timeout=120)
+        for k in range(len(reqs)):
+            assert out[k] is not None and np.array_equal(out[k], alone[k])
+    finally:
+        srv.stop()
