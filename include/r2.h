@@ -236,6 +236,53 @@ int r2_server_cancel(r2_server* s, int64_t id);
 int r2_server_get_stats(r2_server* s, r2_server_stats* out);
 int r2_server_stop(r2_server* s);
 
+/* --------------------------------------------------------- post-process -- */
+/* Tonemap, gamma and bytes of a rendered frame (SFrame), as rt::Raytracer::renderFrame
+ * does after its spp loop (raytracer/raytracer.cu:53-127,192-206): k = exp(sum over
+ * pixels of log(r+g+b+FLT_MIN) / (3 W H)) - W H FLT_MIN, c -> c / (c + k), then
+ * c^(1/gamma) and round(255 c).  hdr / ldr / bytes hold W*H*3 values; ldr or bytes may
+ * be NULL.  on_device: 1 if the three pointers are device pointers. */
+int r2_postprocess(r2_ctx* ctx, const float* hdr, int32_t w, int32_t h, float gamma, float* ldr, uint8_t* bytes,
+                   int32_t on_device);
+
+/* ------------------------------------------------------------ front end -- */
+/* The reference's program: CConfigLoader::loadConfig + CPipeline::run
+ * (raytracer/src/backend/config_loader.cu:345-368, raytracer/pipeline.cu:19-93,
+ * raytracer/main.cu:4-17).  Reads config.json and its scenedescription.json (same keys,
+ * comments allowed), filters the FILTER-masked mesh to its LODs when
+ * Filtering.Active (grid files <out_dir>/filtering/<mesh>/filtered_mesh_<vs>.r2grid and
+ * filtering/volume_description.json), renders the RENDER-masked media under the env map
+ * and writes <out_dir>/output.png (tonemapped, flipped like the reference) and
+ * output.pfm (linear).  err receives a message on failure. */
+typedef struct {
+  int32_t lods_filtered;
+  double filter_ms;
+  int32_t width, height, spp;
+  double render_ms, mpaths_per_s;
+  int32_t media_rendered, objects_skipped;
+} r2_pipeline_report;
+int r2_pipeline_run(const char* config_path, const char* out_dir, int32_t device, r2_pipeline_report* report,
+                    char* err, int32_t err_len);
+
+/* Parse a config (and its scene description) without a device: the fields of SConfig
+ * (raytracer/include/backend/config_loader.hpp:9-33) that drive this library. */
+typedef struct {
+  int32_t frame_width, frame_height, samples;
+  float gamma;
+  int32_t use_brick_grid;
+  int32_t filter_active;
+  float min_voxel_size;
+  int32_t lods, samples_per_voxel;
+  float sigma_t;
+  int32_t estimation_iterations;
+  float alpha;
+  int32_t clip_rays;
+  float fov, cam_pos[3], cam_look_at[3], cam_up[3];
+  int32_t n_media, n_meshes, n_surfaces, n_filter_objects;
+  int32_t max_depth;
+} r2_config_summary_t;
+int r2_config_summary(const char* config_path, r2_config_summary_t* out, char* err, int32_t err_len);
+
 /* ----------------------------------------------------- host utilities -- */
 /* Synthetic inputs of the benchmark configurations (seeded, deterministic):
  * kind 1 = isotropic value-noise volume (S = I, normal +y), kind 2 = anisotropic

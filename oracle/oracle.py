@@ -156,6 +156,9 @@ def lib() -> C.CDLL:
         L.r2o_det_logf.argtypes = [C.c_float]
         L.r2o_det_logf.restype = C.c_float
         L.r2o_det_sincosf.argtypes = [C.c_float, vp, vp]
+        L.r2o_det_powf.argtypes = [C.c_float, C.c_float]
+        L.r2o_det_powf.restype = C.c_float
+        L.r2o_postprocess.argtypes = [vp, C.c_int, C.c_int, C.c_float, vp, vp]
         _lib = L
     return _lib
 
@@ -472,3 +475,13 @@ def decode_records(rec: np.ndarray) -> dict:
         "normal": u16[:, 11:14],
         "ior": u16[:, 14].view(np.float16).astype(np.float32),
     }
+
+
+def postprocess(hdr: np.ndarray, gamma: float = 2.0):
+    """Tonemapped + gamma-corrected frame (float) and its bytes, like SFrame."""
+    hdr = np.ascontiguousarray(hdr, np.float32)
+    h, w = hdr.shape[:2]
+    ldr = np.zeros_like(hdr)
+    by = np.zeros(hdr.shape, np.uint8)
+    lib().r2o_postprocess(_p(hdr), w, h, C.c_float(gamma), _p(ldr), _p(by))
+    return ldr, by

@@ -133,7 +133,13 @@ int r2o_filter_probe(const r2o_filter_ctx* c, const r2o_filter_grid* g, const r2
 int r2o_filter_closest_hit(const r2o_filter_ctx* c, const float o[3], const float d[3], float tmax, float out[3]);
 float r2o_det_expf(float x);
 float r2o_det_logf(float x);
+float r2o_det_powf(float x, float y);
+double r2o_det_exp_d(double x);
+double r2o_det_log_d(double x);
 void r2o_det_sincosf(float x, float* s, float* c);
+
+/* ---- post-process (r2o_post.c) ---- */
+void r2o_postprocess(const float* hdr, int w, int h, float gamma, float* ldr, uint8_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -286,3 +286,105 @@ float r2o_brick_density(const r2o_bricks* b, int ix, int iy, int iz) {
   }
   return r2o_decode_voxel(v, lo, hi);
 }
+
+/* ------------------------------------------------ deterministic math ---- */
+/* sin/cos: quadrant reduction by pi/2 (two-part constant) and Taylor series on
+ * |r| <= pi/4 to degree 17/16, all in double, rounded once to float. */
+void r2o_det_sincosf(float x, float* s, float* c) {
+  const double xd = (double)x;
+  const double q = rint(xd * 0.63661977236758134308);
+  const double r = (xd - q * 1.5707963267948966) - q * 6.123233995736766e-17;
+  const double r2 = r * r;
+  double sp = 1.0 / 355687428096000.0;
+  sp = sp * r2 - 1.0 / 1307674368000.0;
+  sp = sp * r2 + 1.0 / 6227020800.0;
+  sp = sp * r2 - 1.0 / 39916800.0;
+  sp = sp * r2 + 1.0 / 362880.0;
+  sp = sp * r2 - 1.0 / 5040.0;
+  sp = sp * r2 + 1.0 / 120.0;
+  sp = sp * r2 - 1.0 / 6.0;
+  sp = sp * r2 + 1.0;
+  const double sr = sp * r;
+  double cp = 1.0 / 20922789888000.0;
+  cp = cp * r2 - 1.0 / 87178291200.0;
+  cp = cp * r2 + 1.0 / 479001600.0;
+  cp = cp * r2 - 1.0 / 3628800.0;
+  cp = cp * r2 + 1.0 / 40320.0;
+  cp = cp * r2 - 1.0 / 720.0;
+  cp = cp * r2 + 1.0 / 24.0;
+  cp = cp * r2 - 1.0 / 2.0;
+  const double cr = cp * r2 + 1.0;
+  const int iq = ((int)q) & 3;
+  const double so = iq == 0 ? sr : (iq == 1 ? cr : (iq == 2 ? -sr : -cr));
+  const double co = iq == 0 ? cr : (iq == 1 ? -sr : (iq == 2 ? -cr : sr));
+  *s = (float)so;
+  *c = (float)co;
+}
+
+/* exp: n = rint(x / ln2), r = x - n ln2, Taylor series to degree 13, scaled by 2^n */
+double r2o_det_exp_d(double xd) {
+  const double n = rint(xd * 1.4426950408889634);
+  const double r = xd - n * 0.6931471805599453;
+  double p = 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)n);
+}
+
+float r2o_det_expf(float x) {
+  if (x != x) return x;
+  if (x < -104.f) return 0.f;
+  if (x > 89.f) return INFINITY;
+  return (float)r2o_det_exp_d((double)x);
+}
+
+/* log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m-1)/(m+1)) by its series */
+double r2o_det_log_d(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752) { m = m * 2.0; e = e - 1; }
+  const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+  double p = 1.0 / 23.0;
+  p = p * s2 + 1.0 / 21.0;
+  p = p * s2 + 1.0 / 19.0;
+  p = p * s2 + 1.0 / 17.0;
+  p = p * s2 + 1.0 / 15.0;
+  p = p * s2 + 1.0 / 13.0;
+  p = p * s2 + 1.0 / 11.0;
+  p = p * s2 + 1.0 / 9.0;
+  p = p * s2 + 1.0 / 7.0;
+  p = p * s2 + 1.0 / 5.0;
+  p = p * s2 + 1.0 / 3.0;
+  p = p * s2 + 1.0;
+  return 2.0 * s * p + (double)e * 0.6931471805599453;
+}
+
+float r2o_det_logf(float x) {
+  if (x != x || x < 0.f) return NAN;
+  if (x == 0.f) return -INFINITY;
+  if (x == INFINITY) return x;
+  return (float)r2o_det_log_d((double)x);
+}
+
+/* x^y = exp(y log x) in double, rounded once */
+float r2o_det_powf(float x, float y) {
+  if (x != x || y != y || x < 0.f) return NAN;
+  if (x == 0.f) return y > 0.f ? 0.f : (y == 0.f ? 1.f : INFINITY);
+  if (x == INFINITY) return y > 0.f ? INFINITY : 0.f;
+  const double t = (double)y * r2o_det_log_d((double)x);
+  if (t < -745.0) return 0.f;
+  if (t > 709.0) return INFINITY;
+  return (float)r2o_det_exp_d(t);
+}
+

@@ -55,92 +55,10 @@ static inline float gmin(float x, float y) { return (y < x) ? y : x; }
 static inline float gmax(float x, float y) { return (x < y) ? y : x; }
 static inline float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
-/* ------------------------------------------------ deterministic math ---- */
-/* sin/cos: quadrant reduction by pi/2 (two-part constant) and Taylor series on
- * |r| <= pi/4 to degree 17/16, all in double, rounded once to float. */
-static void det_sincosf(float x, float* s, float* c) {
-  const double xd = (double)x;
-  const double q = rint(xd * 0.63661977236758134308);
-  const double r = (xd - q * 1.5707963267948966) - q * 6.123233995736766e-17;
-  const double r2 = r * r;
-  double sp = 1.0 / 355687428096000.0;
-  sp = sp * r2 - 1.0 / 1307674368000.0;
-  sp = sp * r2 + 1.0 / 6227020800.0;
-  sp = sp * r2 - 1.0 / 39916800.0;
-  sp = sp * r2 + 1.0 / 362880.0;
-  sp = sp * r2 - 1.0 / 5040.0;
-  sp = sp * r2 + 1.0 / 120.0;
-  sp = sp * r2 - 1.0 / 6.0;
-  sp = sp * r2 + 1.0;
-  const double sr = sp * r;
-  double cp = 1.0 / 20922789888000.0;
-  cp = cp * r2 - 1.0 / 87178291200.0;
-  cp = cp * r2 + 1.0 / 479001600.0;
-  cp = cp * r2 - 1.0 / 3628800.0;
-  cp = cp * r2 + 1.0 / 40320.0;
-  cp = cp * r2 - 1.0 / 720.0;
-  cp = cp * r2 + 1.0 / 24.0;
-  cp = cp * r2 - 1.0 / 2.0;
-  const double cr = cp * r2 + 1.0;
-  const int iq = ((int)q) & 3;
-  const double so = iq == 0 ? sr : (iq == 1 ? cr : (iq == 2 ? -sr : -cr));
-  const double co = iq == 0 ? cr : (iq == 1 ? -sr : (iq == 2 ? -cr : sr));
-  *s = (float)so;
-  *c = (float)co;
-}
-
-/* exp: n = rint(x / ln2), r = x - n ln2, Taylor series to degree 13, scaled by 2^n */
-static float det_expf(float x) {
-  if (x != x) return x;
-  if (x < -104.f) return 0.f;
-  if (x > 89.f) return INFINITY;
-  const double xd = (double)x;
-  const double n = rint(xd * 1.4426950408889634);
-  const double r = xd - n * 0.6931471805599453;
-  double p = 1.0 / 6227020800.0;
-  p = p * r + 1.0 / 479001600.0;
-  p = p * r + 1.0 / 39916800.0;
-  p = p * r + 1.0 / 3628800.0;
-  p = p * r + 1.0 / 362880.0;
-  p = p * r + 1.0 / 40320.0;
-  p = p * r + 1.0 / 5040.0;
-  p = p * r + 1.0 / 720.0;
-  p = p * r + 1.0 / 120.0;
-  p = p * r + 1.0 / 24.0;
-  p = p * r + 1.0 / 6.0;
-  p = p * r + 0.5;
-  p = p * r + 1.0;
-  p = p * r + 1.0;
-  return (float)ldexp(p, (int)n);
-}
-
-/* log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m-1)/(m+1)) by its series */
-static float det_logf(float x) {
-  if (x != x || x < 0.f) return NAN;
-  if (x == 0.f) return -INFINITY;
-  if (x == INFINITY) return x;
-  int e;
-  double m = frexp((double)x, &e);
-  if (m < 0.70710678118654752) { m = m * 2.0; e = e - 1; }
-  const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
-  double p = 1.0 / 23.0;
-  p = p * s2 + 1.0 / 21.0;
-  p = p * s2 + 1.0 / 19.0;
-  p = p * s2 + 1.0 / 17.0;
-  p = p * s2 + 1.0 / 15.0;
-  p = p * s2 + 1.0 / 13.0;
-  p = p * s2 + 1.0 / 11.0;
-  p = p * s2 + 1.0 / 9.0;
-  p = p * s2 + 1.0 / 7.0;
-  p = p * s2 + 1.0 / 5.0;
-  p = p * s2 + 1.0 / 3.0;
-  p = p * s2 + 1.0;
-  return (float)(2.0 * s * p + (double)e * 0.6931471805599453);
-}
-
-float r2o_det_expf(float x) { return det_expf(x); }
-float r2o_det_logf(float x) { return det_logf(x); }
-void r2o_det_sincosf(float x, float* s, float* c) { det_sincosf(x, s, c); }
+/* deterministic sin/cos/exp/log: r2o_common.c */
+#define det_sincosf r2o_det_sincosf
+#define det_expf r2o_det_expf
+#define det_logf r2o_det_logf
 
 /* ------------------------------------------------------------ matrices -- */
 typedef struct { float c[4][4]; } m44;

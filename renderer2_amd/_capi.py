@@ -116,6 +116,27 @@ class FilterStats(C.Structure):
     ]
 
 
+class PipelineReport(C.Structure):
+    _fields_ = [
+        ("lods_filtered", C.c_int32), ("filter_ms", C.c_double),
+        ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+        ("render_ms", C.c_double), ("mpaths_per_s", C.c_double),
+        ("media_rendered", C.c_int32), ("objects_skipped", C.c_int32),
+    ]
+
+
+class ConfigSummary(C.Structure):
+    _fields_ = [
+        ("frame_width", C.c_int32), ("frame_height", C.c_int32), ("samples", C.c_int32),
+        ("gamma", C.c_float), ("use_brick_grid", C.c_int32), ("filter_active", C.c_int32),
+        ("min_voxel_size", C.c_float), ("lods", C.c_int32), ("samples_per_voxel", C.c_int32),
+        ("sigma_t", C.c_float), ("estimation_iterations", C.c_int32), ("alpha", C.c_float),
+        ("clip_rays", C.c_int32), ("fov", C.c_float), ("cam_pos", f3), ("cam_look_at", f3), ("cam_up", f3),
+        ("n_media", C.c_int32), ("n_meshes", C.c_int32), ("n_surfaces", C.c_int32), ("n_filter_objects", C.c_int32),
+        ("max_depth", C.c_int32),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_inflight", C.c_int32), ("slice_paths", C.c_int32)]
 
@@ -165,6 +186,11 @@ SIGNATURES = [
     ("r2_server_cancel", C.c_int, [C.c_void_p, C.c_int64]),
     ("r2_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     ("r2_server_stop", C.c_int, [C.c_void_p]),
+    ("r2_postprocess", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_void_p, C.c_void_p,
+                                 C.c_int32]),
+    ("r2_pipeline_run", C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(PipelineReport), C.c_char_p,
+                                  C.c_int32]),
+    ("r2_config_summary", C.c_int, [C.c_char_p, C.POINTER(ConfigSummary), C.c_char_p, C.c_int32]),
     ("r2_synth_volume", C.c_int, [C.c_int32, C.c_int32, C.c_uint32, C.c_float, C.c_int32, C.c_void_p]),
     ("r2_synth_env", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     ("r2_synth_mesh", C.c_int, [C.c_int32, C.c_int32, C.c_float, C.c_uint32, C.POINTER(C.c_int32),

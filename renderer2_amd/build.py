@@ -27,9 +27,13 @@ SOURCES = [
     "r2_render.hip",
     "r2_filter.hip",
     "r2_server.cpp",
+    "r2_post.hip",
     "r2_frontend.cpp",
 ]
+# the command-line program (links the library)
+MAIN = "r2_main.cpp"
 
+BIN = Path(__file__).resolve().parent / "bin"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("R2_OFFLOAD_ARCH", "gfx950")
 
@@ -55,7 +59,10 @@ def _compile(src: Path, obj: Path) -> None:
 
 def build_library(jobs: int | None = None, verbose: bool = True) -> Path:
     LIB.mkdir(parents=True, exist_ok=True)
-    srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
+    srcs = [CSRC / s for s in SOURCES]
+    missing = [s.name for s in srcs if not s.exists()]
+    if missing:
+        raise FileNotFoundError(f"missing sources: {missing}")
     objs = [LIB / (s.stem + ".o") for s in srcs]
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -75,7 +82,22 @@ def build_library(jobs: int | None = None, verbose: bool = True) -> Path:
         o.unlink(missing_ok=True)
     if verbose:
         print(f"[build] linked {out}", flush=True)
+    build_cli(verbose)
     return out
+
+
+def build_cli(verbose: bool = True) -> Path:
+    """renderer2 <config.json> [out_dir]: the front end's command-line program."""
+    BIN.mkdir(parents=True, exist_ok=True)
+    exe = BIN / "renderer2"
+    cmd = [HIPCC, "-O2", "-std=c++17", str(CSRC / MAIN), "-o", str(exe), f"-L{LIB}", "-lrenderer2",
+           "-Wl,-rpath,$ORIGIN/../lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"building the command-line program failed:\n{res.stderr[-4000:]}")
+    if verbose:
+        print(f"[build] linked {exe}", flush=True)
+    return exe
 
 
 def build_oracle(verbose: bool = True) -> Path:

@@ -43,11 +43,8 @@ R2_HD void det_sincosf(float x, float* s, float* c) {
   *c = (float)co;
 }
 
-R2_HD float det_expf(float x) {
-  if (x != x) return x;
-  if (x < -104.f) return 0.f;
-  if (x > 89.f) return INFINITY;
-  const double xd = (double)x;
+// exp of a double argument in [-745, 709] by the same series (double result)
+R2_HD double det_exp_d(double xd) {
   const double n = rint(xd * 1.4426950408889634);
   const double r = xd - n * 0.6931471805599453;
   double p = 1.0 / 6227020800.0;
@@ -64,15 +61,20 @@ R2_HD float det_expf(float x) {
   p = p * r + 0.5;
   p = p * r + 1.0;
   p = p * r + 1.0;
-  return (float)ldexp(p, (int)n);
+  return ldexp(p, (int)n);
 }
 
-R2_HD float det_logf(float x) {
-  if (x != x || x < 0.f) return NAN;
-  if (x == 0.f) return -INFINITY;
-  if (x == INFINITY) return x;
+R2_HD float det_expf(float x) {
+  if (x != x) return x;
+  if (x < -104.f) return 0.f;
+  if (x > 89.f) return INFINITY;
+  return (float)det_exp_d((double)x);
+}
+
+// log of a positive finite double (double result)
+R2_HD double det_log_d(double x) {
   int e;
-  double m = frexp((double)x, &e);
+  double m = frexp(x, &e);
   if (m < 0.70710678118654752) {
     m = m * 2.0;
     e = e - 1;
@@ -90,7 +92,25 @@ R2_HD float det_logf(float x) {
   p = p * s2 + 1.0 / 5.0;
   p = p * s2 + 1.0 / 3.0;
   p = p * s2 + 1.0;
-  return (float)(2.0 * s * p + (double)e * 0.6931471805599453);
+  return 2.0 * s * p + (double)e * 0.6931471805599453;
+}
+
+R2_HD float det_logf(float x) {
+  if (x != x || x < 0.f) return NAN;
+  if (x == 0.f) return -INFINITY;
+  if (x == INFINITY) return x;
+  return (float)det_log_d((double)x);
+}
+
+// x^y for x >= 0 as exp(y log x) in double, rounded once (0^y = 0 for y > 0)
+R2_HD float det_powf(float x, float y) {
+  if (x != x || y != y || x < 0.f) return NAN;
+  if (x == 0.f) return y > 0.f ? 0.f : (y == 0.f ? 1.f : INFINITY);
+  if (x == INFINITY) return y > 0.f ? INFINITY : 0.f;
+  const double t = (double)y * det_log_d((double)x);
+  if (t < -745.0) return 0.f;
+  if (t > 709.0) return INFINITY;
+  return (float)det_exp_d(t);
 }
 
 }  // namespace r2

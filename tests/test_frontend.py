@@ -1,0 +1,90 @@
+"""Front end (CPU): config.json / scenedescription.json parsing with the keys,
+required fields and comment handling of CConfigLoader
+(raytracer/src/backend/config_loader.cu).  The fixtures under
+tests/golden/frontend/ are the reference's examples/config.json and
+examples/scenedescription.json, unchanged."""
+import ctypes as C
+import json
+import shutil
+from pathlib import Path
+
+import pytest
+
+from renderer2_amd import _capi as capi
+
+FIX = Path(__file__).resolve().parent / "golden" / "frontend"
+
+
+def summary(path):
+    out = capi.ConfigSummary()
+    err = C.create_string_buffer(512)
+    rc = capi.lib().r2_config_summary(str(path).encode(), C.byref(out), err, 512)
+    return rc, out, err.value.decode()
+
+
+def layout(tmp_path):
+    """The example config names its scene description as ./../../raytracer/scenedescription.json."""
+    cfg_dir = tmp_path / "build" / "release"
+    cfg_dir.mkdir(parents=True)
+    (tmp_path / "raytracer").mkdir()
+    shutil.copy(FIX / "config.json", cfg_dir / "config.json")
+    shutil.copy(FIX / "scenedescription.json", tmp_path / "raytracer" / "scenedescription.json")
+    return cfg_dir / "config.json"
+
+
+def test_example_config_parses(tmp_path):
+    rc, s, err = summary(layout(tmp_path))
+    assert rc == 0, err
+    assert (s.frame_width, s.frame_height, s.samples) == (1920, 1080, 32)
+    assert s.gamma == 2.0 and s.use_brick_grid == 1
+    assert s.filter_active == 0 and s.min_voxel_size == 1.0 and s.lods == 7 and s.samples_per_voxel == 100
+    assert s.sigma_t == 10.0 and s.estimation_iterations == 20 and abs(s.alpha - 0.1) < 1e-7 and s.clip_rays == 1
+    assert s.fov == 90.0
+    assert list(s.cam_pos) == pytest.approx([-680.0, 35.1, 680.0])
+    assert list(s.cam_look_at) == pytest.approx([-675.0, 31.5, 675.0])
+    assert list(s.cam_up) == [0.0, 1.0, 0.0]
+    # one mesh (FILTER + RENDER) and one ground circle are active; the rest is commented out
+    assert (s.n_meshes, s.n_surfaces, s.n_media, s.n_filter_objects) == (1, 1, 0, 1)
+    assert s.max_depth == 50
+
+
+@pytest.mark.parametrize("drop", [("General", "Gamma"), ("Filtering", "ClipRays"), ("Camera", "Up"),
+                                  ("Scene", "Envmap")])
+def test_missing_required_key_fails(tmp_path, drop):
+    cfg = layout(tmp_path)
+    text = cfg.read_text()
+    # strip comments the simple way: the fixture's comments are whole-line or trailing //
+    lines = [ln.split("//")[0] if "//" in ln and '"./' not in ln else ln for ln in text.splitlines()]
+    j = json.loads("\n".join(lines))
+    del j[drop[0]][drop[1]]
+    cfg.write_text(json.dumps(j))
+    rc, _, err = summary(cfg)
+    assert rc < 0 and "Could not parse config" in err
+
+
+def test_medium_objects_and_mask_default(tmp_path):
+    cfg = layout(tmp_path)
+    sd = tmp_path / "raytracer" / "scenedescription.json"
+    sd.write_text("""[
+      /* block comment */
+      { "Type": "Medium", "Path": "./filtering/x/filtered_mesh_0.100000",
+        "SigmaA": [0, 0, 0], "SigmaS": [10, 10, 10], "G": 0.2,
+        "Pos": [0, 0, 0], "Orientation": [0, 1, 0], "Scaling": [1, 1, 1] },   // default mask RENDER
+      { "Type": "Medium", "Path": "p_0.2", "SigmaA": [1, 1, 1], "SigmaS": [1, 1, 1],
+        "Pos": [1, 0, 0], "Orientation": [0, 1, 0], "Scaling": [2, 2, 2], "Mask": ["RENDER"] },
+      { "Type": "Unknown", "Whatever": 1 }
+    ]""")
+    rc, s, err = summary(cfg)
+    assert rc == 0, err
+    assert (s.n_media, s.n_meshes, s.n_surfaces, s.n_filter_objects) == (2, 0, 0, 0)
+    sd.write_text('[ { "Type": "Medium", "Path": "p_0.2", "SigmaA": [1, 1], "SigmaS": [1, 1, 1], '
+                  '"Pos": [1, 0, 0], "Orientation": [0, 1, 0], "Scaling": [2, 2, 2] } ]')
+    rc, _, err = summary(cfg)
+    assert rc < 0  # SigmaA must have three components (parseVec3)
+
+
+def test_malformed_json_reports_line(tmp_path):
+    cfg = tmp_path / "c.json"
+    cfg.write_text('{\n "General": {\n "FrameWidth": 10,,\n }\n}')
+    rc, _, err = summary(cfg)
+    assert rc < 0 and "line 3" in err
