@@ -29,10 +29,10 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "M paths/sec at 1024² on 256³ SGGX grid @relMSE≤1e-3; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per lookup kind: 32-byte voxel record,
+# algorithmic bytes per lookup kind: 32-byte voxel record (24 B packed),
 # 4-byte majorant, brick density = 4 B indirection + 4 B range + 1 B atlas,
 # env = 2 CDF reads (8 B) + 4 RGB texels (48 B); +12 B framebuffer per path
-B_VOX_FP32, B_VOX_PACKED, B_MAJ, B_BRICK, B_ENV, B_FB = 32, 16, 4, 9, 56, 12
+B_VOX_FP32, B_VOX_PACKED, B_MAJ, B_BRICK, B_ENV, B_FB = 32, 24, 4, 9, 56, 12
 GOLDEN = ROOT / "tests" / "golden"
 
 
@@ -71,6 +71,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-stride", type=int, default=2, help="pixel stride of the CPU baseline sample")
     ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-brick-grid", action="store_true")
+    ap.add_argument("--precision", choices=["fp32", "packed"], default=None,
+                    help="voxel record precision (default: the workload's own)")
     ap.add_argument("--cpu-voxels", type=int, default=1_500_000,
                     help="random voxels per LOD of the filter CPU baseline sample")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -79,7 +81,7 @@ def main(argv=None) -> int:
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(rendering shards by pixel tiles / sample ranges with one framebuffer reduce, see DESIGN.md)",
+              "(rendering would shard by pixel tiles or sample ranges with one framebuffer reduce)",
               file=sys.stderr)
         return 2
 
@@ -89,7 +91,7 @@ def main(argv=None) -> int:
     import renderer2_amd as r2
     from renderer2_amd import _hip, scenes
 
-    w = scenes.make_workload(a.workload, seed=1)
+    w = scenes.make_workload(a.workload, seed=1, **({"precision": a.precision} if a.precision else {}))
     cfg = dict(w["render"], use_brick_grid=not a.no_brick_grid)
     W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
     ctx = r2.Context(0)
@@ -165,13 +167,14 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if w["grids"][0].get("precision", "fp32") == "fp32" else "fp16 records / fp32 transport",
         "data": "synthetic",
         "config": {"workload": {"c1": "64^3 isotropic SGGX value-noise volume, 256x256, 16 spp, depth 8",
                                 "c2": "256^3 anisotropic SGGX fibre volume, procedural sky+sun env, "
                                       "1024x1024, 64 spp per step, depth 16",
-                                "c4": "512^3 isotropic volume, packed FP16/BF16 records, 1024x1024, 256 spp"}
+                                "c4": "512^3 isotropic value-noise volume, 1024x1024, 256 spp per step, depth 8"}
                                [a.workload],
+                   "precision": w["grids"][0].get("precision", "fp32"),
                    "grid": w["params"]["n"], "width": W, "height": H, "spp_per_step": spp,
                    "max_depth": cfg["max_depth"], "brick_grid": cfg["use_brick_grid"],
                    "paths_per_step": paths_per_step},
@@ -279,7 +282,7 @@ def bench_filter(a) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if w["grids"][0].get("precision", "fp32") == "fp32" else "fp16 records / fp32 transport",
         "data": "synthetic",
         "config": {"workload": "2.4 M triangle noise-displaced sphere + 200 k quads filtered to 3 LODs, "
                                "64 rays/voxel, 8 iterations, clipped rays",

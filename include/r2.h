@@ -34,7 +34,9 @@ extern "C" {
 
 /* Grid record precision (r2_grid_desc.precision) */
 #define R2_PREC_FP32 0   /* 32-byte SFilteredDataCompact voxel records (filtered_data.hpp:33-61) */
-#define R2_PREC_PACKED 1 /* 16-byte records: FP16 density + 6 SGGX coefficients FP16/BF16 + rest packed */
+#define R2_PREC_PACKED 1 /* 24-byte records: FP16 density, the six SGGX coefficients of S in FP16 after an
+                            FP64 eigen-decomposition (PSD projection), FP16 ior, u8 colours,
+                            octahedral normal */
 
 /* r2_render_params.flags */
 #define R2_FLAG_BRICK_GRID 1u   /* majorant DDA + brick density (config "UseBrickGrid": true) */
@@ -92,8 +94,15 @@ int r2_grid_get_info(const r2_grid* grid, r2_grid_info* out);
  * mips: 3 levels of u32 ranges concatenated.  Any pointer may be NULL. */
 int r2_grid_download_bricks(r2_ctx* ctx, const r2_grid* grid, uint32_t* range, uint8_t* u8, uint32_t* mips);
 /* Parity probe: NanoVDB-layout voxel lookups. For each of n integer coordinates
- * (xyz triples) write the 32-byte record the tree returns (background when inactive). */
+ * (xyz triples) write the 32-byte record the tree returns (background when inactive);
+ * packed grids keep these FP32 records beside their packed ones. */
 int r2_grid_lookup(r2_ctx* ctx, const r2_grid* grid, int32_t n, const int32_t* coords, uint8_t* out_records);
+/* Parity probe: the filtered record (trilinear over the 8 neighbours, decoded) at n
+ * positions normalised to the index bbox, as CNVDBMedium::filteredData
+ * (raytracer/include/medium/nvdb_medium_impl.hpp:47-64) computes it; 17 floats each:
+ * density, S (xx yy zz xy xz yz), diffuse rgb, specular rgb, normal xyz, ior.
+ * A packed grid answers from its packed records. */
+int r2_grid_sample(r2_ctx* ctx, const r2_grid* grid, int32_t n, const float* pos, float* out);
 
 /* ----------------------------------------------------------------- env -- */
 /* Lat-long HDR environment map with sin(theta)-weighted 2-D importance sampling.

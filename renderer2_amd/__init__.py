@@ -219,6 +219,14 @@ class Grid:
             off += c
         return rng, u8, out_m
 
+    def sample(self, pos: np.ndarray) -> np.ndarray:
+        """Filtered (interpolated, decoded) records at normalised positions: (n, 17) floats."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        n = pos.shape[0]
+        out = np.zeros((n, 17), np.float32)
+        capi.check(capi.lib().r2_grid_sample(self.ctx.h, self.h, n, pos.ctypes.data, out.ctypes.data), self.ctx.h)
+        return out
+
     def lookup(self, coords: np.ndarray) -> np.ndarray:
         coords = np.ascontiguousarray(coords, np.int32)
         n = coords.shape[0]

@@ -258,8 +258,13 @@ int r2_grid_download_bricks(r2_ctx* ctx, const r2_grid* g, uint32_t* range, uint
 int r2_grid_lookup(r2_ctx* ctx, const r2_grid* g, int32_t n, const int32_t* coords, uint8_t* out) {
   if (!ctx || !g || n < 0 || (n > 0 && (!coords || !out))) return R2_ERR_INVALID;
   if (n == 0) return R2_OK;
-  if (g->dev.precision != R2_PREC_FP32) return fail(ctx, R2_ERR_UNSUPPORTED, "grid_lookup: FP32 records only");
   return grid_lookup(ctx, g, n, coords, out);
+}
+
+int r2_grid_sample(r2_ctx* ctx, const r2_grid* g, int32_t n, const float* pos, float* out) {
+  if (!ctx || !g || n < 0 || (n > 0 && (!pos || !out))) return R2_ERR_INVALID;
+  if (n == 0) return R2_OK;
+  return grid_sample(ctx, g, n, pos, out);
 }
 
 int r2_env_upload(r2_ctx* ctx, const float* rgb, int32_t W, int32_t H, r2_env** out) {

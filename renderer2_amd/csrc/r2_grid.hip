@@ -7,6 +7,7 @@
 // CNVDBMedium::init / getMaxValue (raytracer/src/medium/nvdb_medium.cu:130-173,308-323),
 // CHostBrickGrid::CHostBrickGrid (raytracer/src/grid_brick/host_grid_brick.cpp:61-152).
 #include "r2_internal.hpp"
+#include "r2_volume.hpp"
 #include <algorithm>
 #include <cstring>
 
@@ -78,6 +79,37 @@ __global__ void k_leaves(const uint4* __restrict__ rec, const uint8_t* __restric
       dst[1] = b;
     }
   }
+}
+
+// packed form of every leaf-pool record (r2_packed.hpp)
+__global__ void k_pack(const uint4* __restrict__ leaves, size_t n, uint4* __restrict__ hot, uint2* __restrict__ cold) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const Rec r = decode_record(leaves[2 * i], leaves[2 * i + 1]);
+    PackIn in;
+    in.density = r.density;
+    for (int k = 0; k < 6; ++k) in.S[k] = r.S[k];
+    in.diffuse[0] = r.diffuse.x; in.diffuse[1] = r.diffuse.y; in.diffuse[2] = r.diffuse.z;
+    in.specular[0] = r.specular.x; in.specular[1] = r.specular.y; in.specular[2] = r.specular.z;
+    in.normal[0] = r.normal.x; in.normal[1] = r.normal.y; in.normal[2] = r.normal.z;
+    in.ior = r.ior;
+    pack_record(in, &hot[i], &cold[i]);
+  }
+}
+
+// filtered (trilinear, decoded) records at normalised positions: 17 floats each
+// (density, S6, diffuse3, specular3, normal3, ior)
+__global__ void k_sample(DGrid g, int n, const float* __restrict__ pos, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Cnt cnt;
+  const Rec r = filtered_data<false>(g, mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), cnt);
+  float* o = out + 17 * (size_t)i;
+  o[0] = r.density;
+  for (int k = 0; k < 6; ++k) o[1 + k] = r.S[k];
+  o[7] = r.diffuse.x; o[8] = r.diffuse.y; o[9] = r.diffuse.z;
+  o[10] = r.specular.x; o[11] = r.specular.y; o[12] = r.specular.z;
+  o[13] = r.normal.x; o[14] = r.normal.y; o[15] = r.normal.z;
+  o[16] = r.ior;
 }
 
 __device__ __forceinline__ float dens_at(const float* __restrict__ dens, int nx, int ny, int nz, int x, int y, int z) {
@@ -263,6 +295,19 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   R2_HIP(ctx, hipGetLastError());
   d.leaves = d_leaves;
   d.leaves_packed = nullptr;
+  d.leaves_cold = nullptr;
+  if (desc->precision == R2_PREC_PACKED) {
+    uint4* d_hot = nullptr;
+    uint2* d_cold = nullptr;
+    const size_t nv = (size_t)nleaves * kLeafVoxels;
+    if ((rc = dmalloc(ctx, g, &d_hot, nv))) return rc;
+    if ((rc = dmalloc(ctx, g, &d_cold, nv))) return rc;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)std::min<size_t>((nv + 255) / 256, 1u << 16)), dim3(256), 0, st, d_leaves,
+                       nv, d_hot, d_cold);
+    R2_HIP(ctx, hipGetLastError());
+    d.leaves_packed = d_hot;
+    d.leaves_cold = d_cold;
+  }
 
   // brick grid
   int nb[3];
@@ -402,6 +447,23 @@ int grid_lookup(r2_ctx* ctx, const r2_grid* g, int n, const int32_t* coords, uin
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipStreamSynchronize(ctx->stream));
   R2_HIP(ctx, hipMemcpy(out, dout, (size_t)n * 32, hipMemcpyDeviceToHost));
+  return R2_OK;
+}
+
+int grid_sample(r2_ctx* ctx, const r2_grid* g, int n, const float* pos, float* out) {
+  void *dp = nullptr, *dout = nullptr;
+  struct F {
+    void** q[2];
+    ~F() { for (auto x : q) if (*x) (void)hipFree(*x); }
+  } f{{&dp, &dout}};
+  R2_HIP(ctx, hipMalloc(&dp, (size_t)n * 12));
+  R2_HIP(ctx, hipMalloc(&dout, (size_t)n * 17 * 4));
+  R2_HIP(ctx, hipMemcpy(dp, pos, (size_t)n * 12, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_sample, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, g->dev, n, (const float*)dp,
+                     (float*)dout);
+  R2_HIP(ctx, hipGetLastError());
+  R2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  R2_HIP(ctx, hipMemcpy(out, dout, (size_t)n * 17 * 4, hipMemcpyDeviceToHost));
   return R2_OK;
 }
 

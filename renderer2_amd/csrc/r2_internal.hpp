@@ -46,6 +46,7 @@ int hip_fail(r2_ctx* ctx, hipError_t e, const char* what);
 // grid build (r2_grid.hip)
 int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g);
 int grid_lookup(r2_ctx* ctx, const r2_grid* g, int n, const int32_t* coords, uint8_t* out);
+int grid_sample(r2_ctx* ctx, const r2_grid* g, int n, const float* pos, float* out);
 // render kernels (r2_render.hip)
 int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum);
 int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,

@@ -22,7 +22,8 @@ constexpr int kLoopCap = 65536;        // safety cap on every traversal loop
 struct DGrid {
   const int32_t* leaf_table;
   const uint4* leaves;          // 2 x uint4 (32 B) per voxel record (FP32 form)
-  const uint4* leaves_packed;   // 1 x uint4 (16 B) per voxel record (packed form)
+  const uint4* leaves_packed;   // packed form: 16 B hot record per voxel (density, S, ior in FP16)
+  const uint2* leaves_cold;     // packed form: 8 B cold record per voxel (u8 colours, octahedral normal)
   int32_t nx, ny, nz;
   int32_t nlx, nly, nlz;
   const uint2* bricks;          // x = encoded range, y = atlas slot (0xffffffff: empty)

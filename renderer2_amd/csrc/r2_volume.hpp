@@ -4,6 +4,7 @@
 // reference routine it cites; arithmetic keeps the reference's evaluation order.
 #pragma once
 #include "r2_types.hpp"
+#include "r2_packed.hpp"
 
 namespace r2 {
 
@@ -107,25 +108,21 @@ __device__ __forceinline__ Rec decode_record(uint4 a, uint4 b) {
   return r;
 }
 
-// Packed 16-byte record: FP16 density | 6 SGGX coefficients (FP16 diagonal,
-// BF16 off-diagonal) | FP16 ior | RGB565 diffuse + RGB565 specular | oct-16 normal.
-__device__ __forceinline__ Rec decode_packed(uint4 p) {
+// packed record (r2_packed.hpp): FP16 density / S / ior, u8 colours, octahedral normal
+__device__ __forceinline__ Rec decode_packed(uint4 h, uint2 c) {
   Rec r;
-  r.density = half_to_float((uint16_t)(p.x & 0xffffu));
-  r.S[0] = half_to_float((uint16_t)(p.x >> 16));
-  r.S[1] = half_to_float((uint16_t)(p.y & 0xffffu));
-  r.S[2] = half_to_float((uint16_t)(p.y >> 16));
-  r.S[3] = bf16_to_float((uint16_t)(p.z & 0xffffu));
-  r.S[4] = bf16_to_float((uint16_t)(p.z >> 16));
-  r.S[5] = bf16_to_float((uint16_t)(p.w & 0xffffu));
-  const uint32_t q = p.w >> 16;
-  // q: 5-bit diffuse grey | 5-bit specular grey | ... see pack_record (r2_grid.hip)
-  r.ior = 1.f + (float)(q & 0x3fu) * (1.f / 63.f);
-  const float dg = (float)((q >> 6) & 0x1fu) * (1.f / 31.f);
-  const float sg = (float)((q >> 11) & 0x1fu) * (1.f / 31.f);
-  r.diffuse = mk(dg, dg, dg);
-  r.specular = mk(sg, sg, sg);
-  r.normal = mk(0.f, 1.f, 0.f);
+  r.density = half_to_float((uint16_t)(h.x & 0xffffu));
+  r.S[0] = half_to_float((uint16_t)(h.x >> 16));
+  r.S[1] = half_to_float((uint16_t)(h.y & 0xffffu));
+  r.S[2] = half_to_float((uint16_t)(h.y >> 16));
+  r.S[3] = half_to_float((uint16_t)(h.z & 0xffffu));
+  r.S[4] = half_to_float((uint16_t)(h.z >> 16));
+  r.S[5] = half_to_float((uint16_t)(h.w & 0xffffu));
+  r.ior = half_to_float((uint16_t)(h.w >> 16));
+  const float inv8 = 1.f / 255.f;
+  r.diffuse = mk((float)(c.x & 0xffu) * inv8, (float)((c.x >> 8) & 0xffu) * inv8, (float)((c.x >> 16) & 0xffu) * inv8);
+  r.specular = mk((float)(c.x >> 24) * inv8, (float)(c.y & 0xffu) * inv8, (float)((c.y >> 8) & 0xffu) * inv8);
+  r.normal = oct_decode(c.y >> 16);
   return r;
 }
 
@@ -152,10 +149,16 @@ __device__ __forceinline__ Rec get_value(const DGrid& g, int x, int y, int z, Cn
   if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny && (unsigned)z < (unsigned)g.nz)
     leaf = g.leaf_table[((size_t)(z >> 3) * g.nly + (y >> 3)) * g.nlx + (x >> 3)];
   if (g.precision == 1) {
-    uint4 p = make_uint4(0u, 0u, 0u, 0u);
-    if (leaf >= 0) p = g.leaves_packed[(size_t)leaf * kLeafVoxels + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7))];
-    else p.w = 0u;  // background: ior code 0 -> 1.0
-    return decode_packed(p);
+    // background: the packed form of the FP32 background record (ior 1, normal
+    // decoded from the zero code, normalize(-1,-1,-1), octahedral code 0xABAB)
+    uint4 h = make_uint4(0u, 0u, 0u, 0x3C00u << 16);
+    uint2 c = make_uint2(0u, 0xABABu << 16);
+    if (leaf >= 0) {
+      const size_t v = (size_t)leaf * kLeafVoxels + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7));
+      h = g.leaves_packed[v];
+      c = g.leaves_cold[v];
+    }
+    return decode_packed(h, c);
   }
   uint4 a = make_uint4(0u, 0u, 0u, 0u), b = make_uint4(0u, 0u, 0u, 0x3C00u);  // background: ior = 1
   if (leaf >= 0) {
@@ -174,6 +177,31 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
   const float psz = p.z * (float)g.size[2] + (float)g.ibb_min[2] - 0.5f;
   const int x = (int)floorf(psx), y = (int)floorf(psy), z = (int)floorf(psz);
   const float dx = psx - (float)x, dy = psy - (float)y, dz = psz - (float)z;
+  if (g.precision == 1) {
+    // packed form: one weighted sum of the eight corners, normal renormalised once
+    Rec o;
+    o.density = 0.f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) o.S[i] = 0.f;
+    o.diffuse = o.specular = o.normal = mk(0.f, 0.f, 0.f);
+    o.ior = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int cx = k & 1, cy = (k >> 1) & 1, cz = k >> 2;
+      const float w = (cx ? dx : 1.f - dx) * (cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz);
+      const Rec r = get_value<kCount>(g, x + cx, y + cy, z + cz, cnt);
+      o.density += w * r.density;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) o.S[i] += w * r.S[i];
+      o.diffuse = o.diffuse + w * r.diffuse;
+      o.specular = o.specular + w * r.specular;
+      o.normal = o.normal + w * r.normal;
+      o.ior += w * r.ior;
+    }
+    const float nl = dot(o.normal, o.normal);
+    o.normal = nl > 0.f ? o.normal * (1.f / sqrtf(nl)) : mk(0.f, 1.f, 0.f);
+    return o;
+  }
   // same arithmetic as d00, d10 -> d0, d01, d11 -> d1, then z; evaluated so that
   // at most three decoded records are live at a time
   Rec a = get_value<kCount>(g, x, y, z, cnt), b = get_value<kCount>(g, x + 1, y, z, cnt);

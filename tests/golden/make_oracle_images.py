@@ -25,7 +25,7 @@ PLAN = {
     # workload: (pixel stride, spp, overrides)
     "c1": (1, 4096, {}),
     "c2": (8, 4096, {}),
-    "c4": (8, 1024, {"precision": "fp32"}),
+    "c4": (8, 4096, {"precision": "fp32"}),
 }
 SEED = 7777
 
