@@ -134,7 +134,10 @@ def main(argv=None) -> int:
         z = np.load(gold, allow_pickle=False)
         st = int(z["stride"])
         sub = acc[::st, ::st]
-        quality = dict(relmse=relmse(sub, z["image"]), spp=a.steps * spp, oracle_spp=int(z["spp"]),
+        from renderer2_amd.flip import flip_hdr
+
+        quality = dict(relmse=relmse(sub, z["image"]), flip=round(flip_hdr(z["image"], sub), 5), spp=a.steps * spp,
+                       oracle_spp=int(z["spp"]),
                        oracle="FP64 CPU oracle, 1/%d strided pixel subset" % (st * st),
                        relmse_1step=relmse(bufs[0].to_host(np.float32, (H, W, 3))[::st, ::st], z["image"]))
 
