@@ -66,13 +66,16 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--no-serving", action="store_true", help="skip the render-server latency probe")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=2, help="pixel stride of the CPU baseline sample")
     ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-brick-grid", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "packed"], default=None,
                     help="voxel record precision (default: the workload's own)")
+    ap.add_argument("--requests", type=int, default=40, help="requests of the serving workload")
+    ap.add_argument("--rate", type=float, default=2.5, help="Poisson arrival rate (requests/s) of the serving workload")
     ap.add_argument("--cpu-voxels", type=int, default=1_500_000,
                     help="random voxels per LOD of the filter CPU baseline sample")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -87,6 +90,8 @@ def main(argv=None) -> int:
 
     if a.workload == "c3":
         return bench_filter(a)
+    if a.workload == "c5":
+        return bench_serving(a)
 
     import renderer2_amd as r2
     from renderer2_amd import _hip, scenes
@@ -141,6 +146,12 @@ def main(argv=None) -> int:
                        oracle="FP64 CPU oracle, 1/%d strided pixel subset" % (st * st),
                        relmse_1step=relmse(bufs[0].to_host(np.float32, (H, W, 3))[::st, ::st], z["image"]))
 
+    serving = None
+    if not a.no_serving:
+        inst = [dict(grid=0, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))]
+        serving = serve_stream(ctx, grids, env, [inst], n_req=24, rate=15.0, seed=5, sizes=(256, 512),
+                               spps=(16, 64), max_depth=cfg["max_depth"])
+
     cpu = None
     counts = None
     if not a.no_cpu_baseline:
@@ -185,6 +196,100 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "quality": quality,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "serving": serving,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def serve_stream(ctx, grids, env, instance_sets, n_req, rate, seed, sizes, spps, max_depth, radius=2.5):
+    """Seeded Poisson request stream through the render server: each request picks
+    a resolution, an spp, an instance set (LOD) and an orbit camera at random."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for k in range(n_req):
+        res = int(rng.choice(sizes))
+        spp = int(rng.choice(spps))
+        inst = instance_sets[int(rng.integers(len(instance_sets)))]
+        az, el = rng.uniform(0, 2 * np.pi), rng.uniform(-0.3, 0.6)
+        cam = (radius * np.cos(el) * np.sin(az), radius * np.sin(el), radius * np.cos(el) * np.cos(az))
+        cfg = dict(width=res, height=res, spp=spp, max_depth=max_depth, cam_pos=cam, cam_look_at=(0, 0, 0),
+                   cam_up=(0, 1, 0), fov_deg=40.0, seed=1000 + k, stream=0, use_brick_grid=True)
+        reqs.append((float(arrivals[k]), cfg, inst))
+    srv = ctx.server(hbm_budget_bytes=16 << 30, max_inflight=16, slice_paths=1 << 23)
+    t0 = time.perf_counter()
+    ids = []
+    for t, cfg, inst in reqs:
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        ids.append(srv.submit(cfg, grids, inst, env))
+    for i in ids:
+        srv.wait(i)
+    elapsed = time.perf_counter() - t0
+    st = srv.stats()
+    srv.stop()
+    paths = sum(c["width"] * c["height"] * c["spp"] for _, c, _ in reqs)
+    return {"requests": n_req, "arrival_rate_per_s": rate, "p50_ms": round(st["p50_ms"], 2),
+            "p99_ms": round(st["p99_ms"], 2), "throughput_mpaths_s": round(paths / elapsed / 1e6, 2),
+            "server_mpaths_s": round(st["paths_per_sec"] / 1e6, 2), "completed": st["completed"],
+            "rejected": st["rejected"], "peak_gib": round(st["peak_bytes"] / 2 ** 30, 3),
+            "sizes": list(sizes), "spps": list(spps), "wall_s": round(elapsed, 2)}
+
+
+def bench_serving(a) -> int:
+    """Render server under a seeded Poisson stream: device-resident LOD grids of the
+    filtered mesh (512^3, 256^3, 128^3) plus the 64^3 value-noise grid; resolution
+    uniform over {256^2, 512^2, 1024^2}, spp over {16, 64, 256}, orbit camera and LOD
+    chosen per request."""
+    import renderer2_amd as r2
+    from renderer2_amd import _hip, scenes
+
+    ctx = r2.Context(0)
+    w = scenes.FILTER_WORKLOADS["c3"]
+    mesh = scenes.make_filter_mesh("c3", seed=1)
+    gm = ctx.upload_mesh(mesh)
+    vs0 = scenes.finest_voxel_size(mesh, w["finest"])
+    grids, keep = [], []
+    ext = float((mesh["positions"].max(axis=0) - mesh["positions"].min(axis=0)).max())
+    for lod in range(w["lods"]):
+        vs = vs0 * 2.0 ** lod
+        fg = gm.setup(vs)
+        buf = _hip.DeviceBuffer(int(fg.num_voxels[0]) * int(fg.num_voxels[1]) * int(fg.num_voxels[2]) * 32)
+        ctx.filter_lod(gm, r2.filter_params(vs, w["samples"], w["iterations"], w["sigma_t"], w["alpha"],
+                                            w["clip_rays"], seed=1, lod=lod), out_dev_ptr=buf.addr)
+        grids.append(ctx.upload_grid(scenes.filtered_grid(fg, vs, albedo=0.9, records_dev=buf.addr)))
+        keep.append(buf)
+    c1 = scenes.make_workload("c1")
+    grids.append(ctx.upload_grid(c1["grids"][0]))
+    env = ctx.upload_env(scenes.synth_env("sky", 512, 256))
+    s = 1.0 / ext
+    inst_sets = [[dict(grid=k, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(s, s, s))] for k in range(3)]
+    inst_sets.append([dict(grid=3, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))])
+    res = serve_stream(ctx, grids, env, inst_sets, n_req=a.requests, rate=a.rate, seed=11, sizes=(256, 512, 1024),
+                       spps=(16, 64, 256), max_depth=16)
+    for b in keep:
+        b.free()
+    line = {
+        "metric": "render server: M paths/s with p50/p99 request latency under a Poisson stream",
+        "value": res["throughput_mpaths_s"],
+        "unit": "M paths/s",
+        "n_gpus": 1,
+        "steps": a.requests,
+        "warmup": 0,
+        "ms_per_step": round(res["wall_s"] * 1e3 / a.requests, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "serving: filtered mesh LODs 512^3/256^3/128^3 + 64^3 value-noise grid resident, "
+                               "Poisson arrivals, resolution {256,512,1024}^2, spp {16,64,256}, orbit camera",
+                   "requests": a.requests, "arrival_rate_per_s": a.rate},
+        "serving": res,
+        "roofline": None,
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -46,11 +46,16 @@ class Context:
 
     # ---------------------------------------------------------------- grid
     def upload_grid(self, g: dict) -> "Grid":
-        rec = np.ascontiguousarray(g["records"])
+        """g: records (host array) or records_dev (device address), shape (nz, ny, nx), map, optics."""
         d = capi.GridDesc()
         d.nz, d.ny, d.nx = g["shape"]
-        d.records = rec.ctypes.data
-        d.records_on_device = 0
+        if g.get("records_dev"):
+            d.records = g["records_dev"]
+            d.records_on_device = 1
+        else:
+            rec = np.ascontiguousarray(g["records"])
+            d.records = rec.ctypes.data
+            d.records_on_device = 0
         d.map_mat[:] = list(g["map_mat"])
         d.map_vec[:] = list(g["map_vec"])
         d.sigma_a[:] = list(g["sigma_a"])

@@ -131,3 +131,19 @@ def make_workload(name: str, seed: int = 1, **overrides) -> dict:
     return dict(name=name, grids=[grid], instances=[dict(grid=0, pos=(0, 0, 0), orientation=(0, 1, 0),
                                                          scaling=(1, 1, 1))],
                 env=env, render=render, params=w)
+
+
+def filtered_grid(fgrid, voxel_size: float, albedo: float = 0.9, records=None, records_dev: int | None = None,
+                  precision: str = "fp32") -> dict:
+    """Grid description of one filtered LOD (r2_filter_grid): the index -> model map
+    is the filter's indexToModel, so the LOD renders in the mesh's model space."""
+    i2m = np.array(fgrid.index_to_model, np.float32).reshape(4, 3)
+    nx, ny, nz = (int(v) for v in fgrid.num_voxels)
+    g = dict(shape=(nz, ny, nx), map_mat=tuple(float(v) for v in i2m[:3].T.reshape(-1)),
+             map_vec=tuple(float(v) for v in i2m[3]), sigma_a=(1.0 - albedo,) * 3, sigma_s=(albedo,) * 3,
+             voxel_size_filtering=float(voxel_size), precision=precision)
+    if records_dev is not None:
+        g["records_dev"] = records_dev
+    else:
+        g["records"] = records
+    return g
