@@ -178,13 +178,16 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
   const int x = (int)floorf(psx), y = (int)floorf(psy), z = (int)floorf(psz);
   const float dx = psx - (float)x, dy = psy - (float)y, dz = psz - (float)z;
   if (g.precision == 1) {
-    // packed form: one weighted sum of the eight corners, normal renormalised once
+    // packed form: density, S, colours and ior as one weighted sum of the eight
+    // corners; normals interpolated pairwise along x, y, z with renormalisation,
+    // as operator*/operator+ do (filtered_data.hpp:115-151)
     Rec o;
     o.density = 0.f;
 #pragma unroll
     for (int i = 0; i < 6; ++i) o.S[i] = 0.f;
-    o.diffuse = o.specular = o.normal = mk(0.f, 0.f, 0.f);
+    o.diffuse = o.specular = mk(0.f, 0.f, 0.f);
     o.ior = 0.f;
+    v3 nx0 = mk(0.f, 0.f, 0.f), ny0 = nx0, nz0 = nx0, nz1 = nx0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int cx = k & 1, cy = (k >> 1) & 1, cz = k >> 2;
@@ -195,11 +198,21 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
       for (int i = 0; i < 6; ++i) o.S[i] += w * r.S[i];
       o.diffuse = o.diffuse + w * r.diffuse;
       o.specular = o.specular + w * r.specular;
-      o.normal = o.normal + w * r.normal;
       o.ior += w * r.ior;
+      if (cx == 0) {
+        nx0 = r.normal;
+        continue;
+      }
+      const v3 e = normalize((1.f - dx) * normalize(nx0) + dx * normalize(r.normal));
+      if (cy == 0) {
+        ny0 = e;
+        continue;
+      }
+      const v3 d = normalize((1.f - dy) * normalize(ny0) + dy * normalize(e));
+      if (cz == 0) nz0 = d;
+      else nz1 = d;
     }
-    const float nl = dot(o.normal, o.normal);
-    o.normal = nl > 0.f ? o.normal * (1.f / sqrtf(nl)) : mk(0.f, 1.f, 0.f);
+    o.normal = normalize((1.f - dz) * normalize(nz0) + dz * normalize(nz1));
     return o;
   }
   // same arithmetic as d00, d10 -> d0, d01, d11 -> d1, then z; evaluated so that
