@@ -390,7 +390,7 @@ def bench_filter(a) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if w["grids"][0].get("precision", "fp32") == "fp32" else "fp16 records / fp32 transport",
+        "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "2.4 M triangle noise-displaced sphere + 200 k quads filtered to 3 LODs, "
                                "64 rays/voxel, 8 iterations, clipped rays",

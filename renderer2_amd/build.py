@@ -25,6 +25,7 @@ SOURCES = [
     "r2_synth.cpp",
     "r2_grid.hip",
     "r2_render.hip",
+    "r2_render_packed.hip",
     "r2_filter.hip",
     "r2_server.cpp",
     "r2_post.hip",

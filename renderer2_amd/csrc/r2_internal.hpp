@@ -40,6 +40,32 @@ struct r2_env {
   std::vector<void*> allocs;
 };
 
+namespace r2 {
+// device entry of a batched launch: a request's scene, chunk sums and item range
+struct BatchEntry {
+  const DScene* scene;  // device copy of the request's scene
+  void* partial;        // chunk sums of the request (Acc[nchunks * npix * 3])
+  uint64_t first;       // first item of this slice in the request's item space
+  uint64_t count;       // items of this slice
+  int chunk;            // samples per item
+  int pad;
+};
+}  // namespace r2
+
+// integrator launchers of the two record-precision instantiations
+#define R2K_DECLARE(NS)                                                                                              \
+  namespace NS {                                                                                                     \
+  void launch_sm(hipStream_t st, int blocks, const r2::DScene* d_scene, uint64_t* counter, uint64_t total, int chunk, \
+                 void* partial, bool fp64, bool single, bool count, const int32_t* pp, const int32_t* ps, float* po, \
+                 uint32_t* pc);                                                                                      \
+  void launch_batch_group(hipStream_t st, int blocks, const void* ents, int n, uint64_t* counters, bool single,      \
+                          bool fp64);                                                                                \
+  void launch_reduce_k(hipStream_t st, const void* partial, bool fp64, int nchunks, uint32_t n3, float* out);        \
+  }
+R2K_DECLARE(r2k_fp32)
+R2K_DECLARE(r2k_packed)
+#undef R2K_DECLARE
+
 namespace r2host {
 int fail(r2_ctx* ctx, int code, const std::string& msg);
 int hip_fail(r2_ctx* ctx, hipError_t e, const char* what);
@@ -57,7 +83,7 @@ struct BatchSlice {
   void* d_partial;
   uint64_t first, count;
   int chunk;
-  bool single, fp64;
+  bool single, fp64, packed;
 };
 constexpr int kMaxBatchEntries = 256;
 void render_chunking(const r2::DScene& scene, int* chunk, int* nchunks);

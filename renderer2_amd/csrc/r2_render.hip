@@ -7,311 +7,15 @@
 // kernel adds the chunk sums in chunk order.  Results are independent of
 // scheduling: every path's random stream is keyed by (seed, stream, pixel,
 // sample) and the reduction order is fixed.
-#include "r2_internal.hpp"
-#include "r2_pathsm.hpp"
-#include <algorithm>
-#include <cstdio>
+#define R2_PREC_MODE 0
+#define R2K_NS r2k_fp32
+#include "r2_render_kernels.hpp"
 
-namespace {
 using namespace r2;
 
-__device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int& y) {
-  if ((W & 7) == 0 && (H & 7) == 0) {
-    const uint32_t tile = q >> 6, in = q & 63u;
-    const uint32_t tx = tile % (uint32_t)(W >> 3), ty = tile / (uint32_t)(W >> 3);
-    x = (int)(tx * 8u + (in & 7u));
-    y = (int)(ty * 8u + (in >> 3));
-  } else {
-    x = (int)(q % (uint32_t)W);
-    y = (int)(q / (uint32_t)W);
-  }
-}
-
-// Persistent path state machine (see r2_pathsm.hpp).  Render mode: item =
-// (sample chunk, pixel), output = chunk sums.  Probe mode (probe_pix != null):
-// item = one (pixel, sample) path, output = its radiance and lookup counters.
-#ifndef R2_EV_MODE
-#define R2_EV_MODE 0      // 0: every event lane advances once per iteration; 1: most populated state first
-#endif
-#ifndef R2_EV_ITERS
-#define R2_EV_ITERS 12    // event iterations per scheduling round
-#endif
-#ifndef R2_TR_MIN_LANES
-#define R2_TR_MIN_LANES 16  // leave the traversal round when fewer lanes are still traversing
-#endif
-#ifndef R2_TR_MIN_ROUNDS
-#define R2_TR_MIN_ROUNDS 8
-#endif
-#ifdef R2_SM_STATS
-// debug-build scheduling statistics: [0] outer iterations, [1] event iterations,
-// [2] traversal rounds, [3] lanes active in traversal rounds, [16+2s] executions
-// of state s, [17+2s] lanes in those executions
-__device__ unsigned long long g_sm_stats[64];
-#define SM_STAT(i, v) do { if (lane == 0) atomicAdd(&g_sm_stats[(i)], (unsigned long long)(v)); } while (0)
-#else
-#define SM_STAT(i, v) do {} while (0)
-#endif
-// end of one path: radiance, NaN guard, accumulation into the item or probe output
-template <typename Acc>
-__device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, int chunk,
-                                            Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
-                                            float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
-  v3 L = p.L;
-  if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))
-    L = mk(0.f, 0.f, 0.f);
-  else
-    L = L / (float)s.spp;
-  if (probe_pix) {
-    probe_out[3 * p.item] = L.x;
-    probe_out[3 * p.item + 1] = L.y;
-    probe_out[3 * p.item + 2] = L.z;
-    if (probe_cnt)
-      for (int c = 0; c < kNumCnt; ++c) probe_cnt[p.item * kNumCnt + c] = cnt.c[c];
-#pragma unroll
-    for (int c = 0; c < kNumCnt; ++c) cnt.c[c] = 0;
-    p.stage = ST_FETCH;
-    return;
-  }
-  if constexpr (sizeof(Acc) == 8) {
-    p.dax += (double)L.x;
-    p.day += (double)L.y;
-    p.daz += (double)L.z;
-  } else {
-    p.ax += L.x;
-    p.ay += L.y;
-    p.az += L.z;
-  }
-  ++p.s_cur;
-  if (p.s_cur < p.s_end) {
-    p.stage = ST_START;
-  } else {
-    const uint32_t c = (uint32_t)(p.item / npix);
-    Acc* o = partial + ((size_t)c * npix + p.pix) * 3;
-    if constexpr (sizeof(Acc) == 8) {
-      o[0] = p.dax;
-      o[1] = p.day;
-      o[2] = p.daz;
-    } else {
-      o[0] = p.ax;
-      o[1] = p.ay;
-      o[2] = p.az;
-    }
-    p.stage = ST_FETCH;
-  }
-}
-
-// One scheduling round of a wavefront: shading events, batched across the
-// wavefront, then traversal rounds in which every traversing lane advances one
-// DDA / tracking step.
-template <bool kSingle, bool kCount, typename Acc>
-__device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt, int lane, uint32_t npix, int chunk,
-                                          Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
-                                          float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
-  // ---- shading events, batched across the wavefront
-#if R2_EV_MODE == 0
-  for (int k = 0; k < R2_EV_ITERS; ++k) {
-    const bool ev = p.stage >= ST_START;
-    if (!__any(ev)) break;
-    SM_STAT(1, 1);
-#ifdef R2_SM_STATS
-    for (int st = ST_START; st <= ST_FINISH; ++st) {
-      const int c = __popcll(__ballot(p.stage == st));
-      if (c) { SM_STAT(16 + 2 * st, 1); SM_STAT(17 + 2 * st, c); }
-    }
-#endif
-    if (ev) {
-      if (p.stage == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-      else event_step<kSingle, kCount>(s, p, cnt, p.stage);
-    }
-  }
-#else
-  // run the most populated event state each iteration (wave-uniform branch)
-  for (int k = 0; k < R2_EV_ITERS; ++k) {
-    int best = -1, bestc = 0;
-    for (int st = ST_START; st <= ST_FINISH; ++st) {
-      const int c = __popcll(__ballot(p.stage == st));
-      if (c > bestc) {
-        bestc = c;
-        best = st;
-      }
-    }
-    if (best < 0) break;
-    if (p.stage == best) {
-      if (best == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-      else event_step<kSingle, kCount>(s, p, cnt, best);
-    }
-  }
-#endif
-  // ---- traversal rounds: every traversing lane advances one DDA / tracking step
-  for (int k = 0; k < 256; ++k) {
-    const uint64_t tm = __ballot(p.stage == ST_TRAV);
-    if (tm == 0) break;
-    if (k >= R2_TR_MIN_ROUNDS && __popcll(tm) < R2_TR_MIN_LANES) break;
-    SM_STAT(2, 1);
-    SM_STAT(3, __popcll(tm));
-    if (p.stage == ST_TRAV) {
-      if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
-    }
-  }
-}
-
-#ifndef R2_SM_WAVES
-#define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for (measured best)
-#endif
-template <bool kSingle, bool kCount, typename Acc>
-__global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __restrict__ sp, uint64_t* __restrict__ counter,
-                                                  uint64_t total, int chunk, Acc* __restrict__ partial,
-                                                  const int32_t* __restrict__ probe_pix,
-                                                  const int32_t* __restrict__ probe_smp, float* __restrict__ probe_out,
-                                                  uint32_t* __restrict__ probe_cnt) {
-  const DScene& s = *sp;
-  const int lane = threadIdx.x & 63;
-  const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
-  PathLane p;
-  p.stage = ST_FETCH;
-  p.item = 0;
-  Cnt cnt;
-#pragma unroll
-  for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
-  for (int guard = 0; guard < (1 << 24); ++guard) {
-    // ---- work fetch: one atomic per wavefront, items handed out by prefix rank
-    const uint64_t need = __ballot(p.stage == ST_FETCH);
-    if (need) {
-      const int n = __popcll(need);
-      const int leader = __ffsll((unsigned long long)need) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)n);
-      base = __shfl(base, leader);
-      if (p.stage == ST_FETCH) {
-        const uint64_t item = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-        if (item < total) {
-          p.item = item;
-          p.ax = p.ay = p.az = 0.f;
-          p.dax = p.day = p.daz = 0.0;
-          if (probe_pix) {
-            p.pix = (uint32_t)probe_pix[item];
-            p.s_cur = probe_smp[item];
-            p.s_end = p.s_cur + 1;
-          } else {
-            const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
-            int x, y;
-            item_pixel(q, s.W, s.H, x, y);
-            p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-            p.s_cur = (int)c * chunk;
-            p.s_end = min(s.spp, p.s_cur + chunk);
-          }
-          p.px = (int)(p.pix % (uint32_t)s.W);
-          p.py = (int)(p.pix / (uint32_t)s.W);
-          p.stage = ST_START;
-        } else {
-          p.stage = ST_IDLE;
-        }
-      }
-    }
-    if (__ballot(p.stage != ST_IDLE) == 0) break;
-    SM_STAT(0, 1);
-    run_round<kSingle, kCount, Acc>(s, p, cnt, lane, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-  }
-}
-
-// Batched form for the render server: one launch serves several requests, each
-// an entry with its own scene, item range and chunk-sum buffer.  A wavefront
-// serves one entry at a time (the scene stays wave-uniform); when the entry is
-// exhausted and all its lanes are idle it moves to the next entry.  Per item the
-// computation is the one of k_paths_sm, so results do not depend on the mix.
-struct BatchEntry {
-  const DScene* scene;  // device copy of the request's scene
-  void* partial;        // chunk sums of the request (Acc[nchunks * npix * 3])
-  uint64_t first;       // first item of this slice in the request's item space
-  uint64_t count;       // items of this slice
-  int chunk;            // samples per item
-  int pad;
-};
-
-template <bool kSingle, typename Acc>
-__global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEntry* __restrict__ ents, int n_ent,
-                                                                  uint64_t* __restrict__ counters) {
-  const int lane = threadIdx.x & 63;
-  int e = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % (unsigned)n_ent));
-  int misses = 0;
-  PathLane p;
-  p.stage = ST_FETCH;
-  p.item = 0;
-  Cnt cnt;
-#pragma unroll
-  for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
-  for (int guard = 0; guard < (1 << 24); ++guard) {
-    const BatchEntry& E = ents[e];
-    const DScene& s = *E.scene;
-    const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
-    const int chunk = E.chunk;
-    const uint64_t need = __ballot(p.stage == ST_FETCH);
-    if (need) {
-      const int n = __popcll(need);
-      const int leader = __ffsll((unsigned long long)need) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd((unsigned long long*)&counters[e], (unsigned long long)n);
-      base = __shfl(base, leader);
-      if (p.stage == ST_FETCH) {
-        const uint64_t local = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-        if (local < E.count) {
-          const uint64_t item = E.first + local;
-          p.item = item;
-          p.ax = p.ay = p.az = 0.f;
-          p.dax = p.day = p.daz = 0.0;
-          const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
-          int x, y;
-          item_pixel(q, s.W, s.H, x, y);
-          p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-          p.s_cur = (int)c * chunk;
-          p.s_end = min(s.spp, p.s_cur + chunk);
-          p.px = (int)(p.pix % (uint32_t)s.W);
-          p.py = (int)(p.pix / (uint32_t)s.W);
-          p.stage = ST_START;
-        } else {
-          p.stage = ST_IDLE;
-        }
-      }
-    }
-    if (__ballot(p.stage != ST_IDLE) == 0) {
-      // this entry has no work left for the wavefront: try the next one
-      if (++misses >= n_ent) break;
-      e = (e + 1 == n_ent) ? 0 : e + 1;
-      p.stage = ST_FETCH;
-      continue;
-    }
-    misses = 0;
-    run_round<kSingle, false, Acc>(s, p, cnt, lane, npix, chunk, (Acc*)E.partial, nullptr, nullptr, nullptr);
-  }
-}
-
-// out[p] = sum over chunks in chunk order (FP32), or a Neumaier-compensated
-// FP64 sum of FP64 chunk sums, rounded once to FP32.
-template <typename Acc>
-__global__ void k_reduce(const Acc* __restrict__ partial, int nchunks, uint32_t n3, float* __restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n3) return;
-  if constexpr (sizeof(Acc) == 4) {
-    float acc = 0.f;
-    for (int c = 0; c < nchunks; ++c) acc += partial[(size_t)c * n3 + i];
-    out[i] = acc;
-  } else {
-    double sum = 0.0, comp = 0.0;
-    for (int c = 0; c < nchunks; ++c) {
-      const double v = partial[(size_t)c * n3 + i];
-      const double t = sum + v;
-      comp += (fabs(sum) >= fabs(v)) ? (sum - t) + v : (v - t) + sum;
-      sum = t;
-    }
-    out[i] = (float)(sum + comp);
-  }
-}
-
-int g_num_cus = 0;
-
-}  // namespace
-
 namespace r2host {
+
+static int g_num_cus = 0;
 
 static int num_cus(r2_ctx* ctx) {
   if (!g_num_cus) {
@@ -322,14 +26,28 @@ static int num_cus(r2_ctx* ctx) {
   return g_num_cus;
 }
 
-template <bool kSingle, bool kCount, typename Acc>
-static void launch_sm(r2_ctx* ctx, int blocks, uint64_t total, int chunk, Acc* partial, const int32_t* pp,
-                      const int32_t* ps, float* po, uint32_t* pc) {
-  hipLaunchKernelGGL((k_paths_sm<kSingle, kCount, Acc>), dim3(blocks), dim3(256), 0, ctx->stream, ctx->d_scene,
-                     (uint64_t*)ctx->d_counter, total, chunk, partial, pp, ps, po, pc);
+// record precision of a scene: 0 FP32, 1 packed; -1 when its instances mix both
+static int scene_precision(const r2::DScene& s) {
+  int p = -2;
+  for (int i = 0; i < s.n_inst; ++i) {
+    const int q = s.grid[s.inst[i].grid].precision == 1 ? 1 : 0;
+    if (p == -2) p = q;
+    else if (p != q) return -1;
+  }
+  return p < 0 ? 0 : p;
 }
 
-size_t batch_entry_bytes() { return sizeof(BatchEntry); }
+static void launch_sm(r2_ctx* ctx, int prec, int blocks, uint64_t total, int chunk, void* partial, bool fp64, bool single,
+                      bool count, const int32_t* pp, const int32_t* ps, float* po, uint32_t* pc) {
+  if (prec == 1)
+    r2k_packed::launch_sm(ctx->stream, blocks, ctx->d_scene, (uint64_t*)ctx->d_counter, total, chunk, partial, fp64,
+                          single, count, pp, ps, po, pc);
+  else
+    r2k_fp32::launch_sm(ctx->stream, blocks, ctx->d_scene, (uint64_t*)ctx->d_counter, total, chunk, partial, fp64,
+                        single, count, pp, ps, po, pc);
+}
+
+size_t batch_entry_bytes() { return sizeof(r2::BatchEntry); }
 
 // samples per work item: about 2^21 items per frame
 void render_chunking(const r2::DScene& scene, int* chunk_out, int* nchunks_out) {
@@ -342,12 +60,7 @@ void render_chunking(const r2::DScene& scene, int* chunk_out, int* nchunks_out) 
 }
 
 int launch_reduce(hipStream_t st, const void* partial, bool fp64, int nchunks, uint32_t n3, float* out) {
-  if (fp64)
-    hipLaunchKernelGGL(k_reduce<double>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const double*)partial, nchunks, n3,
-                       out);
-  else
-    hipLaunchKernelGGL(k_reduce<float>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const float*)partial, nchunks, n3,
-                       out);
+  r2k_fp32::launch_reduce_k(st, partial, fp64, nchunks, n3, out);
   return hipGetLastError() == hipSuccess ? R2_OK : R2_ERR_HIP;
 }
 
@@ -357,12 +70,12 @@ int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& ite
   // one launch per (single instance, accumulator type) group
   std::vector<BatchEntry> host(items.size());
   int off = 0;
-  for (int g = 0; g < 4; ++g) {
-    const bool single = (g & 1) != 0, fp64 = (g & 2) != 0;
+  for (int g = 0; g < 8; ++g) {
+    const bool single = (g & 1) != 0, fp64 = (g & 2) != 0, packed = (g & 4) != 0;
     const int first = off;
     uint64_t total = 0;
     for (const BatchSlice& b : items) {
-      if (b.single != single || b.fp64 != fp64 || b.count == 0) continue;
+      if (b.single != single || b.fp64 != fp64 || b.packed != packed || b.count == 0) continue;
       BatchEntry& e = host[off++];
       e.scene = b.d_scene;
       e.partial = b.d_partial;
@@ -378,14 +91,8 @@ int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& ite
     R2_HIP(ctx, hipMemcpyAsync(de, &host[first], sizeof(BatchEntry) * n, hipMemcpyHostToDevice, st));
     R2_HIP(ctx, hipMemsetAsync(d_counters + first, 0, sizeof(uint64_t) * n, st));
     const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total + 255) / 256);
-    if (single && fp64)
-      hipLaunchKernelGGL((k_paths_batch<true, double>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
-    else if (single)
-      hipLaunchKernelGGL((k_paths_batch<true, float>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
-    else if (fp64)
-      hipLaunchKernelGGL((k_paths_batch<false, double>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
-    else
-      hipLaunchKernelGGL((k_paths_batch<false, float>), dim3(blocks), dim3(256), 0, st, de, n, d_counters + first);
+    if (packed) r2k_packed::launch_batch_group(st, blocks, de, n, d_counters + first, single, fp64);
+    else r2k_fp32::launch_batch_group(st, blocks, de, n, d_counters + first, single, fp64);
     R2_HIP(ctx, hipGetLastError());
   }
   return R2_OK;
@@ -406,25 +113,14 @@ int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64)
   R2_HIP(ctx, hipMemsetAsync(ctx->d_counter, 0, 8, st));
   const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total + 255) / 256);
   const bool single = scene.n_inst == 1;
+  const int prec = scene_precision(scene);
+  if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
   R2_HIP(ctx, hipEventRecord(ctx->ev0, st));
-  if (fp64) {
-    double* part = (double*)ctx->scratch;
-    if (single) launch_sm<true, false, double>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
-    else launch_sm<false, false, double>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
-  } else {
-    float* part = (float*)ctx->scratch;
-    if (single) launch_sm<true, false, float>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
-    else launch_sm<false, false, float>(ctx, blocks, total, chunk, part, nullptr, nullptr, nullptr, nullptr);
-  }
+  launch_sm(ctx, prec, blocks, total, chunk, ctx->scratch, fp64, single, false, nullptr, nullptr, nullptr, nullptr);
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipEventRecord(ctx->ev1, st));
   const uint32_t n3 = (uint32_t)(npix * 3);
-  if (fp64)
-    hipLaunchKernelGGL(k_reduce<double>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const double*)ctx->scratch, nchunks,
-                       n3, d_out);
-  else
-    hipLaunchKernelGGL(k_reduce<float>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const float*)ctx->scratch, nchunks,
-                       n3, d_out);
+  r2k_fp32::launch_reduce_k(st, ctx->scratch, fp64, nchunks, n3, d_out);
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipStreamSynchronize(st));
   float ms = 0.f;
@@ -433,13 +129,13 @@ int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64)
 #ifdef R2_SM_STATS
   {
     unsigned long long st_host[64];
-    R2_HIP(ctx, hipMemcpyFromSymbol(st_host, HIP_SYMBOL(g_sm_stats), sizeof(st_host)));
+    R2_HIP(ctx, hipMemcpyFromSymbol(st_host, HIP_SYMBOL(r2k_fp32::g_sm_stats), sizeof(st_host)));
     fprintf(stderr, "[sm-stats] paths %llu outer %llu ev_iters %llu trav_rounds %llu trav_lanes %llu\n",
             (unsigned long long)(npix * scene.spp), st_host[0], st_host[1], st_host[2], st_host[3]);
     for (int k = ST_START; k <= ST_FINISH; ++k)
       fprintf(stderr, "[sm-stats] state %d: execs %llu lanes %llu\n", k, st_host[16 + 2 * k], st_host[17 + 2 * k]);
     const unsigned long long z[64] = {0};
-    R2_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(g_sm_stats), z, sizeof(z)));
+    R2_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(r2k_fp32::g_sm_stats), z, sizeof(z)));
   }
 #endif
   ctx->stats.last_paths = (int64_t)(npix * scene.spp);
@@ -453,10 +149,9 @@ int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_p
   R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(r2::DScene), hipMemcpyHostToDevice, st));
   R2_HIP(ctx, hipMemsetAsync(ctx->d_counter, 0, 8, st));
   const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, ((uint64_t)n + 255) / 256);
-  if (scene.n_inst == 1)
-    launch_sm<true, true, float>(ctx, blocks, (uint64_t)n, 1, nullptr, d_pix, d_smp, d_out, d_cnt);
-  else
-    launch_sm<false, true, float>(ctx, blocks, (uint64_t)n, 1, nullptr, d_pix, d_smp, d_out, d_cnt);
+  const int prec = scene_precision(scene);
+  if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
+  launch_sm(ctx, prec, blocks, (uint64_t)n, 1, nullptr, false, scene.n_inst == 1, true, d_pix, d_smp, d_out, d_cnt);
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipStreamSynchronize(st));
   return R2_OK;

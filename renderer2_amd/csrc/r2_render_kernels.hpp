@@ -1,0 +1,350 @@
+// Integrator kernels, included once per record precision: the including file
+// defines R2_PREC_MODE (0: FP32 records, 1: packed records) and R2K_NS, the
+// namespace of that instantiation.
+// Volume integrator kernels for gfx950.
+//
+// Work item = (sample chunk, pixel), items numbered so that 64 consecutive items
+// are one 8x8 pixel tile of one chunk (coherent primary rays).  A persistent
+// grid of wavefronts runs the path state machine of r2_pathsm.hpp; each lane
+// runs the K samples of its item in order and writes the chunk sum; a second
+// kernel adds the chunk sums in chunk order.  Results are independent of
+// scheduling: every path's random stream is keyed by (seed, stream, pixel,
+// sample) and the reduction order is fixed.
+#pragma once
+#include "r2_internal.hpp"
+#include "r2_pathsm.hpp"
+#include <algorithm>
+#include <cstdio>
+
+namespace R2K_NS {
+using namespace r2;
+
+__device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int& y) {
+  if ((W & 7) == 0 && (H & 7) == 0) {
+    const uint32_t tile = q >> 6, in = q & 63u;
+    const uint32_t tx = tile % (uint32_t)(W >> 3), ty = tile / (uint32_t)(W >> 3);
+    x = (int)(tx * 8u + (in & 7u));
+    y = (int)(ty * 8u + (in >> 3));
+  } else {
+    x = (int)(q % (uint32_t)W);
+    y = (int)(q / (uint32_t)W);
+  }
+}
+
+// Persistent path state machine (see r2_pathsm.hpp).  Render mode: item =
+// (sample chunk, pixel), output = chunk sums.  Probe mode (probe_pix != null):
+// item = one (pixel, sample) path, output = its radiance and lookup counters.
+#ifndef R2_EV_MODE
+#define R2_EV_MODE 0      // 0: every event lane advances once per iteration; 1: most populated state first
+#endif
+#ifndef R2_EV_ITERS
+#define R2_EV_ITERS 12    // event iterations per scheduling round
+#endif
+#ifndef R2_TR_MIN_LANES
+#define R2_TR_MIN_LANES 16  // leave the traversal round when fewer lanes are still traversing
+#endif
+#ifndef R2_TR_MIN_ROUNDS
+#define R2_TR_MIN_ROUNDS 8
+#endif
+#ifdef R2_SM_STATS
+// debug-build scheduling statistics: [0] outer iterations, [1] event iterations,
+// [2] traversal rounds, [3] lanes active in traversal rounds, [16+2s] executions
+// of state s, [17+2s] lanes in those executions
+__device__ unsigned long long g_sm_stats[64];
+#define SM_STAT(i, v) do { if (lane == 0) atomicAdd(&g_sm_stats[(i)], (unsigned long long)(v)); } while (0)
+#else
+#define SM_STAT(i, v) do {} while (0)
+#endif
+// end of one path: radiance, NaN guard, accumulation into the item or probe output
+template <typename Acc>
+__device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, int chunk,
+                                            Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
+                                            float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
+  v3 L = p.L;
+  if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))
+    L = mk(0.f, 0.f, 0.f);
+  else
+    L = L / (float)s.spp;
+  if (probe_pix) {
+    probe_out[3 * p.item] = L.x;
+    probe_out[3 * p.item + 1] = L.y;
+    probe_out[3 * p.item + 2] = L.z;
+    if (probe_cnt)
+      for (int c = 0; c < kNumCnt; ++c) probe_cnt[p.item * kNumCnt + c] = cnt.c[c];
+#pragma unroll
+    for (int c = 0; c < kNumCnt; ++c) cnt.c[c] = 0;
+    p.stage = ST_FETCH;
+    return;
+  }
+  if constexpr (sizeof(Acc) == 8) {
+    p.dax += (double)L.x;
+    p.day += (double)L.y;
+    p.daz += (double)L.z;
+  } else {
+    p.ax += L.x;
+    p.ay += L.y;
+    p.az += L.z;
+  }
+  ++p.s_cur;
+  if (p.s_cur < p.s_end) {
+    p.stage = ST_START;
+  } else {
+    const uint32_t c = (uint32_t)(p.item / npix);
+    Acc* o = partial + ((size_t)c * npix + p.pix) * 3;
+    if constexpr (sizeof(Acc) == 8) {
+      o[0] = p.dax;
+      o[1] = p.day;
+      o[2] = p.daz;
+    } else {
+      o[0] = p.ax;
+      o[1] = p.ay;
+      o[2] = p.az;
+    }
+    p.stage = ST_FETCH;
+  }
+}
+
+// One scheduling round of a wavefront: shading events, batched across the
+// wavefront, then traversal rounds in which every traversing lane advances one
+// DDA / tracking step.
+template <bool kSingle, bool kCount, typename Acc>
+__device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt, int lane, uint32_t npix, int chunk,
+                                          Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
+                                          float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
+  // ---- shading events, batched across the wavefront
+#if R2_EV_MODE == 0
+  for (int k = 0; k < R2_EV_ITERS; ++k) {
+    const bool ev = p.stage >= ST_START;
+    if (!__any(ev)) break;
+    SM_STAT(1, 1);
+#ifdef R2_SM_STATS
+    for (int st = ST_START; st <= ST_FINISH; ++st) {
+      const int c = __popcll(__ballot(p.stage == st));
+      if (c) { SM_STAT(16 + 2 * st, 1); SM_STAT(17 + 2 * st, c); }
+    }
+#endif
+    if (ev) {
+      if (p.stage == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+      else event_step<kSingle, kCount>(s, p, cnt, p.stage);
+    }
+  }
+#else
+  // run the most populated event state each iteration (wave-uniform branch)
+  for (int k = 0; k < R2_EV_ITERS; ++k) {
+    int best = -1, bestc = 0;
+    for (int st = ST_START; st <= ST_FINISH; ++st) {
+      const int c = __popcll(__ballot(p.stage == st));
+      if (c > bestc) {
+        bestc = c;
+        best = st;
+      }
+    }
+    if (best < 0) break;
+    if (p.stage == best) {
+      if (best == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+      else event_step<kSingle, kCount>(s, p, cnt, best);
+    }
+  }
+#endif
+  // ---- traversal rounds: every traversing lane advances one DDA / tracking step
+  for (int k = 0; k < 256; ++k) {
+    const uint64_t tm = __ballot(p.stage == ST_TRAV);
+    if (tm == 0) break;
+    if (k >= R2_TR_MIN_ROUNDS && __popcll(tm) < R2_TR_MIN_LANES) break;
+    SM_STAT(2, 1);
+    SM_STAT(3, __popcll(tm));
+    if (p.stage == ST_TRAV) {
+      if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
+    }
+  }
+}
+
+#ifndef R2_SM_WAVES
+#define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for (measured best)
+#endif
+template <bool kSingle, bool kCount, typename Acc>
+__global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __restrict__ sp, uint64_t* __restrict__ counter,
+                                                  uint64_t total, int chunk, Acc* __restrict__ partial,
+                                                  const int32_t* __restrict__ probe_pix,
+                                                  const int32_t* __restrict__ probe_smp, float* __restrict__ probe_out,
+                                                  uint32_t* __restrict__ probe_cnt) {
+  const DScene& s = *sp;
+  const int lane = threadIdx.x & 63;
+  const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
+  PathLane p;
+  p.stage = ST_FETCH;
+  p.item = 0;
+  Cnt cnt;
+#pragma unroll
+  for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
+  for (int guard = 0; guard < (1 << 24); ++guard) {
+    // ---- work fetch: one atomic per wavefront, items handed out by prefix rank
+    const uint64_t need = __ballot(p.stage == ST_FETCH);
+    if (need) {
+      const int n = __popcll(need);
+      const int leader = __ffsll((unsigned long long)need) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)n);
+      base = __shfl(base, leader);
+      if (p.stage == ST_FETCH) {
+        const uint64_t item = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (item < total) {
+          p.item = item;
+          p.ax = p.ay = p.az = 0.f;
+          p.dax = p.day = p.daz = 0.0;
+          if (probe_pix) {
+            p.pix = (uint32_t)probe_pix[item];
+            p.s_cur = probe_smp[item];
+            p.s_end = p.s_cur + 1;
+          } else {
+            const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
+            int x, y;
+            item_pixel(q, s.W, s.H, x, y);
+            p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+            p.s_cur = (int)c * chunk;
+            p.s_end = min(s.spp, p.s_cur + chunk);
+          }
+          p.px = (int)(p.pix % (uint32_t)s.W);
+          p.py = (int)(p.pix / (uint32_t)s.W);
+          p.stage = ST_START;
+        } else {
+          p.stage = ST_IDLE;
+        }
+      }
+    }
+    if (__ballot(p.stage != ST_IDLE) == 0) break;
+    SM_STAT(0, 1);
+    run_round<kSingle, kCount, Acc>(s, p, cnt, lane, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+  }
+}
+
+// Batched form for the render server: one launch serves several requests, each
+// an entry with its own scene, item range and chunk-sum buffer.  A wavefront
+// serves one entry at a time (the scene stays wave-uniform); when the entry is
+// exhausted and all its lanes are idle it moves to the next entry.  Per item the
+// computation is the one of k_paths_sm, so results do not depend on the mix.
+
+template <bool kSingle, typename Acc>
+__global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEntry* __restrict__ ents, int n_ent,
+                                                                  uint64_t* __restrict__ counters) {
+  const int lane = threadIdx.x & 63;
+  int e = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % (unsigned)n_ent));
+  int misses = 0;
+  PathLane p;
+  p.stage = ST_FETCH;
+  p.item = 0;
+  Cnt cnt;
+#pragma unroll
+  for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
+  for (int guard = 0; guard < (1 << 24); ++guard) {
+    const BatchEntry& E = ents[e];
+    const DScene& s = *E.scene;
+    const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
+    const int chunk = E.chunk;
+    const uint64_t need = __ballot(p.stage == ST_FETCH);
+    if (need) {
+      const int n = __popcll(need);
+      const int leader = __ffsll((unsigned long long)need) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd((unsigned long long*)&counters[e], (unsigned long long)n);
+      base = __shfl(base, leader);
+      if (p.stage == ST_FETCH) {
+        const uint64_t local = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+        if (local < E.count) {
+          const uint64_t item = E.first + local;
+          p.item = item;
+          p.ax = p.ay = p.az = 0.f;
+          p.dax = p.day = p.daz = 0.0;
+          const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
+          int x, y;
+          item_pixel(q, s.W, s.H, x, y);
+          p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+          p.s_cur = (int)c * chunk;
+          p.s_end = min(s.spp, p.s_cur + chunk);
+          p.px = (int)(p.pix % (uint32_t)s.W);
+          p.py = (int)(p.pix / (uint32_t)s.W);
+          p.stage = ST_START;
+        } else {
+          p.stage = ST_IDLE;
+        }
+      }
+    }
+    if (__ballot(p.stage != ST_IDLE) == 0) {
+      // this entry has no work left for the wavefront: try the next one
+      if (++misses >= n_ent) break;
+      e = (e + 1 == n_ent) ? 0 : e + 1;
+      p.stage = ST_FETCH;
+      continue;
+    }
+    misses = 0;
+    run_round<kSingle, false, Acc>(s, p, cnt, lane, npix, chunk, (Acc*)E.partial, nullptr, nullptr, nullptr);
+  }
+}
+
+// out[p] = sum over chunks in chunk order (FP32), or a Neumaier-compensated
+// FP64 sum of FP64 chunk sums, rounded once to FP32.
+template <typename Acc>
+__global__ void k_reduce(const Acc* __restrict__ partial, int nchunks, uint32_t n3, float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  if constexpr (sizeof(Acc) == 4) {
+    float acc = 0.f;
+    for (int c = 0; c < nchunks; ++c) acc += partial[(size_t)c * n3 + i];
+    out[i] = acc;
+  } else {
+    double sum = 0.0, comp = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+      const double v = partial[(size_t)c * n3 + i];
+      const double t = sum + v;
+      comp += (fabs(sum) >= fabs(v)) ? (sum - t) + v : (v - t) + sum;
+      sum = t;
+    }
+    out[i] = (float)(sum + comp);
+  }
+}
+
+
+
+// host-side launchers of this instantiation
+void launch_sm(hipStream_t st, int blocks, const DScene* d_scene, uint64_t* counter, uint64_t total, int chunk,
+               void* partial, bool fp64, bool single, bool count, const int32_t* pp, const int32_t* ps, float* po,
+               uint32_t* pc) {
+#define R2K_LAUNCH(S, C, A)                                                                                        \
+  hipLaunchKernelGGL((k_paths_sm<S, C, A>), dim3(blocks), dim3(256), 0, st, d_scene, counter, total, chunk, (A*)partial, \
+                     pp, ps, po, pc)
+  if (count) {
+    if (single) R2K_LAUNCH(true, true, float);
+    else R2K_LAUNCH(false, true, float);
+  } else if (fp64) {
+    if (single) R2K_LAUNCH(true, false, double);
+    else R2K_LAUNCH(false, false, double);
+  } else {
+    if (single) R2K_LAUNCH(true, false, float);
+    else R2K_LAUNCH(false, false, float);
+  }
+#undef R2K_LAUNCH
+}
+
+void launch_batch_group(hipStream_t st, int blocks, const void* ents, int n, uint64_t* counters, bool single,
+                        bool fp64) {
+  const BatchEntry* de = (const BatchEntry*)ents;
+  if (single && fp64)
+    hipLaunchKernelGGL((k_paths_batch<true, double>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+  else if (single)
+    hipLaunchKernelGGL((k_paths_batch<true, float>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+  else if (fp64)
+    hipLaunchKernelGGL((k_paths_batch<false, double>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+  else
+    hipLaunchKernelGGL((k_paths_batch<false, float>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+}
+
+void launch_reduce_k(hipStream_t st, const void* partial, bool fp64, int nchunks, uint32_t n3, float* out) {
+  if (fp64)
+    hipLaunchKernelGGL(k_reduce<double>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const double*)partial, nchunks, n3,
+                       out);
+  else
+    hipLaunchKernelGGL(k_reduce<float>, dim3((n3 + 255) / 256), dim3(256), 0, st, (const float*)partial, nchunks, n3,
+                       out);
+}
+
+}  // namespace R2K_NS

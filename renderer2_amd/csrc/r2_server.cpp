@@ -41,7 +41,7 @@ struct SrvReq {
   float* out = nullptr;
   r2::DScene scene{};
   int chunk = 1, nchunks = 1;
-  bool fp64 = false, single = true;
+  bool fp64 = false, single = true, packed = false;
   uint64_t total_items = 0, next_item = 0;
   size_t bytes = 0;
   void* d_mem = nullptr;
@@ -169,7 +169,7 @@ void scheduler(r2_server* s) {
       uint64_t items = std::max<uint64_t>(64 * 64, share / (uint64_t)r.chunk);
       items = (items + 63) & ~(uint64_t)63;
       items = std::min<uint64_t>(items, r.total_items - r.next_item);
-      r2host::BatchSlice b{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.fp64};
+      r2host::BatchSlice b{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.fp64, r.packed};
       slice.push_back(b);
       took.push_back({r.id, items});
     }
@@ -244,6 +244,15 @@ extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
   r2host::render_chunking(r->scene, &r->chunk, &r->nchunks);
   r->fp64 = (r->params.flags & R2_FLAG_FP64_ACCUM) != 0;
   r->single = r->scene.n_inst == 1;
+  {
+    int prec = -2;
+    for (int i = 0; i < r->scene.n_inst; ++i) {
+      const int q = r->scene.grid[r->scene.inst[i].grid].precision == 1 ? 1 : 0;
+      if (prec == -2) prec = q;
+      else if (prec != q) return R2_ERR_UNSUPPORTED;  // instances mix FP32 and packed grids
+    }
+    r->packed = prec == 1;
+  }
   const uint64_t npix = (uint64_t)r->scene.W * r->scene.H;
   r->total_items = (uint64_t)r->nchunks * npix;
   r->bytes = align256(sizeof(r2::DScene)) + align256((size_t)r->nchunks * npix * 3 * (r->fp64 ? 8 : 4)) +

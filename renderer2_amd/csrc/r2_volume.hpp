@@ -108,6 +108,23 @@ __device__ __forceinline__ Rec decode_record(uint4 a, uint4 b) {
   return r;
 }
 
+// Record precision of the lookups: R2_PREC_MODE 0 compiles the FP32-record path
+// only, 1 the packed path only, 2 (default) both behind a runtime test.
+#ifndef R2_PREC_MODE
+#define R2_PREC_MODE 2
+#endif
+__device__ __forceinline__ bool grid_packed(const DGrid& g) {
+#if R2_PREC_MODE == 0
+  (void)g;
+  return false;
+#elif R2_PREC_MODE == 1
+  (void)g;
+  return true;
+#else
+  return g.precision == 1;
+#endif
+}
+
 // packed record (r2_packed.hpp): FP16 density / S / ior, u8 colours, octahedral normal
 __device__ __forceinline__ Rec decode_packed(uint4 h, uint2 c) {
   Rec r;
@@ -148,7 +165,7 @@ __device__ __forceinline__ Rec get_value(const DGrid& g, int x, int y, int z, Cn
   int32_t leaf = -1;
   if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny && (unsigned)z < (unsigned)g.nz)
     leaf = g.leaf_table[((size_t)(z >> 3) * g.nly + (y >> 3)) * g.nlx + (x >> 3)];
-  if (g.precision == 1) {
+  if (grid_packed(g)) {
     // background: the packed form of the FP32 background record (ior 1, normal
     // decoded from the zero code, normalize(-1,-1,-1), octahedral code 0xABAB)
     uint4 h = make_uint4(0u, 0u, 0u, 0x3C00u << 16);
@@ -177,42 +194,43 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
   const float psz = p.z * (float)g.size[2] + (float)g.ibb_min[2] - 0.5f;
   const int x = (int)floorf(psx), y = (int)floorf(psy), z = (int)floorf(psz);
   const float dx = psx - (float)x, dy = psy - (float)y, dz = psz - (float)z;
-  if (g.precision == 1) {
+  if (grid_packed(g)) {
     // packed form: density, S, colours and ior as one weighted sum of the eight
     // corners; normals interpolated pairwise along x, y, z with renormalisation,
-    // as operator*/operator+ do (filtered_data.hpp:115-151)
+    // as operator*/operator+ do (filtered_data.hpp:115-151); corners are taken in
+    // pairs so that at most two decoded records are live
     Rec o;
     o.density = 0.f;
 #pragma unroll
     for (int i = 0; i < 6; ++i) o.S[i] = 0.f;
     o.diffuse = o.specular = mk(0.f, 0.f, 0.f);
     o.ior = 0.f;
-    v3 nx0 = mk(0.f, 0.f, 0.f), ny0 = nx0, nz0 = nx0, nz1 = nx0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int cx = k & 1, cy = (k >> 1) & 1, cz = k >> 2;
-      const float w = (cx ? dx : 1.f - dx) * (cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz);
-      const Rec r = get_value<kCount>(g, x + cx, y + cy, z + cz, cnt);
+    auto acc = [&](float w, const Rec& r) {
       o.density += w * r.density;
 #pragma unroll
       for (int i = 0; i < 6; ++i) o.S[i] += w * r.S[i];
       o.diffuse = o.diffuse + w * r.diffuse;
       o.specular = o.specular + w * r.specular;
       o.ior += w * r.ior;
-      if (cx == 0) {
-        nx0 = r.normal;
-        continue;
+    };
+    auto nlerp = [](float t, v3 u, v3 v) { return normalize((1.f - t) * normalize(u) + t * normalize(v)); };
+    v3 d0 = mk(0.f, 0.f, 0.f), d1 = d0;
+#pragma unroll 1
+    for (int cz = 0; cz < 2; ++cz) {
+      v3 e = mk(0.f, 0.f, 0.f);
+#pragma unroll 1
+      for (int cy = 0; cy < 2; ++cy) {
+        const Rec ra = get_value<kCount>(g, x, y + cy, z + cz, cnt);
+        acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * (1.f - dx), ra);
+        const Rec rb = get_value<kCount>(g, x + 1, y + cy, z + cz, cnt);
+        acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * dx, rb);
+        const v3 f = nlerp(dx, ra.normal, rb.normal);
+        if (cy == 0) e = f;
+        else if (cz == 0) d0 = nlerp(dy, e, f);
+        else d1 = nlerp(dy, e, f);
       }
-      const v3 e = normalize((1.f - dx) * normalize(nx0) + dx * normalize(r.normal));
-      if (cy == 0) {
-        ny0 = e;
-        continue;
-      }
-      const v3 d = normalize((1.f - dy) * normalize(ny0) + dy * normalize(e));
-      if (cz == 0) nz0 = d;
-      else nz1 = d;
     }
-    o.normal = normalize((1.f - dz) * normalize(nz0) + dz * normalize(nz1));
+    o.normal = nlerp(dz, d0, d1);
     return o;
   }
   // same arithmetic as d00, d10 -> d0, d01, d11 -> d1, then z; evaluated so that
