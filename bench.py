@@ -75,7 +75,7 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", choices=["fp32", "packed"], default=None,
                     help="voxel record precision (default: the workload's own)")
     ap.add_argument("--requests", type=int, default=40, help="requests of the serving workload")
-    ap.add_argument("--rate", type=float, default=2.5, help="Poisson arrival rate (requests/s) of the serving workload")
+    ap.add_argument("--rate", type=float, default=1.5, help="Poisson arrival rate (requests/s) of the serving workload")
     ap.add_argument("--cpu-voxels", type=int, default=1_500_000,
                     help="random voxels per LOD of the filter CPU baseline sample")
     ap.add_argument("--copy-calib", type=int, default=0,

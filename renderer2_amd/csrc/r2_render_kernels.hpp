@@ -228,56 +228,55 @@ template <bool kSingle, typename Acc>
 __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEntry* __restrict__ ents, int n_ent,
                                                                   uint64_t* __restrict__ counters) {
   const int lane = threadIdx.x & 63;
-  int e = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % (unsigned)n_ent));
-  int misses = 0;
-  PathLane p;
-  p.stage = ST_FETCH;
-  p.item = 0;
+  const int e0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % (unsigned)n_ent));
   Cnt cnt;
 #pragma unroll
   for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
-  for (int guard = 0; guard < (1 << 24); ++guard) {
+  // one pass over the entries: an entry is left only when none of its items is
+  // left to hand out, so it never needs a second visit
+#pragma unroll 1
+  for (int v = 0; v < n_ent; ++v) {
+    const int e = __builtin_amdgcn_readfirstlane(e0 + v < n_ent ? e0 + v : e0 + v - n_ent);
     const BatchEntry& E = ents[e];
     const DScene& s = *E.scene;
     const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
     const int chunk = E.chunk;
-    const uint64_t need = __ballot(p.stage == ST_FETCH);
-    if (need) {
-      const int n = __popcll(need);
-      const int leader = __ffsll((unsigned long long)need) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd((unsigned long long*)&counters[e], (unsigned long long)n);
-      base = __shfl(base, leader);
-      if (p.stage == ST_FETCH) {
-        const uint64_t local = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-        if (local < E.count) {
-          const uint64_t item = E.first + local;
-          p.item = item;
-          p.ax = p.ay = p.az = 0.f;
-          p.dax = p.day = p.daz = 0.0;
-          const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
-          int x, y;
-          item_pixel(q, s.W, s.H, x, y);
-          p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-          p.s_cur = (int)c * chunk;
-          p.s_end = min(s.spp, p.s_cur + chunk);
-          p.px = (int)(p.pix % (uint32_t)s.W);
-          p.py = (int)(p.pix / (uint32_t)s.W);
-          p.stage = ST_START;
-        } else {
-          p.stage = ST_IDLE;
+    Acc* partial = (Acc*)E.partial;
+    PathLane p;
+    p.stage = ST_FETCH;
+    p.item = 0;
+    for (int guard = 0; guard < (1 << 24); ++guard) {
+      const uint64_t need = __ballot(p.stage == ST_FETCH);
+      if (need) {
+        const int n = __popcll(need);
+        const int leader = __ffsll((unsigned long long)need) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd((unsigned long long*)&counters[e], (unsigned long long)n);
+        base = __shfl(base, leader);
+        if (p.stage == ST_FETCH) {
+          const uint64_t local = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+          if (local < E.count) {
+            const uint64_t item = E.first + local;
+            p.item = item;
+            p.ax = p.ay = p.az = 0.f;
+            p.dax = p.day = p.daz = 0.0;
+            const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
+            int x, y;
+            item_pixel(q, s.W, s.H, x, y);
+            p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+            p.s_cur = (int)c * chunk;
+            p.s_end = min(s.spp, p.s_cur + chunk);
+            p.px = (int)(p.pix % (uint32_t)s.W);
+            p.py = (int)(p.pix / (uint32_t)s.W);
+            p.stage = ST_START;
+          } else {
+            p.stage = ST_IDLE;
+          }
         }
       }
+      if (__ballot(p.stage != ST_IDLE) == 0) break;
+      run_round<kSingle, false, Acc>(s, p, cnt, lane, npix, chunk, partial, nullptr, nullptr, nullptr);
     }
-    if (__ballot(p.stage != ST_IDLE) == 0) {
-      // this entry has no work left for the wavefront: try the next one
-      if (++misses >= n_ent) break;
-      e = (e + 1 == n_ent) ? 0 : e + 1;
-      p.stage = ST_FETCH;
-      continue;
-    }
-    misses = 0;
-    run_round<kSingle, false, Acc>(s, p, cnt, lane, npix, chunk, (Acc*)E.partial, nullptr, nullptr, nullptr);
   }
 }
 

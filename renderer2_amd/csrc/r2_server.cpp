@@ -9,9 +9,11 @@
 //   * runs slices: one batched persistent launch (k_paths_batch) whose work is
 //     an equal share of paths from every resident request, so requests that
 //     arrive while others run join at the next slice and their work overlaps
-//     on the device;
-//   * reduces a request's chunk sums when all its items are done, copies the
-//     frame to the caller's buffer and frees its memory, which admits the next.
+//     on the device; two slices are kept queued on the stream so the GPU does
+//     not idle while the host retires one and plans the next;
+//   * reduces a request's chunk sums when all its items are done (on a second
+//     stream, after the slice that finished it), copies the frame to the
+//     caller's buffer and frees its memory, which admits the next.
 // A request is split into the same work items (sample chunk x pixel) and the
 // same chunk-ordered reduction as a standalone r2_render, so its result is the
 // same alone and in any mix.
@@ -42,7 +44,9 @@ struct SrvReq {
   r2::DScene scene{};
   int chunk = 1, nchunks = 1;
   bool fp64 = false, single = true, packed = false;
-  uint64_t total_items = 0, next_item = 0;
+  uint64_t total_items = 0, next_item = 0;  // items in all, items handed to slices
+  uint64_t done_items = 0;                   // items of completed slices
+  int inflight = 0;                          // slices on the GPU that hold items of this request
   size_t bytes = 0;
   void* d_mem = nullptr;
   r2::DScene* d_scene = nullptr;
@@ -76,8 +80,9 @@ struct r2_server {
   int64_t paths_done = 0;
   bool have_first = false;
   Clock::time_point t_first, t_last;
-  void* d_ents = nullptr;
+  void* d_ents = nullptr;       // two halves: one per slice in flight
   uint64_t* d_counters = nullptr;
+  hipStream_t copy_stream = nullptr;  // chunk reduction and frame copy-out
 };
 
 namespace {
@@ -143,58 +148,111 @@ void admit_locked(r2_server* s) {
   }
 }
 
+struct Slice {
+  std::vector<std::pair<int64_t, uint64_t>> took;  // (request, items)
+  hipEvent_t done = nullptr;
+  int half = 0;
+  int rc = R2_OK;
+};
+
+// reduce a finished request on the copy stream (after its last slice) and copy it out
+void complete_locked(r2_server* s, SrvReq& r, hipEvent_t after, std::unique_lock<std::mutex>& lk) {
+  const uint32_t n3 = (uint32_t)((uint64_t)r.scene.W * r.scene.H * 3);
+  int rr = hipStreamWaitEvent(s->copy_stream, after, 0) == hipSuccess ? R2_OK : R2_ERR_HIP;
+  if (rr == R2_OK) rr = r2host::launch_reduce(s->copy_stream, r.d_partial, r.fp64, r.nchunks, n3, r.d_frame);
+  if (rr == R2_OK &&
+      hipMemcpyAsync(r.out, r.d_frame, (size_t)n3 * 4, hipMemcpyDeviceToHost, s->copy_stream) != hipSuccess)
+    rr = R2_ERR_HIP;
+  lk.unlock();
+  if (rr == R2_OK && hipStreamSynchronize(s->copy_stream) != hipSuccess) rr = R2_ERR_HIP;
+  lk.lock();
+  finish_locked(s, r, rr == R2_OK ? RQ_DONE : RQ_FAILED, rr);
+}
+
 void scheduler(r2_server* s) {
   (void)hipSetDevice(s->ctx->device);
+  std::deque<Slice> flight;  // at most two slices queued on the stream
+  hipEvent_t ev[2];
+  for (auto& e : ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  int next_half = 0;
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     admit_locked(s);
-    // drop cancelled active requests
+    // drop cancelled requests that no slice in flight still uses
     for (int64_t id : std::vector<int64_t>(s->active)) {
       SrvReq& r = *s->reqs[id];
-      if (r.cancel) finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
+      if (r.cancel && r.inflight == 0) finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
     }
-    if (s->active.empty()) {
-      if (s->stopping) break;
-      s->cv_work.wait(lk);
+    // plan and launch a slice while fewer than two are in flight: an equal share
+    // of paths per resident request with items left, in whole 64-item tiles
+    bool launched = false;
+    if (flight.size() < 2) {
+      std::vector<r2host::BatchSlice> slice;
+      Slice sl;
+      std::vector<int64_t> cand;
+      for (int64_t id : s->active) {
+        SrvReq& r = *s->reqs[id];
+        if (!r.cancel && r.next_item < r.total_items) cand.push_back(id);
+      }
+      const int n_act = std::min<int>((int)cand.size(), r2host::kMaxBatchEntries / 2);
+      if (n_act > 0) {
+        const uint64_t share = std::max<uint64_t>(1, (uint64_t)s->cfg.slice_paths / (uint64_t)n_act);
+        for (int i = 0; i < n_act; ++i) {
+          SrvReq& r = *s->reqs[cand[i]];
+          uint64_t items = std::max<uint64_t>(64 * 64, share / (uint64_t)r.chunk);
+          items = (items + 63) & ~(uint64_t)63;
+          items = std::min<uint64_t>(items, r.total_items - r.next_item);
+          slice.push_back(r2host::BatchSlice{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.fp64,
+                                             r.packed});
+          sl.took.push_back({r.id, items});
+          r.next_item += items;
+          ++r.inflight;
+        }
+        sl.half = next_half;
+        next_half ^= 1;
+        sl.done = ev[sl.half];
+        void* ents = (char*)s->d_ents + (size_t)sl.half * r2host::batch_entry_bytes() * (r2host::kMaxBatchEntries / 2);
+        uint64_t* ctr = s->d_counters + (size_t)sl.half * (r2host::kMaxBatchEntries / 2);
+        lk.unlock();
+        sl.rc = r2host::launch_batch(&s->shadow, s->shadow.stream, slice, ents, ctr);
+        if (sl.rc == R2_OK && hipEventRecord(sl.done, s->shadow.stream) != hipSuccess) sl.rc = R2_ERR_HIP;
+        lk.lock();
+        flight.push_back(std::move(sl));
+        launched = true;
+      }
+    }
+    if (flight.empty()) {
+      if (s->stopping && s->active.empty() && s->queue.empty()) break;
+      if (s->active.empty() || !launched) {
+        bool work = false;
+        for (int64_t id : s->active) work = work || s->reqs[id]->cancel;
+        if (!work && !s->stopping) s->cv_work.wait(lk);
+        else if (s->stopping && s->active.empty()) break;
+      }
       continue;
     }
-    // plan a slice: an equal share of paths per resident request, in whole
-    // 64-item tiles, at most kMaxBatchEntries requests
-    std::vector<r2host::BatchSlice> slice;
-    std::vector<std::pair<int64_t, uint64_t>> took;
-    const int n_act = std::min<int>((int)s->active.size(), r2host::kMaxBatchEntries);
-    const uint64_t share = std::max<uint64_t>(1, (uint64_t)s->cfg.slice_paths / (uint64_t)n_act);
-    for (int i = 0; i < n_act; ++i) {
-      SrvReq& r = *s->reqs[s->active[i]];
-      uint64_t items = std::max<uint64_t>(64 * 64, share / (uint64_t)r.chunk);
-      items = (items + 63) & ~(uint64_t)63;
-      items = std::min<uint64_t>(items, r.total_items - r.next_item);
-      r2host::BatchSlice b{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.fp64, r.packed};
-      slice.push_back(b);
-      took.push_back({r.id, items});
-    }
+    if (launched && flight.size() < 2) continue;  // keep the stream two slices deep
+    // retire the oldest slice
+    Slice sl = std::move(flight.front());
+    flight.pop_front();
     lk.unlock();
-    int rc = r2host::launch_batch(&s->shadow, s->shadow.stream, slice, s->d_ents, s->d_counters);
-    if (rc == R2_OK && hipStreamSynchronize(s->shadow.stream) != hipSuccess) rc = R2_ERR_HIP;
+    int rc = sl.rc;
+    if (rc == R2_OK && hipEventSynchronize(sl.done) != hipSuccess) rc = R2_ERR_HIP;
     lk.lock();
-    for (auto& t : took) {
+    for (auto& t : sl.took) {
       SrvReq& r = *s->reqs[t.first];
+      --r.inflight;
       if (rc != R2_OK) {
-        finish_locked(s, r, RQ_FAILED, rc);
+        if (r.inflight == 0) finish_locked(s, r, RQ_FAILED, rc);
+        else r.cancel = true;
         continue;
       }
-      r.next_item += t.second;
-      if (r.next_item < r.total_items || r.cancel) continue;
-      // all items done: reduce in chunk order and copy the frame out
-      const uint32_t n3 = (uint32_t)((uint64_t)r.scene.W * r.scene.H * 3);
-      int rr = r2host::launch_reduce(s->shadow.stream, r.d_partial, r.fp64, r.nchunks, n3, r.d_frame);
-      if (rr == R2_OK && hipMemcpyAsync(r.out, r.d_frame, (size_t)n3 * 4, hipMemcpyDeviceToHost, s->shadow.stream) !=
-                             hipSuccess)
-        rr = R2_ERR_HIP;
-      if (rr == R2_OK && hipStreamSynchronize(s->shadow.stream) != hipSuccess) rr = R2_ERR_HIP;
-      finish_locked(s, r, rr == R2_OK ? RQ_DONE : RQ_FAILED, rr);
+      r.done_items += t.second;
+      if (r.cancel || r.done_items < r.total_items) continue;
+      complete_locked(s, r, sl.done, lk);
     }
   }
+  for (auto& e : ev) (void)hipEventDestroy(e);
 }
 
 }  // namespace
@@ -207,14 +265,19 @@ extern "C" int r2_server_start(r2_ctx* ctx, const r2_server_config* cfg, r2_serv
   if (cfg) s->cfg = *cfg;
   if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = 8ll << 30;
   if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 16;
-  s->cfg.max_inflight = std::min(s->cfg.max_inflight, r2host::kMaxBatchEntries);
+  s->cfg.max_inflight = std::min(s->cfg.max_inflight, r2host::kMaxBatchEntries / 2);
   if (s->cfg.slice_paths <= 0) s->cfg.slice_paths = 1 << 24;
   s->shadow.device = ctx->device;
   if (hipSetDevice(ctx->device) != hipSuccess) return r2host::fail(ctx, R2_ERR_HIP, "server: hipSetDevice");
   if (hipStreamCreate(&s->shadow.stream) != hipSuccess) return r2host::fail(ctx, R2_ERR_HIP, "server: stream");
+  if (hipStreamCreate(&s->copy_stream) != hipSuccess) {
+    (void)hipStreamDestroy(s->shadow.stream);
+    return r2host::fail(ctx, R2_ERR_HIP, "server: stream");
+  }
   if (hipMalloc(&s->d_ents, r2host::batch_entry_bytes() * r2host::kMaxBatchEntries) != hipSuccess ||
       hipMalloc((void**)&s->d_counters, sizeof(uint64_t) * r2host::kMaxBatchEntries) != hipSuccess) {
     if (s->d_ents) (void)hipFree(s->d_ents);
+    (void)hipStreamDestroy(s->copy_stream);
     (void)hipStreamDestroy(s->shadow.stream);
     return r2host::fail(ctx, R2_ERR_OOM, "server: device tables");
   }
@@ -355,6 +418,7 @@ extern "C" int r2_server_stop(r2_server* s) {
   }
   (void)hipFree(s->d_ents);
   (void)hipFree(s->d_counters);
+  (void)hipStreamDestroy(s->copy_stream);
   (void)hipStreamDestroy(s->shadow.stream);
   delete s;
   return R2_OK;
