@@ -170,14 +170,16 @@ __global__ void k_task_mark(const unsigned long long* __restrict__ tasks, const 
   task_tri[begin[t]] = (uint32_t)t;
 }
 
-// one wave per 64-voxel task; kEmit = false counts, true writes keys
+// one wave per 64-voxel task, one voxel per lane.  kEmit = false: the task's
+// pair count goes to task_cnt[task]; kEmit = true: the pairs are written from
+// task_off[task] on, in lane order (no atomics: the scan of the counts places them)
 template <bool kEmit>
 __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, FConst c,
                                                const TriRange* __restrict__ rng,
                                                const unsigned long long* __restrict__ task_begin,
                                                const uint32_t* __restrict__ task_tri, unsigned long long n_tasks,
-                                               unsigned long long* __restrict__ counter,
-                                               unsigned long long* __restrict__ keys, unsigned long long cap) {
+                                               uint32_t* __restrict__ task_cnt, const uint32_t* __restrict__ task_off,
+                                               unsigned long long* __restrict__ keys) {
   const int lane = threadIdx.x & 63;
   const unsigned long long task =
       (unsigned long long)blockIdx.x * kWaves + (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -200,17 +202,13 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, F
     vid = (uint32_t)ix + (uint32_t)c.nx * ((uint32_t)iy + (uint32_t)c.ny * (uint32_t)iz);
   }
   const unsigned long long ball = __ballot(pass);
-  const int cnt = __popcll(ball);
-  if (cnt == 0) return;
   if (!kEmit) {
-    if (lane == 0) atomicAdd(counter, (unsigned long long)cnt);
+    if (lane == 0) task_cnt[task] = (uint32_t)__popcll(ball);
     return;
   }
-  unsigned long long base = 0;
-  if (lane == 0) base = atomicAdd(counter, (unsigned long long)cnt);
-  base = __shfl(base, 0);
-  const unsigned long long slot = base + (unsigned long long)__popcll(ball & ((1ull << lane) - 1ull));
-  if (pass && slot < cap) {
+  if (pass) {
+    const unsigned long long slot =
+        (unsigned long long)task_off[task] + (unsigned long long)__popcll(ball & ((1ull << lane) - 1ull));
     keys[slot] = ((unsigned long long)vid << 32) | (unsigned long long)(uint32_t)t;
   }
 }
@@ -611,7 +609,8 @@ struct Binning {
 // context pool slots of the binning buffers
 enum {
   kSlotRange = 0, kSlotTasks, kSlotBegin, kSlotCounter, kSlotTaskMark, kSlotTaskTri, kSlotKeys, kSlotKeys2,
-  kSlotVox, kSlotCand, kSlotAct, kSlotCnt, kSlotOff, kSlotTemp, kSlotRuns, kSlotRecords, kSlotIds
+  kSlotVox, kSlotCand, kSlotAct, kSlotCnt, kSlotOff, kSlotTemp, kSlotRuns, kSlotRecords, kSlotIds, kSlotTaskCnt,
+  kSlotTaskOff
 };
 
 template <class T>
@@ -627,11 +626,10 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   hipStream_t s = ctx->stream;
   const int n = m->n_triangles;
   TriRange* rng = nullptr;
-  unsigned long long *tasks = nullptr, *begin = nullptr, *counter = nullptr;
+  unsigned long long *tasks = nullptr, *begin = nullptr;
   R2F_HIP(pooled(ctx, kSlotRange, n, &rng), "filter: alloc ranges");
   R2F_HIP(pooled(ctx, kSlotTasks, n, &tasks), "filter: alloc tasks");
   R2F_HIP(pooled(ctx, kSlotBegin, n, &begin), "filter: alloc task offsets");
-  R2F_HIP(pooled(ctx, kSlotCounter, 2, &counter), "filter: alloc counter");
   hipLaunchKernelGGL(k_tri_prep, dim3((n + 255) / 256), dim3(256), 0, s, m->tri, n, c, rng, tasks);
   R2F_HIP(hipGetLastError(), "filter: k_tri_prep");
   void* tmp = nullptr;
@@ -657,27 +655,32 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   R2F_HIP(r2host::pool_get(ctx, kSlotTemp, tb, &tmp), "filter: alloc scan temp");
   R2F_HIP(hipcub::DeviceScan::InclusiveScan(tmp, tb, mark, task_tri, hipcub::Max(), nt, s), "filter: task map");
   const dim3 grid((unsigned)((B->n_tasks + kWaves - 1) / kWaves));
-  R2F_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s), "filter: clear counter");
-  hipLaunchKernelGGL(k_pairs<false>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, counter,
-                     (unsigned long long*)nullptr, 0ull);
+  uint32_t *task_cnt = nullptr, *task_off = nullptr;
+  R2F_HIP(pooled(ctx, kSlotTaskCnt, nt, &task_cnt), "filter: alloc task counts");
+  R2F_HIP(pooled(ctx, kSlotTaskOff, nt, &task_off), "filter: alloc task offsets");
+  hipLaunchKernelGGL(k_pairs<false>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, task_cnt,
+                     (const uint32_t*)nullptr, (unsigned long long*)nullptr);
   R2F_HIP(hipGetLastError(), "filter: k_pairs count");
-  unsigned long long np = 0;
-  R2F_HIP(hipMemcpyAsync(&np, counter, sizeof(np), hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, task_cnt, task_off, nt, s), "filter: scan size");
+  R2F_HIP(r2host::pool_get(ctx, kSlotTemp, tb, &tmp), "filter: alloc scan temp");
+  R2F_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, task_cnt, task_off, nt, s), "filter: scan pair counts");
+  uint32_t last[2] = {0, 0};
+  R2F_HIP(hipMemcpyAsync(&last[0], task_off + (nt - 1), 4, hipMemcpyDeviceToHost, s), "filter: d2h");
+  R2F_HIP(hipMemcpyAsync(&last[1], task_cnt + (nt - 1), 4, hipMemcpyDeviceToHost, s), "filter: d2h");
   R2F_HIP(hipStreamSynchronize(s), "filter: sync");
+  // each task holds at most 64 pairs, so the u32 offsets are exact while the total fits
+  if ((unsigned long long)nt * 64ull > 0xFFFFFFFFull)
+    return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: too many binning tasks");
+  const unsigned long long np = (unsigned long long)last[0] + last[1];
   B->n_pairs = np;
   if (np == 0) return R2_OK;
   if (np > 0x7FFFFFFFull) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: more than 2^31 voxel/triangle pairs");
   unsigned long long *keys = nullptr, *keys2 = nullptr;
   R2F_HIP(pooled(ctx, kSlotKeys, np, &keys), "filter: alloc keys");
   R2F_HIP(pooled(ctx, kSlotKeys2, np, &keys2), "filter: alloc keys");
-  R2F_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s), "filter: clear counter");
-  hipLaunchKernelGGL(k_pairs<true>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, counter, keys,
-                     np);
+  hipLaunchKernelGGL(k_pairs<true>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, task_cnt,
+                     (const uint32_t*)task_off, keys);
   R2F_HIP(hipGetLastError(), "filter: k_pairs emit");
-  unsigned long long np2 = 0;
-  R2F_HIP(hipMemcpyAsync(&np2, counter, sizeof(np2), hipMemcpyDeviceToHost, s), "filter: d2h");
-  R2F_HIP(hipStreamSynchronize(s), "filter: sync");
-  if (np2 != np) return r2host::fail(ctx, R2_ERR_HIP, "filter: candidate emit count differs from the count pass");
   const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
   int vbits = 1;
   while ((1ull << vbits) < nvox) ++vbits;
