@@ -36,37 +36,65 @@ enum : int {
 
 enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO = 4 };
 
-struct PathLane {
-  // work item
+// Path state touched only at the start and end of a path and in shading events
+// lives in LDS, one slot per lane (43 dwords: an odd stride, so the 64 lanes of
+// a wavefront fall on distinct banks); the state the traversal loop and the RNG
+// touch stays in registers.  This keeps the kernel within the register budget
+// of three wavefronts per SIMD without spilling to scratch.
+struct ColdLane {
   uint32_t pix;
   int32_t px, py;
   int32_t s_cur, s_end;
-  uint64_t item;
+  uint32_t item;
   float ax, ay, az;     // FP32 item sum (same order as the reference's framebuffer +=)
+  v3 Ld, pdir, pLe;     // pending NEE sample (light direction / phase direction, radiance)
+  float ppdf;
+  v3 mi_pos, mi_n;      // scatter event
+  float S[6];
+  float mi_ior;
+  int32_t mi_inst;
+  v3 sh_p0, sh_p1, si_pos;
+  uint32_t pad;
+};
+static_assert(sizeof(ColdLane) == 43 * 4, "ColdLane must keep an odd dword stride");
+
+struct PathLane {
+  // work item (LDS)
+  uint32_t& pix;
+  int32_t &px, &py;
+  int32_t &s_cur, &s_end;
+  uint32_t& item;
+  float &ax, &ay, &az;
   double dax, day, daz;  // FP64 item sum (compensated framebuffer form)
   // path
   Rng rng;
   int stage, b, iters;
-  v3 L, thr, Ld;
+  v3 L, thr;
+  v3& Ld;
   Ray ray;
-  v3 si_pos;
+  v3& si_pos;
   int si_medium, si_hit;
-  // scatter event
-  v3 mi_pos, mi_n;
-  float S[6];
-  float mi_ior;
-  int mi_inst;
-  // pending NEE sample (light direction / phase direction, radiance, pdf)
-  v3 pdir, pLe;
-  float ppdf;
+  // scatter event (LDS)
+  v3 &mi_pos, &mi_n;
+  float (&S)[6];
+  float& mi_ior;
+  int32_t& mi_inst;
+  // pending NEE sample (LDS)
+  v3 &pdir, &pLe;
+  float& ppdf;
   // shadow ray walk (intersectTr)
-  v3 sh_p0, sh_p1;
+  v3 &sh_p0, &sh_p1;
   int sh_cur, sh_next, sh_hit, sh_seg, sh_ret;
   float sh_Tr;
   // traversal
   int tk, tinst, tsteps, tres;
   v3 to, td, tri;
   float tt, ttau, tmip, ttmax, ttr;
+
+  __device__ explicit PathLane(ColdLane& c)
+      : pix(c.pix), px(c.px), py(c.py), s_cur(c.s_cur), s_end(c.s_end), item(c.item), ax(c.ax), ay(c.ay), az(c.az),
+        Ld(c.Ld), si_pos(c.si_pos), mi_pos(c.mi_pos), mi_n(c.mi_n), S(c.S), mi_ior(c.mi_ior), mi_inst(c.mi_inst),
+        pdir(c.pdir), pLe(c.pLe), ppdf(c.ppdf), sh_p0(c.sh_p0), sh_p1(c.sh_p1) {}
 };
 
 template <bool kSingle>

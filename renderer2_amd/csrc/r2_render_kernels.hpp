@@ -171,7 +171,8 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
   const DScene& s = *sp;
   const int lane = threadIdx.x & 63;
   const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
-  PathLane p;
+  __shared__ ColdLane cold[256];
+  PathLane p(cold[threadIdx.x]);
   p.stage = ST_FETCH;
   p.item = 0;
   Cnt cnt;
@@ -229,6 +230,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
                                                                   uint64_t* __restrict__ counters) {
   const int lane = threadIdx.x & 63;
   const int e0 = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * 4u + (threadIdx.x >> 6)) % (unsigned)n_ent));
+  __shared__ ColdLane cold[256];
   Cnt cnt;
 #pragma unroll
   for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
     const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
     const int chunk = E.chunk;
     Acc* partial = (Acc*)E.partial;
-    PathLane p;
+    PathLane p(cold[threadIdx.x]);
     p.stage = ST_FETCH;
     p.item = 0;
     for (int guard = 0; guard < (1 << 24); ++guard) {
