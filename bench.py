@@ -74,8 +74,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-brick-grid", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "packed"], default=None,
                     help="voxel record precision (default: the workload's own)")
-    ap.add_argument("--requests", type=int, default=40, help="requests of the serving workload")
-    ap.add_argument("--rate", type=float, default=1.5, help="Poisson arrival rate (requests/s) of the serving workload")
+    ap.add_argument("--requests", type=int, default=80, help="requests of the serving workload")
+    ap.add_argument("--rate", type=float, default=5.0, help="Poisson arrival rate (requests/s) of the serving workload")
     ap.add_argument("--cpu-voxels", type=int, default=1_500_000,
                     help="random voxels per LOD of the filter CPU baseline sample")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -106,13 +106,7 @@ def main(argv=None) -> int:
     bufs = [_hip.DeviceBuffer(frame_bytes) for _ in range(a.steps)]
     warm = _hip.DeviceBuffer(frame_bytes)
 
-    if a.copy_calib > 0:
-        nb = a.copy_calib << 20
-        src, dst = _hip.DeviceBuffer(nb), _hip.DeviceBuffer(nb)
-        _hip.check(_hip.hip().hipMemset(src.ptr, 1, nb), "hipMemset")
-        _hip.check(_hip.hip().hipMemcpy(dst.ptr, src.ptr, nb, _hip.D2D), "hipMemcpy D2D")
-        _hip.synchronize()
-        del src, dst
+    copy_calibration(a.copy_calib)
     for k in range(a.warmup):
         ctx.render_device(dict(cfg, sample_begin=1_000_000 + k * spp), grids, w["instances"], env, warm.addr)
     _hip.synchronize()
@@ -202,11 +196,27 @@ def main(argv=None) -> int:
     return 0
 
 
-def serve_stream(ctx, grids, env, instance_sets, n_req, rate, seed, sizes, spps, max_depth, radius=2.5):
-    """Seeded Poisson request stream through the render server: each request picks
-    a resolution, an spp, an instance set (LOD) and an orbit camera at random."""
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy of a known size, so that a PMC byte counter can be
+    scaled against it (with HSA_ENABLE_SDMA=0 the copy runs as a blit kernel)."""
+    if mib <= 0:
+        return
+    from renderer2_amd import _hip
+
+    nb = mib << 20
+    src, dst = _hip.DeviceBuffer(nb), _hip.DeviceBuffer(nb)
+    _hip.check(_hip.hip().hipMemset(src.ptr, 1, nb), "hipMemset")
+    _hip.check(_hip.hip().hipMemcpy(dst.ptr, src.ptr, nb, _hip.D2D), "hipMemcpy D2D")
+    _hip.synchronize()
+    src.free()
+    dst.free()
+
+
+def make_requests(n_req, rate, seed, instance_sets, sizes, spps, max_depth, radius=2.5):
+    """Seeded Poisson request stream: each request picks a resolution, an spp, an
+    instance set (LOD) and an orbit camera at random.  rate None: all at t = 0."""
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req)) if rate else np.zeros(n_req)
     reqs = []
     for k in range(n_req):
         res = int(rng.choice(sizes))
@@ -217,7 +227,13 @@ def serve_stream(ctx, grids, env, instance_sets, n_req, rate, seed, sizes, spps,
         cfg = dict(width=res, height=res, spp=spp, max_depth=max_depth, cam_pos=cam, cam_look_at=(0, 0, 0),
                    cam_up=(0, 1, 0), fov_deg=40.0, seed=1000 + k, stream=0, use_brick_grid=True)
         reqs.append((float(arrivals[k]), cfg, inst))
-    srv = ctx.server(hbm_budget_bytes=16 << 30, max_inflight=16, slice_paths=1 << 23)
+    return reqs
+
+
+def run_server(ctx, grids, env, reqs, budget=16 << 30, max_inflight=16, slice_paths=1 << 24):
+    """Submit each request at its arrival time, wait for all; latency is per request
+    from submission to the frame in host memory (the server's own clock)."""
+    srv = ctx.server(hbm_budget_bytes=budget, max_inflight=max_inflight, slice_paths=slice_paths)
     t0 = time.perf_counter()
     ids = []
     for t, cfg, inst in reqs:
@@ -231,46 +247,51 @@ def serve_stream(ctx, grids, env, instance_sets, n_req, rate, seed, sizes, spps,
     st = srv.stats()
     srv.stop()
     paths = sum(c["width"] * c["height"] * c["spp"] for _, c, _ in reqs)
-    return {"requests": n_req, "arrival_rate_per_s": rate, "p50_ms": round(st["p50_ms"], 2),
-            "p99_ms": round(st["p99_ms"], 2), "throughput_mpaths_s": round(paths / elapsed / 1e6, 2),
+    return {"requests": len(reqs), "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+            "throughput_mpaths_s": round(paths / elapsed / 1e6, 2),
             "server_mpaths_s": round(st["paths_per_sec"] / 1e6, 2), "completed": st["completed"],
-            "rejected": st["rejected"], "peak_gib": round(st["peak_bytes"] / 2 ** 30, 3),
-            "sizes": list(sizes), "spps": list(spps), "wall_s": round(elapsed, 2)}
+            "rejected": st["rejected"], "peak_gib": round(st["peak_bytes"] / 2 ** 30, 3), "wall_s": round(elapsed, 2),
+            "paths": paths}
+
+
+def serve_stream(ctx, grids, env, instance_sets, n_req, rate, seed, sizes, spps, max_depth, radius=2.5):
+    reqs = make_requests(n_req, rate, seed, instance_sets, sizes, spps, max_depth, radius)
+    res = run_server(ctx, grids, env, reqs, slice_paths=1 << 23)
+    res.update(arrival_rate_per_s=rate, sizes=list(sizes), spps=list(spps))
+    return res
 
 
 def bench_serving(a) -> int:
     """Render server under a seeded Poisson stream: device-resident LOD grids of the
-    filtered mesh (512^3, 256^3, 128^3) plus the 64^3 value-noise grid; resolution
-    uniform over {256^2, 512^2, 1024^2}, spp over {16, 64, 256}, orbit camera and LOD
-    chosen per request."""
+    256^3 anisotropic field at 128^3, 256^3 and 512^3 plus the 64^3 value-noise
+    grid; resolution uniform over {256^2, 512^2, 1024^2}, spp over {16, 64, 256},
+    orbit camera and LOD chosen per request.  The same request set is first run
+    closed-loop (all submitted at once) to measure the server's capacity; the
+    Poisson run's offered load is stated against it."""
     import renderer2_amd as r2
-    from renderer2_amd import _hip, scenes
+    from renderer2_amd import scenes
 
     ctx = r2.Context(0)
-    w = scenes.FILTER_WORKLOADS["c3"]
-    mesh = scenes.make_filter_mesh("c3", seed=1)
-    gm = ctx.upload_mesh(mesh)
-    vs0 = scenes.finest_voxel_size(mesh, w["finest"])
-    grids, keep = [], []
-    ext = float((mesh["positions"].max(axis=0) - mesh["positions"].min(axis=0)).max())
-    for lod in range(w["lods"]):
-        vs = vs0 * 2.0 ** lod
-        fg = gm.setup(vs)
-        buf = _hip.DeviceBuffer(int(fg.num_voxels[0]) * int(fg.num_voxels[1]) * int(fg.num_voxels[2]) * 32)
-        ctx.filter_lod(gm, r2.filter_params(vs, w["samples"], w["iterations"], w["sigma_t"], w["alpha"],
-                                            w["clip_rays"], seed=1, lod=lod), out_dev_ptr=buf.addr)
-        grids.append(ctx.upload_grid(scenes.filtered_grid(fg, vs, albedo=0.9, records_dev=buf.addr)))
-        keep.append(buf)
+    grids = []
+    for n in (128, 256, 512):
+        w = scenes.make_workload("c2", n=n)
+        grids.append(ctx.upload_grid(w["grids"][0]))
+        del w
     c1 = scenes.make_workload("c1")
     grids.append(ctx.upload_grid(c1["grids"][0]))
     env = ctx.upload_env(scenes.synth_env("sky", 512, 256))
-    s = 1.0 / ext
-    inst_sets = [[dict(grid=k, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(s, s, s))] for k in range(3)]
-    inst_sets.append([dict(grid=3, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))])
-    res = serve_stream(ctx, grids, env, inst_sets, n_req=a.requests, rate=a.rate, seed=11, sizes=(256, 512, 1024),
-                       spps=(16, 64, 256), max_depth=16)
-    for b in keep:
-        b.free()
+    inst_sets = [[dict(grid=k, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))] for k in range(4)]
+    kw = dict(instance_sets=inst_sets, sizes=(256, 512, 1024), spps=(16, 64, 256), max_depth=16)
+    ctx.render(dict(make_requests(1, None, 0, **kw)[0][1], spp=1), grids, inst_sets[0], env)  # warm-up
+    cap = run_server(ctx, grids, env, make_requests(a.requests, None, 11, **kw))
+    mean_paths = cap["paths"] / a.requests
+    capacity_rps = cap["throughput_mpaths_s"] * 1e6 / mean_paths
+    res = run_server(ctx, grids, env, make_requests(a.requests, a.rate, 11, **kw))
+    offered = a.rate * mean_paths / 1e6
+    res.update(arrival_rate_per_s=a.rate, offered_mpaths_s=round(offered, 2),
+               capacity_mpaths_s=cap["throughput_mpaths_s"], capacity_req_s=round(capacity_rps, 2),
+               utilisation=round(offered / cap["throughput_mpaths_s"], 3), closed_loop_p50_ms=cap["p50_ms"],
+               closed_loop_p99_ms=cap["p99_ms"], sizes=[256, 512, 1024], spps=[16, 64, 256])
     line = {
         "metric": "render server: M paths/s with p50/p99 request latency under a Poisson stream",
         "value": res["throughput_mpaths_s"],
@@ -284,8 +305,9 @@ def bench_serving(a) -> int:
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "serving: filtered mesh LODs 512^3/256^3/128^3 + 64^3 value-noise grid resident, "
-                               "Poisson arrivals, resolution {256,512,1024}^2, spp {16,64,256}, orbit camera",
+        "config": {"workload": "serving: 256^3 anisotropic field at 128^3/256^3/512^3 + 64^3 value-noise grid "
+                               "resident, Poisson arrivals, resolution {256,512,1024}^2, spp {16,64,256}, "
+                               "orbit camera, depth 16",
                    "requests": a.requests, "arrival_rate_per_s": a.rate},
         "serving": res,
         "roofline": None,
@@ -332,6 +354,7 @@ def bench_filter(a) -> int:
     w = scenes.FILTER_WORKLOADS["c3"]
     mesh = scenes.make_filter_mesh("c3", seed=1)
     ctx = r2.Context(0)
+    copy_calibration(a.copy_calib)
     gm = ctx.upload_mesh(mesh)
     vs0 = scenes.finest_voxel_size(mesh, w["finest"])
     lods = [vs0 * 2.0 ** l for l in range(w["lods"])]

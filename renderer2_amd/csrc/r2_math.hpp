@@ -13,6 +13,20 @@
 
 namespace r2 {
 
+// Read-only launch data (a scene, the entry table of a batched launch) is read
+// through the constant address space: wave-uniform fields then load with scalar
+// instructions, and the device pointers read from it are known to address global
+// memory, so loads through them are global loads instead of flat ones (which
+// also wait on the LDS counter).  The empty asm keeps the optimiser from folding
+// the address-space round trip away.  The pointer must be wave-uniform and the
+// data must not change while the kernel runs.
+template <typename T>
+__device__ __forceinline__ const T& kconst(const T* p) {
+  uint64_t v = (uint64_t)(uintptr_t)p;
+  asm volatile("" : "+s"(v));
+  return *(const T*)(const __attribute__((address_space(4))) T*)v;
+}
+
 struct v3 { float x, y, z; };
 R2_HD v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
 R2_HD v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }

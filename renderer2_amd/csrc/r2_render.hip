@@ -49,10 +49,12 @@ static void launch_sm(r2_ctx* ctx, int prec, int blocks, uint64_t total, int chu
 
 size_t batch_entry_bytes() { return sizeof(r2::BatchEntry); }
 
-// samples per work item: about 2^21 items per frame
+// samples per work item: about 2^24 items per frame, so that a frame (and each
+// share of one that the render server runs as a slice) has many short items for
+// the 200 K lanes of the chip; the chunk sums take 12 B per item
 void render_chunking(const r2::DScene& scene, int* chunk_out, int* nchunks_out) {
   const uint64_t npix = (uint64_t)scene.W * scene.H;
-  const uint64_t target = 1ull << 21;
+  const uint64_t target = 1ull << 24;
   const int chunk =
       (int)std::min<uint64_t>((uint64_t)scene.spp, std::max<uint64_t>(1, (npix * scene.spp + target - 1) / target));
   *chunk_out = chunk;

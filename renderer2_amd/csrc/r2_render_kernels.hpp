@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
                                                   const int32_t* __restrict__ probe_pix,
                                                   const int32_t* __restrict__ probe_smp, float* __restrict__ probe_out,
                                                   uint32_t* __restrict__ probe_cnt) {
-  const DScene& s = *sp;
+  const DScene& s = kconst(sp);
   const int lane = threadIdx.x & 63;
   const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
   __shared__ ColdLane cold[256];
@@ -239,8 +239,8 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
 #pragma unroll 1
   for (int v = 0; v < n_ent; ++v) {
     const int e = __builtin_amdgcn_readfirstlane(e0 + v < n_ent ? e0 + v : e0 + v - n_ent);
-    const BatchEntry& E = ents[e];
-    const DScene& s = *E.scene;
+    const BatchEntry& E = (&kconst(ents))[e];
+    const DScene& s = kconst(E.scene);
     const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
     const int chunk = E.chunk;
     Acc* partial = (Acc*)E.partial;

@@ -69,7 +69,7 @@ def test_server_budget_admission_and_rejection(ctx, world):
     reqs = _requests(world)
     cfg, inst = reqs[2]  # 128x128 x 16 spp
     npix, spp = 128 * 128, 16
-    chunk = min(spp, max(1, -(-npix * spp // (1 << 21))))  # samples per work item
+    chunk = min(spp, max(1, -(-npix * spp // (1 << 24))))  # samples per work item
     nchunks = -(-spp // chunk)
     # scene copy (< 64 KiB) + chunk sums + frame: room for exactly one request at a time
     one = (1 << 16) + nchunks * npix * 12 + npix * 12
