@@ -307,6 +307,26 @@ int r2_env_upload(r2_ctx* ctx, const float* rgb, int32_t W, int32_t H, r2_env** 
   }
   const float marg_int = build_1d(marg_func.data(), H, marg_cdf.data());
   for (int j = 0; j < H; ++j) row_int[j] = build_1d(row_func.data() + (size_t)j * W, W, row_cdf.data() + (size_t)j * (W + 1));
+  // guide tables: g = smallest power of two >= the number of intervals, so that
+  // u * g is exact and a guided search brackets about one interval
+  auto pow2_at_least = [](int v) {
+    int g = 1;
+    while (g < v && g < (1 << 16)) g <<= 1;
+    return g;
+  };
+  auto build_guide = [](const float* cdf, int size, int g, int32_t* out) {
+    int cnt = 0;  // number of cdf values <= j / g (the CDF is non-decreasing)
+    for (int j = 0; j <= g; ++j) {
+      const float v = (float)j / (float)g;
+      while (cnt < size && cdf[cnt] <= v) ++cnt;
+      out[j] = cnt;
+    }
+  };
+  const int marg_g = pow2_at_least(H), row_g = pow2_at_least(W);
+  std::vector<int32_t> marg_guide(marg_g + 1), row_guide((size_t)H * (row_g + 1));
+  build_guide(marg_cdf.data(), H + 1, marg_g, marg_guide.data());
+  for (int j = 0; j < H; ++j)
+    build_guide(row_cdf.data() + (size_t)j * (W + 1), W + 1, row_g, row_guide.data() + (size_t)j * (row_g + 1));
   r2_env* e = new (std::nothrow) r2_env();
   if (!e) return R2_ERR_OOM;
   auto up = [&](const void* src, size_t bytes) -> void* {
@@ -324,8 +344,13 @@ int r2_env_upload(r2_ctx* ctx, const float* rgb, int32_t W, int32_t H, r2_env** 
   e->dev.row_int = (const float*)up(row_int.data(), H * 4);
   e->dev.marg_func = (const float*)up(marg_func.data(), H * 4);
   e->dev.marg_cdf = (const float*)up(marg_cdf.data(), (H + 1) * 4);
+  e->dev.marg_guide = (const int32_t*)up(marg_guide.data(), marg_guide.size() * 4);
+  e->dev.row_guide = (const int32_t*)up(row_guide.data(), row_guide.size() * 4);
+  e->dev.marg_g = marg_g;
+  e->dev.row_g = row_g;
   e->dev.marg_int = marg_int;
-  if (!e->dev.tex || !e->dev.row_func || !e->dev.row_cdf || !e->dev.row_int || !e->dev.marg_func || !e->dev.marg_cdf) {
+  if (!e->dev.tex || !e->dev.row_func || !e->dev.row_cdf || !e->dev.row_int || !e->dev.marg_func || !e->dev.marg_cdf ||
+      !e->dev.marg_guide || !e->dev.row_guide) {
     r2_env_free(ctx, e);
     return fail(ctx, R2_ERR_HIP, "env: device upload failed");
   }

@@ -134,8 +134,10 @@ int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64)
     R2_HIP(ctx, hipMemcpyFromSymbol(st_host, HIP_SYMBOL(r2k_fp32::g_sm_stats), sizeof(st_host)));
     fprintf(stderr, "[sm-stats] paths %llu outer %llu ev_iters %llu trav_rounds %llu trav_lanes %llu\n",
             (unsigned long long)(npix * scene.spp), st_host[0], st_host[1], st_host[2], st_host[3]);
+    fprintf(stderr, "[sm-stats] wave cycles: events %llu traversal %llu\n", st_host[4], st_host[5]);
     for (int k = ST_START; k <= ST_FINISH; ++k)
-      fprintf(stderr, "[sm-stats] state %d: execs %llu lanes %llu\n", k, st_host[16 + 2 * k], st_host[17 + 2 * k]);
+      fprintf(stderr, "[sm-stats] state %d: execs %llu lanes %llu cycles %llu\n", k, st_host[16 + 2 * k],
+              st_host[17 + 2 * k], st_host[48 + k]);
     const unsigned long long z[64] = {0};
     R2_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(r2k_fp32::g_sm_stats), z, sizeof(z)));
   }

@@ -35,7 +35,8 @@ __device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int
 // (sample chunk, pixel), output = chunk sums.  Probe mode (probe_pix != null):
 // item = one (pixel, sample) path, output = its radiance and lookup counters.
 #ifndef R2_EV_MODE
-#define R2_EV_MODE 0      // 0: every event lane advances once per iteration; 1: most populated state first
+#define R2_EV_MODE 2      // 0: every event lane advances once per iteration; 1: most populated state
+                          // first; 2: states in the forward order of the state graph (measured best)
 #endif
 #ifndef R2_EV_ITERS
 #define R2_EV_ITERS 12    // event iterations per scheduling round
@@ -47,13 +48,36 @@ __device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int
 #define R2_TR_MIN_ROUNDS 8
 #endif
 #ifdef R2_SM_STATS
-// debug-build scheduling statistics: [0] outer iterations, [1] event iterations,
-// [2] traversal rounds, [3] lanes active in traversal rounds, [16+2s] executions
-// of state s, [17+2s] lanes in those executions
+// debug-build scheduling statistics, kept per wavefront in LDS and added to the
+// global table once when the wavefront exits: [0] outer iterations, [1] event
+// iterations, [2] traversal rounds, [3] lanes active in traversal rounds,
+// [4] / [5] wave cycles in event passes / traversal rounds, [16+2s] executions
+// of state s, [17+2s] lanes in those executions, [48+s] wave cycles in state s
 __device__ unsigned long long g_sm_stats[64];
-#define SM_STAT(i, v) do { if (lane == 0) atomicAdd(&g_sm_stats[(i)], (unsigned long long)(v)); } while (0)
+__device__ __forceinline__ unsigned long long* wave_stats() {
+  __shared__ unsigned long long st[4][64];
+  return st[(threadIdx.x >> 6) & 3];
+}
+__device__ __forceinline__ void stats_begin() { wave_stats()[threadIdx.x & 63] = 0ull; }
+__device__ __forceinline__ void stats_end() {
+  atomicAdd(&g_sm_stats[threadIdx.x & 63], wave_stats()[threadIdx.x & 63]);
+}
+// leader lane of the active lanes adds v to counter i
+__device__ __forceinline__ void stat_add(int i, uint64_t v) {
+  const uint64_t m = __ballot(true);
+  if ((threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) wave_stats()[i] += v;
+}
+#define SM_STAT(i, v) stat_add((i), (uint64_t)(v))
+__device__ __forceinline__ void stat_cycles(int i, uint64_t c) { stat_add(i, c); }
+__device__ __forceinline__ void stat_state(int st) {
+  stat_add(16 + 2 * st, 1);
+  stat_add(17 + 2 * st, __popcll(__ballot(true)));
+}
 #else
 #define SM_STAT(i, v) do {} while (0)
+__device__ __forceinline__ void stat_state(int) {}
+__device__ __forceinline__ void stats_begin() {}
+__device__ __forceinline__ void stats_end() {}
 #endif
 // end of one path: radiance, NaN guard, accumulation into the item or probe output
 template <typename Acc>
@@ -104,6 +128,22 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
   }
 }
 
+// Run state kSt for the lanes that are in it (a divergent branch the wave skips
+// when no lane is).
+template <bool kSingle, bool kCount, int kSt>
+__device__ __forceinline__ void event_if(const DScene& s, PathLane& p, Cnt& cnt) {
+  if (p.stage == kSt) {
+#ifdef R2_SM_STATS
+    stat_state(kSt);
+    const uint64_t t0 = clock64();
+    event_step<kSingle, kCount>(s, p, cnt, kSt);
+    stat_cycles(48 + kSt, clock64() - t0);
+#else
+    event_step<kSingle, kCount>(s, p, cnt, kSt);
+#endif
+  }
+}
+
 // One scheduling round of a wavefront: shading events, batched across the
 // wavefront, then traversal rounds in which every traversing lane advances one
 // DDA / tracking step.
@@ -128,6 +168,42 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
       else event_step<kSingle, kCount>(s, p, cnt, p.stage);
     }
   }
+#elif R2_EV_MODE == 2
+#ifdef R2_SM_STATS
+  const uint64_t te0 = clock64();
+#endif
+  // pass-through: the states run in the forward order of the path's state graph,
+  // so within one pass a lane advances through every state that follows its own
+  // in that order (e.g. SH_NEXT -> BRDF_DONE -> DIRECT_DONE -> BOUNCE -> a new
+  // traversal), and each state's code runs at most once per pass for the wave
+  for (int k = 0; k < R2_EV_ITERS; ++k) {
+    if (!__any(p.stage >= ST_START)) break;
+    SM_STAT(1, 1);
+    if (p.stage == ST_FINISH) {
+      stat_state(ST_FINISH);
+#ifdef R2_SM_STATS
+      const uint64_t t0 = clock64();
+#endif
+      finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+#ifdef R2_SM_STATS
+      stat_cycles(48 + ST_FINISH, clock64() - t0);
+#endif
+    }
+    event_if<kSingle, kCount, ST_START>(s, p, cnt);
+    event_if<kSingle, kCount, ST_SH_NEXT>(s, p, cnt);
+    event_if<kSingle, kCount, ST_AFTER_SAMPLE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_NOHIT>(s, p, cnt);
+    event_if<kSingle, kCount, ST_LIGHT_DONE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_BRDF_DONE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_DIRECT_DONE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_BOUNCE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_SCATTER>(s, p, cnt);
+    event_if<kSingle, kCount, ST_BRDF_SAMPLE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_SH_SEG>(s, p, cnt);
+  }
+#ifdef R2_SM_STATS
+  SM_STAT(4, clock64() - te0);
+#endif
 #else
   // run the most populated event state each iteration (wave-uniform branch)
   for (int k = 0; k < R2_EV_ITERS; ++k) {
@@ -147,6 +223,9 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
   }
 #endif
   // ---- traversal rounds: every traversing lane advances one DDA / tracking step
+#ifdef R2_SM_STATS
+  const uint64_t tt0 = clock64();
+#endif
   for (int k = 0; k < 256; ++k) {
     const uint64_t tm = __ballot(p.stage == ST_TRAV);
     if (tm == 0) break;
@@ -157,6 +236,9 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
       if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
     }
   }
+#ifdef R2_SM_STATS
+  SM_STAT(5, clock64() - tt0);
+#endif
 }
 
 #ifndef R2_SM_WAVES
@@ -178,6 +260,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
   Cnt cnt;
 #pragma unroll
   for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
+  stats_begin();
   for (int guard = 0; guard < (1 << 24); ++guard) {
     // ---- work fetch: one atomic per wavefront, items handed out by prefix rank
     const uint64_t need = __ballot(p.stage == ST_FETCH);
@@ -217,6 +300,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
     SM_STAT(0, 1);
     run_round<kSingle, kCount, Acc>(s, p, cnt, lane, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
   }
+  stats_end();
 }
 
 // Batched form for the render server: one launch serves several requests, each
@@ -234,6 +318,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
   Cnt cnt;
 #pragma unroll
   for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
+  stats_begin();
   // one pass over the entries: an entry is left only when none of its items is
   // left to hand out, so it never needs a second visit
 #pragma unroll 1
@@ -280,6 +365,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
       run_round<kSingle, false, Acc>(s, p, cnt, lane, npix, chunk, partial, nullptr, nullptr, nullptr);
     }
   }
+  stats_end();
 }
 
 // out[p] = sum over chunks in chunk order (FP32), or a Neumaier-compensated

@@ -53,6 +53,12 @@ struct DEnv {
   const float* row_int;    // h
   const float* marg_func;  // h
   const float* marg_cdf;   // h + 1
+  // guide tables of the CDF searches: entry j of a table of g + 1 entries is the
+  // number of CDF values <= j / g, which brackets the search for any u in
+  // [j / g, (j + 1) / g)
+  const int32_t* marg_guide;  // marg_g + 1
+  const int32_t* row_guide;   // h * (row_g + 1)
+  int32_t marg_g, row_g;
   float marg_int;
 };
 

@@ -158,6 +158,13 @@ __device__ __forceinline__ Rec rec_lerp(float t, const Rec& a, const Rec& b) {
   return o;
 }
 
+// leaf index of voxel (x, y, z), or -1 outside the grid or in an empty leaf
+__device__ __forceinline__ int32_t leaf_of(const DGrid& g, int x, int y, int z) {
+  if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny && (unsigned)z < (unsigned)g.nz)
+    return g.leaf_table[((size_t)(z >> 3) * g.nly + (y >> 3)) * g.nlx + (x >> 3)];
+  return -1;
+}
+
 // NanoVDB ReadAccessor<Vec4d>::getValue over the collapsed tree.
 template <bool kCount>
 __device__ __forceinline__ Rec get_value(const DGrid& g, int x, int y, int z, Cnt& cnt) {
@@ -233,18 +240,39 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
     o.normal = nlerp(dz, d0, d1);
     return o;
   }
-  // same arithmetic as d00, d10 -> d0, d01, d11 -> d1, then z; evaluated so that
-  // at most three decoded records are live at a time
-  Rec a = get_value<kCount>(g, x, y, z, cnt), b = get_value<kCount>(g, x + 1, y, z, cnt);
+  // FP32 form: the leaf-table entries of the eight corners, then all sixteen
+  // record words are requested before any is used (two memory round trips
+  // instead of eight); then the same arithmetic as d00, d10 -> d0, d01, d11 ->
+  // d1, then z, with at most three decoded records live at a time
+  uint4 ra[8], rb[8];
+  {
+    int32_t lf[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) lf[c] = leaf_of(g, x + (c & 1), y + ((c >> 1) & 1), z + (c >> 2));
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      ra[c] = make_uint4(0u, 0u, 0u, 0u);
+      rb[c] = make_uint4(0u, 0u, 0u, 0x3C00u);  // background: ior = 1
+      if (lf[c] >= 0) {
+        const int vx = x + (c & 1), vy = y + ((c >> 1) & 1), vz = z + (c >> 2);
+        const uint4* q = g.leaves + ((size_t)lf[c] * kLeafVoxels + (((vz & 7) << 6) | ((vy & 7) << 3) | (vx & 7))) * 2;
+        ra[c] = q[0];
+        rb[c] = q[1];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) count<kCount>(cnt, kCntVox);
+  Rec a = decode_record(ra[0], rb[0]), b = decode_record(ra[1], rb[1]);
   Rec e = rec_lerp(dx, a, b);
-  a = get_value<kCount>(g, x, y + 1, z, cnt);
-  b = get_value<kCount>(g, x + 1, y + 1, z, cnt);
+  a = decode_record(ra[2], rb[2]);
+  b = decode_record(ra[3], rb[3]);
   const Rec d0 = rec_lerp(dy, e, rec_lerp(dx, a, b));
-  a = get_value<kCount>(g, x, y, z + 1, cnt);
-  b = get_value<kCount>(g, x + 1, y, z + 1, cnt);
+  a = decode_record(ra[4], rb[4]);
+  b = decode_record(ra[5], rb[5]);
   e = rec_lerp(dx, a, b);
-  a = get_value<kCount>(g, x, y + 1, z + 1, cnt);
-  b = get_value<kCount>(g, x + 1, y + 1, z + 1, cnt);
+  a = decode_record(ra[6], rb[6]);
+  b = decode_record(ra[7], rb[7]);
   const Rec d1 = rec_lerp(dy, e, rec_lerp(dx, a, b));
   return rec_lerp(dz, d0, d1);
 }
@@ -622,8 +650,14 @@ __device__ __forceinline__ float phase_sample(const float S6[6], v3 n, float ior
 }
 
 // ------------------------------------------------------------------ env --
-__device__ __forceinline__ int find_interval(const float* cdf, int size, float u) {
-  int first = 0, len = size;
+// find_interval over the bracket a guide table gives: the CDF is non-decreasing,
+// so the number of values <= u lies between the counts at j / g and (j + 1) / g
+// for j = floor(u * g), and the same bisection over that bracket returns the
+// same interval as over the whole table (u in [0, 1), g a power of two)
+__device__ __forceinline__ int find_interval_guided(const float* cdf, int size, float u, const int32_t* guide, int g) {
+  int j = (int)(u * (float)g);
+  j = j < 0 ? 0 : (j > g - 1 ? g - 1 : j);
+  int first = guide[j], len = guide[j + 1] - first;
   while (len > 0) {
     const int half = len >> 1, mid = first + half;
     if (cdf[mid] <= u) {
@@ -638,8 +672,8 @@ __device__ __forceinline__ int find_interval(const float* cdf, int size, float u
 }
 // CDistribution1D::sampleContinuous (distribution_1d.hpp:54-75)
 __device__ __forceinline__ float sample_1d(const float* f, const float* cdf, float fint, int n, float u, float& pdf,
-                                           int& off) {
-  const int o = find_interval(cdf, n + 1, u);
+                                           int& off, const int32_t* guide, int g) {
+  const int o = find_interval_guided(cdf, n + 1, u, guide, g);
   off = o;
   const float c0 = cdf[o], c1 = cdf[o + 1];
   float du = u - c0;
@@ -676,9 +710,9 @@ __device__ __forceinline__ v3 env_sample(const DEnv& e, Rng& rng, v3& dir, float
   count<kCount>(cnt, kCntEnv);
   float p1, p0;
   int v, dummy;
-  const float y = sample_1d(e.marg_func, e.marg_cdf, e.marg_int, e.h, rng.uni(), p1, v);
+  const float y = sample_1d(e.marg_func, e.marg_cdf, e.marg_int, e.h, rng.uni(), p1, v, e.marg_guide, e.marg_g);
   const float x = sample_1d(e.row_func + (size_t)v * e.w, e.row_cdf + (size_t)v * (e.w + 1), e.row_int[v], e.w,
-                            rng.uni(), p0, dummy);
+                            rng.uni(), p0, dummy, e.row_guide + (size_t)v * (e.row_g + 1), e.row_g);
   pdf = p0 * p1;
   const float phi = x * 2.f * kPi;
   const float theta = y * kPi;
