@@ -204,8 +204,8 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
   if (grid_packed(g)) {
     // packed form: density, S, colours and ior as one weighted sum of the eight
     // corners; normals interpolated pairwise along x, y, z with renormalisation,
-    // as operator*/operator+ do (filtered_data.hpp:115-151); corners are taken in
-    // pairs so that at most two decoded records are live
+    // as operator*/operator+ do (filtered_data.hpp:115-151); corners are decoded
+    // in pairs so that at most two decoded records are live
     Rec o;
     o.density = 0.f;
 #pragma unroll
@@ -221,21 +221,50 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
       o.ior += w * r.ior;
     };
     auto nlerp = [](float t, v3 u, v3 v) { return normalize((1.f - t) * normalize(u) + t * normalize(v)); };
-    v3 d0 = mk(0.f, 0.f, 0.f), d1 = d0;
-#pragma unroll 1
-    for (int cz = 0; cz < 2; ++cz) {
-      v3 e = mk(0.f, 0.f, 0.f);
-#pragma unroll 1
-      for (int cy = 0; cy < 2; ++cy) {
-        const Rec ra = get_value<kCount>(g, x, y + cy, z + cz, cnt);
-        acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * (1.f - dx), ra);
-        const Rec rb = get_value<kCount>(g, x + 1, y + cy, z + cz, cnt);
-        acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * dx, rb);
-        const v3 f = nlerp(dx, ra.normal, rb.normal);
-        if (cy == 0) e = f;
-        else if (cz == 0) d0 = nlerp(dy, e, f);
-        else d1 = nlerp(dy, e, f);
+    // all corner loads are issued before any is decoded (two round trips)
+    uint4 hw[8];
+    uint2 cw[8];
+    {
+      int32_t lf[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) lf[c] = leaf_of(g, x + (c & 1), y + ((c >> 1) & 1), z + (c >> 2));
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        // background: the packed form of the FP32 background record (ior 1,
+        // normal decoded from the zero code, normalize(-1,-1,-1): octahedral 0xABAB)
+        hw[c] = make_uint4(0u, 0u, 0u, 0x3C00u << 16);
+        cw[c] = make_uint2(0u, 0xABABu << 16);
+        if (lf[c] >= 0) {
+          const int vx = x + (c & 1), vy = y + ((c >> 1) & 1), vz = z + (c >> 2);
+          const size_t v = (size_t)lf[c] * kLeafVoxels + (((vz & 7) << 6) | ((vy & 7) << 3) | (vx & 7));
+          hw[c] = g.leaves_packed[v];
+          cw[c] = g.leaves_cold[v];
+        }
       }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) count<kCount>(cnt, kCntVox);
+    v3 e = mk(0.f, 0.f, 0.f), d0 = e, d1 = e;
+#pragma unroll 1
+    for (int pr = 0; pr < 4; ++pr) {
+      const int cy = pr & 1, cz = pr >> 1;
+      // the pair is always in slots 0 and 1: the words rotate down by two each
+      // iteration, so every access has a constant index and stays in registers
+      const uint4 ha = hw[0], hb = hw[1];
+      const uint2 ca = cw[0], cb = cw[1];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        hw[c] = hw[c + 2];
+        cw[c] = cw[c + 2];
+      }
+      const Rec ra = decode_packed(ha, ca);
+      acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * (1.f - dx), ra);
+      const Rec rb = decode_packed(hb, cb);
+      acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * dx, rb);
+      const v3 f = nlerp(dx, ra.normal, rb.normal);
+      if (cy == 0) e = f;
+      else if (cz == 0) d0 = nlerp(dy, e, f);
+      else d1 = nlerp(dy, e, f);
     }
     o.normal = nlerp(dz, d0, d1);
     return o;
