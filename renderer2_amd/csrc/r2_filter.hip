@@ -243,9 +243,11 @@ __device__ __forceinline__ void ray_uniforms(uint32_t seed, uint32_t lod, uint32
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     const uint32_t k = k0 + (uint32_t)j;
-    const uint32_t* src = ((k >> 2) == b0) ? p0 : p1;
+    // values (not pointers) are selected, so both blocks stay in registers
     const uint32_t s = k & 3u;
-    const uint32_t x = s == 0 ? src[0] : (s == 1 ? src[1] : (s == 2 ? src[2] : src[3]));
+    const uint32_t x0 = s == 0 ? p0[0] : (s == 1 ? p0[1] : (s == 2 ? p0[2] : p0[3]));
+    const uint32_t x1 = s == 0 ? p1[0] : (s == 1 ? p1[1] : (s == 2 ? p1[2] : p1[3]));
+    const uint32_t x = ((k >> 2) == b0) ? x0 : x1;
     u[j] = (float)(x >> 8) * 5.9604644775390625e-08f;
   }
 }
