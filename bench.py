@@ -164,6 +164,8 @@ def main(argv=None) -> int:
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "bytes_per_path": round(b_path, 1), "kernel": "k_paths_sm", "kernel_ms": round(avg_kernel_ms, 3)}
+        if a.workload == "c2" and w["grids"][0].get("precision", "fp32") == "fp32":
+            roof["traffic"], roof["traffic_source"] = profile_traffic("k_paths_sm_c2_256cube_1024sq_64spp")
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -194,6 +196,21 @@ def main(argv=None) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+PROFILES = Path(__file__).resolve().parent / "profiles"
+
+
+def profile_traffic(name: str):
+    """HBM bytes per launch of the profiled kernel, from the rocprofv3 summary of the
+    same workload (calibrated FETCH_SIZE + WRITE_SIZE passes), or None."""
+    import re
+
+    f = PROFILES / f"{name}.md"
+    if not f.exists():
+        return None, None
+    m = re.search(r"\*\*([0-9.]+) MB per launch\*\*", f.read_text())
+    return (int(float(m.group(1)) * 1e6), f"profiles/{f.name}") if m else (None, None)
 
 
 def copy_calibration(mib: int) -> None:
@@ -391,6 +408,7 @@ def bench_filter(a) -> int:
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_filter",
             "kernel_ms": round(kms[0], 3), "lod": 0}
+    roof["traffic"], roof["traffic_source"] = profile_traffic("k_filter_c3_mesh_512_256_128")
     cpu = None
     parity = None
     if not a.no_cpu_baseline:
