@@ -1,0 +1,167 @@
+"""Edge cases of the render path on the GPU against the CPU oracle.
+
+Covered: ragged and tiny frames (8x8 tiles cut by the frame border, one pixel,
+one sample), depth 0, grids whose size is not a multiple of the 8^3 leaf or
+brick (1^3, 13^3, non-cubic), an all-zero grid, a scene without media, an
+environment map with zero rows and columns (flat stretches of the CDFs that the
+guided search must skip exactly as the full bisection does), and argument errors.
+
+Bars as in test_gpu_render: same seeds, FP32 against the FP32 oracle; per-path
+agreement >= 99% (the rest diverge through last-ulp differences of device and
+host transcendentals), frames relMSE <= 1e-5, lookups and brick grids bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import renderer2_amd as r2
+from renderer2_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def relmse(x, r):
+    x = np.asarray(x, np.float64)
+    r = np.asarray(r, np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return scenes.make_workload("c1", width=32, height=32, spp=4)
+
+
+def _grid(records, vs=1.0 / 16, albedo=0.8):
+    nz, ny, nx = records.shape[:3]
+    return dict(records=np.ascontiguousarray(records), shape=(nz, ny, nx),
+                map_mat=(vs, 0.0, 0.0, 0.0, vs, 0.0, 0.0, 0.0, vs),
+                map_vec=(-0.5 * nx * vs, -0.5 * ny * vs, -0.5 * nz * vs),
+                sigma_a=(1.0 - albedo,) * 3, sigma_s=(albedo,) * 3, voxel_size_filtering=vs)
+
+
+def _paths_agree(ctx, grids_desc, instances, env_rgb, cfg):
+    grids = [ctx.upload_grid(g) for g in grids_desc]
+    env = ctx.upload_env(env_rgb)
+    W, H = cfg["width"], cfg["height"]
+    pix = np.repeat(np.arange(W * H, dtype=np.int32), 2)
+    smp = np.tile(np.arange(2, dtype=np.int32), W * H)
+    gpu = ctx.render_paths(cfg, grids, instances, env, pix, smp)
+    ref = O.Scene(grids_desc, instances, env_rgb, "f32").render_paths(cfg, pix, smp)
+    return float(np.mean(np.all(np.abs(gpu - ref) <= 1e-3 * np.maximum(np.abs(ref), 1e-3), axis=1)))
+
+
+@pytest.mark.parametrize("W,H,spp", [(1, 1, 1), (7, 5, 3), (13, 29, 2), (64, 3, 5)])
+def test_ragged_and_tiny_frames(ctx, c1, W, H, spp):
+    w = c1
+    grids = [ctx.upload_grid(g) for g in w["grids"]]
+    env = ctx.upload_env(w["env"])
+    cfg = dict(w["render"], width=W, height=H, spp=spp)
+    img = ctx.render(cfg, grids, w["instances"], env)
+    ref = O.Scene(w["grids"], w["instances"], w["env"], "f32").render(cfg)
+    assert img.shape == (H, W, 3)
+    assert np.isfinite(img).all()
+    assert relmse(img, ref) <= 1e-5
+
+
+def test_depth_zero_is_black(ctx, c1):
+    w = c1
+    grids = [ctx.upload_grid(g) for g in w["grids"]]
+    env = ctx.upload_env(w["env"])
+    cfg = dict(w["render"], max_depth=0)
+    img = ctx.render(cfg, grids, w["instances"], env)
+    ref = O.Scene(w["grids"], w["instances"], w["env"], "f32").render(cfg)
+    assert np.array_equal(img, ref.astype(np.float32))
+    assert not img.any()
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (13, 13, 13), (20, 9, 5), (24, 16, 8)])
+def test_grid_sizes(ctx, shape):
+    nz, ny, nx = shape
+    n = max(shape)
+    full = scenes.synth_volume(1, n, 7, 4.0, 3)
+    rec = np.ascontiguousarray(full[:nz, :ny, :nx])
+    desc = _grid(rec)
+    g = ctx.upload_grid(desc)
+    # lookups: active voxels (density > 0 or a bbox corner) return their record
+    coords = np.stack(np.meshgrid(np.arange(-1, nx + 1), np.arange(-1, ny + 1), np.arange(-1, nz + 1),
+                                  indexing="ij"), -1).reshape(-1, 3).astype(np.int32)
+    got = g.lookup(coords)
+    dens = rec[..., :4].copy().view(np.float32)[..., 0]
+    bg = np.zeros(32, np.uint8)
+    bg[28:30] = np.frombuffer(np.uint16(0x3C00).tobytes(), np.uint8)
+    for (x, y, z), r in zip(coords, got):
+        inside = 0 <= x < nx and 0 <= y < ny and 0 <= z < nz
+        corner = x in (0, nx - 1) and y in (0, ny - 1) and z in (0, nz - 1)
+        want = rec[z, y, x] if inside and (dens[z, y, x] > 0 or corner) else bg
+        assert np.array_equal(r, want), (x, y, z)
+    # brick grid
+    bgo = O.Brickgrid(dens)
+    brange, _, mips = g.download_bricks()
+    assert np.array_equal(brange, bgo.range)
+    for lvl in range(3):
+        assert np.array_equal(mips[lvl], bgo.mips[lvl])
+    # paths through the grid
+    cfg = dict(width=16, height=16, spp=2, max_depth=8, cam_pos=(0.3, 0.2, 2.0), cam_look_at=(0, 0, 0),
+               cam_up=(0, 1, 0), fov_deg=40.0, seed=5, stream=0, use_brick_grid=True)
+    inst = [dict(grid=0, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))]
+    assert _paths_agree(ctx, [desc], inst, scenes.synth_env("sky", 64, 32), cfg) >= 0.99
+
+
+def test_zero_density_grid_shows_the_environment(ctx, c1):
+    rec = np.array(c1["grids"][0]["records"])
+    rec[..., :4] = 0  # density 0 everywhere
+    desc = dict(c1["grids"][0], records=rec)
+    env_rgb = scenes.synth_env("sky", 64, 32)
+    g = ctx.upload_grid(desc)
+    env = ctx.upload_env(env_rgb)
+    cfg = c1["render"]
+    img = ctx.render(cfg, [g], c1["instances"], env)
+    ref = O.Scene([desc], c1["instances"], env_rgb, "f32").render(cfg)
+    assert np.isfinite(img).all() and img.max() > 0
+    assert relmse(img, ref) <= 1e-5
+    rng, _, _ = g.download_bricks()
+    assert np.all((rng & 0xFFFF) == (rng >> 16))  # every brick empty (min == max)
+
+
+def test_scene_without_media(ctx, c1):
+    env_rgb = scenes.synth_env("sky", 64, 32)
+    env = ctx.upload_env(env_rgb)
+    cfg = c1["render"]
+    img = ctx.render(cfg, [], [], env)
+    ref = O.Scene([], [], env_rgb, "f32").render(cfg)
+    assert img.max() > 0
+    assert relmse(img, ref) <= 1e-6
+
+
+def test_env_with_zero_rows_and_columns(ctx, c1):
+    """Importance sampling of an env whose CDFs have long flat stretches and ties."""
+    H, W = 32, 64
+    env_rgb = np.zeros((H, W, 3), np.float32)
+    env_rgb[10:12, 40:43] = 500.0          # a small bright source
+    env_rgb[20:24, :] = 0.5                # one lit band of rows
+    env_rgb[:, 5] = 0.0                    # a dark column through the band
+    cfg = dict(c1["render"], width=24, height=24, spp=2)
+    assert _paths_agree(ctx, c1["grids"], c1["instances"], env_rgb, cfg) >= 0.99
+    env = ctx.upload_env(env_rgb)
+    grids = [ctx.upload_grid(g) for g in c1["grids"]]
+    img = ctx.render(cfg, grids, c1["instances"], env)
+    ref = O.Scene(c1["grids"], c1["instances"], env_rgb, "f32").render(cfg)
+    assert relmse(img, ref) <= 1e-5
+
+
+def test_argument_errors(ctx, c1):
+    grids = [ctx.upload_grid(g) for g in c1["grids"]]
+    env = ctx.upload_env(c1["env"])
+    bad = [dict(c1["render"], width=0), dict(c1["render"], spp=0), dict(c1["render"], max_depth=-1)]
+    for cfg in bad:
+        with pytest.raises(r2.R2Error) as e:
+            ctx.render(cfg, grids, c1["instances"], env)
+        assert e.value.code == -1  # R2_ERR_INVALID
+    packed = ctx.upload_grid(dict(c1["grids"][0], precision="packed"))
+    two = [dict(grid=0, pos=(-0.3, 0, 0)), dict(grid=1, pos=(0.3, 0, 0))]
+    with pytest.raises(r2.R2Error) as e:
+        ctx.render(c1["render"], [grids[0], packed], two, env)
+    assert e.value.code == -4  # R2_ERR_UNSUPPORTED: one record precision per frame
+    with pytest.raises(r2.R2Error) as e:
+        ctx.render(c1["render"], grids * 5, [dict(grid=k) for k in range(5)], env)
+    assert e.value.code == -4  # at most 4 medium instances
