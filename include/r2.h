@@ -103,6 +103,14 @@ int r2_grid_lookup(r2_ctx* ctx, const r2_grid* grid, int32_t n, const int32_t* c
  * density, S (xx yy zz xy xz yz), diffuse rgb, specular rgb, normal xyz, ior.
  * A packed grid answers from its packed records. */
 int r2_grid_sample(r2_ctx* ctx, const r2_grid* grid, int32_t n, const float* pos, float* out);
+/* Parity probe: the majorant-DDA walk of the free-flight loop of CNVDBMedium::sampleDDA
+ * (raytracer/include/medium/nvdb_medium_impl.hpp:139-187) with stepDDA (:241-246) and the
+ * mip schedule (start 3, +0.25 per step, -2 after a tentative collision, :13-15,158,184),
+ * without density decisions.  rays7: n x (origin xyz, direction xyz, tmax) in voxel space;
+ * collide_every = k > 0 applies a collision's mip drop after every k-th step (0: never).
+ * cells: n x max_cells x (brick x, y, z, mip) of the majorant lookups; counts: cells per ray. */
+int r2_dda_cells(r2_ctx* ctx, int32_t n, const float* rays7, float sigma_t, int32_t collide_every,
+                 int32_t max_cells, int32_t* cells, int32_t* counts);
 
 /* ----------------------------------------------------------------- env -- */
 /* Lat-long HDR environment map with sin(theta)-weighted 2-D importance sampling.

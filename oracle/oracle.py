@@ -120,7 +120,7 @@ def lib() -> C.CDLL:
             g("r2o_scene_info").argtypes = [vp, C.c_int, vp]
             g("r2o_filtered_data").argtypes = [vp, C.c_int, vp, vp]
             g("r2o_decode_record").argtypes = [vp, vp]
-            g("r2o_dda_cells").argtypes = [vp, C.c_int, vp, R, C.c_int, vp]
+            g("r2o_dda_cells").argtypes = [vp, C.c_int, vp, R, C.c_int, C.c_int, vp]
             g("r2o_dda_cells").restype = C.c_int
             g("r2o_fresnel").argtypes = [R, R]
             g("r2o_fresnel").restype = R
@@ -354,10 +354,13 @@ class Scene:
         hit = self._fn("r2o_intersect")(self.h, _p(r), _p(out))
         return hit, out
 
-    def dda_cells(self, ray7, sigma_t: float = 1.0, max_cells: int = 4096, medium: int = 0) -> np.ndarray:
+    def dda_cells(self, ray7, sigma_t: float = 1.0, max_cells: int = 4096, medium: int = 0,
+                  collide_every: int = 0) -> np.ndarray:
+        """(brick x, y, z, mip) of every majorant lookup of a DDA walk; ray7 = origin,
+        direction, tmax in voxel space."""
         r = np.ascontiguousarray(ray7, self.R)
         cells = np.zeros((max_cells, 4), np.int32)
-        n = self._fn("r2o_dda_cells")(self.h, medium, _p(r), sigma_t, max_cells, _p(cells))
+        n = self._fn("r2o_dda_cells")(self.h, medium, _p(r), sigma_t, collide_every, max_cells, _p(cells))
         return cells[:n].copy()
 
 

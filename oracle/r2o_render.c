@@ -1406,10 +1406,14 @@ void SFX(r2o_decode_record)(const uint8_t* rec, R* out17) {
   out17[13] = r.normal.x; out17[14] = r.normal.y; out17[15] = r.normal.z;
   out17[16] = r.ior;
 }
-/* DDA cell walk of one free-flight traversal, with an infinite tau so that it
- * never collides: records (brick x, y, z, mip) of every majorant lookup.
+/* DDA cell walk of one free-flight traversal (the loop of sampleDDA,
+ * nvdb_medium_impl.hpp:139-187) without its density decisions: records (brick x, y,
+ * z, mip) of every majorant lookup.  collide_every = k > 0 applies the mip drop of a
+ * tentative collision (mip - 2, floored at 0, nvdb_medium_impl.hpp:184) after every
+ * k-th step, so that the finer mips are walked too; 0 never collides.
  * ray: origin(3), dir(3), tmax in voxel ("scaled index") space. */
-int SFX(r2o_dda_cells)(const scene_t* s, int medium, const R* ray7, R sigma_t, int max_cells, int32_t* cells) {
+int SFX(r2o_dda_cells)(const scene_t* s, int medium, const R* ray7, R sigma_t, int collide_every, int max_cells,
+                       int32_t* cells) {
   const medium_t* m = &s->media[medium];
   const v3 o = V(ray7[0], ray7[1], ray7[2]), d = V(ray7[3], ray7[4], ray7[5]);
   const R tmax = ray7[6];
@@ -1428,6 +1432,7 @@ int SFX(r2o_dda_cells)(const scene_t* s, int medium, const R* ray7, R sigma_t, i
     t += dt / sigma_t;
     if (t == prev) break;
     mip = gmin(mip + C(0.25), C(3));
+    if (collide_every > 0 && n % collide_every == 0) mip = gmax(C(0), mip - C(2));
   }
   return n;
 }

@@ -123,6 +123,17 @@ class Context:
                                                out.ctypes.data, cnt.ctypes.data if counters else None))
         return (out, cnt[:, :7]) if counters else out
 
+    def dda_cells(self, rays7, sigma_t: float = 1.0, collide_every: int = 0, max_cells: int = 4096):
+        """Majorant-DDA walks (parity probe): for each ray (origin, direction, tmax in
+        voxel space) the (brick x, y, z, mip) of every majorant lookup."""
+        rays7 = np.ascontiguousarray(rays7, np.float32).reshape(-1, 7)
+        n = rays7.shape[0]
+        cells = np.zeros((n, max_cells, 4), np.int32)
+        counts = np.zeros(n, np.int32)
+        self._check(capi.lib().r2_dda_cells(self.h, n, rays7.ctypes.data, C.c_float(sigma_t), collide_every,
+                                            max_cells, cells.ctypes.data, counts.ctypes.data))
+        return [cells[i, :counts[i]].copy() for i in range(n)]
+
     # -------------------------------------------------------------- filter
     def upload_mesh(self, mesh: dict) -> "Mesh":
         keep = {}

@@ -166,6 +166,8 @@ SIGNATURES = [
     ("r2_grid_download_bricks", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("r2_grid_lookup", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("r2_dda_cells", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
+                               C.c_void_p]),
     ("r2_env_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     ("r2_env_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("r2_render", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_int32]),

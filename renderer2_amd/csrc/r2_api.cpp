@@ -267,6 +267,16 @@ int r2_grid_sample(r2_ctx* ctx, const r2_grid* g, int32_t n, const float* pos, f
   return grid_sample(ctx, g, n, pos, out);
 }
 
+int r2_dda_cells(r2_ctx* ctx, int32_t n, const float* rays7, float sigma_t, int32_t collide_every, int32_t max_cells,
+                 int32_t* cells, int32_t* counts) {
+  if (!ctx || n < 0 || max_cells <= 0 || !(sigma_t > 0.f) || collide_every < 0 ||
+      (n > 0 && (!rays7 || !cells || !counts)))
+    return R2_ERR_INVALID;
+  if ((int64_t)n * max_cells > (1ll << 28)) return fail(ctx, R2_ERR_UNSUPPORTED, "dda_cells: at most 2^28 cells");
+  if (n == 0) return R2_OK;
+  return dda_cells(ctx, n, rays7, sigma_t, collide_every, max_cells, cells, counts);
+}
+
 int r2_env_upload(r2_ctx* ctx, const float* rgb, int32_t W, int32_t H, r2_env** out) {
   if (!ctx || !rgb || !out || W < 2 || H < 2) return fail(ctx, R2_ERR_INVALID, "env: need at least 2x2 texels");
   *out = nullptr;

@@ -96,6 +96,39 @@ __global__ void k_pack(const uint4* __restrict__ leaves, size_t n, uint4* __rest
   }
 }
 
+// majorant-DDA walk of the free-flight loop without its density decisions: the
+// same position, mip and step arithmetic as traversal_step, recording (brick x, y,
+// z, mip) of every majorant lookup; collide_every = k > 0 applies a tentative
+// collision's mip drop after every k-th step
+__global__ void k_dda_cells(const float* __restrict__ rays, int n, float sigma_t, int collide_every, int max_cells,
+                            int32_t* __restrict__ cells, int32_t* __restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = rays + (size_t)7 * i;
+  const v3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+  const float tmax = r[6];
+  const v3 ri = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+  float t = 0.f, mip = 3.f;
+  int c = 0;
+  int32_t* out = cells + (size_t)i * max_cells * 4;
+  for (int it = 0; it < kLoopCap && t < tmax && c < max_cells; ++it) {
+    const v3 cur = o + t * d;
+    const int imip = (int)roundf(mip);
+    out[4 * c + 0] = ((int)floorf(cur.x)) >> 3;
+    out[4 * c + 1] = ((int)floorf(cur.y)) >> 3;
+    out[4 * c + 2] = ((int)floorf(cur.z)) >> 3;
+    out[4 * c + 3] = imip;
+    ++c;
+    const float dt = step_dda(cur, ri, imip);
+    const float prev = t;
+    t += dt / sigma_t;
+    if (t == prev) break;
+    mip = gmin(mip + 0.25f, 3.f);
+    if (collide_every > 0 && c % collide_every == 0) mip = gmax(0.f, mip - 2.f);
+  }
+  counts[i] = c;
+}
+
 // filtered (trilinear, decoded) records at normalised positions: 17 floats each
 // (density, S6, diffuse3, specular3, normal3, ior)
 __global__ void k_sample(DGrid g, int n, const float* __restrict__ pos, float* __restrict__ out) {
@@ -464,6 +497,27 @@ int grid_sample(r2_ctx* ctx, const r2_grid* g, int n, const float* pos, float* o
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipStreamSynchronize(ctx->stream));
   R2_HIP(ctx, hipMemcpy(out, dout, (size_t)n * 17 * 4, hipMemcpyDeviceToHost));
+  return R2_OK;
+}
+
+int dda_cells(r2_ctx* ctx, int n, const float* rays, float sigma_t, int collide_every, int max_cells, int32_t* cells,
+              int32_t* counts) {
+  void *dr = nullptr, *dc = nullptr, *dn = nullptr;
+  struct F {
+    void** q[3];
+    ~F() { for (auto x : q) if (*x) (void)hipFree(*x); }
+  } f{{&dr, &dc, &dn}};
+  const size_t cbytes = (size_t)n * max_cells * 16;
+  R2_HIP(ctx, hipMalloc(&dr, (size_t)n * 28));
+  R2_HIP(ctx, hipMalloc(&dc, cbytes));
+  R2_HIP(ctx, hipMalloc(&dn, (size_t)n * 4));
+  R2_HIP(ctx, hipMemcpy(dr, rays, (size_t)n * 28, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_dda_cells, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, (const float*)dr, n, sigma_t,
+                     collide_every, max_cells, (int32_t*)dc, (int32_t*)dn);
+  R2_HIP(ctx, hipGetLastError());
+  R2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  R2_HIP(ctx, hipMemcpy(cells, dc, cbytes, hipMemcpyDeviceToHost));
+  R2_HIP(ctx, hipMemcpy(counts, dn, (size_t)n * 4, hipMemcpyDeviceToHost));
   return R2_OK;
 }
 

@@ -73,6 +73,8 @@ int hip_fail(r2_ctx* ctx, hipError_t e, const char* what);
 int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g);
 int grid_lookup(r2_ctx* ctx, const r2_grid* g, int n, const int32_t* coords, uint8_t* out);
 int grid_sample(r2_ctx* ctx, const r2_grid* g, int n, const float* pos, float* out);
+int dda_cells(r2_ctx* ctx, int n, const float* rays, float sigma_t, int collide_every, int max_cells, int32_t* cells,
+              int32_t* counts);
 // render kernels (r2_render.hip)
 int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum);
 int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,

@@ -460,3 +460,65 @@ def test_empty_volume_shows_env():
         O.lib().r2o_camera_ray_f64(C.byref(p), int(k % 16), int(k // 16), 0, r.ctypes.data)
         le = sc.env_le(r[3:6])
         assert np.allclose(out[k], le[:3], rtol=1e-9)
+
+
+def _textbook_cells(o, d, tmax, size):
+    """Cells of edge `size` crossed by o + t d, t in [0, tmax), in order, with the
+    t-length of each crossing (Amanatides & Woo 1987, in float64)."""
+    o = np.asarray(o, np.float64)
+    d = np.asarray(d, np.float64)
+    cell = np.floor(o / size).astype(np.int64)
+    step = np.where(d >= 0, 1, -1)
+    with np.errstate(divide="ignore"):
+        nxt = np.where(d >= 0, (cell + 1) * size, cell * size)
+        tnext = np.where(d != 0, (nxt - o) / d, np.inf)
+        tdelta = np.where(d != 0, size / np.abs(d), np.inf)
+    out, t = [], 0.0
+    while t < tmax:
+        a = int(np.argmin(tnext))
+        t_exit = min(tnext[a], tmax)
+        out.append((tuple(cell), t_exit - t))
+        t = tnext[a]
+        cell[a] += step[a]
+        tnext[a] += tdelta[a]
+    return out
+
+
+def test_dda_walk_axis_aligned_known_answer():
+    """stepDDA lands half a voxel past each face of the 64-voxel mip-3 cells:
+    x = 0.25, 64.25, 128.5, 192.5, then 256.5 ends the walk."""
+    sc, _ = _scene(np.ones((4, 8, 3), np.float32), precision="f32")
+    cells = sc.dda_cells([0.25, 3.0, 3.0, 1.0, 0.0, 0.0, 256.0])
+    assert cells.tolist() == [[0, 0, 0, 3], [8, 0, 0, 3], [16, 0, 0, 3], [24, 0, 0, 3]]
+    back = sc.dda_cells([255.75, 3.0, 3.0, -1.0, 0.0, 0.0, 256.0])
+    assert [c[0] >> 3 for c in back] == [3, 2, 1, 0]
+
+
+def test_dda_walk_follows_the_line():
+    """Against the textbook cell enumeration: the walk's mip-3 cells appear in the
+    order the line crosses them, and every cell the line crosses for longer than
+    the walk's half-voxel overshoot is visited."""
+    sc, _ = _scene(np.ones((4, 8, 3), np.float32), precision="f64")
+    rng = np.random.default_rng(17)
+    for _ in range(300):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        if np.min(np.abs(d)) < 0.3:
+            continue
+        o = rng.uniform(0, 256, 3)
+        tmax = float(rng.uniform(50, 400))
+        walk = sc.dda_cells(list(o) + list(d) + [tmax])
+        seen = []
+        for c in walk:
+            assert c[3] == 3
+            cc = (int(c[0]) >> 3, int(c[1]) >> 3, int(c[2]) >> 3)
+            if not seen or seen[-1] != cc:
+                seen.append(cc)
+        book = _textbook_cells(o, d, tmax, 64.0)
+        order = [c for c, _ in book]
+        pos = [order.index(c) for c in seen]
+        assert pos == sorted(pos) and len(set(pos)) == len(pos)
+        overshoot = 0.5 / np.min(np.abs(d))
+        for c, chord in book:
+            if chord > overshoot + 1e-6:
+                assert c in seen, (o, d, tmax, c, chord)
