@@ -337,9 +337,8 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + (uint32_t)wave));
   if (w >= n_work) return;
-  float* V = lds + wave * (kVals * 64 + 64 + ts_cap);  // [64][kVals] per-ray values (later: exp terms)
-  int* F = (int*)(V + kVals * 64);                      // [64] hit flags
-  float* TS = V + kVals * 64 + 64;                      // [ts_cap] clip distances of the hits, in ray order
+  float* V = lds + wave * (kVals * 64 + ts_cap);  // [64][kVals] per-ray values (later: exp terms)
+  float* TS = V + kVals * 64;                      // [ts_cap] clip distances of the hits, in ray order
 
   uint32_t vid, beg = 0, nc = 0;
   size_t orec;
@@ -445,14 +444,47 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
     }
 #pragma unroll
     for (int q = 0; q < kVals; ++q) V[lane * kVals + q] = val[q];
-    F[lane] = hit ? 1 : 0;
     const unsigned long long hb = __ballot(hit);
     if (hit) TS[hits + (uint32_t)__popcll(hb & ((1ull << lane) - 1ull))] = cur_tmax;
     wave_sync();
-    const int nr = min(64, c.n_samples - r0);
+    // ray-order sums over the hit rays of this batch: the wave-uniform hit mask
+    // drives the loop (no per-ray flag read), four hits per step so that their
+    // LDS reads are in flight together; the additions keep ray order
     if (lane < kVals) {
-      for (int r = 0; r < nr; ++r)
-        if (F[r]) acc += V[r * kVals + lane];
+      unsigned long long m = hb;
+      while (m) {
+        const int ra = __ffsll(m) - 1;
+        m &= m - 1;
+        if (!m) {
+          acc += V[ra * kVals + lane];
+          break;
+        }
+        const int rb = __ffsll(m) - 1;
+        m &= m - 1;
+        if (!m) {
+          const float va = V[ra * kVals + lane], vb = V[rb * kVals + lane];
+          acc += va;
+          acc += vb;
+          break;
+        }
+        const int rc = __ffsll(m) - 1;
+        m &= m - 1;
+        if (!m) {
+          const float va = V[ra * kVals + lane], vb = V[rb * kVals + lane], vc = V[rc * kVals + lane];
+          acc += va;
+          acc += vb;
+          acc += vc;
+          break;
+        }
+        const int rd = __ffsll(m) - 1;
+        m &= m - 1;
+        const float va = V[ra * kVals + lane], vb = V[rb * kVals + lane], vc = V[rc * kVals + lane],
+                    vd = V[rd * kVals + lane];
+        acc += va;
+        acc += vb;
+        acc += vc;
+        acc += vd;
+      }
     }
     hits += (uint32_t)__popcll(hb);
     wave_sync();
@@ -468,9 +500,19 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
     const float P = (float)hits / (float)c.n_samples;
     float dens = -det_logf(1.f - P) / (avg * c.sigma_t);
     const float na = c.alpha / c.sigma_t;
+    // with at most 64 rays, lane h keeps the clip distance of hit h in a register
+    // and the hit-order sum of the exp terms reads them lane by lane (readlane)
+    const bool lane_sums = c.clip && c.n_samples <= 64;
+    const float ts_l = (lane_sums && (uint32_t)lane < hits) ? TS[lane] : 0.f;
     for (int it = 0; it < c.iterations; ++it) {
       float Pv;
-      if (c.clip) {
+      if (lane_sums) {
+        const float term = (uint32_t)lane < hits ? det_expf(-dens * c.sigma_t * ts_l) : 0.f;
+        const int tb = __float_as_int(term);
+        float e = 0.f;
+        for (uint32_t h = 0; h < hits; ++h) e += __int_as_float(__builtin_amdgcn_readlane(tb, (int)h));
+        Pv = 1.f - e / (float)hits;
+      } else if (c.clip) {
         for (uint32_t h = (uint32_t)lane; h < hits; h += 64) V[h] = det_expf(-dens * c.sigma_t * TS[h]);
         wave_sync();
         float e = 0.f;
@@ -722,7 +764,7 @@ int launch_filter(r2_ctx* ctx, const r2_mesh* m, const FConst& c, const Binning&
     return R2_OK;
   }
   const int ts_cap = ((c.n_samples + 63) / 64) * 64;
-  const size_t lds = sizeof(float) * kWaves * (kVals * 64 + 64 + ts_cap);
+  const size_t lds = sizeof(float) * kWaves * (kVals * 64 + ts_cap);
   const unsigned blocks = (n_work + kWaves - 1) / kWaves;
   R2F_HIP(hipEventRecord(ctx->ev0, ctx->stream), "filter: event");
   hipLaunchKernelGGL(k_filter, dim3(blocks), dim3(256), lds, ctx->stream, c, work, n_work, direct ? 1 : 0, B.act,
