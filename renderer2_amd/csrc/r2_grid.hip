@@ -454,6 +454,21 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
     const float bc = (b < c) ? c : b;
     d.sigma_t = (a < bc) ? bc : a;
   }
+  {
+    // a positive normal float is a power of two iff its mantissa bits are zero
+    auto pow2 = [](float v) {
+      uint32_t u;
+      memcpy(&u, &v, 4);
+      return v > 0.f && (u & 0x007fffffu) == 0u && ((u >> 23) & 0xffu) != 0u && ((u >> 23) & 0xffu) != 0xffu;
+    };
+    d.size_pow2 = 1;
+    for (int a = 0; a < 3; ++a) {
+      d.inv_size[a] = 1.f / (float)d.size[a];
+      d.size_pow2 = d.size_pow2 && pow2((float)d.size[a]);
+    }
+    d.inv_sigma_t = 1.f / d.sigma_t;
+    d.sigma_t_pow2 = pow2(d.sigma_t) ? 1 : 0;
+  }
   for (int a = 0; a < 3; ++a) {
     d.sigma_s[a] = desc->sigma_s[a];
     g->sigma_a[a] = desc->sigma_a[a];

@@ -150,7 +150,7 @@ __device__ __forceinline__ bool traversal_step(const DScene& s, PathLane& p, Cnt
     const float maj = ratio * lookup_majorant<kCount>(g, cur, imip, cnt);
     const float dt = step_dda(cur, p.tri, imip);
     const float prev = p.tt;
-    p.tt += dt / g.sigma_t;
+    p.tt += div_sigma_t(g, dt);
     if (p.tt == prev) return true;
     p.ttau -= maj * dt;
     p.tmip = gmin(p.tmip + 0.25f, 3.f);
@@ -185,7 +185,7 @@ __device__ __forceinline__ bool traversal_step(const DScene& s, PathLane& p, Cnt
   ++p.tsteps;
   count<kCount>(cnt, kCntFlight);
   const float inv_max = 1.f / (ratio / g.inv_max_density);
-  p.tt -= logf(1.f - p.rng.uni()) * inv_max / g.sigma_t;
+  p.tt -= div_sigma_t(g, logf(1.f - p.rng.uni()) * inv_max);
   if (p.tt >= p.ttmax) return true;
   const Rec fd = filtered_data<kCount>(g, p.to + p.tt * p.td, cnt);
   if (p.tk == TK_FLIGHT_DELTA) {
@@ -219,8 +219,8 @@ __device__ __forceinline__ v3 flight_collision(const DScene& s, PathLane& p, Cnt
   const Ray ir = mkray(p.to, p.td, p.tt, -1);
   if (p.tk == TK_FLIGHT_DDA) {
     const v3 sp = p.to + p.tt * p.td;
-    const v3 pos = mk((sp.x - (float)g.ibb_min[0]) / (float)g.size[0], (sp.y - (float)g.ibb_min[1]) / (float)g.size[1],
-                      (sp.z - (float)g.ibb_min[2]) / (float)g.size[2]);
+    const v3 pos = mk(div_size(g, sp.x - (float)g.ibb_min[0], 0), div_size(g, sp.y - (float)g.ibb_min[1], 1),
+                      div_size(g, sp.z - (float)g.ibb_min[2], 2));
     fd = filtered_data<kCount>(g, pos, cnt);
     rw = ray_transform2(ir, g.s2m);
   } else {

@@ -36,6 +36,11 @@ struct DGrid {
   float sigma_s[3];
   float sigma_t;
   float inv_max_density;
+  // reciprocals of the index extent and of sigma_t, used in place of a division
+  // when the divisor is a power of two (then x * (1/d) and x / d are the same number)
+  float inv_size[3];
+  float inv_sigma_t;
+  int32_t size_pow2, sigma_t_pow2;
 };
 
 struct DInstance {

@@ -507,6 +507,16 @@ __device__ float tr_dda(const DGrid& g, const Ray& rmodel, Rng& rng, float ratio
   return tr;
 }
 
+// x / size[i] and x / sigma_t of a grid: with a power-of-two divisor the product
+// with the exact reciprocal is the same number as the IEEE quotient, so the
+// (wave-uniform) branch skips the division without changing any result
+__device__ __forceinline__ float div_size(const DGrid& g, float x, int i) {
+  return g.size_pow2 ? x * g.inv_size[i] : x / (float)g.size[i];
+}
+__device__ __forceinline__ float div_sigma_t(const DGrid& g, float x) {
+  return g.sigma_t_pow2 ? x * g.inv_sigma_t : x / g.sigma_t;
+}
+
 // CNVDBMedium::moveToVoxelBorder (nvdb_medium_impl.hpp:296-320), ray in model space
 __device__ __forceinline__ Ray medium_move_to_voxel_border(const DGrid& g, const Ray& ray) {
   const Ray ri = ray_transform(ray, g.m2i);
@@ -518,8 +528,8 @@ __device__ __forceinline__ Ray medium_move_to_voxel_border(const DGrid& g, const
   for (int i = 0; i < 3; ++i) {
     const float size = (float)g.size[i], ibb = (float)g.ibb_min[i];
     const float ip = o[i] * size + ibb;
-    const float vmin = (floorf(ip) - ibb) / size;
-    const float vmax = (ceilf(ip) - ibb) / size;
+    const float vmin = div_size(g, floorf(ip) - ibb, i);
+    const float vmax = div_size(g, ceilf(ip) - ibb, i);
     const float inv = 1.f / d[i];
     float tn = (vmin - o[i]) * inv;
     float tf = (vmax - o[i]) * inv;
