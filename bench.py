@@ -78,6 +78,8 @@ def main(argv=None) -> int:
     ap.add_argument("--rate", type=float, default=5.0, help="Poisson arrival rate (requests/s) of the serving workload")
     ap.add_argument("--cpu-voxels", type=int, default=1_500_000,
                     help="random voxels per LOD of the filter CPU baseline sample")
+    ap.add_argument("--render-steps", type=int, default=2,
+                    help="c3: 1024^2 x 64 spp renders of each filtered LOD (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB copied device-to-device before rendering (reference kernel for PMC byte counters)")
     a = ap.parse_args(argv)
@@ -420,6 +422,39 @@ def bench_filter(a) -> int:
         got = ctx.filter_voxels(gm, params[0], ids.astype(np.uint32))
         parity = {"records_bitexact": float(np.mean(np.all(got == ref, axis=1))), "voxels": int(len(ids)),
                   "nonempty": int(np.sum(ref[:, :4].copy().view(np.float32)[:, 0] > 0))}
+    # each LOD rendered at 1024^2, 64 spp under the constant white environment of
+    # the 64^3 workload (its camera and depth); the mesh is scaled to unit extent
+    # and the media take sigma_a = 0, sigma_s = the filter's SigmaT, as the
+    # reference's example scene does for filtered meshes
+    render = None
+    if a.render_steps > 0:
+        rw = scenes.make_workload("c1")
+        env = ctx.upload_env(rw["env"])
+        ext = float((mesh["positions"].max(axis=0) - mesh["positions"].min(axis=0)).max())
+        sc = 1.0 / ext
+        inst = [dict(grid=0, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(sc, sc, sc))]
+        rcfg = dict(rw["render"], width=1024, height=1024, spp=64)
+        frame = _hip.DeviceBuffer(1024 * 1024 * 12)
+        mps, kern = [], []
+        for fg, vs, b in zip(grids, lods, bufs):
+            gd = scenes.filtered_grid(fg, vs, records_dev=b.addr)
+            gd["sigma_a"], gd["sigma_s"] = (0.0,) * 3, (float(w["sigma_t"]),) * 3
+            g = ctx.upload_grid(gd)
+            ctx.render_device(dict(rcfg, sample_begin=1_000_000), [g], inst, env, frame.addr)
+            _hip.synchronize()
+            t0 = time.perf_counter()
+            km = 0.0
+            for k in range(a.render_steps):
+                ctx.render_device(dict(rcfg, sample_begin=k * 64), [g], inst, env, frame.addr)
+                km += ctx.stats()["kernel_ms"]
+            _hip.synchronize()
+            mps.append(round(1024 * 1024 * 64 * a.render_steps / (time.perf_counter() - t0) / 1e6, 2))
+            kern.append(round(km / a.render_steps, 3))
+            g.free()
+        frame.free()
+        env.free()
+        render = {"mpaths_s": mps, "kernel_ms": kern, "frame": "1024x1024, 64 spp per launch, constant white env, "
+                  "depth 8", "steps": a.render_steps}
     line = {
         "metric": "M voxels/s mesh->SGGX filter (3 LODs 512^3/256^3/128^3, 64 rays/voxel)",
         "value": round(value, 3),
@@ -444,6 +479,7 @@ def bench_filter(a) -> int:
         "cpu_baseline": cpu,
         "quality": parity,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "lod_render": render,
     }
     print(json.dumps(line), flush=True)
     for b in bufs:
