@@ -34,12 +34,8 @@ __device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int
 // Persistent path state machine (see r2_pathsm.hpp).  Render mode: item =
 // (sample chunk, pixel), output = chunk sums.  Probe mode (probe_pix != null):
 // item = one (pixel, sample) path, output = its radiance and lookup counters.
-#ifndef R2_EV_MODE
-#define R2_EV_MODE 2      // 0: every event lane advances once per iteration; 1: most populated state
-                          // first; 2: states in the forward order of the state graph (measured best)
-#endif
 #ifndef R2_EV_ITERS
-#define R2_EV_ITERS 12    // event iterations per scheduling round
+#define R2_EV_ITERS 12    // event passes per scheduling round
 #endif
 #ifndef R2_TR_MIN_LANES
 #define R2_TR_MIN_LANES 16  // leave the traversal round when fewer lanes are still traversing
@@ -152,23 +148,6 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
                                           Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
                                           float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
   // ---- shading events, batched across the wavefront
-#if R2_EV_MODE == 0
-  for (int k = 0; k < R2_EV_ITERS; ++k) {
-    const bool ev = p.stage >= ST_START;
-    if (!__any(ev)) break;
-    SM_STAT(1, 1);
-#ifdef R2_SM_STATS
-    for (int st = ST_START; st <= ST_FINISH; ++st) {
-      const int c = __popcll(__ballot(p.stage == st));
-      if (c) { SM_STAT(16 + 2 * st, 1); SM_STAT(17 + 2 * st, c); }
-    }
-#endif
-    if (ev) {
-      if (p.stage == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-      else event_step<kSingle, kCount>(s, p, cnt, p.stage);
-    }
-  }
-#elif R2_EV_MODE == 2
 #ifdef R2_SM_STATS
   const uint64_t te0 = clock64();
 #endif
@@ -203,24 +182,6 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
   }
 #ifdef R2_SM_STATS
   SM_STAT(4, clock64() - te0);
-#endif
-#else
-  // run the most populated event state each iteration (wave-uniform branch)
-  for (int k = 0; k < R2_EV_ITERS; ++k) {
-    int best = -1, bestc = 0;
-    for (int st = ST_START; st <= ST_FINISH; ++st) {
-      const int c = __popcll(__ballot(p.stage == st));
-      if (c > bestc) {
-        bestc = c;
-        best = st;
-      }
-    }
-    if (best < 0) break;
-    if (p.stage == best) {
-      if (best == ST_FINISH) finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-      else event_step<kSingle, kCount>(s, p, cnt, best);
-    }
-  }
 #endif
   // ---- traversal rounds: every traversing lane advances one DDA / tracking step
 #ifdef R2_SM_STATS

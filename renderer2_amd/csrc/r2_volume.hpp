@@ -385,128 +385,6 @@ __device__ __forceinline__ v3 collision_albedo(const DGrid& g, const Rec& fd, co
   return (b * mk(g.sigma_s[0], g.sigma_s[1], g.sigma_s[2])) / g.sigma_t;
 }
 
-// CNVDBMedium::sampleInternal (nvdb_medium_impl.hpp:94-137)
-template <bool kCount>
-__device__ v3 sample_internal(const DGrid& g, Ray& rmodel, Rng& rng, float ratio, MHit& mi, Cnt& cnt) {
-  Ray ray = ray_transform2(rmodel, g.m2i);
-  float t = 0.f;
-  const float inv_max = 1.f / (ratio / g.inv_max_density);
-  for (int it = 0; it < kLoopCap; ++it) {
-    count<kCount>(cnt, kCntFlight);
-    t -= logf(1.f - rng.uni()) * inv_max / g.sigma_t;
-    if (t >= ray.tmax) break;
-    const Rec fd = filtered_data<kCount>(g, ray.o + t * ray.d, cnt);
-    if (ratio * fd.density * inv_max > rng.uni()) {
-      ray.tmax = t;
-      const Ray rw = ray_transform2(ray, g.i2m);
-      rmodel.tmax = rw.tmax;
-      return collision_albedo(g, fd, rw, mi);
-    }
-  }
-  return mk(1.f, 1.f, 1.f);
-}
-
-// CNVDBMedium::sampleDDA (nvdb_medium_impl.hpp:139-187)
-template <bool kCount>
-__device__ v3 sample_dda(const DGrid& g, Ray& rmodel, Rng& rng, float ratio, MHit& mi, Cnt& cnt) {
-  Ray iray = ray_transform2(rmodel, g.m2s);
-  const v3 ri = mk(1.f / iray.d.x, 1.f / iray.d.y, 1.f / iray.d.z);
-  float t = 0.f;
-  float tau = -logf(1.f - rng.uni());
-  float mip = 3.f;
-  for (int it = 0; it < kLoopCap && t < iray.tmax; ++it) {
-    count<kCount>(cnt, kCntDda);
-    const v3 cur = iray.o + t * iray.d;
-    const int imip = (int)roundf(mip);
-    const float maj = ratio * lookup_majorant<kCount>(g, cur, imip, cnt);
-    const float dt = step_dda(cur, ri, imip);
-    const float prev = t;
-    t += dt / g.sigma_t;
-    if (t == prev) break;
-    tau -= maj * dt;
-    mip = gmin(mip + 0.25f, 3.f);
-    if (tau > 0.f) continue;
-    t += tau / (maj * g.sigma_t);
-    if (t >= iray.tmax) break;
-    const float u0 = rng.uni(), u1 = rng.uni(), u2 = rng.uni();
-    const float d = lookup_density<kCount>(g, iray.o + t * iray.d, mk(u0, u1, u2), cnt);
-    if (rng.uni() * maj < d * ratio) {
-      iray.tmax = t;
-      const v3 sp = iray.o + t * iray.d;
-      const v3 pos = mk((sp.x - (float)g.ibb_min[0]) / (float)g.size[0], (sp.y - (float)g.ibb_min[1]) / (float)g.size[1],
-                        (sp.z - (float)g.ibb_min[2]) / (float)g.size[2]);
-      const Rec fd = filtered_data<kCount>(g, pos, cnt);
-      const Ray rw = ray_transform2(iray, g.s2m);
-      rmodel.tmax = rw.tmax;
-      return collision_albedo(g, fd, rw, mi);
-    }
-    tau = -logf(1.f - rng.uni());
-    mip = gmax(0.f, mip - 2.f);
-  }
-  return mk(1.f, 1.f, 1.f);
-}
-
-// CNVDBMedium::trInternal (nvdb_medium_impl.hpp:203-239)
-template <bool kCount>
-__device__ float tr_internal(const DGrid& g, const Ray& rmodel, Rng& rng, float ratio, Cnt& cnt) {
-  const Ray ray = ray_transform2(rmodel, g.m2i);
-  float tr = 1.f, t = 0.f;
-  const float inv_max = 1.f / (ratio / g.inv_max_density);
-  for (int it = 0; it < kLoopCap; ++it) {
-    count<kCount>(cnt, kCntFlight);
-    t -= logf(1.f - rng.uni()) * inv_max / g.sigma_t;
-    if (t >= ray.tmax) break;
-    const float d = filtered_data<kCount>(g, ray.o + t * ray.d, cnt).density;
-    tr *= 1.f - gmax(0.f, ratio * d * inv_max);
-    if (tr < 1.f) {
-      const float p = 1.f - tr;
-      if (rng.uni() < p) return 0.f;
-      tr /= 1.f - p;
-    }
-  }
-  return tr;
-}
-
-// CNVDBMedium::trDDA (nvdb_medium_impl.hpp:248-294)
-template <bool kCount>
-__device__ float tr_dda(const DGrid& g, const Ray& rmodel, Rng& rng, float ratio, Cnt& cnt) {
-  const Ray iray = ray_transform2(rmodel, g.m2s);
-  const v3 ri = mk(1.f / iray.d.x, 1.f / iray.d.y, 1.f / iray.d.z);
-  float tr = 1.f, t = 0.f;
-  const float inv_max = 1.f / (ratio / g.inv_max_density);
-  float tau = -logf(1.f - rng.uni());
-  float mip = 3.f;
-  const float vol_maj = 1.f / inv_max;
-  for (int it = 0; it < kLoopCap && t < iray.tmax; ++it) {
-    count<kCount>(cnt, kCntDda);
-    const v3 cur = iray.o + t * iray.d;
-    const int imip = (int)roundf(mip);
-    const float maj = ratio * lookup_majorant<kCount>(g, cur, imip, cnt);
-    const float dt = step_dda(cur, ri, imip);
-    const float prev = t;
-    t += dt / g.sigma_t;
-    if (t == prev) break;
-    tau -= maj * dt;
-    mip = gmin(mip + 0.25f, 3.f);
-    if (tau > 0.f) continue;
-    t += tau / (maj * g.sigma_t);
-    if (t >= iray.tmax) break;
-    const float u0 = rng.uni(), u1 = rng.uni(), u2 = rng.uni();
-    const float d = lookup_density<kCount>(g, iray.o + t * iray.d, mk(u0, u1, u2), cnt);
-    if (rng.uni() * maj < d * ratio) {
-      tr *= gmax(0.f, 1.f - vol_maj / maj);
-      if (tr < 0.1f) {
-        const float prob = 1.f - tr;
-        if (rng.uni() < prob) return 0.f;
-        tr /= 1.f - prob;
-      }
-    }
-    tau = -logf(1.f - rng.uni());
-    mip = gmax(0.f, mip - 2.f);
-  }
-  return tr;
-}
-
 // x / size[i] and x / sigma_t of a grid: with a power-of-two divisor the product
 // with the exact reciprocal is the same number as the IEEE quotient, so the
 // (wave-uniform) branch skips the division without changing any result
@@ -845,36 +723,6 @@ __device__ __forceinline__ Hit scene_intersect(const DScene& s, Ray& ray) {
   return h;
 }
 
-// CMediumInstance::sample (medium_instance_impl.hpp:7-18)
-template <bool kCount>
-__device__ v3 inst_sample(const DScene& s, int k, Ray& rw, Rng& rng, bool dda, MHit& mi, Cnt& cnt) {
-  const DInstance& in = s.inst[k];
-  const DGrid& g = s.grid[in.grid];
-  Ray rm = ray_transform2(rw, in.w2m);
-  MHit h;
-  h.valid = 0;
-  const v3 alb = dda ? sample_dda<kCount>(g, rm, rng, in.ratio, h, cnt) : sample_internal<kCount>(g, rm, rng, in.ratio, h, cnt);
-  const Ray rwn = ray_transform2(rm, in.m2w);
-  rw.tmax = rwn.tmax;
-  mi.valid = h.valid;
-  if (h.valid) {
-    mi.inst = k;
-    mi.pos = mul_point(in.m2w, h.pos);
-    mi.normal = normalize(mul_dir(in.m2w, h.normal));
-#pragma unroll
-    for (int i = 0; i < 6; ++i) mi.S[i] = h.S[i];
-    mi.ior = h.ior;
-    count<kCount>(cnt, kCntScatter);
-  }
-  return alb;
-}
-template <bool kCount>
-__device__ float inst_tr(const DScene& s, int k, const Ray& rw, Rng& rng, bool dda, Cnt& cnt) {
-  const DInstance& in = s.inst[k];
-  const DGrid& g = s.grid[in.grid];
-  const Ray rm = ray_transform2(rw, in.w2m);
-  return dda ? tr_dda<kCount>(g, rm, rng, in.ratio, cnt) : tr_internal<kCount>(g, rm, rng, in.ratio, cnt);
-}
 // CMediumInstance::moveToVoxelBorder (medium_instance_impl.hpp:38-41)
 __device__ __forceinline__ Ray inst_move_to_voxel_border(const DScene& s, int k, const Ray& ray) {
   const DInstance& in = s.inst[k];
@@ -884,65 +732,6 @@ __device__ __forceinline__ Ray inst_move_to_voxel_border(const DScene& s, int k,
   ray_offset(r, -ray.d);
   return r;
 }
-// CDeviceScene::intersectTr (device_scene_impl.hpp:93-114): 1 = blocked by a surface hit
-template <bool kCount>
-__device__ int intersect_tr(const DScene& s, const Ray& ray, Rng& rng, bool dda, float& Tr, Cnt& cnt) {
-  Tr = 1.f;
-  v3 p0 = ray.o;
-  const v3 p1 = p0 + ray.tmax * ray.d;
-  int cur = ray.medium;
-  for (int it = 0; it < 64; ++it) {
-    Ray r = spawn_ray(p0, p1, cur);
-    const Hit h = scene_intersect(s, r);
-    if (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur)) Tr *= inst_tr<kCount>(s, r.medium, r, rng, dda, cnt);
-    if (!h.hit) return 0;
-    p0 = h.pos;
-    cur = r.medium < 0 ? h.medium : -1;
-  }
-  return 1;
-}
-
-// direct (path_integrator_impl.hpp:23-118), medium interactions
-template <bool kCount>
-__device__ v3 direct(const DScene& s, const MHit& mi, v3 wo, Rng& rng, bool dda, Cnt& cnt) {
-  v3 L = mk(0.f, 0.f, 0.f);
-  {
-    v3 ldir;
-    float light_pdf = 0.f;
-    const v3 Le = env_sample<kCount>(s.env, rng, ldir, light_pdf, cnt);
-    if (light_pdf > 0.f) {
-      ldir = normalize(ldir);
-      Ray rl = mkray(mi.pos, ldir, kDefaultTmax, mi.inst);
-      ray_offset(rl, mi.normal);
-      rl = inst_move_to_voxel_border(s, mi.inst, rl);
-      float trs;
-      const int hit = intersect_tr<kCount>(s, rl, rng, dda, trs, cnt);
-      const float p = phase_p(mi.S, mi.normal, mi.ior, wo, ldir, rng);
-      const float mis = (1.f * light_pdf) / (1.f * light_pdf + 1.f * p);
-      const float tr = hit ? 0.f : trs;
-      L = L + ((Le * (mis * p)) * tr) / light_pdf;
-    }
-  }
-  {
-    v3 wi;
-    const float p = phase_sample(mi.S, mi.normal, mi.ior, wo, wi, rng);
-    if (p > 0.f) {
-      wi = normalize(wi);
-      Ray rb = mkray(mi.pos, wi, kDefaultTmax, mi.inst);
-      ray_offset(rb, mi.normal);
-      rb = inst_move_to_voxel_border(s, mi.inst, rb);
-      float trs;
-      const int hit = intersect_tr<kCount>(s, rb, rng, dda, trs, cnt);
-      float light_pdf;
-      const v3 Le = env_le<kCount>(s.env, wi, light_pdf, cnt);
-      const float mis = (1.f * p) / (1.f * p + 1.f * light_pdf);
-      const float tr = hit ? 0.f : trs;
-      L = L + ((Le * (mis * p)) * tr) / p;
-    }
-  }
-  return L;
-}
-
 // CPixelSampler::samplePixel (pixel_sampler.hpp:36-47)
 __device__ __forceinline__ Ray camera_ray(const DScene& s, int px, int py, Rng& rng) {
   const float rh = rng.uni();
@@ -952,63 +741,6 @@ __device__ __forceinline__ Ray camera_ray(const DScene& s, int px, int py, Rng& 
   const v3 d = normalize(mk(h, v, -s.near_dist));
   const Ray vr = mkray(mk(0.f, 0.f, 0.f), d, kDefaultTmax, -1);
   return ray_transform(vr, s.view_to_world);
-}
-
-// CPathIntegrator::Li (path_integrator_impl.hpp:120-247) for media-only scenes
-template <bool kCount>
-__device__ v3 Li(const DScene& s, int px, int py, Rng& rng, Cnt& cnt) {
-  const bool dda = (s.flags & 1u) != 0u;
-  v3 L = mk(0.f, 0.f, 0.f), thr = mk(1.f, 1.f, 1.f);
-  Ray ray = camera_ray(s, px, py, rng);
-  int iters = 0;
-  for (int b = 0; b < s.max_depth; ++b) {
-    if (++iters > 4 * s.max_depth + 64) break;
-    const Hit si = scene_intersect(s, ray);
-    MHit mi;
-    mi.valid = 0;
-    if (si.medium >= 0) {
-      if (ray.medium >= 0) {
-        if (si.medium != ray.medium) break;
-        thr = thr * inst_sample<kCount>(s, ray.medium, ray, rng, dda, mi, cnt);
-      } else {
-        Ray mr = mkray(si.pos, normalize(ray.d), kDefaultTmax, si.medium);
-        ray_offset(mr, ray.d);
-        const Hit se = scene_intersect(s, mr);
-        if (se.hit) {
-          if (se.medium >= 0 && si.medium != se.medium) break;
-          thr = thr * inst_sample<kCount>(s, mr.medium, mr, rng, dda, mi, cnt);
-        }
-      }
-      if (thr.x == 0.f || thr.y == 0.f || thr.z == 0.f) break;
-    }
-    if (mi.valid) {
-      const v3 wo = -ray.d;
-      L = L + thr * direct<kCount>(s, mi, wo, rng, dda, cnt);
-      v3 wi;
-      (void)phase_sample(mi.S, mi.normal, mi.ior, wo, wi, rng);
-      Ray nr = mkray(mi.pos, normalize(wi), kDefaultTmax, mi.inst);
-      ray_offset(nr, wi);
-      ray = inst_move_to_voxel_border(s, mi.inst, nr);
-    } else {
-      if (b == 0 && !si.hit) {
-        float p;
-        L = L + env_le<kCount>(s.env, ray.d, p, cnt) * thr;
-      }
-      if (!si.hit) break;
-      Ray nr = mkray(si.pos, normalize(ray.d), kDefaultTmax, ray.medium < 0 ? si.medium : -1);
-      ray_offset(nr, ray.d);
-      ray = nr;
-      --b;
-      continue;
-    }
-    if (b > 3) {
-      const float p = 1.f - (thr.x + thr.y + thr.z) / 3.f;
-      if (rng.uni() <= p) break;
-      thr = thr / (1.f - p);
-    }
-  }
-  if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z)) return mk(0.f, 0.f, 0.f);
-  return L / (float)s.spp;
 }
 
 }  // namespace r2
