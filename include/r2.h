@@ -246,7 +246,9 @@ typedef struct {
 int r2_server_start(r2_ctx* ctx, const r2_server_config* cfg, r2_server** out);
 /* Returns a request id >= 0, or R2_ERR_BUSY / R2_ERR_OOM (request larger than the budget). */
 int64_t r2_server_submit(r2_server* s, const r2_request* req);
-/* 1 = done (result written), 0 = pending, <0 = error / cancelled. */
+/* 1 = done (result written), 0 = pending, <0 = error / cancelled.  The outcome of a
+ * finished request is reported once, by the first poll or wait that sees it; the id
+ * is then forgotten (later calls return R2_ERR_INVALID). */
 int r2_server_poll(r2_server* s, int64_t id, double* latency_ms);
 int r2_server_wait(r2_server* s, int64_t id, double* latency_ms);
 int r2_server_cancel(r2_server* s, int64_t id);

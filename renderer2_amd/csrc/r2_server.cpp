@@ -336,13 +336,17 @@ extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
   return id;
 }
 
+// state of a request; a finished one is reported once and then forgotten, so the
+// table holds only requests whose outcome has not been collected yet
 static int poll_locked(r2_server* s, int64_t id, double* lat) {
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return R2_ERR_INVALID;
   SrvReq& r = *it->second;
   if (r.state == RQ_QUEUED || r.state == RQ_ACTIVE) return 0;
   if (lat) *lat = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
-  return r.state == RQ_DONE ? 1 : (r.rc < 0 ? r.rc : R2_ERR_HIP);
+  const int rc = r.state == RQ_DONE ? 1 : (r.rc < 0 ? r.rc : R2_ERR_HIP);
+  s->reqs.erase(it);
+  return rc;
 }
 
 extern "C" int r2_server_poll(r2_server* s, int64_t id, double* latency_ms) {

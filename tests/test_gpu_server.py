@@ -57,6 +57,9 @@ def test_server_matches_standalone_renders(ctx, world):
             img, lat = srv.wait(rid)
             assert lat > 0
             assert np.array_equal(img, alone[k]), f"request {k}: max diff {np.abs(img - alone[k]).max()}"
+        with pytest.raises(r2.R2Error) as e:  # an outcome is collected once, then the id is forgotten
+            srv.wait(ids[0])
+        assert e.value.code == -1
         st = srv.stats()
         assert st["completed"] == len(reqs) and st["rejected"] == 0
         assert 0 < st["p50_ms"] <= st["p99_ms"]
