@@ -411,6 +411,7 @@ std::vector<float> load_hdr(const fs::path& p, int* w, int* h) {
   if (sscanf(line.c_str(), "%2s %d %2s %d", ys, h, xs, w) != 4 || std::string(ys) != "-Y" || std::string(xs) != "+X")
     throw std::runtime_error("unsupported HDR orientation: " + line);
   const int W = *w, H = *h;
+  if (W < 2 || H < 2 || (int64_t)W * H > (1ll << 28)) throw std::runtime_error("bad HDR size: " + line);
   std::vector<float> out((size_t)W * H * 3);
   std::vector<uint8_t> scan((size_t)W * 4);
   for (int y = 0; y < H; ++y) {
@@ -438,6 +439,7 @@ std::vector<float> load_hdr(const fs::path& p, int* w, int* h) {
       memcpy(scan.data(), hdr4, 4);
       f.read((char*)scan.data() + 4, (std::streamsize)W * 4 - 4);
     }
+    if (!f) throw std::runtime_error("truncated HDR file");
     for (int x = 0; x < W; ++x) {
       const uint8_t* q = &scan[4 * x];
       float* o = &out[3 * ((size_t)y * W + x)];
