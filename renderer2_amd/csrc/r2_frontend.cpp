@@ -486,6 +486,8 @@ GridFile read_grid(const fs::path& p) {
   f.read((char*)&g.vs, 4);
   if (!f || memcmp(magic, "R2GRID01", 8) != 0 || g.n[0] <= 0 || g.n[1] <= 0 || g.n[2] <= 0)
     throw std::runtime_error("not a grid file: " + p.string());
+  if (g.n[0] > 8192 || g.n[1] > 8192 || g.n[2] > 8192)
+    throw std::runtime_error("grid file larger than 8192 voxels per axis: " + p.string());
   const size_t bytes = (size_t)g.n[0] * g.n[1] * g.n[2] * 32;
   g.rec.resize(bytes);
   f.read((char*)g.rec.data(), (std::streamsize)bytes);
@@ -495,17 +497,18 @@ GridFile read_grid(const fs::path& p) {
 
 // ----------------------------------------------------------------- images ---
 uint32_t crc32(const uint8_t* d, size_t n, uint32_t c = 0xFFFFFFFFu) {
-  static uint32_t table[256];
-  static bool init = false;
-  if (!init) {
-    for (uint32_t i = 0; i < 256; ++i) {
-      uint32_t v = i;
-      for (int k = 0; k < 8; ++k) v = (v & 1) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
-      table[i] = v;
+  struct Table {
+    uint32_t v[256];
+    Table() {
+      for (uint32_t i = 0; i < 256; ++i) {
+        uint32_t x = i;
+        for (int k = 0; k < 8; ++k) x = (x & 1) ? 0xEDB88320u ^ (x >> 1) : x >> 1;
+        v[i] = x;
+      }
     }
-    init = true;
-  }
-  for (size_t i = 0; i < n; ++i) c = table[(c ^ d[i]) & 0xFF] ^ (c >> 8);
+  };
+  static const Table table;  // initialised once, thread-safe
+  for (size_t i = 0; i < n; ++i) c = table.v[(c ^ d[i]) & 0xFF] ^ (c >> 8);
   return c;
 }
 
