@@ -84,6 +84,7 @@ typedef struct {
   float max_density, inv_max_density;
   float bbox_min[3], bbox_max[3];                /* model-space bbox of the grid */
   int64_t device_bytes;
+  float min_density;                             /* over all voxels, inactive ones reading 0 */
 } r2_grid_info;
 
 int r2_grid_upload(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid** out);
@@ -282,6 +283,12 @@ typedef struct {
 } r2_pipeline_report;
 int r2_pipeline_run(const char* config_path, const char* out_dir, int32_t device, r2_pipeline_report* report,
                     char* err, int32_t err_len);
+
+/* Write the brick majorant grid of a grid as the reference's .brickgrid file: a
+ * CHostBrickGrid in cereal 1.3.2's portable binary archive (rt::write_grid,
+ * raytracer/src/grid_brick/serialization.cpp:37-57; fields host_grid_brick.hpp:47-72).
+ * r2_pipeline_run writes one beside each filtered LOD. */
+int r2_grid_write_brickgrid(r2_ctx* ctx, const r2_grid* grid, const char* path, char* err, int32_t err_len);
 
 /* Parse a config (and its scene description) without a device: the fields of SConfig
  * (raytracer/include/backend/config_loader.hpp:9-33) that drive this library. */

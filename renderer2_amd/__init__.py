@@ -219,6 +219,13 @@ class Grid:
         return {k: (np.array(getattr(i, k)) if not isinstance(getattr(i, k), (int, float)) else getattr(i, k))
                 for k, _ in i._fields_}
 
+    def write_brickgrid(self, path: str) -> None:
+        """The reference's .brickgrid file of this grid's brick majorant grid."""
+        err = C.create_string_buffer(512)
+        rc = capi.lib().r2_grid_write_brickgrid(self.ctx.h, self.h, str(path).encode(), err, 512)
+        if rc != 0:
+            raise capi.R2Error(rc, err.value.decode(errors="replace"))
+
     def download_bricks(self):
         inf = self.info()
         nbx, nby, nbz = [int(x) for x in inf["n_bricks"]]

@@ -55,6 +55,7 @@ class GridInfo(C.Structure):
         ("max_density", C.c_float), ("inv_max_density", C.c_float),
         ("bbox_min", f3), ("bbox_max", f3),
         ("device_bytes", C.c_int64),
+        ("min_density", C.c_float),
     ]
 
 
@@ -166,6 +167,7 @@ SIGNATURES = [
     ("r2_grid_download_bricks", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("r2_grid_lookup", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("r2_grid_write_brickgrid", C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32]),
     ("r2_dda_cells", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                C.c_void_p]),
     ("r2_env_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

@@ -495,6 +495,96 @@ GridFile read_grid(const fs::path& p) {
   return g;
 }
 
+// ------------------------------------------------------------ brick grid ---
+// CHostBrickGrid in cereal 1.3.2's portable binary archive (serialization.cpp:37-47,
+// host_grid_brick.hpp:47-72): one byte 1 (little-endian data), then little-endian
+// fields in declaration order; a std::vector is a uint64 length and its elements;
+// a glm vector its components; std::pair first, second; std::atomic its value.
+//   transform mat4 (identity, host_grid_brick.cpp:63), n_bricks uvec3, min_maj pair,
+//   brick_counter size_t, indirection / range / atlas Buf3D (stride uvec3, data
+//   vector), range_mipmaps vector<Buf3D<uint32>>.
+// Atlas slots are handed out in the order of the reference's brick loop (z, y, x:
+// one of the orders its parallel atomic allocation can produce); slot id -> atlas
+// brick coordinates as indirection.to_coord (buf3d.hpp:34-36), pointers encoded as
+// encode_ptr (common.hpp:31-38) and the atlas pruned in z (host_grid_brick.cpp:117).
+struct LeWriter {
+  std::vector<uint8_t> b;
+  template <typename T>
+  void put(const T& v) {
+    const uint8_t* p = (const uint8_t*)&v;
+    b.insert(b.end(), p, p + sizeof(T));
+  }
+  template <typename T>
+  void vec(const std::vector<T>& v) {
+    put<uint64_t>((uint64_t)v.size());
+    const uint8_t* p = (const uint8_t*)v.data();
+    b.insert(b.end(), p, p + v.size() * sizeof(T));
+  }
+  void uvec3(uint32_t x, uint32_t y, uint32_t z) { put(x), put(y), put(z); }
+};
+
+void write_brickgrid(r2_ctx* ctx, const r2_grid* g, const fs::path& path) {
+  r2_grid_info inf{};
+  if (r2_grid_get_info(g, &inf) != R2_OK) throw std::runtime_error("brickgrid: grid info");
+  const uint32_t nbx = (uint32_t)inf.n_bricks[0], nby = (uint32_t)inf.n_bricks[1], nbz = (uint32_t)inf.n_bricks[2];
+  const size_t nb = (size_t)nbx * nby * nbz;
+  const size_t dx = (size_t)nbx * 8, dy = (size_t)nby * 8, dz = (size_t)nbz * 8;
+  size_t nmip = 0;
+  for (int l = 0; l < 3; ++l) nmip += (size_t)(nbx >> (l + 1)) * (nby >> (l + 1)) * (nbz >> (l + 1));
+  std::vector<uint32_t> range(nb), mips(nmip);
+  std::vector<uint8_t> dense(dx * dy * dz);
+  if (r2_grid_download_bricks(ctx, g, range.data(), dense.data(), mips.data()) != R2_OK)
+    throw std::runtime_error(std::string("brickgrid: ") + r2_last_error(ctx));
+  auto empty = [](uint32_t r) { return (r & 0xFFFFu) == (r >> 16); };
+  uint64_t count = 0;
+  for (uint32_t r : range) count += empty(r) ? 0 : 1;
+  const size_t zs = 8 * (size_t)std::lround(std::ceil((double)count / (double)(nbx * nby)));
+  std::vector<uint32_t> ind(nb, 0u);
+  std::vector<uint8_t> atlas(dx * dy * zs, 0);
+  uint64_t id = 0;
+  for (uint32_t bz = 0; bz < nbz; ++bz)
+    for (uint32_t by = 0; by < nby; ++by)
+      for (uint32_t bx = 0; bx < nbx; ++bx) {
+        const size_t bi = ((size_t)bz * nby + by) * nbx + bx;
+        if (empty(range[bi])) continue;
+        const uint32_t px = (uint32_t)(id % nbx), py = (uint32_t)((id / nbx) % nby), pz = (uint32_t)(id / ((uint64_t)nbx * nby));
+        ++id;
+        ind[bi] = (std::min(px, 1023u) << 22) | (std::min(py, 1023u) << 12) | (std::min(pz, 1023u) << 2);
+        for (uint32_t z = 0; z < 8; ++z)
+          for (uint32_t y = 0; y < 8; ++y)
+            for (uint32_t x = 0; x < 8; ++x)
+              atlas[((size_t)(pz * 8 + z) * dy + (py * 8 + y)) * dx + (px * 8 + x)] =
+                  dense[((size_t)(bz * 8 + z) * dy + (by * 8 + y)) * dx + (bx * 8 + x)];
+      }
+  LeWriter w;
+  w.put<uint8_t>(1);
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) w.put<float>(c == r ? 1.f : 0.f);
+  w.uvec3(nbx, nby, nbz);
+  w.put<float>(inf.min_density);
+  w.put<float>(inf.max_density);
+  w.put<uint64_t>(count);
+  w.uvec3(nbx, nby, nbz);
+  w.vec(ind);
+  w.uvec3(nbx, nby, nbz);
+  w.vec(range);
+  w.uvec3((uint32_t)dx, (uint32_t)dy, (uint32_t)zs);
+  w.vec(atlas);
+  w.put<uint64_t>(3);
+  size_t off = 0;
+  for (int l = 0; l < 3; ++l) {
+    const uint32_t mx = nbx >> (l + 1), my = nby >> (l + 1), mz = nbz >> (l + 1);
+    const size_t n = (size_t)mx * my * mz;
+    w.uvec3(mx, my, mz);
+    w.vec(std::vector<uint32_t>(mips.begin() + off, mips.begin() + off + n));
+    off += n;
+  }
+  std::ofstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open " + path.string() + " for writing");
+  f.write((const char*)w.b.data(), (std::streamsize)w.b.size());
+  if (!f) throw std::runtime_error("brickgrid: write failed: " + path.string());
+}
+
 // ----------------------------------------------------------------- images ---
 uint32_t crc32(const uint8_t* d, size_t n, uint32_t c = 0xFFFFFFFFu) {
   struct Table {
@@ -649,6 +739,31 @@ void run_filter(r2_ctx* ctx, const Config& c, const fs::path& out_root, r2_pipel
     std::vector<uint8_t> rec((size_t)g.num_voxels[0] * g.num_voxels[1] * g.num_voxels[2] * 32);
     check(ctx, r2_filter_lod(ctx, mesh, &p, rec.data(), 0, &g), "filter");
     write_grid(dir / ("filtered_mesh_" + voxel_size_name(vs) + ".r2grid"), g, vs, rec);
+    {
+      // the LOD's brick majorant grid beside it, as writeToFile does (openvdb_backend.cpp:143-156)
+      r2_grid_desc d{};
+      d.nx = g.num_voxels[0];
+      d.ny = g.num_voxels[1];
+      d.nz = g.num_voxels[2];
+      d.records = rec.data();
+      d.records_on_device = 0;
+      for (int r = 0; r < 3; ++r) {
+        for (int col = 0; col < 3; ++col) d.map_mat[r * 3 + col] = g.index_to_model[col * 3 + r];
+        d.map_vec[r] = g.index_to_model[9 + r];
+        d.sigma_s[r] = 1.f;
+      }
+      d.voxel_size_filtering = vs;
+      d.precision = R2_PREC_FP32;
+      r2_grid* bg = nullptr;
+      check(ctx, r2_grid_upload(ctx, &d, &bg), "brick grid build");
+      try {
+        write_brickgrid(ctx, bg, dir / ("filtered_mesh_" + voxel_size_name(vs) + ".brickgrid"));
+      } catch (...) {
+        r2_grid_free(ctx, bg);
+        throw;
+      }
+      r2_grid_free(ctx, bg);
+    }
     // model-space box of the LOD: getRotation(orientation) * worldToModel applied to the world box
     const r2::m44 R = r2host::get_rotation(r2::mk(mo->orientation[0], mo->orientation[1], mo->orientation[2]));
     r2::m43 w2m;
@@ -805,6 +920,21 @@ extern "C" int r2_pipeline_run(const char* config_path, const char* out_dir, int
 }
 
 // Config parsing only (no device): fills a summary for front-end tests.
+extern "C" int r2_grid_write_brickgrid(r2_ctx* ctx, const r2_grid* grid, const char* path, char* err,
+                                       int32_t err_len) {
+  if (!ctx || !grid || !path) return R2_ERR_INVALID;
+  try {
+    write_brickgrid(ctx, grid, fs::path(path));
+  } catch (const std::exception& e) {
+    if (err && err_len > 0) {
+      strncpy(err, e.what(), (size_t)err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return R2_ERR_IO;
+  }
+  return R2_OK;
+}
+
 extern "C" int r2_config_summary(const char* config_path, r2_config_summary_t* out, char* err, int32_t err_len) {
   if (!config_path || !out) return R2_ERR_INVALID;
   try {

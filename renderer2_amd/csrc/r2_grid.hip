@@ -20,11 +20,19 @@ __device__ __forceinline__ bool is_corner(int x, int y, int z, int nx, int ny, i
 }
 
 // effective density (active rule) + leaf occupancy + bbox + max density
+// order-preserving unsigned key of a float (for atomicMin over any sign)
+__device__ __forceinline__ uint32_t float_key(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// max_bits[0]: max active density (float bits); max_bits[1]: order key of the
+// minimum over all voxels, inactive ones reading as 0 (the brick grid's minorant)
 __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, float* __restrict__ dens,
                          uint8_t* __restrict__ active, uint32_t* __restrict__ leaf_flag, int nlx, int nly,
                          int* __restrict__ bbox, uint32_t* __restrict__ max_bits) {
   const size_t n = (size_t)nx * ny * nz;
-  float lmax = 0.f;
+  float lmax = 0.f, lmin = FLT_MAX;
   int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
@@ -32,6 +40,7 @@ __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, 
     const bool a = (d > 0.f) || is_corner(x, y, z, nx, ny, nz);
     active[i] = a ? 1 : 0;
     dens[i] = a ? d : 0.f;
+    lmin = fminf(lmin, a ? d : 0.f);
     if (a) {
       leaf_flag[((size_t)(z >> 3) * nly + (y >> 3)) * nlx + (x >> 3)] = 1u;
       lmax = fmaxf(lmax, d);
@@ -42,6 +51,7 @@ __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, 
   // wave-level reduction, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     lmax = fmaxf(lmax, __shfl_xor(lmax, off));
+    lmin = fminf(lmin, __shfl_xor(lmin, off));
     for (int a = 0; a < 3; ++a) {
       lo[a] = min(lo[a], __shfl_xor(lo[a], off));
       hi[a] = max(hi[a], __shfl_xor(hi[a], off));
@@ -49,6 +59,7 @@ __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, 
   }
   if ((threadIdx.x & 63) == 0) {
     atomicMax(max_bits, __float_as_uint(fmaxf(lmax, 0.f)));
+    atomicMin(max_bits + 1, float_key(lmin));
     for (int a = 0; a < 3; ++a) {
       atomicMin(&bbox[a], lo[a]);
       atomicMax(&bbox[3 + a], hi[a]);
@@ -289,10 +300,11 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   R2_HIP(ctx, tmalloc((void**)&d_active, nv));
   R2_HIP(ctx, tmalloc((void**)&d_leaf_flag, nleaf_total * 4));
   R2_HIP(ctx, tmalloc((void**)&d_bbox, 6 * 4));
-  R2_HIP(ctx, tmalloc((void**)&d_max, 4));
+  R2_HIP(ctx, tmalloc((void**)&d_max, 8));
   const int bbox_init[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
   R2_HIP(ctx, hipMemcpyAsync(d_bbox, bbox_init, sizeof(bbox_init), hipMemcpyHostToDevice, st));
-  R2_HIP(ctx, hipMemsetAsync(d_max, 0, 4, st));
+  const uint32_t mm_init[2] = {0u, 0xFFFFFFFFu};
+  R2_HIP(ctx, hipMemcpyAsync(d_max, mm_init, 8, hipMemcpyHostToDevice, st));
   R2_HIP(ctx, hipMemsetAsync(d_leaf_flag, 0, nleaf_total * 4, st));
   {
     const int blocks = (int)std::min<size_t>((nv + 255) / 256, 65536);
@@ -303,10 +315,12 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   // leaf allocation: deterministic prefix order over the leaf table
   std::vector<uint32_t> flags(nleaf_total);
   int bbox[6];
-  uint32_t maxbits = 0;
+  uint32_t mm[2] = {0u, 0u};
   R2_HIP(ctx, hipMemcpyAsync(flags.data(), d_leaf_flag, nleaf_total * 4, hipMemcpyDeviceToHost, st));
   R2_HIP(ctx, hipMemcpyAsync(bbox, d_bbox, sizeof(bbox), hipMemcpyDeviceToHost, st));
-  R2_HIP(ctx, hipMemcpyAsync(&maxbits, d_max, 4, hipMemcpyDeviceToHost, st));
+  R2_HIP(ctx, hipMemcpyAsync(mm, d_max, 8, hipMemcpyDeviceToHost, st));
+  const uint32_t maxbits = mm[0];
+  const uint32_t minbits = (mm[1] & 0x80000000u) ? (mm[1] & 0x7FFFFFFFu) : ~mm[1];
   R2_HIP(ctx, hipStreamSynchronize(st));
   std::vector<int32_t> table(nleaf_total);
   int64_t nleaves = 0;
@@ -439,6 +453,7 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   float maxd;
   memcpy(&maxd, &maxbits, 4);
   info.max_density = maxd;
+  memcpy(&info.min_density, &minbits, 4);
   info.inv_max_density = 1.f / maxd;
   info.n_leaves = nleaves;
   info.n_nonempty_bricks = nne;

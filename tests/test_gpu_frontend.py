@@ -20,6 +20,8 @@ import oracle as O
 import renderer2_amd as r2
 from renderer2_amd import capi, scenes
 
+from brickgrid_io import check_against_device
+
 pytestmark = pytest.mark.gpu
 
 
@@ -134,6 +136,10 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
         ref, g = ctx.filter_lod(gm, r2.filter_params(float(v), 32, 4, 10.0, 0.1, True, seed=3, lod=lod))
         assert tuple(g.num_voxels) == (nx, ny, nz)
         assert np.array_equal(rec, ref)
+        # the LOD's .brickgrid beside it equals the brick grid of the same records
+        grid = ctx.upload_grid(scenes.filtered_grid(g, float(v), records=ref))
+        check_against_device(grid, path.with_suffix(".brickgrid"), ref)
+        grid.free()
     desc = json.loads((tmp_path / "filtering" / "volume_description.json").read_text())
     assert set(desc["blob"].keys()) == {"%f" % vs, "%f" % np.float32(vs * 2)}
     # output.png = post-process(output.pfm), rows flipped
