@@ -14,12 +14,44 @@ import numpy as np
 from . import _capi as capi
 from ._capi import R2Error
 
-__all__ = ["Context", "Grid", "Env", "Mesh", "Server", "R2Error", "capi", "filter_params"]
+__all__ = ["Context", "Grid", "Env", "Mesh", "Server", "R2Error", "capi", "filter_params", "nvdb_write",
+           "nvdb_read"]
 
 
 def filter_params(voxel_size: float, samples: int = 64, iterations: int = 8, sigma_t: float = 1.0,
                   alpha: float = 1.0, clip_rays: bool = True, seed: int = 0, lod: int = 0) -> "capi.FilterParams":
     return capi.FilterParams(voxel_size, samples, iterations, sigma_t, alpha, int(bool(clip_rays)), seed, lod)
+
+
+def nvdb_write(path, records: np.ndarray) -> None:
+    """NanoVDB file (one Vec4d grid) of (nz, ny, nx, 32) u8 records: the voxels with
+    density > 0 and the 8 grid corners are active.  Host only."""
+    rec = np.ascontiguousarray(records, np.uint8)
+    if rec.ndim != 4 or rec.shape[3] != 32:
+        raise ValueError("records must be (nz, ny, nx, 32) uint8")
+    nz, ny, nx = rec.shape[:3]
+    err = C.create_string_buffer(512)
+    rc = capi.lib().r2_nvdb_write(str(path).encode(), nx, ny, nz, rec.ctypes.data, err, 512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
+
+
+def nvdb_read(path, with_records: bool = True):
+    """(info dict, (nz, ny, nx, 32) u8 records over the index box or None).  Host only."""
+    info = capi.NvdbInfo()
+    err = C.create_string_buffer(512)
+    rc = capi.lib().r2_nvdb_read(str(path).encode(), C.byref(info), None, 0, err, 512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
+    d = {k: (np.array(getattr(info, k)) if not isinstance(getattr(info, k), (int, float)) else getattr(info, k))
+         for k, _ in info._fields_}
+    if not with_records:
+        return d, None
+    rec = np.empty((info.nz, info.ny, info.nx, 32), np.uint8)
+    rc = capi.lib().r2_nvdb_read(str(path).encode(), C.byref(info), rec.ctypes.data, rec.nbytes, err, 512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
+    return d, rec
 
 
 class Context:

@@ -59,6 +59,14 @@ class GridInfo(C.Structure):
     ]
 
 
+class NvdbInfo(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("ibb_min", i3),
+        ("active_voxels", C.c_int64), ("map_mat", f9), ("map_vec", f3),
+        ("leaf_count", C.c_int32), ("lower_count", C.c_int32), ("upper_count", C.c_int32),
+    ]
+
+
 class Instance(C.Structure):
     _fields_ = [("grid", C.c_void_p), ("pos", f3), ("orientation", f3), ("scaling", f3)]
 
@@ -168,6 +176,8 @@ SIGNATURES = [
     ("r2_grid_lookup", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_write_brickgrid", C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32]),
+    ("r2_nvdb_write", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
+    ("r2_nvdb_read", C.c_int, [C.c_char_p, C.POINTER(NvdbInfo), C.c_void_p, C.c_int64, C.c_char_p, C.c_int32]),
     ("r2_dda_cells", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                C.c_void_p]),
     ("r2_env_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

@@ -30,6 +30,7 @@ SOURCES = [
     "r2_server.cpp",
     "r2_post.hip",
     "r2_frontend.cpp",
+    "r2_nvdb.cpp",
 ]
 # the command-line program (links the library)
 MAIN = "r2_main.cpp"

@@ -739,8 +739,11 @@ void run_filter(r2_ctx* ctx, const Config& c, const fs::path& out_root, r2_pipel
     std::vector<uint8_t> rec((size_t)g.num_voxels[0] * g.num_voxels[1] * g.num_voxels[2] * 32);
     check(ctx, r2_filter_lod(ctx, mesh, &p, rec.data(), 0, &g), "filter");
     write_grid(dir / ("filtered_mesh_" + voxel_size_name(vs) + ".r2grid"), g, vs, rec);
+    // the LOD as a NanoVDB grid and its brick majorant grid beside it, as writeToFile
+    // does (openvdb_backend.cpp:134-157)
+    r2host::nvdb_write((dir / ("filtered_mesh_" + voxel_size_name(vs) + ".nvdb")).string(), g.num_voxels[0],
+                       g.num_voxels[1], g.num_voxels[2], rec.data());
     {
-      // the LOD's brick majorant grid beside it, as writeToFile does (openvdb_backend.cpp:143-156)
       r2_grid_desc d{};
       d.nx = g.num_voxels[0];
       d.ny = g.num_voxels[1];

@@ -140,6 +140,15 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
         grid = ctx.upload_grid(scenes.filtered_grid(g, float(v), records=ref))
         check_against_device(grid, path.with_suffix(".brickgrid"), ref)
         grid.free()
+        # and its .nvdb: the active voxels (density > 0, the 8 corners) hold their records
+        info, nv = r2.nvdb_read(path.with_suffix(".nvdb"))
+        assert (info["nx"], info["ny"], info["nz"]) == (nx, ny, nz)
+        dens = ref[..., :4].copy().view(np.float32)[..., 0]
+        act = dens > 0
+        act[::nz - 1, ::ny - 1, ::nx - 1] = True
+        bg = np.zeros(32, np.uint8)
+        bg[29] = 0x3C
+        assert np.array_equal(nv, np.where(act[..., None], ref, bg))
     desc = json.loads((tmp_path / "filtering" / "volume_description.json").read_text())
     assert set(desc["blob"].keys()) == {"%f" % vs, "%f" % np.float32(vs * 2)}
     # output.png = post-process(output.pfm), rows flipped
