@@ -31,6 +31,7 @@ enum : int {
   ST_DIRECT_DONE,
   ST_SH_SEG,
   ST_SH_NEXT,
+  ST_MOVE,        // spawn a ray from the scatter point and move it to the voxel border
   ST_FINISH,
 };
 
@@ -82,7 +83,7 @@ struct PathLane {
   // pending NEE sample (LDS)
   v3 &pdir, &pLe;
   float& ppdf;
-  // shadow ray walk (intersectTr)
+  // shadow ray walk (intersectTr); sh_ret is also the state that follows ST_MOVE
   v3 &sh_p0, &sh_p1;
   int sh_cur, sh_next, sh_hit, sh_seg, sh_ret;
   float sh_Tr;
@@ -283,30 +284,25 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
       p.si_hit = si.hit;
       p.tres = 0;
       p.tk = 0;
+      // the free flight starts on the current ray (inside a medium) or on the ray
+      // entering the medium that was hit; one traversal set-up serves both
+      int next = ST_NOHIT;
+      Ray fr = p.ray;
       if (si.medium >= 0) {
         if (p.ray.medium >= 0) {
-          if (si.medium != p.ray.medium) {
-            p.stage = ST_FINISH;
-            break;
-          }
-          start_traversal<kSingle>(s, p, p.ray.medium, p.ray, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
+          next = si.medium != p.ray.medium ? ST_FINISH : ST_TRAV;
         } else {
           Ray mr = mkray(si.pos, normalize(p.ray.d), kDefaultTmax, si.medium);
           ray_offset(mr, p.ray.d);
           const Hit se = scene_intersect(s, mr);
-          if (se.hit) {
-            if (se.medium >= 0 && si.medium != se.medium) {
-              p.stage = ST_FINISH;
-              break;
-            }
-            start_traversal<kSingle>(s, p, mr.medium, mr, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
-          } else {
-            p.stage = ST_AFTER_SAMPLE;
-          }
+          next = !se.hit ? ST_AFTER_SAMPLE : ((se.medium >= 0 && si.medium != se.medium) ? ST_FINISH : ST_TRAV);
+          fr = mr;
         }
-      } else {
-        p.stage = ST_NOHIT;
       }
+      if (next == ST_TRAV)
+        start_traversal<kSingle>(s, p, fr.medium, fr, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
+      else
+        p.stage = next;
       break;
     }
     case ST_AFTER_SAMPLE: {
@@ -341,14 +337,11 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
       float lpdf = 0.f;
       const v3 Le = env_sample<kCount>(s.env, p.rng, ldir, lpdf, cnt);
       if (lpdf > 0.f) {
-        ldir = normalize(ldir);
-        Ray rl = mkray(p.mi_pos, ldir, kDefaultTmax, p.mi_inst);
-        ray_offset(rl, p.mi_n);
-        rl = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, rl);
-        p.pdir = ldir;
+        p.pdir = normalize(ldir);
         p.pLe = Le;
         p.ppdf = lpdf;
-        start_shadow(p, rl, ST_LIGHT_DONE);
+        p.sh_ret = ST_LIGHT_DONE;  // shadow ray toward the light sample (ST_MOVE)
+        p.stage = ST_MOVE;
       } else {
         p.stage = ST_BRDF_SAMPLE;
       }
@@ -364,20 +357,38 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
       break;
     }
     case ST_BRDF_SAMPLE: {
+      // also runs the lanes in ST_DIRECT_DONE: both start with a phase sample at
+      // the scatter point (direct()'s phase-sampled light direction, then Li's
+      // continuation direction), so a wavefront runs that code once for both
+      const bool cont = p.stage == ST_DIRECT_DONE;
+      if (cont) p.L = p.L + p.thr * p.Ld;
       const v3 wo = -p.ray.d;
       v3 wi;
       const float ph = phase_sample(p.S, p.mi_n, p.mi_ior, wo, wi, p.rng);
-      if (ph > 0.f) {
-        wi = normalize(wi);
-        Ray rb = mkray(p.mi_pos, wi, kDefaultTmax, p.mi_inst);
-        ray_offset(rb, p.mi_n);
-        rb = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, rb);
-        p.pdir = wi;
-        p.ppdf = ph;
-        start_shadow(p, rb, ST_BRDF_DONE);
-      } else {
-        p.stage = ST_DIRECT_DONE;
+      if (!cont) {
+        if (ph > 0.f) {
+          p.pdir = normalize(wi);
+          p.ppdf = ph;
+          p.sh_ret = ST_BRDF_DONE;  // shadow ray along the phase sample (ST_MOVE)
+          p.stage = ST_MOVE;
+        } else {
+          p.stage = ST_DIRECT_DONE;
+        }
+        break;
       }
+      p.pdir = normalize(wi);
+      p.Ld = wi;  // the continuation ray is offset along the unnormalised sample
+      if (p.b > 3) {
+        const float q = 1.f - (p.thr.x + p.thr.y + p.thr.z) / 3.f;
+        if (p.rng.uni() <= q) {
+          p.stage = ST_FINISH;
+          break;
+        }
+        p.thr = p.thr / (1.f - q);
+      }
+      ++p.b;
+      p.sh_ret = ST_BOUNCE;
+      p.stage = ST_MOVE;
       break;
     }
     case ST_BRDF_DONE: {
@@ -390,24 +401,20 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
       p.stage = ST_DIRECT_DONE;
       break;
     }
-    case ST_DIRECT_DONE: {
-      p.L = p.L + p.thr * p.Ld;
-      const v3 wo = -p.ray.d;
-      v3 wi;
-      (void)phase_sample(p.S, p.mi_n, p.mi_ior, wo, wi, p.rng);
-      Ray nr = mkray(p.mi_pos, normalize(wi), kDefaultTmax, p.mi_inst);
-      ray_offset(nr, wi);
-      p.ray = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, nr);
-      if (p.b > 3) {
-        const float q = 1.f - (p.thr.x + p.thr.y + p.thr.z) / 3.f;
-        if (p.rng.uni() <= q) {
-          p.stage = ST_FINISH;
-          break;
-        }
-        p.thr = p.thr / (1.f - q);
+    case ST_MOVE: {
+      // the ray from the scatter point along pdir, offset along the normal (shadow
+      // rays) or along the unnormalised direction sample (continuation), moved to
+      // the voxel border: the tails of direct() and of Li's bounce
+      const bool cont = p.sh_ret == ST_BOUNCE;
+      Ray r = mkray(p.mi_pos, p.pdir, kDefaultTmax, p.mi_inst);
+      ray_offset(r, cont ? p.Ld : p.mi_n);
+      r = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, r);
+      if (cont) {
+        p.ray = r;
+        p.stage = ST_BOUNCE;
+      } else {
+        start_shadow(p, r, p.sh_ret);
       }
-      ++p.b;
-      p.stage = ST_BOUNCE;
       break;
     }
     case ST_SH_SEG: {

@@ -124,11 +124,11 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
   }
 }
 
-// Run state kSt for the lanes that are in it (a divergent branch the wave skips
-// when no lane is).
-template <bool kSingle, bool kCount, int kSt>
+// Run state kSt for the lanes that are in it, or in kSt2 when the code of kSt
+// also serves kSt2 (a divergent branch the wave skips when no lane is).
+template <bool kSingle, bool kCount, int kSt, int kSt2 = -1>
 __device__ __forceinline__ void event_if(const DScene& s, PathLane& p, Cnt& cnt) {
-  if (p.stage == kSt) {
+  if (p.stage == kSt || p.stage == kSt2) {
 #ifdef R2_SM_STATS
     stat_state(kSt);
     const uint64_t t0 = clock64();
@@ -153,8 +153,10 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
 #endif
   // pass-through: the states run in the forward order of the path's state graph,
   // so within one pass a lane advances through every state that follows its own
-  // in that order (e.g. SH_NEXT -> BRDF_DONE -> DIRECT_DONE -> BOUNCE -> a new
-  // traversal), and each state's code runs at most once per pass for the wave
+  // in that order (e.g. SH_NEXT -> BRDF_DONE -> DIRECT_DONE -> MOVE -> BOUNCE ->
+  // a new traversal), and each state's code runs at most once per pass for the
+  // wave; the voxel-border move of the three rays spawned at a scatter point
+  // (ST_MOVE) and the phase sample of BRDF_SAMPLE / DIRECT_DONE are each one block
   for (int k = 0; k < R2_EV_ITERS; ++k) {
     if (!__any(p.stage >= ST_START)) break;
     SM_STAT(1, 1);
@@ -174,10 +176,10 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
     event_if<kSingle, kCount, ST_NOHIT>(s, p, cnt);
     event_if<kSingle, kCount, ST_LIGHT_DONE>(s, p, cnt);
     event_if<kSingle, kCount, ST_BRDF_DONE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_DIRECT_DONE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_BOUNCE>(s, p, cnt);
     event_if<kSingle, kCount, ST_SCATTER>(s, p, cnt);
-    event_if<kSingle, kCount, ST_BRDF_SAMPLE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_BRDF_SAMPLE, ST_DIRECT_DONE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_MOVE>(s, p, cnt);
+    event_if<kSingle, kCount, ST_BOUNCE>(s, p, cnt);
     event_if<kSingle, kCount, ST_SH_SEG>(s, p, cnt);
   }
 #ifdef R2_SM_STATS

@@ -544,19 +544,22 @@ __device__ __forceinline__ float phase_p(const float S6[6], v3 n, float ior, v3 
   const float pd = gmax(dot(wi, wm), 0.f) / kPi;
   return pd * (1.f - w) + sggx_spec_p(g, wo, wi) * w;
 }
-// CSGGXPhaseFunction::sampleP (sggx_phase_function.hpp:220-229)
+// CSGGXPhaseFunction::sampleP (sggx_phase_function.hpp:220-229).  Both lobes
+// start by drawing (ux, uy) and sampling the VNDF with them (the specular lobe's
+// microflake normal, the diffuse lobe's frame normal), so that part runs once,
+// before the lobe choice: same draws in the same order, same arithmetic, and a
+// wavefront whose lanes choose different lobes does not run it twice.
 __device__ __forceinline__ float phase_sample(const float S6[6], v3 n, float ior, v3 wo, v3& wi, Rng& rng) {
   const Sggx g = sggx_make(S6, n);
   const float pf = fresnel(fabsf(dot(normalize(wo), n)), ior);
-  if (rng.uni() < pf) {
-    const float ux = rng.uni(), uy = rng.uni();
-    const v3 wm = sggx_vndf(g, ux, uy);
-    const v3 I = -wo;
-    wi = I - (wm * dot(wm, I)) * 2.f;
-    return sggx_spec_p(g, wo, wi);
-  }
+  const bool specular = rng.uni() < pf;
   const float ux = rng.uni(), uy = rng.uni();
   const v3 wn = sggx_vndf(g, ux, uy);
+  if (specular) {
+    const v3 I = -wo;
+    wi = I - (wn * dot(wn, I)) * 2.f;
+    return sggx_spec_p(g, wo, wi);
+  }
   v3 T, B;
   frame_basic(wn, T, B);
   const v3 wm = cosine_hemisphere(rng);
