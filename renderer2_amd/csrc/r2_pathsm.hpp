@@ -38,10 +38,11 @@ enum : int {
 enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO = 4 };
 
 // Path state touched only at the start and end of a path and in shading events
-// lives in LDS, one slot per lane (43 dwords: an odd stride, so the 64 lanes of
-// a wavefront fall on distinct banks); the state the traversal loop and the RNG
-// touch stays in registers.  This keeps the kernel within the register budget
-// of three wavefronts per SIMD without spilling to scratch.
+// lives in LDS, one slot per lane (51 dwords: an odd stride, so the 64 lanes of
+// a wavefront fall on distinct banks; 3 blocks of 256 lanes fit a CU's 160 KB);
+// the state the traversal loop touches stays in registers.  This keeps the
+// kernel within the register budget of three wavefronts per SIMD without
+// spilling to scratch.
 struct ColdLane {
   uint32_t pix;
   int32_t px, py;
@@ -56,8 +57,61 @@ struct ColdLane {
   int32_t mi_inst;
   v3 sh_p0, sh_p1, si_pos;
   uint32_t pad;
+  uint32_t ring[8];     // random words of the path (PathRng)
 };
-static_assert(sizeof(ColdLane) == 43 * 4, "ColdLane must keep an odd dword stride");
+static_assert(sizeof(ColdLane) == 51 * 4, "ColdLane must keep an odd dword stride");
+
+// The random stream of one path: Philox4x32-10 keyed (seed, stream), counter
+// (pixel, sample, block, 0); draw k is word k & 3 of block k >> 2, the same
+// stream as Rng.  Words are read from an 8-word ring in LDS that holds blocks
+// b and b + 1 (block b at words 4 (b & 1) ..).  top_up() computes the next block
+// ahead of use at points that the whole wavefront passes (the top of an event
+// pass, of a traversal round), so that after it every lane has at least five
+// words in hand; a draw computes a block itself only when its lane has run out.
+// A wavefront therefore runs about one block computation per pass or round,
+// where a lazily refilled buffer runs one at every draw site at which any of
+// its lanes happens to cross into a new block.
+struct PathRng {
+  const uint32_t k0, k1;  // launch-uniform: kept in scalar registers
+  uint32_t c0, c1;
+  uint32_t n;   // words drawn
+  uint32_t hi;  // blocks 0 .. hi - 1 computed; the ring holds hi - 2 and hi - 1
+  uint32_t (&ring)[8];
+  __device__ PathRng(uint32_t (&r)[8], uint32_t seed, uint32_t stream) : k0(seed), k1(stream), ring(r) {}
+  __device__ __forceinline__ void block(uint32_t b) {
+    uint32_t c[4] = {c0, c1, b, 0u};
+    philox4x32_10(c, k0, k1);
+    const uint32_t o = (b & 1u) * 4u;
+    ring[o] = c[0];
+    ring[o + 1] = c[1];
+    ring[o + 2] = c[2];
+    ring[o + 3] = c[3];
+  }
+  __device__ __forceinline__ void init(uint32_t ctr0, uint32_t ctr1) {
+    c0 = ctr0;
+    c1 = ctr1;
+    n = 0;
+    block(0);
+    hi = 1;
+  }
+  // make words n .. 4 (n / 4 + 2) - 1 available
+  __device__ __forceinline__ void top_up() {
+    const uint32_t want = (n >> 2) + 2u;
+    if (hi < want) {
+      if (hi + 1u < want) block(want - 2u);
+      block(want - 1u);
+      hi = want;
+    }
+  }
+  __device__ __forceinline__ uint32_t next_u32() {
+    if ((n >> 2) >= hi) {  // ran past the words in hand: compute the block now
+      block(n >> 2);
+      hi = (n >> 2) + 1u;
+    }
+    return ring[n++ & 7u];
+  }
+  __device__ __forceinline__ float uni() { return (float)(next_u32() >> 8) * 5.9604644775390625e-08f; }
+};
 
 struct PathLane {
   // work item (LDS)
@@ -68,7 +122,7 @@ struct PathLane {
   float &ax, &ay, &az;
   double dax, day, daz;  // FP64 item sum (compensated framebuffer form)
   // path
-  Rng rng;
+  PathRng rng;
   int stage, b, iters;
   v3 L, thr;
   v3& Ld;
@@ -92,8 +146,9 @@ struct PathLane {
   v3 to, td, tri;
   float tt, ttau, tmip, ttmax, ttr;
 
-  __device__ explicit PathLane(ColdLane& c)
+  __device__ PathLane(ColdLane& c, const DScene& s)
       : pix(c.pix), px(c.px), py(c.py), s_cur(c.s_cur), s_end(c.s_end), item(c.item), ax(c.ax), ay(c.ay), az(c.az),
+        rng(c.ring, s.seed, s.stream),
         Ld(c.Ld), si_pos(c.si_pos), mi_pos(c.mi_pos), mi_n(c.mi_n), S(c.S), mi_ior(c.mi_ior), mi_inst(c.mi_inst),
         pdir(c.pdir), pLe(c.pLe), ppdf(c.ppdf), sh_p0(c.sh_p0), sh_p1(c.sh_p1) {}
 };
@@ -117,7 +172,7 @@ __device__ __forceinline__ void start_traversal(const DScene& s, PathLane& p, in
   p.tres = 0;
   p.tt = 0.f;
   p.ttr = 1.f;
-  if (kind == TK_FLIGHT_DDA || kind == TK_TR_DDA) {
+  if ((s.flags & 1u) != 0u) {  // TK_FLIGHT_DDA / TK_TR_DDA (brick grid on: same for every lane)
     const Ray ir = ray_transform2(rm, g.m2s);
     p.to = ir.o;
     p.td = ir.d;
@@ -264,7 +319,7 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
   const bool dda = (s.flags & 1u) != 0u;
   switch (st) {
     case ST_START: {
-      p.rng.init(s.seed, s.stream, p.pix, (uint32_t)(s.sample_begin + p.s_cur), 0u);
+      p.rng.init(p.pix, (uint32_t)(s.sample_begin + p.s_cur));
       p.ray = camera_ray(s, p.px, p.py, p.rng);
       p.L = mk(0.f, 0.f, 0.f);
       p.thr = mk(1.f, 1.f, 1.f);
@@ -274,33 +329,53 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
       break;
     }
     case ST_BOUNCE: {
-      if (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64) {
-        p.stage = ST_FINISH;
+      // also runs the lanes in ST_SH_SEG: both intersect the scene and may start a
+      // traversal (Li's free flight, or the transmittance of one intersectTr
+      // segment), so a wavefront runs that code once for both
+      const bool sh = p.stage == ST_SH_SEG;
+      if (sh ? ++p.sh_seg > 64 : (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64)) {
+        if (sh) {
+          p.sh_hit = 1;
+          p.stage = p.sh_ret;
+        } else {
+          p.stage = ST_FINISH;
+        }
         break;
       }
-      const Hit si = scene_intersect(s, p.ray);
-      p.si_pos = si.pos;
-      p.si_medium = si.medium;
-      p.si_hit = si.hit;
-      p.tres = 0;
-      p.tk = 0;
-      // the free flight starts on the current ray (inside a medium) or on the ray
-      // entering the medium that was hit; one traversal set-up serves both
-      int next = ST_NOHIT;
-      Ray fr = p.ray;
-      if (si.medium >= 0) {
-        if (p.ray.medium >= 0) {
-          next = si.medium != p.ray.medium ? ST_FINISH : ST_TRAV;
-        } else {
-          Ray mr = mkray(si.pos, normalize(p.ray.d), kDefaultTmax, si.medium);
-          ray_offset(mr, p.ray.d);
-          const Hit se = scene_intersect(s, mr);
-          next = !se.hit ? ST_AFTER_SAMPLE : ((se.medium >= 0 && si.medium != se.medium) ? ST_FINISH : ST_TRAV);
-          fr = mr;
+      Ray r = sh ? spawn_ray(p.sh_p0, p.sh_p1, p.sh_cur) : p.ray;
+      const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
+      Ray fr = r;
+      int next;
+      if (sh) {
+        p.sh_hit = h.hit;
+        p.sh_next = r.medium < 0 ? h.medium : -1;
+        p.sh_p0 = h.pos;
+        next = (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == p.sh_cur)) ? ST_TRAV : ST_SH_NEXT;
+      } else {
+        p.ray = r;
+        p.si_pos = h.pos;
+        p.si_medium = h.medium;
+        p.si_hit = h.hit;
+        p.tres = 0;
+        p.tk = 0;
+        // the free flight starts on the current ray (inside a medium) or on the ray
+        // entering the medium that was hit
+        next = ST_NOHIT;
+        if (h.medium >= 0) {
+          if (r.medium >= 0) {
+            next = h.medium != r.medium ? ST_FINISH : ST_TRAV;
+          } else {
+            Ray mr = mkray(h.pos, normalize(r.d), kDefaultTmax, h.medium);
+            ray_offset(mr, r.d);
+            const Hit se = scene_intersect(s, mr);
+            next = !se.hit ? ST_AFTER_SAMPLE : ((se.medium >= 0 && h.medium != se.medium) ? ST_FINISH : ST_TRAV);
+            fr = mr;
+          }
         }
       }
       if (next == ST_TRAV)
-        start_traversal<kSingle>(s, p, fr.medium, fr, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
+        start_traversal<kSingle>(s, p, fr.medium, fr,
+                                 sh ? (dda ? TK_TR_DDA : TK_TR_RATIO) : (dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA));
       else
         p.stage = next;
       break;
@@ -317,10 +392,23 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
       break;
     }
     case ST_NOHIT: {
-      if (p.b == 0 && !p.si_hit) {
-        float pdf;
-        p.L = p.L + env_le<kCount>(s.env, p.ray.d, pdf, cnt) * p.thr;
+      // also runs the lanes in ST_BRDF_DONE: both look up the environment (the
+      // radiance of an escaped camera ray, or the light along direct()'s phase
+      // sample), so a wavefront runs that lookup once for both
+      const bool brdf = p.stage == ST_BRDF_DONE;
+      const bool escaped = !brdf && p.b == 0 && !p.si_hit;
+      float lpdf = 0.f;
+      v3 Le = mk(0.f, 0.f, 0.f);
+      if (brdf || escaped) Le = env_le<kCount>(s.env, brdf ? p.pdir : p.ray.d, lpdf, cnt);
+      if (brdf) {
+        const float ph = p.ppdf;
+        const float mis = (1.f * ph) / (1.f * ph + 1.f * lpdf);
+        const float tr = p.sh_hit ? 0.f : p.sh_Tr;
+        p.Ld = p.Ld + ((Le * (mis * ph)) * tr) / ph;
+        p.stage = ST_DIRECT_DONE;
+        break;
       }
+      if (escaped) p.L = p.L + Le * p.thr;
       if (!p.si_hit) {
         p.stage = ST_FINISH;
         break;
@@ -391,16 +479,6 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
       p.stage = ST_MOVE;
       break;
     }
-    case ST_BRDF_DONE: {
-      float lpdf;
-      const v3 Le = env_le<kCount>(s.env, p.pdir, lpdf, cnt);
-      const float ph = p.ppdf;
-      const float mis = (1.f * ph) / (1.f * ph + 1.f * lpdf);
-      const float tr = p.sh_hit ? 0.f : p.sh_Tr;
-      p.Ld = p.Ld + ((Le * (mis * ph)) * tr) / ph;
-      p.stage = ST_DIRECT_DONE;
-      break;
-    }
     case ST_MOVE: {
       // the ray from the scatter point along pdir, offset along the normal (shadow
       // rays) or along the unnormalised direction sample (continuation), moved to
@@ -414,25 +492,6 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
         p.stage = ST_BOUNCE;
       } else {
         start_shadow(p, r, p.sh_ret);
-      }
-      break;
-    }
-    case ST_SH_SEG: {
-      if (++p.sh_seg > 64) {
-        p.sh_hit = 1;
-        p.stage = p.sh_ret;
-        break;
-      }
-      Ray rr = spawn_ray(p.sh_p0, p.sh_p1, p.sh_cur);
-      const Hit h = scene_intersect(s, rr);
-      p.sh_hit = h.hit;
-      p.sh_next = rr.medium < 0 ? h.medium : -1;
-      const bool trav = h.hit && rr.medium >= 0 && (h.medium < 0 || h.medium == p.sh_cur);
-      p.sh_p0 = h.pos;
-      if (trav) {
-        start_traversal<kSingle>(s, p, rr.medium, rr, dda ? TK_TR_DDA : TK_TR_RATIO);
-      } else {
-        p.stage = ST_SH_NEXT;
       }
       break;
     }

@@ -40,6 +40,9 @@ __device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int
 #ifndef R2_TR_MIN_LANES
 #define R2_TR_MIN_LANES 16  // leave the traversal round when fewer lanes are still traversing
 #endif
+#ifndef R2_RNG_TRAV_TOPUP
+#define R2_RNG_TRAV_TOPUP 1  // top up the random words of traversing lanes every traversal round
+#endif
 #ifndef R2_TR_MIN_ROUNDS
 #define R2_TR_MIN_ROUNDS 8
 #endif
@@ -155,11 +158,15 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
   // so within one pass a lane advances through every state that follows its own
   // in that order (e.g. SH_NEXT -> BRDF_DONE -> DIRECT_DONE -> MOVE -> BOUNCE ->
   // a new traversal), and each state's code runs at most once per pass for the
-  // wave; the voxel-border move of the three rays spawned at a scatter point
-  // (ST_MOVE) and the phase sample of BRDF_SAMPLE / DIRECT_DONE are each one block
+  // wave; states whose code shares its expensive part run as one block (BOUNCE /
+  // SH_SEG: scene intersection + traversal set-up; NOHIT / BRDF_DONE: environment
+  // lookup; BRDF_SAMPLE / DIRECT_DONE: phase sample; ST_MOVE: the voxel-border
+  // move of every ray spawned at a scatter point)
   for (int k = 0; k < R2_EV_ITERS; ++k) {
     if (!__any(p.stage >= ST_START)) break;
     SM_STAT(1, 1);
+    // random words for this pass (a lane in ST_START starts a new stream)
+    if (p.stage > ST_START && p.stage != ST_FINISH) p.rng.top_up();
     if (p.stage == ST_FINISH) {
       stat_state(ST_FINISH);
 #ifdef R2_SM_STATS
@@ -173,14 +180,12 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
     event_if<kSingle, kCount, ST_START>(s, p, cnt);
     event_if<kSingle, kCount, ST_SH_NEXT>(s, p, cnt);
     event_if<kSingle, kCount, ST_AFTER_SAMPLE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_NOHIT>(s, p, cnt);
+    event_if<kSingle, kCount, ST_NOHIT, ST_BRDF_DONE>(s, p, cnt);
     event_if<kSingle, kCount, ST_LIGHT_DONE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_BRDF_DONE>(s, p, cnt);
     event_if<kSingle, kCount, ST_SCATTER>(s, p, cnt);
     event_if<kSingle, kCount, ST_BRDF_SAMPLE, ST_DIRECT_DONE>(s, p, cnt);
     event_if<kSingle, kCount, ST_MOVE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_BOUNCE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_SH_SEG>(s, p, cnt);
+    event_if<kSingle, kCount, ST_BOUNCE, ST_SH_SEG>(s, p, cnt);
   }
 #ifdef R2_SM_STATS
   SM_STAT(4, clock64() - te0);
@@ -195,6 +200,9 @@ __device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt
     if (k >= R2_TR_MIN_ROUNDS && __popcll(tm) < R2_TR_MIN_LANES) break;
     SM_STAT(2, 1);
     SM_STAT(3, __popcll(tm));
+#if R2_RNG_TRAV_TOPUP
+    if (p.stage == ST_TRAV) p.rng.top_up();
+#endif
     if (p.stage == ST_TRAV) {
       if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
     }
@@ -217,7 +225,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
   const int lane = threadIdx.x & 63;
   const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
   __shared__ ColdLane cold[256];
-  PathLane p(cold[threadIdx.x]);
+  PathLane p(cold[threadIdx.x], s);
   p.stage = ST_FETCH;
   p.item = 0;
   Cnt cnt;
@@ -292,7 +300,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
     const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
     const int chunk = E.chunk;
     Acc* partial = (Acc*)E.partial;
-    PathLane p(cold[threadIdx.x]);
+    PathLane p(cold[threadIdx.x], s);
     p.stage = ST_FETCH;
     p.item = 0;
     for (int guard = 0; guard < (1 << 24); ++guard) {

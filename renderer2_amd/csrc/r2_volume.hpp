@@ -517,7 +517,8 @@ __device__ __forceinline__ float sggx_spec_p(const Sggx& g, v3 wo, v3 wi) {
   return sggx_D(g, wh) / (4.f * sggx_sigma(g, wo));
 }
 // concentricSampleDisk + cosineSampleHemisphere (sampler.hpp:52-80)
-__device__ __forceinline__ v3 cosine_hemisphere(Rng& rng) {
+template <typename R>
+__device__ __forceinline__ v3 cosine_hemisphere(R& rng) {
   const float r1 = rng.uni(), r2 = rng.uni();
   const float ox = 2.f * r1 - 1.f, oy = 2.f * r2 - 1.f;
   v3 d = mk(0.f, 0.f, 0.f);
@@ -536,7 +537,8 @@ __device__ __forceinline__ v3 cosine_hemisphere(Rng& rng) {
   return mk(d.x, d.y, z);
 }
 // CSGGXPhaseFunction::p (sggx_phase_function.hpp:214-218)
-__device__ __forceinline__ float phase_p(const float S6[6], v3 n, float ior, v3 wo, v3 wi, Rng& rng) {
+template <typename R>
+__device__ __forceinline__ float phase_p(const float S6[6], v3 n, float ior, v3 wo, v3 wi, R& rng) {
   const Sggx g = sggx_make(S6, n);
   const float w = fresnel(fabsf(dot(normalize(wo), n)), ior);
   const float ux = rng.uni(), uy = rng.uni();
@@ -549,7 +551,8 @@ __device__ __forceinline__ float phase_p(const float S6[6], v3 n, float ior, v3 
 // microflake normal, the diffuse lobe's frame normal), so that part runs once,
 // before the lobe choice: same draws in the same order, same arithmetic, and a
 // wavefront whose lanes choose different lobes does not run it twice.
-__device__ __forceinline__ float phase_sample(const float S6[6], v3 n, float ior, v3 wo, v3& wi, Rng& rng) {
+template <typename R>
+__device__ __forceinline__ float phase_sample(const float S6[6], v3 n, float ior, v3 wo, v3& wi, R& rng) {
   const Sggx g = sggx_make(S6, n);
   const float pf = fresnel(fabsf(dot(normalize(wo), n)), ior);
   const bool specular = rng.uni() < pf;
@@ -625,8 +628,8 @@ __device__ __forceinline__ v3 env_tex(const DEnv& e, float x, float y) {
   return mk(r[0], r[1], r[2]);
 }
 // CEnvironmentMap::sample (environmentmap.hpp:28-42)
-template <bool kCount>
-__device__ __forceinline__ v3 env_sample(const DEnv& e, Rng& rng, v3& dir, float& pdf, Cnt& cnt) {
+template <bool kCount, typename R>
+__device__ __forceinline__ v3 env_sample(const DEnv& e, R& rng, v3& dir, float& pdf, Cnt& cnt) {
   count<kCount>(cnt, kCntEnv);
   float p1, p0;
   int v, dummy;
@@ -736,7 +739,8 @@ __device__ __forceinline__ Ray inst_move_to_voxel_border(const DScene& s, int k,
   return r;
 }
 // CPixelSampler::samplePixel (pixel_sampler.hpp:36-47)
-__device__ __forceinline__ Ray camera_ray(const DScene& s, int px, int py, Rng& rng) {
+template <typename R>
+__device__ __forceinline__ Ray camera_ray(const DScene& s, int px, int py, R& rng) {
   const float rh = rng.uni();
   const float h = ((float)(px - s.W / 2) + rh) * s.pixel_size;
   (void)rng.uni();  // randomVertical: drawn, unused
