@@ -45,7 +45,7 @@ enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO 
 // spilling to scratch.
 struct ColdLane {
   uint32_t pix;
-  int32_t px, py;
+  int32_t sh_next, sh_seg;  // shadow ray walk: medium after the segment, segments walked
   int32_t s_cur, s_end;
   uint32_t item;
   float ax, ay, az;     // FP32 item sum (same order as the reference's framebuffer +=)
@@ -116,7 +116,6 @@ struct PathRng {
 struct PathLane {
   // work item (LDS)
   uint32_t& pix;
-  int32_t &px, &py;
   int32_t &s_cur, &s_end;
   uint32_t& item;
   float &ax, &ay, &az;
@@ -139,7 +138,8 @@ struct PathLane {
   float& ppdf;
   // shadow ray walk (intersectTr); sh_ret is also the state that follows ST_MOVE
   v3 &sh_p0, &sh_p1;
-  int sh_cur, sh_next, sh_hit, sh_seg, sh_ret;
+  int32_t &sh_next, &sh_seg;
+  int sh_cur, sh_hit, sh_ret;
   float sh_Tr;
   // traversal
   int tk, tinst, tsteps, tres;
@@ -147,10 +147,11 @@ struct PathLane {
   float tt, ttau, tmip, ttmax, ttr;
 
   __device__ PathLane(ColdLane& c, const DScene& s)
-      : pix(c.pix), px(c.px), py(c.py), s_cur(c.s_cur), s_end(c.s_end), item(c.item), ax(c.ax), ay(c.ay), az(c.az),
+      : pix(c.pix), s_cur(c.s_cur), s_end(c.s_end), item(c.item), ax(c.ax), ay(c.ay), az(c.az),
         rng(c.ring, s.seed, s.stream),
         Ld(c.Ld), si_pos(c.si_pos), mi_pos(c.mi_pos), mi_n(c.mi_n), S(c.S), mi_ior(c.mi_ior), mi_inst(c.mi_inst),
-        pdir(c.pdir), pLe(c.pLe), ppdf(c.ppdf), sh_p0(c.sh_p0), sh_p1(c.sh_p1) {}
+        pdir(c.pdir), pLe(c.pLe), ppdf(c.ppdf), sh_p0(c.sh_p0), sh_p1(c.sh_p1), sh_next(c.sh_next),
+        sh_seg(c.sh_seg) {}
 };
 
 template <bool kSingle>
@@ -320,7 +321,7 @@ __device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cn
   switch (st) {
     case ST_START: {
       p.rng.init(p.pix, (uint32_t)(s.sample_begin + p.s_cur));
-      p.ray = camera_ray(s, p.px, p.py, p.rng);
+      p.ray = camera_ray(s, (int)(p.pix % (uint32_t)s.W), (int)(p.pix / (uint32_t)s.W), p.rng);
       p.L = mk(0.f, 0.f, 0.f);
       p.thr = mk(1.f, 1.f, 1.f);
       p.b = 0;

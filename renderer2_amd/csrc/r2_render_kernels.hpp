@@ -259,8 +259,6 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
             p.s_cur = (int)c * chunk;
             p.s_end = min(s.spp, p.s_cur + chunk);
           }
-          p.px = (int)(p.pix % (uint32_t)s.W);
-          p.py = (int)(p.pix / (uint32_t)s.W);
           p.stage = ST_START;
         } else {
           p.stage = ST_IDLE;
@@ -324,8 +322,6 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
             p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
             p.s_cur = (int)c * chunk;
             p.s_end = min(s.spp, p.s_cur + chunk);
-            p.px = (int)(p.pix % (uint32_t)s.W);
-            p.py = (int)(p.pix / (uint32_t)s.W);
             p.stage = ST_START;
           } else {
             p.stage = ST_IDLE;
