@@ -294,8 +294,10 @@ int r2_grid_write_brickgrid(r2_ctx* ctx, const r2_grid* grid, const char* path, 
  * only.  r2_nvdb_write replaces COpenvdbBackend::setValues + writeToFile
  * (raytracer/src/filtering/openvdb_backend.cpp:108-157: openToNanoVDB +
  * nanovdb::io::writeGrid): records are nx*ny*nz 32-byte SFilteredDataCompact records,
- * x fastest; active voxels are those with density > 0 and the 8 grid corners; the
- * transform is the identity; node statistics are written as absent.
+ * x fastest; active voxels are those with density > 0 and the 8 grid corners;
+ * map_mat (row-major 3x3) and map_vec are the index -> model transform that setupGrid
+ * gives the grid (openvdb_backend.cpp:84-95), stored as the NanoVDB map (NULL: the
+ * identity); node statistics are written as absent.
  * r2_nvdb_read replaces the grid load of CNVDBMedium (raytracer/src/medium/
  * nvdb_medium.cu:130-173): info gets the index box and map; with records != NULL it
  * also gets the dense records over the index box (x fastest, inactive voxels = the
@@ -306,12 +308,12 @@ typedef struct {
   int32_t nx, ny, nz;     /* extent of the index bounding box */
   int32_t ibb_min[3];     /* its lower corner */
   int64_t active_voxels;
-  float map_mat[9];       /* index -> world, row major (mMatF) */
+  float map_mat[9];       /* index -> model, row major (mMatF) */
   float map_vec[3];       /* translation (mVecF) */
   int32_t leaf_count, lower_count, upper_count;
 } r2_nvdb_info;
-int r2_nvdb_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const void* records, char* err,
-                  int32_t err_len);
+int r2_nvdb_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const void* records, const float* map_mat,
+                  const float* map_vec, char* err, int32_t err_len);
 int r2_nvdb_read(const char* path, r2_nvdb_info* info, void* records, int64_t records_bytes, char* err,
                  int32_t err_len);
 

@@ -23,15 +23,20 @@ def filter_params(voxel_size: float, samples: int = 64, iterations: int = 8, sig
     return capi.FilterParams(voxel_size, samples, iterations, sigma_t, alpha, int(bool(clip_rays)), seed, lod)
 
 
-def nvdb_write(path, records: np.ndarray) -> None:
+def nvdb_write(path, records: np.ndarray, map_mat=None, map_vec=None) -> None:
     """NanoVDB file (one Vec4d grid) of (nz, ny, nx, 32) u8 records: the voxels with
-    density > 0 and the 8 grid corners are active.  Host only."""
+    density > 0 and the 8 grid corners are active; map_mat (3x3, row major) and map_vec
+    are the index -> model map (default: the identity).  Host only."""
     rec = np.ascontiguousarray(records, np.uint8)
     if rec.ndim != 4 or rec.shape[3] != 32:
         raise ValueError("records must be (nz, ny, nx, 32) uint8")
     nz, ny, nx = rec.shape[:3]
+    mm = None if map_mat is None else np.ascontiguousarray(np.asarray(map_mat, np.float32).reshape(9))
+    mv = None if map_vec is None else np.ascontiguousarray(np.asarray(map_vec, np.float32).reshape(3))
     err = C.create_string_buffer(512)
-    rc = capi.lib().r2_nvdb_write(str(path).encode(), nx, ny, nz, rec.ctypes.data, err, 512)
+    rc = capi.lib().r2_nvdb_write(str(path).encode(), nx, ny, nz, rec.ctypes.data,
+                                  None if mm is None else mm.ctypes.data, None if mv is None else mv.ctypes.data,
+                                  err, 512)
     if rc != 0:
         raise R2Error(rc, err.value.decode(errors="replace"))
 

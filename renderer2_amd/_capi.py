@@ -176,7 +176,8 @@ SIGNATURES = [
     ("r2_grid_lookup", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_write_brickgrid", C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32]),
-    ("r2_nvdb_write", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_char_p, C.c_int32]),
+    ("r2_nvdb_write", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_char_p, C.c_int32]),
     ("r2_nvdb_read", C.c_int, [C.c_char_p, C.POINTER(NvdbInfo), C.c_void_p, C.c_int64, C.c_char_p, C.c_int32]),
     ("r2_dda_cells", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                C.c_void_p]),

@@ -16,7 +16,8 @@
 // missing normals become flat face normals as with assimp's GenNormals),
 // Radiance .hdr environment maps (RGBE, flat or run-length scanlines), and this
 // build's grid files "<Path>.r2grid" (dense SFilteredDataCompact records with the
-// index -> model transform).  Surface objects (Circle, Sphere, meshes with the
+// index -> model transform) or, where none exists, the reference's NanoVDB file
+// "<Path>.nvdb"; filtering writes both, and the .brickgrid beside them.  Surface objects (Circle, Sphere, meshes with the
 // RENDER mask) are parsed but not rendered: the surface path is not part of this
 // library.
 #include "r2_internal.hpp"
@@ -739,11 +740,9 @@ void run_filter(r2_ctx* ctx, const Config& c, const fs::path& out_root, r2_pipel
     std::vector<uint8_t> rec((size_t)g.num_voxels[0] * g.num_voxels[1] * g.num_voxels[2] * 32);
     check(ctx, r2_filter_lod(ctx, mesh, &p, rec.data(), 0, &g), "filter");
     write_grid(dir / ("filtered_mesh_" + voxel_size_name(vs) + ".r2grid"), g, vs, rec);
-    // the LOD as a NanoVDB grid and its brick majorant grid beside it, as writeToFile
-    // does (openvdb_backend.cpp:134-157)
-    r2host::nvdb_write((dir / ("filtered_mesh_" + voxel_size_name(vs) + ".nvdb")).string(), g.num_voxels[0],
-                       g.num_voxels[1], g.num_voxels[2], rec.data());
     {
+      // the LOD as a NanoVDB grid (map = the grid's index -> model transform) and its
+      // brick majorant grid beside it, as writeToFile does (openvdb_backend.cpp:134-157)
       r2_grid_desc d{};
       d.nx = g.num_voxels[0];
       d.ny = g.num_voxels[1];
@@ -757,6 +756,8 @@ void run_filter(r2_ctx* ctx, const Config& c, const fs::path& out_root, r2_pipel
       }
       d.voxel_size_filtering = vs;
       d.precision = R2_PREC_FP32;
+      r2host::nvdb_write((dir / ("filtered_mesh_" + voxel_size_name(vs) + ".nvdb")).string(), d.nx, d.ny, d.nz,
+                         rec.data(), d.map_mat, d.map_vec);
       r2_grid* bg = nullptr;
       check(ctx, r2_grid_upload(ctx, &d, &bg), "brick grid build");
       try {
@@ -825,7 +826,31 @@ void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeli
     if (it != cache.end()) {
       g = it->second;
     } else {
-      const GridFile gf = read_grid(resolve(c, o.path + ".r2grid"));
+      // the library's own grid file, or else the reference's NanoVDB file of the
+      // same path (CNVDBMedium loads "<Path>.nvdb", nvdb_medium.cu:130-173)
+      const fs::path own = resolve(c, o.path + ".r2grid"), nano = resolve(c, o.path + ".nvdb");
+      GridFile gf;
+      if (fs::exists(own) || !fs::exists(nano)) {
+        gf = read_grid(own);
+      } else {
+        r2_nvdb_info ni{};
+        r2host::nvdb_read(nano.string(), &ni, nullptr, 0);
+        if (ni.nx > 8192 || ni.ny > 8192 || ni.nz > 8192)
+          throw std::runtime_error("grid file larger than 8192 voxels per axis: " + nano.string());
+        const size_t bytes = (size_t)ni.nx * ni.ny * ni.nz * 32;
+        gf.rec.resize(bytes);
+        r2host::nvdb_read(nano.string(), &ni, gf.rec.data(), (int64_t)bytes);
+        gf.n[0] = ni.nx, gf.n[1] = ni.ny, gf.n[2] = ni.nz;
+        // an index box starting at ibb_min != 0: fold the offset into the translation
+        const bool shifted = ni.ibb_min[0] != 0 || ni.ibb_min[1] != 0 || ni.ibb_min[2] != 0;
+        for (int r = 0; r < 3; ++r) {
+          for (int col = 0; col < 3; ++col) gf.i2m[col * 3 + r] = ni.map_mat[r * 3 + col];
+          gf.i2m[9 + r] = !shifted ? ni.map_vec[r]
+                                   : ni.map_vec[r] + ni.map_mat[r * 3] * (float)ni.ibb_min[0] +
+                                         ni.map_mat[r * 3 + 1] * (float)ni.ibb_min[1] +
+                                         ni.map_mat[r * 3 + 2] * (float)ni.ibb_min[2];
+        }
+      }
       r2_grid_desc d{};
       d.nx = gf.n[0];
       d.ny = gf.n[1];

@@ -101,7 +101,8 @@ hipError_t pool_get(r2_ctx* ctx, int slot, size_t bytes, void** out);
 // getRotation (functions.hpp:128-142), host only
 r2::m44 get_rotation(r2::v3 n);
 // NanoVDB grid files of dense 32-byte records (r2_nvdb.cpp)
-void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* records);
+void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* records, const float* map_mat,
+                const float* map_vec);
 void nvdb_read(const std::string& path, r2_nvdb_info* info, uint8_t* records, int64_t records_bytes);
 }  // namespace r2host
 

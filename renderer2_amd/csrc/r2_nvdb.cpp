@@ -8,7 +8,9 @@
 // SFilteredDataCompact record in the bits of a Vec4d; a voxel is active when its
 // density is > 0 or it is one of the 8 corners of the grid (setValues,
 // openvdb_backend.cpp:108-132); the background is the all-zero record with ior = 1
-// (openvdb_backend.cpp:30-42); the transform is OpenVDB's default (identity).
+// (openvdb_backend.cpp:30-42); the transform is the filter grid's index -> model
+// matrix (setupGrid, openvdb_backend.cpp:84-95), which openToNanoVDB stores as the
+// NanoVDB map (row-major 3x3 + translation, with its inverse, in float and double).
 //
 // The format is written and read here without the NanoVDB library, from the
 // published layout of NanoVDB 32.3 (the version shipped with OpenVDB 9.0):
@@ -19,10 +21,13 @@
 //           (8^3), each level in tree order (root key, then child index: x major);
 //           child references are signed byte offsets from the referring node; root
 //           tiles are keyed by the upper node's origin, 21 bits per axis.
-// Node statistics (min / max / average / deviation) are written as absent: the grid
-// flags carry HasBBox and IsBreadthFirst only, and the checksum field holds the
-// "none" value (all ones).  Nodes are streamed to the file one at a time.
+// The world bounding box is the index box (max + 1) mapped through the map in
+// double, corner by corner; the voxel size is the length of each column of the
+// map.  Node statistics (min / max / average / deviation) are written as absent:
+// the grid flags carry HasBBox and IsBreadthFirst only, and the checksum field
+// holds the "none" value (all ones).  Nodes are streamed to the file one at a time.
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <climits>
 #include <cstring>
@@ -150,6 +155,37 @@ void write_internal(std::ofstream& f, std::vector<uint8_t>& nb, size_t header, s
   f.write((const char*)p, (std::streamsize)nb.size());
 }
 
+// NanoVDB Map of an affine index -> world transform: A (row-major 3x3), t, and the
+// inverse of A (the inverse map subtracts t first, so it needs no translation).
+struct MapD {
+  double A[9], t[3], Ai[9];
+  double voxel[3];
+  void apply(const double x[3], double y[3]) const {
+    for (int r = 0; r < 3; ++r) y[r] = A[3 * r] * x[0] + A[3 * r + 1] * x[1] + A[3 * r + 2] * x[2] + t[r];
+  }
+};
+MapD make_map(const float* mat, const float* vec) {
+  MapD m{};
+  for (int i = 0; i < 9; ++i) m.A[i] = mat ? (double)mat[i] : (i % 4 == 0 ? 1.0 : 0.0);
+  for (int i = 0; i < 3; ++i) m.t[i] = vec ? (double)vec[i] : 0.0;
+  const double* a = m.A;
+  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+  if (!(det != 0.0) || !std::isfinite(det)) throw std::invalid_argument("nvdb: the index -> world map is singular");
+  const double id = 1.0 / det;
+  m.Ai[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+  m.Ai[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  m.Ai[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  m.Ai[3] = (a[5] * a[6] - a[3] * a[8]) * id;
+  m.Ai[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  m.Ai[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  m.Ai[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+  m.Ai[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  m.Ai[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+  for (int c = 0; c < 3; ++c) m.voxel[c] = std::sqrt(a[c] * a[c] + a[3 + c] * a[3 + c] + a[6 + c] * a[6 + c]);
+  return m;
+}
+
 void copy_err(char* err, int32_t err_len, const char* what) {
   if (err && err_len > 0) {
     strncpy(err, what, (size_t)err_len - 1);
@@ -161,9 +197,11 @@ void copy_err(char* err, int32_t err_len, const char* what) {
 
 namespace r2host {
 
-void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* records) {
+void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* records, const float* map_mat,
+                const float* map_vec) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || nx > kMaxExtent || ny > kMaxExtent || nz > kMaxExtent)
     throw std::invalid_argument("nvdb: grid size outside 1..65536 per axis");
+  const MapD map = make_map(map_mat, map_vec);
   const Rec bg = background_record();
   // leaves with at least one active voxel
   const int lx = (nx + 7) / 8, ly = (ny + 7) / 8, lz = (nz + 7) / 8;
@@ -193,6 +231,15 @@ void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* 
   std::sort(leaves.begin(), leaves.end(), [](const Leaf& a, const Leaf& b) { return tree_key(a.o) < tree_key(b.o); });
   const std::vector<Inner> lowers = group(leaves, 7), uppers = group(lowers, 12);
 
+  // world box: the 8 corners of the index box (max + 1) through the map
+  double wlo[3] = {INFINITY, INFINITY, INFINITY}, whi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int k = 0; k < 8; ++k) {
+    const double x[3] = {(double)((k & 1) ? all.hi[0] + 1 : all.lo[0]), (double)((k & 2) ? all.hi[1] + 1 : all.lo[1]),
+                         (double)((k & 4) ? all.hi[2] + 1 : all.lo[2])};
+    double y[3];
+    map.apply(x, y);
+    for (int a = 0; a < 3; ++a) wlo[a] = std::min(wlo[a], y[a]), whi[a] = std::max(whi[a], y[a]);
+  }
   const size_t n_leaf = leaves.size(), n_lower = lowers.size(), n_upper = uppers.size();
   const size_t off_tree = kGridData, off_root = off_tree + kTreeData;
   const size_t off_upper = off_root + kRootData + n_upper * kRootTile;
@@ -217,11 +264,11 @@ void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* 
     put<uint32_t>(m, 32, kGridTypeVec4d);
     put<uint32_t>(m, 36, kGridClassUnknown);
     for (int a = 0; a < 3; ++a) {
-      put<double>(m, 40 + 8 * a, (double)all.lo[a]);
-      put<double>(m, 64 + 8 * a, (double)all.hi[a] + 1.0);
+      put<double>(m, 40 + 8 * a, wlo[a]);
+      put<double>(m, 64 + 8 * a, whi[a]);
       put<int32_t>(m, 88 + 4 * a, all.lo[a]);
       put<int32_t>(m, 100 + 4 * a, all.hi[a]);
-      put<double>(m, 112 + 8 * a, 1.0);
+      put<double>(m, 112 + 8 * a, map.voxel[a]);
     }
     put<uint32_t>(m, 136, 1u);  // name size, terminating NUL included
     put<uint32_t>(m, 140, (uint32_t)n_leaf);
@@ -242,19 +289,23 @@ void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* 
     put<uint32_t>(g, 24, 0u);  // grid index
     put<uint32_t>(g, 28, 1u);  // grid count
     put<uint64_t>(g, 32, (uint64_t)total);
-    // map (identity): mMatF, mInvMatF, mVecF, mTaperF, mMatD, mInvMatD, mVecD, mTaperD
+    // map: mMatF, mInvMatF, mVecF, mTaperF, mMatD, mInvMatD, mVecD, mTaperD
+    for (int i = 0; i < 9; ++i) {
+      put<float>(g, kGdMap + 4 * i, (float)map.A[i]);
+      put<float>(g, kGdMap + 36 + 4 * i, (float)map.Ai[i]);
+      put<double>(g, kGdMap + 88 + 8 * i, map.A[i]);
+      put<double>(g, kGdMap + 160 + 8 * i, map.Ai[i]);
+    }
     for (int i = 0; i < 3; ++i) {
-      put<float>(g, kGdMap + 4 * (4 * i), 1.f);
-      put<float>(g, kGdMap + 36 + 4 * (4 * i), 1.f);
-      put<double>(g, kGdMap + 88 + 8 * (4 * i), 1.0);
-      put<double>(g, kGdMap + 160 + 8 * (4 * i), 1.0);
+      put<float>(g, kGdMap + 72 + 4 * i, (float)map.t[i]);
+      put<double>(g, kGdMap + 232 + 8 * i, map.t[i]);
     }
     put<float>(g, kGdMap + 84, 1.f);
     put<double>(g, kGdMap + 256, 1.0);
     for (int a = 0; a < 3; ++a) {
-      put<double>(g, kGdWorldBBox + 8 * a, (double)all.lo[a]);
-      put<double>(g, kGdWorldBBox + 24 + 8 * a, (double)all.hi[a] + 1.0);
-      put<double>(g, kGdVoxelSize + 8 * a, 1.0);
+      put<double>(g, kGdWorldBBox + 8 * a, wlo[a]);
+      put<double>(g, kGdWorldBBox + 24 + 8 * a, whi[a]);
+      put<double>(g, kGdVoxelSize + 8 * a, map.voxel[a]);
     }
     put<uint32_t>(g, kGdClass, kGridClassUnknown);
     put<uint32_t>(g, kGdType, kGridTypeVec4d);
@@ -410,11 +461,11 @@ void nvdb_read(const std::string& path, r2_nvdb_info* info, uint8_t* records, in
 
 }  // namespace r2host
 
-extern "C" int r2_nvdb_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const void* records, char* err,
-                             int32_t err_len) {
+extern "C" int r2_nvdb_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const void* records,
+                             const float* map_mat, const float* map_vec, char* err, int32_t err_len) {
   if (!path || !records || nx <= 0 || ny <= 0 || nz <= 0) return R2_ERR_INVALID;
   try {
-    r2host::nvdb_write(path, nx, ny, nz, (const uint8_t*)records);
+    r2host::nvdb_write(path, nx, ny, nz, (const uint8_t*)records, map_mat, map_vec);
   } catch (const std::invalid_argument& e) {
     copy_err(err, err_len, e.what());
     return R2_ERR_INVALID;

@@ -149,6 +149,10 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
         bg = np.zeros(32, np.uint8)
         bg[29] = 0x3C
         assert np.array_equal(nv, np.where(act[..., None], ref, bg))
+        # its map is the grid's index -> model transform (row-major, as NanoVDB's mMatF)
+        i2m = np.frombuffer(data[20:68], np.float32)
+        assert np.array_equal(info["map_mat"], i2m[:9].reshape(3, 3).T.ravel())
+        assert np.array_equal(info["map_vec"], i2m[9:12])
     desc = json.loads((tmp_path / "filtering" / "volume_description.json").read_text())
     assert set(desc["blob"].keys()) == {"%f" % vs, "%f" % np.float32(vs * 2)}
     # output.png = post-process(output.pfm), rows flipped
@@ -157,3 +161,16 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
     assert np.isfinite(hdr).all() and hdr.mean() > 0
     _, by = O.postprocess(np.ascontiguousarray(hdr), 2.0)
     assert np.array_equal(png, by[::-1])
+    # the same scene rendered from the .nvdb files alone (the reference's own filter
+    # output format): the .r2grid files removed, filtering off -> the same image
+    for f in (tmp_path / "filtering" / "blob").glob("*.r2grid"):
+        f.unlink()
+    cfg["Filtering"]["Active"] = False
+    (tmp_path / "config.json").write_text(json.dumps(cfg))
+    out2 = tmp_path / "from_nvdb"
+    out2.mkdir()
+    rc = capi.lib().r2_pipeline_run(str(tmp_path / "config.json").encode(), str(out2).encode(), 0, C.byref(rep),
+                                    err, 1024)
+    assert rc == 0, err.value.decode()
+    assert rep.lods_filtered == 0 and rep.media_rendered == 1
+    assert np.array_equal(read_pfm(out2 / "output.pfm"), hdr)

@@ -168,6 +168,39 @@ def test_file_header(tmp_path):
     assert wmin == (0.0, 0.0, 0.0) and wmax == (9.0, 9.0, 9.0)
 
 
+def test_map_world_box_and_voxel_size(tmp_path):
+    """The index -> model map (setupGrid's indexToModel) is stored as the NanoVDB map;
+    the world box is the index box (max + 1) through the map, the voxel size the
+    column lengths, and the stored inverse inverts the stored matrix."""
+    rec, _ = records((6, 10, 14), 2, 0.3)
+    c, s_ = np.cos(0.3), np.sin(0.3)
+    A = (np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]]) @ np.diag([0.02, 0.03, 0.05])).astype(np.float32)
+    t = np.array([-0.4, 0.25, 1.5], np.float32)
+    p = tmp_path / "g.nvdb"
+    r2.nvdb_write(p, rec, A, t)
+    info, got = r2.nvdb_read(p)
+    assert np.array_equal(info["map_mat"], A.ravel()) and np.array_equal(info["map_vec"], t)
+    data = p.read_bytes()
+    g = 16 + 176 + 1
+    Ad = np.array(struct.unpack_from("<9d", data, g + 296 + 88)).reshape(3, 3)
+    Aid = np.array(struct.unpack_from("<9d", data, g + 296 + 160)).reshape(3, 3)
+    td = np.array(struct.unpack_from("<3d", data, g + 296 + 232))
+    assert np.array_equal(Ad, A.astype(np.float64)) and np.array_equal(td, t.astype(np.float64))
+    assert np.abs(Ad @ Aid - np.eye(3)).max() < 1e-12
+    corners = np.array([[x, y, z] for x in (0, 14) for y in (0, 10) for z in (0, 6)], np.float64)
+    w = corners @ Ad.T + td
+    wmin = np.array(struct.unpack_from("<3d", data, g + 560))
+    wmax = np.array(struct.unpack_from("<3d", data, g + 584))
+    assert np.allclose(wmin, w.min(0), rtol=0, atol=1e-12) and np.allclose(wmax, w.max(0), rtol=0, atol=1e-12)
+    vs = np.array(struct.unpack_from("<3d", data, g + 608))
+    assert np.allclose(vs, [0.02, 0.03, 0.05], rtol=1e-6)
+    meta_w = np.array(struct.unpack_from("<6d", data, 16 + 40))
+    assert np.array_equal(meta_w, np.concatenate([wmin, wmax]))
+    with pytest.raises(r2.R2Error) as e:
+        r2.nvdb_write(p, rec, np.zeros((3, 3), np.float32), t)
+    assert e.value.code == -1  # singular map
+
+
 def test_errors(tmp_path):
     rec, _ = records((8, 8, 8), 1, 0.5)
     p = tmp_path / "g.nvdb"
@@ -188,5 +221,5 @@ def test_errors(tmp_path):
     buf = np.zeros(100, np.uint8)
     rc = capi.lib().r2_nvdb_read(str(p).encode(), info, buf.ctypes.data, buf.nbytes, None, 0)
     assert rc == -1  # R2_ERR_INVALID: buffer smaller than 8^3 records
-    rc = capi.lib().r2_nvdb_write(str(p).encode(), 0, 1, 1, buf.ctypes.data, None, 0)
+    rc = capi.lib().r2_nvdb_write(str(p).encode(), 0, 1, 1, buf.ctypes.data, None, None, None, 0)
     assert rc == -1
