@@ -75,7 +75,7 @@ struct PathRng {
   const uint32_t k0, k1;  // launch-uniform: kept in scalar registers
   uint32_t c0, c1;
   uint32_t n;   // words drawn
-  uint32_t hi;  // blocks 0 .. hi - 1 computed; the ring holds hi - 2 and hi - 1
+  uint32_t hi;  // blocks 0 .. hi - 1 computed; the ring holds the last two of them (block 0 alone after init)
   uint32_t (&ring)[8];
   __device__ PathRng(uint32_t (&r)[8], uint32_t seed, uint32_t stream) : k0(seed), k1(stream), ring(r) {}
   __device__ __forceinline__ void block(uint32_t b) {
