@@ -32,10 +32,19 @@ __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, 
                          uint8_t* __restrict__ active, uint32_t* __restrict__ leaf_flag, int nlx, int nly,
                          int* __restrict__ bbox, uint32_t* __restrict__ max_bits) {
   const size_t n = (size_t)nx * ny * nz;
+  const bool small = n <= 0xFFFFFFFFull;  // 32-bit index arithmetic (the common case)
   float lmax = 0.f, lmin = FLT_MAX;
   int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    int x, y, z;
+    if (small) {
+      const uint32_t i32 = (uint32_t)i, r = i32 / (uint32_t)nx;
+      x = (int)(i32 - r * (uint32_t)nx);
+      y = (int)(r % (uint32_t)ny);
+      z = (int)(r / (uint32_t)ny);
+    } else {
+      x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    }
     const float d = __uint_as_float(rec[2 * i].x);
     const bool a = (d > 0.f) || is_corner(x, y, z, nx, ny, nz);
     active[i] = a ? 1 : 0;
@@ -57,7 +66,24 @@ __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, 
       hi[a] = max(hi[a], __shfl_xor(hi[a], off));
     }
   }
+  // then across the block's wavefronts in LDS, and one set of atomics per block
+  // (same-address atomics from every wavefront serialise in L2)
+  __shared__ float s_max[4], s_min[4];
+  __shared__ int s_lo[4][3], s_hi[4][3];
+  const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    s_max[wave] = lmax;
+    s_min[wave] = lmin;
+    for (int a = 0; a < 3; ++a) s_lo[wave][a] = lo[a], s_hi[wave][a] = hi[a];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 1; w < nw; ++w) {
+      lmax = fmaxf(lmax, s_max[w]);
+      lmin = fminf(lmin, s_min[w]);
+      for (int a = 0; a < 3; ++a) lo[a] = min(lo[a], s_lo[w][a]), hi[a] = max(hi[a], s_hi[w][a]);
+    }
     atomicMax(max_bits, __float_as_uint(fmaxf(lmax, 0.f)));
     atomicMin(max_bits + 1, float_key(lmin));
     for (int a = 0; a < 3; ++a) {
@@ -307,7 +333,8 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   R2_HIP(ctx, hipMemcpyAsync(d_max, mm_init, 8, hipMemcpyHostToDevice, st));
   R2_HIP(ctx, hipMemsetAsync(d_leaf_flag, 0, nleaf_total * 4, st));
   {
-    const int blocks = (int)std::min<size_t>((nv + 255) / 256, 65536);
+    // grid-stride over the voxels: 8 blocks per CU are enough to stream the records
+    const int blocks = (int)std::min<size_t>((nv + 255) / 256, 2048);
     hipLaunchKernelGGL(k_active, dim3(blocks), dim3(256), 0, st, d_rec, nx, ny, nz, d_dens, d_active, d_leaf_flag, nlx,
                        nly, d_bbox, d_max);
     R2_HIP(ctx, hipGetLastError());
