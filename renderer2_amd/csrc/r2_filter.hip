@@ -7,7 +7,8 @@
 //                   can reach it, split into 64-voxel tasks.
 //   2. k_pairs      one wave per task, one voxel per lane: exact point/triangle
 //                   distance against the voxel's reach radius; passing (voxel, tri)
-//                   pairs are counted, then emitted as 64-bit keys voxel<<32 | tri.
+//                   pairs are counted (with the task's lane mask), then emitted
+//                   from the masks as 64-bit keys voxel<<32 | tri.
 //   3. radix sort + run-length encode: sorted candidate list per active voxel.
 //   4. k_filter     one wave per active voxel, the 64 lanes are the voxel's rays.
 //                   Every lane tests every candidate triangle (uniform loop; the
@@ -170,46 +171,72 @@ __global__ void k_task_mark(const unsigned long long* __restrict__ tasks, const 
   task_tri[begin[t]] = (uint32_t)t;
 }
 
-// one wave per 64-voxel task, one voxel per lane.  kEmit = false: the task's
-// pair count goes to task_cnt[task]; kEmit = true: the pairs are written from
-// task_off[task] on, in lane order (no atomics: the scan of the counts places them)
+// voxel of lane `local` of a triangle's voxel range (x fastest), in 32-bit
+// arithmetic when the range allows it
+__device__ __forceinline__ void range_voxel(const TriRange& r, unsigned long long local, int& ix, int& iy, int& iz) {
+  const unsigned long long dxy = (unsigned long long)r.dim[0] * r.dim[1];
+  if (dxy * (unsigned long long)r.dim[2] <= 0xFFFFFFFFull) {
+    const uint32_t l = (uint32_t)local, d0 = (uint32_t)r.dim[0], d01 = (uint32_t)dxy;
+    const uint32_t z = l / d01, rem = l - z * d01, y = rem / d0;
+    iz = r.lo[2] + (int)z;
+    iy = r.lo[1] + (int)y;
+    ix = r.lo[0] + (int)(rem - y * d0);
+  } else {
+    const unsigned long long rem = local % dxy;
+    iz = r.lo[2] + (int)(local / dxy);
+    iy = r.lo[1] + (int)(rem / r.dim[0]);
+    ix = r.lo[0] + (int)(rem % r.dim[0]);
+  }
+}
+
+// one wave per 64-voxel task, one voxel per lane.  kEmit = false: the exact
+// point/triangle test; the task's pair count goes to task_cnt[task] and its lane
+// mask to task_mask[task].  kEmit = true: the pairs of the stored mask are written
+// from task_off[task] on, in lane order (no atomics: the scan of the counts places
+// them), without testing again
 template <bool kEmit>
 __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, FConst c,
                                                const TriRange* __restrict__ rng,
                                                const unsigned long long* __restrict__ task_begin,
                                                const uint32_t* __restrict__ task_tri, unsigned long long n_tasks,
-                                               uint32_t* __restrict__ task_cnt, const uint32_t* __restrict__ task_off,
+                                               uint32_t* __restrict__ task_cnt,
+                                               unsigned long long* __restrict__ task_mask,
+                                               const uint32_t* __restrict__ task_off,
                                                unsigned long long* __restrict__ keys) {
   const int lane = threadIdx.x & 63;
   const unsigned long long task =
       (unsigned long long)blockIdx.x * kWaves + (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (task >= n_tasks) return;
+  if (kEmit) {
+    const unsigned long long ball = task_mask[task];
+    if (ball == 0) return;
+    if (!((ball >> lane) & 1ull)) return;
+    const int t = __builtin_amdgcn_readfirstlane((int)task_tri[task]);
+    const TriRange r = rng[t];
+    int ix, iy, iz;
+    range_voxel(r, (task - task_begin[t]) * 64ull + (unsigned long long)lane, ix, iy, iz);
+    const uint32_t vid = (uint32_t)ix + (uint32_t)c.nx * ((uint32_t)iy + (uint32_t)c.ny * (uint32_t)iz);
+    const unsigned long long slot =
+        (unsigned long long)task_off[task] + (unsigned long long)__popcll(ball & ((1ull << lane) - 1ull));
+    keys[slot] = ((unsigned long long)vid << 32) | (unsigned long long)(uint32_t)t;
+    return;
+  }
   const int t = __builtin_amdgcn_readfirstlane((int)task_tri[task]);
   const TriRange r = rng[t];
   const unsigned long long local = (task - task_begin[t]) * 64ull + (unsigned long long)lane;
   const unsigned long long vol = (unsigned long long)r.dim[0] * r.dim[1] * r.dim[2];
   bool pass = false;
-  uint32_t vid = 0;
   if (local < vol) {
-    const unsigned long long dxy = (unsigned long long)r.dim[0] * r.dim[1];
-    const int iz = r.lo[2] + (int)(local / dxy);
-    const unsigned long long rem = local % dxy;
-    const int iy = r.lo[1] + (int)(rem / r.dim[0]);
-    const int ix = r.lo[0] + (int)(rem % r.dim[0]);
+    int ix, iy, iz;
+    range_voxel(r, local, ix, iy, iz);
     const v3 ctr = voxel_center(c, ix, iy, iz);
     const v3 a = f4v(tri[3 * t]), b = f4v(tri[3 * t + 1]), d = f4v(tri[3 * t + 2]);
     pass = dist2_point_triangle(ctr, a, b, d) <= c.reach * c.reach;
-    vid = (uint32_t)ix + (uint32_t)c.nx * ((uint32_t)iy + (uint32_t)c.ny * (uint32_t)iz);
   }
   const unsigned long long ball = __ballot(pass);
-  if (!kEmit) {
-    if (lane == 0) task_cnt[task] = (uint32_t)__popcll(ball);
-    return;
-  }
-  if (pass) {
-    const unsigned long long slot =
-        (unsigned long long)task_off[task] + (unsigned long long)__popcll(ball & ((1ull << lane) - 1ull));
-    keys[slot] = ((unsigned long long)vid << 32) | (unsigned long long)(uint32_t)t;
+  if (lane == 0) {
+    task_cnt[task] = (uint32_t)__popcll(ball);
+    task_mask[task] = ball;
   }
 }
 
@@ -654,7 +681,7 @@ struct Binning {
 enum {
   kSlotRange = 0, kSlotTasks, kSlotBegin, kSlotCounter, kSlotTaskMark, kSlotTaskTri, kSlotKeys, kSlotKeys2,
   kSlotVox, kSlotCand, kSlotAct, kSlotCnt, kSlotOff, kSlotTemp, kSlotRuns, kSlotRecords, kSlotIds, kSlotTaskCnt,
-  kSlotTaskOff
+  kSlotTaskOff, kSlotTaskMask
 };
 
 template <class T>
@@ -700,10 +727,12 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   R2F_HIP(hipcub::DeviceScan::InclusiveScan(tmp, tb, mark, task_tri, hipcub::Max(), nt, s), "filter: task map");
   const dim3 grid((unsigned)((B->n_tasks + kWaves - 1) / kWaves));
   uint32_t *task_cnt = nullptr, *task_off = nullptr;
+  unsigned long long* task_mask = nullptr;
   R2F_HIP(pooled(ctx, kSlotTaskCnt, nt, &task_cnt), "filter: alloc task counts");
   R2F_HIP(pooled(ctx, kSlotTaskOff, nt, &task_off), "filter: alloc task offsets");
+  R2F_HIP(pooled(ctx, kSlotTaskMask, nt, &task_mask), "filter: alloc task masks");
   hipLaunchKernelGGL(k_pairs<false>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, task_cnt,
-                     (const uint32_t*)nullptr, (unsigned long long*)nullptr);
+                     task_mask, (const uint32_t*)nullptr, (unsigned long long*)nullptr);
   R2F_HIP(hipGetLastError(), "filter: k_pairs count");
   R2F_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, task_cnt, task_off, nt, s), "filter: scan size");
   R2F_HIP(r2host::pool_get(ctx, kSlotTemp, tb, &tmp), "filter: alloc scan temp");
@@ -723,7 +752,7 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   R2F_HIP(pooled(ctx, kSlotKeys, np, &keys), "filter: alloc keys");
   R2F_HIP(pooled(ctx, kSlotKeys2, np, &keys2), "filter: alloc keys");
   hipLaunchKernelGGL(k_pairs<true>, grid, dim3(256), 0, s, m->tri, c, rng, begin, task_tri, B->n_tasks, task_cnt,
-                     (const uint32_t*)task_off, keys);
+                     task_mask, (const uint32_t*)task_off, keys);
   R2F_HIP(hipGetLastError(), "filter: k_pairs emit");
   const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
   int vbits = 1;
