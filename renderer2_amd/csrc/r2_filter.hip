@@ -189,11 +189,26 @@ __device__ __forceinline__ void range_voxel(const TriRange& r, unsigned long lon
   }
 }
 
-// one wave per 64-voxel task, one voxel per lane.  kEmit = false: the exact
-// point/triangle test; the task's pair count goes to task_cnt[task] and its lane
-// mask to task_mask[task].  kEmit = true: the pairs of the stored mask are written
-// from task_off[task] on, in lane order (no atomics: the scan of the counts places
-// them), without testing again
+// 64-voxel tasks.  A block takes 256 consecutive tasks: its threads first load
+// the tasks' metadata (triangle, voxel range, first task of the triangle, and the
+// triangle or the stored mask) into LDS, one task each, so that the dependent
+// loads of all 256 tasks are in flight together.
+//   kEmit = false: the exact point/triangle test of every (task, voxel) pair.  The
+//     block's pairs are packed densely onto its lanes through a prefix sum of the
+//     tasks' voxel counts (a lane finds its task by binary search).  About nine in
+//     ten pairs of a triangle's range are out of reach, so each pair first takes a
+//     conservative sphere test: the distance to the triangle is at least the
+//     distance to its centroid minus the largest vertex distance, and a pair is
+//     dropped only when that bound exceeds the reach by a margin (0.1 % relative
+//     plus 1e-5 of the coordinates' magnitude) far above the rounding of either
+//     test, so no pair the exact test would accept is dropped.  The survivors
+//     queue up per wave in LDS and take the exact test 64 at a time; a passing
+//     pair sets its bit in the task's mask with an LDS atomic OR.  Each task's
+//     mask and pair count then go to task_mask / task_cnt.
+//   kEmit = true: one wave per task, one bit per lane: the pairs of the stored
+//     mask are written from task_off[task] on, in lane order (no atomics: the
+//     scan of the counts places them), without testing again.
+constexpr int kTasksPerBlock = 256;
 template <bool kEmit>
 __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, FConst c,
                                                const TriRange* __restrict__ rng,
@@ -203,40 +218,149 @@ __global__ __launch_bounds__(256) void k_pairs(const float4* __restrict__ tri, F
                                                unsigned long long* __restrict__ task_mask,
                                                const uint32_t* __restrict__ task_off,
                                                unsigned long long* __restrict__ keys) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long task =
-      (unsigned long long)blockIdx.x * kWaves + (unsigned)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (task >= n_tasks) return;
-  if (kEmit) {
-    const unsigned long long ball = task_mask[task];
-    if (ball == 0) return;
-    if (!((ball >> lane) & 1ull)) return;
-    const int t = __builtin_amdgcn_readfirstlane((int)task_tri[task]);
+  __shared__ uint32_t s_t[kTasksPerBlock];
+  __shared__ TriRange s_r[kTasksPerBlock];
+  __shared__ unsigned long long s_beg[kTasksPerBlock];
+  __shared__ unsigned long long s_mask[kTasksPerBlock];
+  __shared__ uint32_t s_off[kTasksPerBlock];  // emit: task_off; test: exclusive prefix of the voxel counts
+  __shared__ uint32_t s_wsum[4];
+  __shared__ float4 s_tri[kEmit ? 1 : 3 * kTasksPerBlock];
+  __shared__ float4 s_sph[kEmit ? 1 : kTasksPerBlock];             // centroid, radius
+  __shared__ uint32_t s_q[kEmit ? 1 : 4][kEmit ? 1 : 128];          // queued pairs: task << 8 | voxel
+  __shared__ float4 s_qc[kEmit ? 1 : 4][kEmit ? 1 : 128];           // their voxel centres
+  const int i = threadIdx.x, lane = i & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(i >> 6);
+  const unsigned long long base = (unsigned long long)blockIdx.x * kTasksPerBlock;
+  const unsigned long long mine = base + (unsigned long long)i;
+  const int nk = (int)(n_tasks - base < (unsigned long long)kTasksPerBlock ? n_tasks - base : kTasksPerBlock);
+  uint32_t act = 0;  // voxels of this thread's task
+  if (mine < n_tasks) {
+    const uint32_t t = task_tri[mine];
     const TriRange r = rng[t];
-    int ix, iy, iz;
-    range_voxel(r, (task - task_begin[t]) * 64ull + (unsigned long long)lane, ix, iy, iz);
-    const uint32_t vid = (uint32_t)ix + (uint32_t)c.nx * ((uint32_t)iy + (uint32_t)c.ny * (uint32_t)iz);
-    const unsigned long long slot =
-        (unsigned long long)task_off[task] + (unsigned long long)__popcll(ball & ((1ull << lane) - 1ull));
-    keys[slot] = ((unsigned long long)vid << 32) | (unsigned long long)(uint32_t)t;
+    const unsigned long long beg = task_begin[t];
+    s_t[i] = t;
+    s_r[i] = r;
+    s_beg[i] = beg;
+    if (kEmit) {
+      s_mask[i] = task_mask[mine];
+      s_off[i] = task_off[mine];
+    } else {
+      const float4 ta = tri[3 * t], tb = tri[3 * t + 1], tc = tri[3 * t + 2];
+      s_tri[3 * i] = ta;
+      s_tri[3 * i + 1] = tb;
+      s_tri[3 * i + 2] = tc;
+      const v3 va = f4v(ta), vb = f4v(tb), vc = f4v(tc);
+      const v3 cen = (va + vb + vc) * (1.f / 3.f);
+      const float r2 = fmaxf(dot(va - cen, va - cen), fmaxf(dot(vb - cen, vb - cen), dot(vc - cen, vc - cen)));
+      s_sph[i] = make_float4(cen.x, cen.y, cen.z, sqrtf(r2));
+      s_mask[i] = 0ull;
+      const unsigned long long vol = (unsigned long long)r.dim[0] * r.dim[1] * r.dim[2];
+      const unsigned long long first = (mine - beg) * 64ull;
+      act = vol > first ? (uint32_t)(vol - first < 64ull ? vol - first : 64ull) : 0u;
+    }
+  }
+  if (kEmit) {
+    __syncthreads();
+    for (int k = wave; k < nk; k += (int)(blockDim.x >> 6)) {
+      const unsigned long long ball = s_mask[k];
+      if ((ball >> lane) & 1ull) {
+        int ix, iy, iz;
+        range_voxel(s_r[k], (base + (unsigned long long)k - s_beg[k]) * 64ull + (unsigned long long)lane, ix, iy, iz);
+        const uint32_t vid = (uint32_t)ix + (uint32_t)c.nx * ((uint32_t)iy + (uint32_t)c.ny * (uint32_t)iz);
+        const unsigned long long slot =
+            (unsigned long long)s_off[k] + (unsigned long long)__popcll(ball & ((1ull << lane) - 1ull));
+        keys[slot] = ((unsigned long long)vid << 32) | (unsigned long long)s_t[k];
+      }
+    }
     return;
   }
-  const int t = __builtin_amdgcn_readfirstlane((int)task_tri[task]);
-  const TriRange r = rng[t];
-  const unsigned long long local = (task - task_begin[t]) * 64ull + (unsigned long long)lane;
-  const unsigned long long vol = (unsigned long long)r.dim[0] * r.dim[1] * r.dim[2];
-  bool pass = false;
-  if (local < vol) {
-    int ix, iy, iz;
-    range_voxel(r, local, ix, iy, iz);
-    const v3 ctr = voxel_center(c, ix, iy, iz);
-    const v3 a = f4v(tri[3 * t]), b = f4v(tri[3 * t + 1]), d = f4v(tri[3 * t + 2]);
-    pass = dist2_point_triangle(ctr, a, b, d) <= c.reach * c.reach;
+  // exclusive prefix of the voxel counts over the block: within each wave, then
+  // across the four waves
+  uint32_t incl = act;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+    if (lane >= o) incl += v;
   }
-  const unsigned long long ball = __ballot(pass);
-  if (lane == 0) {
-    task_cnt[task] = (uint32_t)__popcll(ball);
-    task_mask[task] = ball;
+  if (lane == 63) s_wsum[wave] = incl;
+  __syncthreads();
+  uint32_t wbase = 0, total = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) wbase += s_wsum[w];
+    total += s_wsum[w];
+  }
+  s_off[i] = wbase + incl - act;
+  __syncthreads();
+  // the block's (task, voxel) pairs, 64 per wave step: sphere test, then the
+  // survivors through the per-wave queue to the exact test
+  uint32_t* q = s_q[kEmit ? 0 : wave];
+  float4* qc = s_qc[kEmit ? 0 : wave];
+  int qn = 0;  // queued pairs (wave-uniform)
+  auto exact = [&](int n) {
+    if (lane < n) {
+      const uint32_t e = q[lane];
+      const int k = (int)(e >> 8), j = (int)(e & 63u);
+      const float4 ctr = qc[lane];
+      if (dist2_point_triangle(mk(ctr.x, ctr.y, ctr.z), f4v(s_tri[3 * k]), f4v(s_tri[3 * k + 1]),
+                               f4v(s_tri[3 * k + 2])) <= c.reach * c.reach)
+        atomicOr(&s_mask[k], 1ull << j);
+    }
+  };
+  for (uint32_t p0 = (uint32_t)wave * 64u; p0 < total; p0 += 256u) {
+    const uint32_t p = p0 + (uint32_t)lane;
+    bool cand = false;
+    int k = 0;
+    uint32_t j = 0;
+    v3 ctr = mk(0.f, 0.f, 0.f);
+    if (p < total) {
+      int lo = 0, hi = nk - 1;  // last task with prefix <= p
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_off[mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      k = lo;
+      j = p - s_off[k];
+      int ix, iy, iz;
+      range_voxel(s_r[k], (base + (unsigned long long)k - s_beg[k]) * 64ull + (unsigned long long)j, ix, iy, iz);
+      ctr = voxel_center(c, ix, iy, iz);
+      const float4 sp = s_sph[k];
+      const v3 dv = ctr - mk(sp.x, sp.y, sp.z);
+      const float mag = fmaxf(fabsf(ctr.x), fmaxf(fabsf(ctr.y), fabsf(ctr.z)));
+      cand = !(sqrtf(dot(dv, dv)) > (sp.w + c.reach) * 1.001f + 1e-5f * mag);  // NaN: keep
+    }
+    const unsigned long long b = __ballot(cand);
+    if (cand) {
+      const int slot = qn + __popcll(b & ((1ull << lane) - 1ull));
+      q[slot] = ((uint32_t)k << 8) | j;
+      qc[slot] = make_float4(ctr.x, ctr.y, ctr.z, 0.f);
+    }
+    qn += __popcll(b);
+    if (qn >= 64) {
+      wave_sync();
+      exact(64);
+      // move the rest (fewer than 64) to the front of the queue
+      uint32_t e = 0;
+      float4 ec = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane < qn - 64) {
+        e = q[64 + lane];
+        ec = qc[64 + lane];
+      }
+      wave_sync();
+      if (lane < qn - 64) {
+        q[lane] = e;
+        qc[lane] = ec;
+      }
+      qn -= 64;
+      wave_sync();
+    }
+  }
+  if (qn > 0) {
+    wave_sync();
+    exact(qn);
+  }
+  __syncthreads();
+  if (mine < n_tasks) {
+    task_mask[mine] = s_mask[i];
+    task_cnt[mine] = (uint32_t)__popcll(s_mask[i]);
   }
 }
 
@@ -725,7 +849,7 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   R2F_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, tb, mark, task_tri, hipcub::Max(), nt, s), "filter: scan size");
   R2F_HIP(r2host::pool_get(ctx, kSlotTemp, tb, &tmp), "filter: alloc scan temp");
   R2F_HIP(hipcub::DeviceScan::InclusiveScan(tmp, tb, mark, task_tri, hipcub::Max(), nt, s), "filter: task map");
-  const dim3 grid((unsigned)((B->n_tasks + kWaves - 1) / kWaves));
+  const dim3 grid((unsigned)((B->n_tasks + kTasksPerBlock - 1) / kTasksPerBlock));
   uint32_t *task_cnt = nullptr, *task_off = nullptr;
   unsigned long long* task_mask = nullptr;
   R2F_HIP(pooled(ctx, kSlotTaskCnt, nt, &task_cnt), "filter: alloc task counts");
@@ -759,9 +883,11 @@ int bin_triangles(r2_ctx* ctx, const r2_mesh* m, const FConst& c, Binning* B) {
   while ((1ull << vbits) < nvox) ++vbits;
   const int npi = (int)np;
   size_t sb = 0;
-  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sb, keys, keys2, npi, 0, 32 + vbits, s), "filter: sort size");
+  // the keys arrive in task order, i.e. ascending triangle; the radix sort is
+  // stable, so sorting the voxel bits alone gives the same (voxel, triangle) order
+  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sb, keys, keys2, npi, 32, 32 + vbits, s), "filter: sort size");
   R2F_HIP(r2host::pool_get(ctx, kSlotTemp, sb, &tmp), "filter: alloc sort temp");
-  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, sb, keys, keys2, npi, 0, 32 + vbits, s), "filter: sort");
+  R2F_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, sb, keys, keys2, npi, 32, 32 + vbits, s), "filter: sort");
   uint32_t *vox = nullptr, *runs = nullptr;
   R2F_HIP(pooled(ctx, kSlotVox, np, &vox), "filter: alloc voxels");
   R2F_HIP(pooled(ctx, kSlotCand, np, &B->cand), "filter: alloc candidates");
