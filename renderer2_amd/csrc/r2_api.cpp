@@ -158,6 +158,8 @@ int r2_create(r2_ctx** out, int device) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void**)&c->d_scene, sizeof(DScene)) != hipSuccess || hipMalloc((void**)&c->d_counter, 64) != hipSuccess) {
     r2_destroy(c);
     return R2_ERR_HIP;
@@ -170,6 +172,7 @@ int r2_destroy(r2_ctx* c) {
   if (!c) return R2_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->d_scene) (void)hipFree(c->d_scene);
   if (c->d_counter) (void)hipFree(c->d_counter);
@@ -177,6 +180,8 @@ int r2_destroy(r2_ctx* c) {
     if (c->pool[i]) (void)hipFree(c->pool[i]);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->side_done) (void)hipEventDestroy(c->side_done);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return R2_OK;

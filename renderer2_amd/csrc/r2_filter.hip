@@ -1048,8 +1048,7 @@ extern "C" int r2_filter_setup(const r2_mesh* m, float voxel_size, r2_filter_gri
   return setup_grid(m, voxel_size, out);
 }
 
-static int filter_common(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p, r2_filter_grid* g, FConst* c,
-                         Binning* B) {
+static int filter_prepare(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p, r2_filter_grid* g, FConst* c) {
   int rc = check_params(ctx, m, p);
   if (rc) return rc;
   rc = setup_grid(m, p->voxel_size, g);
@@ -1057,7 +1056,13 @@ static int filter_common(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* 
   const unsigned long long nvox = (unsigned long long)g->num_voxels[0] * g->num_voxels[1] * g->num_voxels[2];
   if (nvox >= (1ull << 32)) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "filter: more than 2^32 voxels");
   *c = make_consts(m, *g, *p);
-  return bin_triangles(ctx, m, *c, B);
+  return R2_OK;
+}
+
+static int filter_common(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p, r2_filter_grid* g, FConst* c,
+                         Binning* B) {
+  const int rc = filter_prepare(ctx, m, p, g, c);
+  return rc ? rc : bin_triangles(ctx, m, *c, B);
 }
 
 extern "C" int r2_filter_lod(r2_ctx* ctx, const r2_mesh* m, const r2_filter_params* p, void* out_records,
@@ -1067,7 +1072,7 @@ extern "C" int r2_filter_lod(r2_ctx* ctx, const r2_mesh* m, const r2_filter_para
   r2_filter_grid g{};
   FConst c{};
   Binning B;
-  int rc = filter_common(ctx, m, p, &g, &c, &B);
+  int rc = filter_prepare(ctx, m, p, &g, &c);
   if (rc) return rc;
   const unsigned long long nvox = (unsigned long long)c.nx * c.ny * c.nz;
   uint4* rec = nullptr;
@@ -1076,8 +1081,18 @@ extern "C" int r2_filter_lod(r2_ctx* ctx, const r2_mesh* m, const r2_filter_para
   } else {
     R2F_HIP(pooled(ctx, kSlotRecords, 2ull * nvox, &rec), "filter: alloc records");
   }
-  hipLaunchKernelGGL(k_fill_empty, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream, rec, nvox);
+  // the no-hit fill of all records streams on the side stream while the binning
+  // (latency-bound) runs on the main one; the main stream waits for it before
+  // the filter writes the active voxels
+  hipLaunchKernelGGL(k_fill_empty, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->side, rec, nvox);
   R2F_HIP(hipGetLastError(), "filter: k_fill_empty");
+  R2F_HIP(hipEventRecord(ctx->side_done, ctx->side), "filter: record fill");
+  rc = bin_triangles(ctx, m, c, &B);
+  if (rc) {
+    (void)hipStreamSynchronize(ctx->side);
+    return rc;
+  }
+  R2F_HIP(hipStreamWaitEvent(ctx->stream, ctx->side_done, 0), "filter: join fill");
   float kms = 0.f;
   rc = launch_filter(ctx, m, c, B, nullptr, B.n_act, true, rec, &kms);
   if (rc) return rc;

@@ -10,6 +10,10 @@ struct r2_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // second stream for work that overlaps the main one (the filter's record fill),
+  // joined back through side_done
+  hipStream_t side = nullptr;
+  hipEvent_t side_done = nullptr;
   std::string err;
   r2_stats stats{};
   // reusable device scratch for renders
