@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
                                                 const float* __restrict__ nrm, const int32_t* __restrict__ tmat,
                                                 const float* __restrict__ matf, uint4* __restrict__ out,
                                                 int ts_cap) {
-  extern __shared__ float lds[];
+  extern __shared__ __align__(16) float lds[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + (uint32_t)wave));
   if (w >= n_work) return;
@@ -651,17 +651,28 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
     const float P = (float)hits / (float)c.n_samples;
     float dens = -det_logf(1.f - P) / (avg * c.sigma_t);
     const float na = c.alpha / c.sigma_t;
-    // with at most 64 rays, lane h keeps the clip distance of hit h in a register
-    // and the hit-order sum of the exp terms reads them lane by lane (readlane)
+    // with at most 64 rays, lane h keeps the clip distance of hit h in a register,
+    // computes its exp term and stores it to the wave's LDS area (free after the
+    // ray sums; 16-byte aligned); the hit-order sum reads four terms per load
     const bool lane_sums = c.clip && c.n_samples <= 64;
     const float ts_l = (lane_sums && (uint32_t)lane < hits) ? TS[lane] : 0.f;
     for (int it = 0; it < c.iterations; ++it) {
       float Pv;
       if (lane_sums) {
-        const float term = (uint32_t)lane < hits ? det_expf(-dens * c.sigma_t * ts_l) : 0.f;
-        const int tb = __float_as_int(term);
+        V[lane] = (uint32_t)lane < hits ? det_expf(-dens * c.sigma_t * ts_l) : 0.f;
+        wave_sync();
+        const float4* V4 = reinterpret_cast<const float4*>(V);
         float e = 0.f;
-        for (uint32_t h = 0; h < hits; ++h) e += __int_as_float(__builtin_amdgcn_readlane(tb, (int)h));
+        uint32_t h = 0;
+        for (; h + 4u <= hits; h += 4u) {
+          const float4 q = V4[h >> 2];
+          e += q.x;
+          e += q.y;
+          e += q.z;
+          e += q.w;
+        }
+        for (; h < hits; ++h) e += V[h];
+        wave_sync();
         Pv = 1.f - e / (float)hits;
       } else if (c.clip) {
         for (uint32_t h = (uint32_t)lane; h < hits; h += 64) V[h] = det_expf(-dens * c.sigma_t * TS[h]);
