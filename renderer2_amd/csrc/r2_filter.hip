@@ -46,6 +46,7 @@ struct r2_mesh {
 
 namespace {
 
+constexpr int kRow = 68;  // LDS row of one value over a batch's hits: 16-byte aligned, rows 4 banks apart
 constexpr int kWaves = 4;  // waves per 256-thread block
 constexpr int kVals = 17;  // per-ray sums: S6, diffuse3, specular3, normal3, ior, distance
 constexpr uint32_t kTag = 0xF17Eu;
@@ -488,8 +489,10 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + (uint32_t)wave));
   if (w >= n_work) return;
-  float* V = lds + wave * (kVals * 64 + ts_cap);  // [64][kVals] per-ray values (later: exp terms)
-  float* TS = V + kVals * 64;                      // [ts_cap] clip distances of the hits, in ray order
+  // [kVals][kRow] per-hit values of a ray batch, value-major, hits compacted in
+  // ray order (later: the exp terms of the density iteration)
+  float* V = lds + wave * (kVals * kRow + ts_cap);
+  float* TS = V + kVals * kRow;  // [ts_cap] clip distances of the hits, in ray order
 
   uint32_t vid, beg = 0, nc = 0;
   size_t orec;
@@ -593,49 +596,29 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
       val[15] = mf[7];
       val[16] = cur_tmax;
     }
-#pragma unroll
-    for (int q = 0; q < kVals; ++q) V[lane * kVals + q] = val[q];
+    // the hit rays' values, compacted by hit rank (ray order) into rows of one
+    // value each; lane q < kVals then sums row q in order, four hits per load
     const unsigned long long hb = __ballot(hit);
-    if (hit) TS[hits + (uint32_t)__popcll(hb & ((1ull << lane) - 1ull))] = cur_tmax;
+    const uint32_t rank = (uint32_t)__popcll(hb & ((1ull << lane) - 1ull));
+    if (hit) {
+#pragma unroll
+      for (int q = 0; q < kVals; ++q) V[q * kRow + rank] = val[q];
+      TS[hits + rank] = cur_tmax;
+    }
     wave_sync();
-    // ray-order sums over the hit rays of this batch: the wave-uniform hit mask
-    // drives the loop (no per-ray flag read), four hits per step so that their
-    // LDS reads are in flight together; the additions keep ray order
+    const uint32_t nh = (uint32_t)__popcll(hb);
     if (lane < kVals) {
-      unsigned long long m = hb;
-      while (m) {
-        const int ra = __ffsll(m) - 1;
-        m &= m - 1;
-        if (!m) {
-          acc += V[ra * kVals + lane];
-          break;
-        }
-        const int rb = __ffsll(m) - 1;
-        m &= m - 1;
-        if (!m) {
-          const float va = V[ra * kVals + lane], vb = V[rb * kVals + lane];
-          acc += va;
-          acc += vb;
-          break;
-        }
-        const int rc = __ffsll(m) - 1;
-        m &= m - 1;
-        if (!m) {
-          const float va = V[ra * kVals + lane], vb = V[rb * kVals + lane], vc = V[rc * kVals + lane];
-          acc += va;
-          acc += vb;
-          acc += vc;
-          break;
-        }
-        const int rd = __ffsll(m) - 1;
-        m &= m - 1;
-        const float va = V[ra * kVals + lane], vb = V[rb * kVals + lane], vc = V[rc * kVals + lane],
-                    vd = V[rd * kVals + lane];
-        acc += va;
-        acc += vb;
-        acc += vc;
-        acc += vd;
+      const float* row = V + lane * kRow;
+      const float4* row4 = reinterpret_cast<const float4*>(row);
+      uint32_t h = 0;
+      for (; h + 4u <= nh; h += 4u) {
+        const float4 v = row4[h >> 2];
+        acc += v.x;
+        acc += v.y;
+        acc += v.z;
+        acc += v.w;
       }
+      for (; h < nh; ++h) acc += row[h];
     }
     hits += (uint32_t)__popcll(hb);
     wave_sync();
@@ -930,7 +913,7 @@ int launch_filter(r2_ctx* ctx, const r2_mesh* m, const FConst& c, const Binning&
     return R2_OK;
   }
   const int ts_cap = ((c.n_samples + 63) / 64) * 64;
-  const size_t lds = sizeof(float) * kWaves * (kVals * 64 + ts_cap);
+  const size_t lds = sizeof(float) * kWaves * (kVals * kRow + ts_cap);
   const unsigned blocks = (n_work + kWaves - 1) / kWaves;
   R2F_HIP(hipEventRecord(ctx->ev0, ctx->stream), "filter: event");
   hipLaunchKernelGGL(k_filter, dim3(blocks), dim3(256), lds, ctx->stream, c, work, n_work, direct ? 1 : 0, B.act,
