@@ -158,10 +158,18 @@ __device__ __forceinline__ Rec rec_lerp(float t, const Rec& a, const Rec& b) {
   return o;
 }
 
+// index of entry (x, y, z) of a row-major 3-D table with rows of sx and planes of
+// sx * sy entries, for the leaf, brick and mip tables.  The grid build allows
+// fewer than 1024 bricks (and so leaves) per axis, so every operand stays below
+// 2^20 and every product below 2^30: the full-rate 24-bit multiply is exact.
+__device__ __forceinline__ uint32_t idx3(uint32_t x, uint32_t y, uint32_t z, uint32_t sx, uint32_t sy) {
+  return __umul24(__umul24(z, sy) + y, sx) + x;
+}
+
 // leaf index of voxel (x, y, z), or -1 outside the grid or in an empty leaf
 __device__ __forceinline__ int32_t leaf_of(const DGrid& g, int x, int y, int z) {
   if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny && (unsigned)z < (unsigned)g.nz)
-    return g.leaf_table[((size_t)(z >> 3) * g.nly + (y >> 3)) * g.nlx + (x >> 3)];
+    return g.leaf_table[idx3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3, (uint32_t)g.nlx, (uint32_t)g.nly)];
   return -1;
 }
 
@@ -171,7 +179,7 @@ __device__ __forceinline__ Rec get_value(const DGrid& g, int x, int y, int z, Cn
   count<kCount>(cnt, kCntVox);
   int32_t leaf = -1;
   if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny && (unsigned)z < (unsigned)g.nz)
-    leaf = g.leaf_table[((size_t)(z >> 3) * g.nly + (y >> 3)) * g.nlx + (x >> 3)];
+    leaf = g.leaf_table[idx3((uint32_t)x >> 3, (uint32_t)y >> 3, (uint32_t)z >> 3, (uint32_t)g.nlx, (uint32_t)g.nly)];
   if (grid_packed(g)) {
     // background: the packed form of the FP32 background record (ior 1, normal
     // decoded from the zero code, normalize(-1,-1,-1), octahedral code 0xABAB)
@@ -307,14 +315,21 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
 }
 
 // CDeviceBrickGrid::lookupMajorant (device_grid_brick_impl.hpp:29-33)
+// texel of brick (bx, by, bz) at mip level `mip` of the range texture.  Brick
+// counts are multiples of 2^3 (host_grid_brick.cpp:64), so the point-sampled texel
+// at the normalised coordinate (b + 0.5) / n of a level of extent n >> mip
+// (tex3DLod, device_grid_brick_impl.hpp:29-33) is b >> mip.
+__device__ __forceinline__ uint32_t mip_texel(const DGrid& g, int bx, int by, int bz, int mip) {
+  return idx3((uint32_t)(bx >> mip), (uint32_t)(by >> mip), (uint32_t)(bz >> mip), (uint32_t)(g.nb[0] >> mip),
+              (uint32_t)(g.nb[1] >> mip));
+}
 template <bool kCount>
 __device__ __forceinline__ float lookup_majorant(const DGrid& g, v3 ipos, int mip, Cnt& cnt) {
   count<kCount>(cnt, kCntMaj);
   const int bx = ((int)floorf(ipos.x)) >> 3, by = ((int)floorf(ipos.y)) >> 3, bz = ((int)floorf(ipos.z)) >> 3;
   if ((unsigned)bx >= (unsigned)g.nb[0] || (unsigned)by >= (unsigned)g.nb[1] || (unsigned)bz >= (unsigned)g.nb[2])
     return 0.f;
-  const int sx = g.nb[0] >> mip, sy = g.nb[1] >> mip;
-  return g.maj[mip][((size_t)(bz >> mip) * sy + (by >> mip)) * sx + (bx >> mip)];
+  return g.maj[mip][mip_texel(g, bx, by, bz, mip)];
 }
 // CDeviceBrickGrid::lookupDensity (device_grid_brick_impl.hpp:9-27)
 template <bool kCount>
@@ -325,7 +340,7 @@ __device__ __forceinline__ float lookup_density(const DGrid& g, v3 ipos, v3 u, C
   const int bx = ix >> 3, by = iy >> 3, bz = iz >> 3;
   if ((unsigned)bx >= (unsigned)g.nb[0] || (unsigned)by >= (unsigned)g.nb[1] || (unsigned)bz >= (unsigned)g.nb[2])
     return 0.f;
-  const uint2 br = g.bricks[((size_t)bz * g.nb[1] + by) * g.nb[0] + bx];
+  const uint2 br = g.bricks[idx3((uint32_t)bx, (uint32_t)by, (uint32_t)bz, (uint32_t)g.nb[0], (uint32_t)g.nb[1])];
   const float lo = range_lo(br.x), hi = range_hi(br.x);
   uint8_t v = 0;
   if (br.y != 0xffffffffu) v = g.atlas[(size_t)br.y * 512 + (((iz & 7) << 6) | ((iy & 7) << 3) | (ix & 7))];
