@@ -252,7 +252,8 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
             p.s_cur = probe_smp[item];
             p.s_end = p.s_cur + 1;
           } else {
-            const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
+            // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
+            const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
             int x, y;
             item_pixel(q, s.W, s.H, x, y);
             p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
@@ -316,7 +317,8 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
             p.item = item;
             p.ax = p.ay = p.az = 0.f;
             p.dax = p.day = p.daz = 0.0;
-            const uint32_t c = (uint32_t)(item / npix), q = (uint32_t)(item % npix);
+            // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
+            const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
             int x, y;
             item_pixel(q, s.W, s.H, x, y);
             p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
