@@ -272,8 +272,8 @@ int r2_postprocess(r2_ctx* ctx, const float* hdr, int32_t w, int32_t h, float ga
  * comments allowed), filters the FILTER-masked mesh to its LODs when
  * Filtering.Active (grid files <out_dir>/filtering/<mesh>/filtered_mesh_<vs>.r2grid and
  * filtering/volume_description.json), renders the RENDER-masked media under the env map
- * and writes <out_dir>/output.png (tonemapped, flipped like the reference) and
- * output.pfm (linear).  err receives a message on failure. */
+ * and writes <out_dir>/output.jpg (quality 100) and output.png (tonemapped, flipped
+ * like the reference, raytracer/pipeline.cu:84-85) and output.pfm (linear).  err receives a message on failure. */
 typedef struct {
   int32_t lods_filtered;
   double filter_ms;
@@ -283,6 +283,19 @@ typedef struct {
 } r2_pipeline_report;
 int r2_pipeline_run(const char* config_path, const char* out_dir, int32_t device, r2_pipeline_report* report,
                     char* err, int32_t err_len);
+
+/* Image file of a post-processed frame: CImageWriter::writeToFile
+ * (visualisation/src/image_writer.cpp:13-21).  rgb holds w*h 8-bit RGB pixels, rows
+ * bottom to top as r2_postprocess writes them; the file is flipped to top-down, as
+ * stbi_flip_vertically_on_write(true) makes the reference's.  R2_IMAGE_PNG: an
+ * uncompressed PNG (stbi_write_png_compression_level = 0).  R2_IMAGE_JPG: a baseline
+ * JFIF JPEG at quality 1..100 (the pipeline writes 100), chroma at full resolution above
+ * 90 and 2x2-averaged at 90 and below.  R2_ERR_INVALID for a bad size, format or
+ * quality; R2_ERR_IO when the file cannot be written.  Host only. */
+#define R2_IMAGE_PNG 0
+#define R2_IMAGE_JPG 1
+int r2_image_write(const char* path, int32_t w, int32_t h, const uint8_t* rgb, int32_t format, int32_t quality,
+                   char* err, int32_t err_len);
 
 /* Write the brick majorant grid of a grid as the reference's .brickgrid file: a
  * CHostBrickGrid in cereal 1.3.2's portable binary archive (rt::write_grid,

@@ -14,13 +14,30 @@ import numpy as np
 from . import _capi as capi
 from ._capi import R2Error
 
-__all__ = ["Context", "Grid", "Env", "Mesh", "Server", "R2Error", "capi", "filter_params", "nvdb_write",
+__all__ = ["Context", "Grid", "Env", "Mesh", "Server", "R2Error", "capi", "filter_params", "image_write", "nvdb_write",
            "nvdb_read"]
 
 
 def filter_params(voxel_size: float, samples: int = 64, iterations: int = 8, sigma_t: float = 1.0,
                   alpha: float = 1.0, clip_rays: bool = True, seed: int = 0, lod: int = 0) -> "capi.FilterParams":
     return capi.FilterParams(voxel_size, samples, iterations, sigma_t, alpha, int(bool(clip_rays)), seed, lod)
+
+
+def image_write(path, rgb: np.ndarray, fmt: str = "png", quality: int = 100) -> None:
+    """Write an (h, w, 3) u8 frame, rows bottom to top as r2_postprocess gives them, as
+    the reference's output.png (fmt "png") or output.jpg (fmt "jpg", quality 1..100);
+    the file is flipped to top-down.  Host only."""
+    img = np.ascontiguousarray(rgb, np.uint8)
+    if img.ndim != 3 or img.shape[2] != 3:
+        raise ValueError("rgb must be (h, w, 3) uint8")
+    f = {"png": capi.R2_IMAGE_PNG, "jpg": capi.R2_IMAGE_JPG}.get(fmt)
+    if f is None:
+        raise ValueError("fmt must be 'png' or 'jpg'")
+    err = C.create_string_buffer(512)
+    rc = capi.lib().r2_image_write(str(path).encode(), img.shape[1], img.shape[0], img.ctypes.data, f, int(quality),
+                                   err, 512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
 
 
 def nvdb_write(path, records: np.ndarray, map_mat=None, map_vec=None) -> None:

@@ -26,6 +26,8 @@ R2_ERR = {
 }
 R2_PREC_FP32 = 0
 R2_PREC_PACKED = 1
+R2_IMAGE_PNG = 0
+R2_IMAGE_JPG = 1
 R2_FLAG_BRICK_GRID = 1
 R2_FLAG_FP64_ACCUM = 2
 
@@ -206,6 +208,8 @@ SIGNATURES = [
                                  C.c_int32]),
     ("r2_pipeline_run", C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(PipelineReport), C.c_char_p,
                                   C.c_int32]),
+    ("r2_image_write", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_char_p,
+                                 C.c_int32]),
     ("r2_config_summary", C.c_int, [C.c_char_p, C.POINTER(ConfigSummary), C.c_char_p, C.c_int32]),
     ("r2_synth_volume", C.c_int, [C.c_int32, C.c_int32, C.c_uint32, C.c_float, C.c_int32, C.c_void_p]),
     ("r2_synth_env", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),

@@ -31,6 +31,7 @@ SOURCES = [
     "r2_post.hip",
     "r2_frontend.cpp",
     "r2_nvdb.cpp",
+    "r2_image.cpp",
 ]
 # the command-line program (links the library)
 MAIN = "r2_main.cpp"

@@ -9,8 +9,8 @@
 //     LOD in ./filtering/<mesh name>/filtered_mesh_<voxel size> and the volume
 //     description JSON (raytracer/src/filtering/volume_description_manager.cpp:50-144).
 //   * Raytracer::renderFrame + post-process (raytracer/raytracer.cu:161-206) and
-//     the vertically flipped, uncompressed PNG of CImageWriter
-//     (raytracer/visualisation/image_writer.cpp:7-21), plus a linear PFM.
+//     CImageWriter's vertically flipped quality-100 JPEG and uncompressed PNG
+//     (visualisation/src/image_writer.cpp:8-21, r2_image.cpp), plus a linear PFM.
 // Assets: Wavefront OBJ + MTL for meshes (Kd, Ks, Ns -> roughness
 // (2/(Ns+2))^0.5 and Ni -> ior with 1 mapped to 1.5, raytracer/src/material/material.cu:90-148;
 // missing normals become flat face normals as with assimp's GenNormals),
@@ -586,80 +586,6 @@ void write_brickgrid(r2_ctx* ctx, const r2_grid* g, const fs::path& path) {
   if (!f) throw std::runtime_error("brickgrid: write failed: " + path.string());
 }
 
-// ----------------------------------------------------------------- images ---
-uint32_t crc32(const uint8_t* d, size_t n, uint32_t c = 0xFFFFFFFFu) {
-  struct Table {
-    uint32_t v[256];
-    Table() {
-      for (uint32_t i = 0; i < 256; ++i) {
-        uint32_t x = i;
-        for (int k = 0; k < 8; ++k) x = (x & 1) ? 0xEDB88320u ^ (x >> 1) : x >> 1;
-        v[i] = x;
-      }
-    }
-  };
-  static const Table table;  // initialised once, thread-safe
-  for (size_t i = 0; i < n; ++i) c = table.v[(c ^ d[i]) & 0xFF] ^ (c >> 8);
-  return c;
-}
-
-void put32(std::vector<uint8_t>& v, uint32_t x) {
-  v.push_back((uint8_t)(x >> 24));
-  v.push_back((uint8_t)(x >> 16));
-  v.push_back((uint8_t)(x >> 8));
-  v.push_back((uint8_t)x);
-}
-
-void chunk(std::ofstream& f, const char* type, const std::vector<uint8_t>& data) {
-  std::vector<uint8_t> b;
-  put32(b, (uint32_t)data.size());
-  b.insert(b.end(), type, type + 4);
-  b.insert(b.end(), data.begin(), data.end());
-  const uint32_t c = crc32(b.data() + 4, b.size() - 4) ^ 0xFFFFFFFFu;
-  put32(b, c);
-  f.write((const char*)b.data(), (std::streamsize)b.size());
-}
-
-// RGB8 PNG with stored (uncompressed) deflate blocks, rows written bottom-up
-void write_png(const fs::path& p, int w, int h, const std::vector<uint8_t>& rgb) {
-  std::ofstream f(p, std::ios::binary);
-  if (!f) throw std::runtime_error("cannot write " + p.string());
-  const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
-  f.write((const char*)sig, 8);
-  std::vector<uint8_t> ihdr;
-  put32(ihdr, (uint32_t)w);
-  put32(ihdr, (uint32_t)h);
-  ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});
-  chunk(f, "IHDR", ihdr);
-  std::vector<uint8_t> raw;
-  raw.reserve((size_t)h * (3 * w + 1));
-  for (int y = h - 1; y >= 0; --y) {
-    raw.push_back(0);
-    raw.insert(raw.end(), rgb.begin() + (size_t)y * w * 3, rgb.begin() + (size_t)(y + 1) * w * 3);
-  }
-  std::vector<uint8_t> z = {0x78, 0x01};
-  uint32_t a = 1, b = 0;
-  for (uint8_t v : raw) {
-    a = (a + v) % 65521u;
-    b = (b + a) % 65521u;
-  }
-  size_t off = 0;
-  do {
-    const size_t n = std::min<size_t>(65535, raw.size() - off);
-    const bool last = off + n == raw.size();
-    z.push_back(last ? 1 : 0);
-    z.push_back((uint8_t)(n & 0xFF));
-    z.push_back((uint8_t)(n >> 8));
-    z.push_back((uint8_t)(~n & 0xFF));
-    z.push_back((uint8_t)((~n >> 8) & 0xFF));
-    z.insert(z.end(), raw.begin() + off, raw.begin() + off + n);
-    off += n;
-  } while (off < raw.size());
-  put32(z, (b << 16) | a);
-  chunk(f, "IDAT", z);
-  chunk(f, "IEND", {});
-}
-
 // linear HDR frame as PFM (rows bottom to top, the frame's own order)
 void write_pfm(const fs::path& p, int w, int h, const std::vector<float>& rgb) {
   std::ofstream f(p, std::ios::binary);
@@ -907,7 +833,8 @@ void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeli
   std::vector<uint8_t> bytes(hdr.size());
   check(ctx, r2_postprocess(ctx, hdr.data(), p.width, p.height, c.gamma, nullptr, bytes.data(), 0), "postprocess");
   fs::create_directories(out_dir);
-  write_png(out_dir / "output.png", p.width, p.height, bytes);
+  r2host::write_jpg((out_dir / "output.jpg").string(), p.width, p.height, bytes.data(), 100);
+  r2host::write_png((out_dir / "output.png").string(), p.width, p.height, bytes.data());
   write_pfm(out_dir / "output.pfm", p.width, p.height, hdr);
   r2_env_free(ctx, env);
   for (r2_grid* g : grids) r2_grid_free(ctx, g);

@@ -5,7 +5,8 @@
 * r2_pipeline_run with a config that filters a mesh (OBJ + MTL written here) to
   two LODs and renders the finer LOD: the grid files hold exactly the records of
   r2_filter_lod for the same mesh and parameters, the volume description lists
-  both LODs, and output.png is the post-processed output.pfm, flipped.
+  both LODs, and output.png is the post-processed output.pfm, flipped (output.jpg
+  the same image at quality 100).
 """
 import ctypes as C
 import json
@@ -161,6 +162,10 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
     assert np.isfinite(hdr).all() and hdr.mean() > 0
     _, by = O.postprocess(np.ascontiguousarray(hdr), 2.0)
     assert np.array_equal(png, by[::-1])
+    # output.jpg: the same bytes as a quality-100 JPEG (decoded by libjpeg through PIL)
+    from PIL import Image
+    jpg = np.asarray(Image.open(tmp_path / "output.jpg").convert("RGB")).astype(np.int64)
+    assert jpg.shape == png.shape and np.abs(jpg - png.astype(np.int64)).max() <= 3
     # the same scene rendered from the .nvdb files alone (the reference's own filter
     # output format): the .r2grid files removed, filtering off -> the same image
     for f in (tmp_path / "filtering" / "blob").glob("*.r2grid"):
