@@ -143,6 +143,9 @@ def lib() -> C.CDLL:
             g("r2o_move_to_voxel_border").argtypes = [vp, vp, vp]
             g("r2o_intersect").argtypes = [vp, vp, vp]
             g("r2o_intersect").restype = C.c_int
+            g("r2o_direct_mean").argtypes = [vp, C.c_int, vp, vp, vp, R, vp, C.c_int, C.c_uint32, C.c_uint32,
+                                             C.c_int64, vp, vp]
+            g("r2o_phase_p_mean").argtypes = [vp, vp, R, vp, vp, C.c_int64, C.c_uint32, C.c_uint32, C.c_int, vp]
         L.r2o_filter_setup.argtypes = [C.POINTER(MeshDesc), C.c_float, C.POINTER(FilterGrid)]
         L.r2o_filter_new.argtypes = [C.POINTER(MeshDesc)]
         L.r2o_filter_new.restype = vp
@@ -331,6 +334,14 @@ class Scene:
         ints = np.zeros(h + 1, self.R)
         self._fn("r2o_env_tables")(self.h, _p(marg), _p(rows), _p(ints))
         return marg, rows, ints
+
+    def direct_mean(self, inst, pos, S6, n, ior, wo, use_dda: bool, key: int, c0: int, count: int):
+        """(mean, variance) per channel of direct() at a medium interaction."""
+        a = [np.ascontiguousarray(x, self.R) for x in (pos, S6, n, wo)]
+        mean, var = np.zeros(3), np.zeros(3)
+        self._fn("r2o_direct_mean")(self.h, inst, _p(a[0]), _p(a[1]), _p(a[2]), ior, _p(a[3]), int(use_dda), key, c0,
+                                    count, _p(mean), _p(var))
+        return mean, var
 
     def tr_segment(self, ray7, use_dda: bool, key: int, c0: int) -> float:
         r = np.ascontiguousarray(ray7, self.R)

@@ -1515,3 +1515,46 @@ int SFX(r2o_intersect)(const scene_t* s, const R* ray7, R* out5) {
   out5[0] = h.t; out5[1] = h.pos.x; out5[2] = h.pos.y; out5[3] = h.pos.z; out5[4] = (R)h.medium;
   return h.hit;
 }
+/* direct() at a medium interaction of instance `inst` at world position pos with
+ * voxel data (S6, n, ior), seen from wo: mean and variance per channel over `count`
+ * draws with counters (key, c0 + i), accumulated in double */
+void SFX(r2o_direct_mean)(const scene_t* s, int inst, const R* pos, const R* S6, const R* n, R ior, const R* wo,
+                          int use_dda, uint32_t key, uint32_t c0, int64_t count, double* mean3, double* var3) {
+  mhit_t mi;
+  mi.valid = 1;
+  mi.inst = inst;
+  mi.pos = V(pos[0], pos[1], pos[2]);
+  mi.normal = V(n[0], n[1], n[2]);
+  for (int k = 0; k < 6; ++k) mi.S[k] = S6[k];
+  mi.ior = ior;
+  double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+  for (int64_t i = 0; i < count; ++i) {
+    r2o_rng rng;
+    r2o_rng_init(&rng, key, 0, c0 + (uint32_t)i, 0, 0);
+    const v3 L = direct(s, &mi, V(wo[0], wo[1], wo[2]), &rng, use_dda, NULL);
+    const double l[3] = {(double)L.x, (double)L.y, (double)L.z};
+    for (int k = 0; k < 3; ++k) {
+      sum[k] += l[k];
+      sq[k] += l[k] * l[k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    mean3[k] = sum[k] / (double)count;
+    var3[k] = sq[k] / (double)count - mean3[k] * mean3[k];
+  }
+}
+/* phase_p averaged over the VNDF draws of counters (key, c0 .. c0 + n_c - 1), for each
+ * of n_wi directions */
+void SFX(r2o_phase_p_mean)(const R* S6, const R* n, R ior, const R* wo, const R* wi, int64_t n_wi, uint32_t key,
+                           uint32_t c0, int n_c, double* out) {
+  for (int64_t j = 0; j < n_wi; ++j) {
+    double acc = 0;
+    for (int c = 0; c < n_c; ++c) {
+      r2o_rng rng;
+      r2o_rng_init(&rng, key, 0, c0 + (uint32_t)c, 0, 0);
+      acc += (double)phase_p(S6, V(n[0], n[1], n[2]), ior, V(wo[0], wo[1], wo[2]),
+                             V(wi[3 * j], wi[3 * j + 1], wi[3 * j + 2]), &rng);
+    }
+    out[j] = acc / n_c;
+  }
+}
