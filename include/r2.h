@@ -29,6 +29,7 @@ extern "C" {
 #define R2_ERR_IO (-5)          /* file missing or malformed */
 #define R2_ERR_CANCELLED (-6)   /* request cancelled / abort callback returned true */
 #define R2_ERR_BUSY (-7)        /* server queue full */
+#define R2_ERR_DEADLINE (-8)    /* server request not finished by its deadline (dropped) */
 
 #define R2_VERSION 1
 
@@ -235,6 +236,11 @@ typedef struct {
 typedef struct {
   r2_render_params params;     /* instances / env must stay valid until the result is taken */
   float* out_rgb;              /* host buffer W*H*3, filled when the request completes */
+  double deadline_ms;          /* 0: none; else the time after submission by which the frame is
+                                  wanted.  Queued requests are admitted earliest deadline first
+                                  (those without one last, in arrival order); a request still
+                                  unfinished at its deadline is dropped and reports
+                                  R2_ERR_DEADLINE (checked at least once per slice). */
 } r2_request;
 
 typedef struct {
@@ -242,6 +248,7 @@ typedef struct {
   double paths_per_sec;
   double p50_ms, p99_ms;
   int64_t peak_bytes;
+  int64_t expired;             /* requests dropped at their deadline */
 } r2_server_stats;
 
 int r2_server_start(r2_ctx* ctx, const r2_server_config* cfg, r2_server** out);

@@ -356,10 +356,12 @@ class Server:
         self.ctx, self.h = ctx, h
         self._req = {}
 
-    def submit(self, cfg: dict, grids, instances, env) -> int:
+    def submit(self, cfg: dict, grids, instances, env, deadline_ms: float = 0.0) -> int:
+        """deadline_ms > 0: admitted earliest deadline first, dropped (R2_ERR_DEADLINE)
+        if unfinished that long after submission."""
         p, keep = Context.render_params(cfg, grids, instances, env)
         out = np.zeros((p.height, p.width, 3), np.float32)
-        r = capi.Request(p, out.ctypes.data)
+        r = capi.Request(p, out.ctypes.data, float(deadline_ms))
         rid = capi.lib().r2_server_submit(self.h, C.byref(r))
         if rid < 0:
             raise capi.R2Error(int(rid), "r2_server_submit")

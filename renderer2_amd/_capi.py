@@ -23,6 +23,7 @@ R2_ERR = {
     -5: "R2_ERR_IO",
     -6: "R2_ERR_CANCELLED",
     -7: "R2_ERR_BUSY",
+    -8: "R2_ERR_DEADLINE",
 }
 R2_PREC_FP32 = 0
 R2_PREC_PACKED = 1
@@ -153,14 +154,14 @@ class ServerConfig(C.Structure):
 
 
 class Request(C.Structure):
-    _fields_ = [("params", RenderParams), ("out_rgb", C.c_void_p)]
+    _fields_ = [("params", RenderParams), ("out_rgb", C.c_void_p), ("deadline_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("cancelled", C.c_int64),
         ("paths_per_sec", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
-        ("peak_bytes", C.c_int64),
+        ("peak_bytes", C.c_int64), ("expired", C.c_int64),
     ]
 
 

@@ -3,9 +3,12 @@
 // The reference has no server; its nearest relative is the interactive preview
 // loop (raytracer/pipeline.cu:41-78) that renders one frame at a time.  Here a
 // host scheduler thread owns a HIP stream and
-//   * admits queued requests in arrival order while the sum of their device
-//     memory (scene copy, chunk sums, frame) fits the HBM budget and fewer than
-//     max_inflight are resident;
+//   * admits queued requests earliest deadline first (requests without a
+//     deadline after those with one, in arrival order) while the sum of their
+//     device memory (scene copy, chunk sums, frame) fits the HBM budget and
+//     fewer than max_inflight are resident;
+//   * drops a request, queued or resident, that is still unfinished at its
+//     deadline (checked at least once per slice; R2_ERR_DEADLINE);
 //   * runs slices: one batched persistent launch (k_paths_batch) whose work is
 //     an equal share of paths from every resident request, so requests that
 //     arrive while others run join at the next slice and their work overlaps
@@ -34,7 +37,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-enum ReqState { RQ_QUEUED, RQ_ACTIVE, RQ_DONE, RQ_CANCELLED, RQ_FAILED };
+enum ReqState { RQ_QUEUED, RQ_ACTIVE, RQ_DONE, RQ_CANCELLED, RQ_FAILED, RQ_EXPIRED };
 
 struct SrvReq {
   int64_t id = 0;
@@ -54,8 +57,10 @@ struct SrvReq {
   float* d_frame = nullptr;
   ReqState state = RQ_QUEUED;
   bool cancel = false;
+  bool expired = false;                      // dropped at its deadline (with cancel set)
+  bool has_deadline = false;
   int rc = R2_OK;
-  Clock::time_point t_submit, t_done;
+  Clock::time_point t_submit, t_done, deadline;
 };
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -75,7 +80,7 @@ struct r2_server {
   bool stopping = false;
   int64_t next_id = 0;
   int64_t inflight_bytes = 0, peak_bytes = 0;
-  int64_t n_submitted = 0, n_completed = 0, n_rejected = 0, n_cancelled = 0;
+  int64_t n_submitted = 0, n_completed = 0, n_rejected = 0, n_cancelled = 0, n_expired = 0;
   std::vector<double> latencies;
   int64_t paths_done = 0;
   bool have_first = false;
@@ -104,30 +109,74 @@ void finish_locked(r2_server* s, SrvReq& r, ReqState st, int rc) {
     s->t_last = r.t_done;
   } else if (st == RQ_CANCELLED) {
     ++s->n_cancelled;
+  } else if (st == RQ_EXPIRED) {
+    ++s->n_expired;
   }
   s->cv_done.notify_all();
 }
 
-// admit queued requests in order while memory and slots allow (lock held)
-void admit_locked(r2_server* s) {
-  while (!s->queue.empty() && (int)s->active.size() < s->cfg.max_inflight) {
-    SrvReq& r = *s->reqs[s->queue.front()];
+// a request that no slice in flight uses any more and that was cancelled or has
+// expired: release it with the matching outcome (lock held)
+void drop_locked(r2_server* s, SrvReq& r) {
+  if (r.expired) finish_locked(s, r, RQ_EXPIRED, R2_ERR_DEADLINE);
+  else finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
+}
+
+// mark requests whose deadline has passed; queued requests that expired or were
+// cancelled leave the queue at once, resident ones are dropped once no slice in
+// flight still uses them (lock held)
+void expire_locked(r2_server* s) {
+  const Clock::time_point now = Clock::now();
+  for (auto it = s->queue.begin(); it != s->queue.end();) {
+    SrvReq& r = *s->reqs[*it];
+    if (r.has_deadline && !r.cancel && now >= r.deadline) r.cancel = r.expired = true;
     if (r.cancel) {
-      s->queue.pop_front();
-      finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
+      it = s->queue.erase(it);
+      drop_locked(s, r);
+    } else {
+      ++it;
+    }
+  }
+  for (int64_t id : s->active) {
+    SrvReq& r = *s->reqs[id];
+    if (r.has_deadline && !r.cancel && now >= r.deadline) r.cancel = r.expired = true;
+  }
+}
+
+// the queued request to admit next: earliest deadline, then arrival order
+std::deque<int64_t>::iterator next_queued(r2_server* s) {
+  auto best = s->queue.begin();
+  for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+    const SrvReq& r = *s->reqs[*it];
+    const SrvReq& b = *s->reqs[*best];
+    if (r.has_deadline && (!b.has_deadline || r.deadline < b.deadline)) best = it;
+  }
+  return best;
+}
+
+// admit queued requests, earliest deadline first, while memory and slots allow
+// (lock held); the next request waits for memory rather than being overtaken
+void admit_locked(r2_server* s) {
+  expire_locked(s);
+  while (!s->queue.empty() && (int)s->active.size() < s->cfg.max_inflight) {
+    const auto qi = next_queued(s);
+    SrvReq& r = *s->reqs[*qi];
+    if (r.cancel) {
+      s->queue.erase(qi);
+      drop_locked(s, r);
       continue;
     }
     if (s->inflight_bytes + (int64_t)r.bytes > s->cfg.hbm_budget_bytes) break;
     void* mem = nullptr;
     if (hipMalloc(&mem, r.bytes) != hipSuccess) {
       if (s->active.empty()) {  // nothing to wait for: fail the request
-        s->queue.pop_front();
+        s->queue.erase(qi);
         finish_locked(s, r, RQ_FAILED, R2_ERR_OOM);
         continue;
       }
       break;
     }
-    s->queue.pop_front();
+    s->queue.erase(qi);
     r.d_mem = mem;
     r.d_scene = (r2::DScene*)mem;
     r.d_partial = (char*)mem + align256(sizeof(r2::DScene));
@@ -178,10 +227,10 @@ void scheduler(r2_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     admit_locked(s);
-    // drop cancelled requests that no slice in flight still uses
+    // drop cancelled or expired requests that no slice in flight still uses
     for (int64_t id : std::vector<int64_t>(s->active)) {
       SrvReq& r = *s->reqs[id];
-      if (r.cancel && r.inflight == 0) finish_locked(s, r, RQ_CANCELLED, R2_ERR_CANCELLED);
+      if (r.cancel && r.inflight == 0) drop_locked(s, r);
     }
     // plan and launch a slice while fewer than two are in flight: an equal share
     // of paths per resident request with items left, in whole 64-item tiles
@@ -289,6 +338,7 @@ extern "C" int r2_server_start(r2_ctx* ctx, const r2_server_config* cfg, r2_serv
 
 extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
   if (!s || !req || !req->out_rgb) return R2_ERR_INVALID;
+  if (!(req->deadline_ms >= 0.0) || !std::isfinite(req->deadline_ms)) return R2_ERR_INVALID;
   auto r = std::make_unique<SrvReq>();
   r->params = req->params;
   if (r->params.n_instances < 0 || r->params.n_instances > r2::kMaxInstances ||
@@ -330,6 +380,11 @@ extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
   const int64_t id = s->next_id++;
   r->id = id;
   r->t_submit = Clock::now();
+  if (req->deadline_ms > 0.0) {
+    r->has_deadline = true;
+    r->deadline = r->t_submit + std::chrono::duration_cast<Clock::duration>(
+                                    std::chrono::duration<double, std::milli>(req->deadline_ms));
+  }
   s->queue.push_back(id);
   s->reqs[id] = std::move(r);
   s->cv_work.notify_one();
@@ -383,6 +438,7 @@ extern "C" int r2_server_get_stats(r2_server* s, r2_server_stats* out) {
   out->completed = s->n_completed;
   out->rejected = s->n_rejected;
   out->cancelled = s->n_cancelled;
+  out->expired = s->n_expired;
   out->peak_bytes = s->peak_bytes;
   if (s->have_first && s->n_completed > 0) {
     const double span = std::chrono::duration<double>(s->t_last - s->t_first).count();

@@ -132,3 +132,42 @@ def test_server_concurrent_submitters(ctx, world):
             assert out[k] is not None and np.array_equal(out[k], alone[k])
     finally:
         srv.stop()
+
+
+def test_server_deadlines(ctx, world):
+    """Earliest-deadline-first admission, and requests dropped at their deadline
+    (R2_ERR_DEADLINE) whether still queued or already resident; requests that make
+    their deadline equal their standalone renders."""
+    cfg, inst = _requests(world)[2]  # 128x128
+    g, env = world["grids"], world["env"]
+    cb, cc = dict(cfg, spp=64, seed=11), dict(cfg, spp=64, seed=12)
+    ref_b, ref_c = ctx.render(cb, g, inst, env), ctx.render(cc, g, inst, env)
+    srv = ctx.server(max_inflight=1, slice_paths=1 << 14)
+    try:
+        with pytest.raises(r2.R2Error) as e:
+            srv.submit(dict(cfg, spp=8), g, inst, env, deadline_ms=-1.0)
+        assert e.value.code == -1  # R2_ERR_INVALID
+        a = srv.submit(dict(cfg, spp=256), g, inst, env)  # resident first, no deadline
+        b = srv.submit(cb, g, inst, env, deadline_ms=60000.0)
+        c = srv.submit(cc, g, inst, env, deadline_ms=30000.0)  # earlier deadline: admitted before b
+        d = srv.submit(dict(cfg, spp=8), g, inst, env, deadline_ms=1e-3)  # expires while queued
+        with pytest.raises(r2.R2Error) as e:
+            srv.wait(d)
+        assert e.value.code == -8  # R2_ERR_DEADLINE
+        srv.wait(a)
+        img_c, lat_c = srv.wait(c)
+        img_b, lat_b = srv.wait(b)
+        assert lat_c < lat_b, (lat_c, lat_b)
+        assert np.array_equal(img_b, ref_b) and np.array_equal(img_c, ref_c)
+        # a long request alone on the device, with a deadline far shorter than its run
+        f = srv.submit(dict(cfg, width=512, height=512, spp=1024), g, inst, env, deadline_ms=20.0)
+        with pytest.raises(r2.R2Error) as e:
+            srv.wait(f)
+        assert e.value.code == -8
+        st = srv.stats()
+        assert st["expired"] == 2 and st["completed"] == 3 and st["cancelled"] == 0
+        # the server keeps serving after drops
+        img, _ = srv.wait(srv.submit(cb, g, inst, env, deadline_ms=60000.0))
+        assert np.array_equal(img, ref_b)
+    finally:
+        srv.stop()
