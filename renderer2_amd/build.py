@@ -53,8 +53,17 @@ COMMON_FLAGS = [
 ]
 
 
+# The render kernels let the compiler fuse multiplies and adds into FMAs (nvcc's
+# default for the reference's kernels too): about 5 % more paths per second.  Their
+# results are held to statistical bars against the oracle.  Every other source
+# keeps each operation separately rounded, because the filter, grid build and
+# post-process are bit-exact with the oracle.
+FMA_SOURCES = {"r2_render.hip", "r2_render_packed.hip"}
+
+
 def _compile(src: Path, obj: Path) -> None:
-    cmd = [HIPCC, *COMMON_FLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
+    extra = ["-ffp-contract=fast"] if src.name in FMA_SOURCES else []
+    cmd = [HIPCC, *COMMON_FLAGS, *extra, "-x", "hip", "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr[-6000:]}")
