@@ -110,3 +110,13 @@ def test_create_without_gpu_fails_cleanly():
     h = C.c_void_p()
     rc = capi.lib().r2_create(C.byref(h), 0)
     assert rc < 0 and not h.value
+
+
+def test_fast_math_only_in_render_sources():
+    """The bit-exact kernels (filter, grid build, post-process) must keep separately
+    and correctly rounded arithmetic; only the render kernels take RENDER_FLAGS."""
+    from renderer2_amd import build as b
+    assert b.RENDER_SOURCES == {"r2_render.hip", "r2_render_packed.hip"}
+    assert "-ffp-contract=off" in b.COMMON_FLAGS
+    assert not any("fast" in f or "denormals" in f or "correctly-rounded" in f for f in b.COMMON_FLAGS)
+    assert {"r2_filter.hip", "r2_grid.hip", "r2_post.hip"} <= set(b.SOURCES)
