@@ -667,7 +667,12 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
       } else {
         Pv = 1.f - det_expf(-dens * c.sigma_t * c.tmax_fixed);
       }
-      dens = dens - na * (Pv - P);
+      // each iteration is a function of dens alone: once it returns dens unchanged
+      // (bit for bit), every later one would too, so the remaining ones are skipped
+      // (the result is that of all iterations; dens is wave-uniform)
+      const float nd = dens - na * (Pv - P);
+      if (nd == dens) break;
+      dens = nd;
     }
     const float ih = 1.f / (float)hits;
     const float S6[6] = {sum[0] * ih, sum[1] * ih, sum[2] * ih, sum[3] * ih, sum[4] * ih, sum[5] * ih};
