@@ -120,3 +120,64 @@ def test_fast_math_only_in_render_sources():
     assert "-ffp-contract=off" in b.COMMON_FLAGS
     assert not any("fast" in f or "denormals" in f or "correctly-rounded" in f for f in b.COMMON_FLAGS)
     assert {"r2_filter.hip", "r2_grid.hip", "r2_post.hip"} <= set(b.SOURCES)
+
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "r2.h"
+
+int main(int argc, char** argv) {
+  if (argc < 2) return 2;
+  char path[1024], err[256];
+  if (r2_version() != R2_VERSION) return 3;
+  /* a synthetic 16^3 volume of 32-byte records, written and read back as .nvdb */
+  const int n = 16;
+  unsigned char* rec = malloc((size_t)n * n * n * 32);
+  unsigned char* back = malloc((size_t)n * n * n * 32);
+  if (!rec || !back) return 4;
+  if (r2_synth_volume(1, n, 7u, 1.0f, 2, rec) != R2_OK) return 5;
+  snprintf(path, sizeof path, "%s/v.nvdb", argv[1]);
+  if (r2_nvdb_write(path, n, n, n, rec, NULL, NULL, err, (int32_t)sizeof err) != R2_OK) return 6;
+  r2_nvdb_info info;
+  memset(&info, 0, sizeof info);
+  if (r2_nvdb_read(path, &info, back, (int64_t)n * n * n * 32, err, (int32_t)sizeof err) != R2_OK) return 7;
+  if (info.nx != n || info.ny != n || info.nz != n || info.active_voxels <= 0) return 8;
+  /* a 4x3 frame as the reference's two image files */
+  unsigned char rgb[4 * 3 * 3];
+  for (int i = 0; i < 36; ++i) rgb[i] = (unsigned char)(i * 7);
+  snprintf(path, sizeof path, "%s/output.png", argv[1]);
+  if (r2_image_write(path, 4, 3, rgb, R2_IMAGE_PNG, 0, err, (int32_t)sizeof err) != R2_OK) return 9;
+  snprintf(path, sizeof path, "%s/output.jpg", argv[1]);
+  if (r2_image_write(path, 4, 3, rgb, R2_IMAGE_JPG, 100, err, (int32_t)sizeof err) != R2_OK) return 10;
+  if (r2_image_write(path, 4, 3, rgb, R2_IMAGE_JPG, 0, err, (int32_t)sizeof err) != R2_ERR_INVALID) return 11;
+  printf("%lld\n", (long long)info.active_voxels);
+  free(rec);
+  free(back);
+  return 0;
+}
+"""
+
+
+def test_header_is_c_and_abi_callable_from_c(tmp_path):
+    """include/r2.h compiles as strict C11 and the host-only entry points work from a
+    C program linked against the library (the binding a C host would use)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "abi.c"
+    src.write_text(C_PROGRAM)
+    exe = tmp_path / "abi"
+    libdir = capi.LIB_PATH.parent
+    cmd = [gcc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I", str(HEADER.parent), str(src),
+           "-o", str(exe), "-L", str(libdir), "-lrenderer2", "-L/opt/rocm/lib", f"-Wl,-rpath,{libdir}",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, (out.returncode, out.stderr[-2000:])
+    assert int(out.stdout.strip()) > 0
+    assert (tmp_path / "output.png").stat().st_size > 0 and (tmp_path / "output.jpg").stat().st_size > 0
