@@ -42,6 +42,32 @@ def relmse(x, r):
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
 
+def host_cpus() -> dict:
+    """The host cores this process may use: nproc, the affinity mask, the cgroup CPU
+    quota (cpu.max) when one is set, and the CPU model.  The CPU baseline runs one
+    OpenMP thread per usable core: the affinity count, capped by the quota."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cgroup_quota_cpus": None,
+            "model": None, "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            info["cgroup_quota_cpus"] = round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                info["model"] = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    n = info["affinity"]
+    if info["cgroup_quota_cpus"]:
+        n = min(n, max(1, int(info["cgroup_quota_cpus"] + 0.5)))
+    info["threads_used"] = n
+    return info
+
+
 def cpu_baseline(w, threads: int, stride: int, spp: int):
     """Time the FP32 oracle (CPU restatement, OpenMP) on a strided pixel subset."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -69,12 +95,19 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--no-serving", action="store_true", help="skip the render-server latency probe")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=2, help="pixel stride of the CPU baseline sample")
-    ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel of the CPU baseline sample")
+    ap.add_argument("--cpu-stride", type=int, default=None,
+                    help="pixel stride of the CPU baseline sample (default: 1 for c1, 2 otherwise)")
+    ap.add_argument("--cpu-spp", type=int, default=None,
+                    help="samples per pixel of the CPU baseline sample (default: the workload's, at most 64)")
+    ap.add_argument("--quality-spp", type=int, default=2048,
+                    help="samples per pixel of the frame whose relMSE against the FP64 oracle is reported; "
+                         "rendered after the timed region (the timed steps' samples count towards it)")
     ap.add_argument("--no-brick-grid", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "packed"], default=None,
                     help="voxel record precision (default: the workload's own)")
-    ap.add_argument("--requests", type=int, default=80, help="requests of the serving workload")
+    ap.add_argument("--requests", type=int, default=80, help="requests of the serving workload (--workload c5)")
+    ap.add_argument("--serving-requests", type=int, default=40,
+                    help="requests of the serving block of the render workloads' line")
     ap.add_argument("--rate", type=float, default=5.0, help="Poisson arrival rate (requests/s) of the serving workload")
     ap.add_argument("--cpu-voxels", type=int, default=1_500_000,
                     help="random voxels per LOD of the filter CPU baseline sample")
@@ -124,38 +157,55 @@ def main(argv=None) -> int:
     value = paths_per_step * a.steps / elapsed / 1e6
     avg_kernel_ms = float(np.mean(kernel_ms))
 
-    # accumulated K*spp frame (each step is already divided by its own spp)
-    acc = np.zeros((H, W, 3), np.float64)
-    for b in bufs:
-        acc += b.to_host(np.float32, (H, W, 3))
-    acc /= a.steps
+    # quality frame: the timed steps' frames plus further launches of the following
+    # samples (outside the timed region) up to --quality-spp samples per pixel; each
+    # frame is already divided by its own spp, so the frame is their mean
     quality = None
     gold = GOLDEN / f"{a.workload}_oracle_fp64.npz"
     if gold.exists():
         z = np.load(gold, allow_pickle=False)
         st = int(z["stride"])
-        sub = acc[::st, ::st]
+        ref = z["image"].astype(np.float64)
+        n_q = max(a.steps, -(-a.quality_spp // spp))
+        acc = np.zeros(ref.shape, np.float64)
+        curve = {}
+        for k in range(n_q):
+            if k < a.steps:
+                fr = bufs[k].to_host(np.float32, (H, W, 3))
+            else:
+                ctx.render_device(dict(cfg, sample_begin=k * spp), grids, w["instances"], env, warm.addr)
+                fr = warm.to_host(np.float32, (H, W, 3))
+            acc += fr[::st, ::st]
+            if ((k + 1) & k) == 0 or k + 1 == n_q:  # 1, 2, 4, ... launches and the last
+                curve[(k + 1) * spp] = relmse(acc / (k + 1), ref)
         from renderer2_amd.flip import flip_hdr
 
-        quality = dict(relmse=relmse(sub, z["image"]), flip=round(flip_hdr(z["image"], sub), 5), spp=a.steps * spp,
+        img = acc / n_q
+        reach = [n for n, e in sorted(curve.items()) if e <= 1e-3]
+        quality = dict(relmse=relmse(img, ref), flip=round(flip_hdr(z["image"], img), 5), spp=n_q * spp,
                        oracle_spp=int(z["spp"]),
-                       oracle="FP64 CPU oracle, 1/%d strided pixel subset" % (st * st),
-                       relmse_1step=relmse(bufs[0].to_host(np.float32, (H, W, 3))[::st, ::st], z["image"]))
+                       oracle="FP64 CPU oracle, %s" % ("full frame" if st == 1 else "1/%d strided pixel subset" % (st * st)),
+                       relmse_by_spp={str(k): round(v, 7) for k, v in sorted(curve.items())},
+                       spp_to_relmse_1e3=reach[0] if reach else None,
+                       timed_frame_spp=a.steps * spp,
+                       relmse_timed_frame=curve.get(a.steps * spp))
 
     serving = None
     if not a.no_serving:
-        inst = [dict(grid=0, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))]
-        serving = serve_stream(ctx, grids, env, [inst], n_req=24, rate=15.0, seed=5, sizes=(256, 512),
-                               spps=(16, 64), max_depth=cfg["max_depth"])
+        serving = serve_c5(ctx, a, n_req=a.serving_requests)
 
     cpu = None
     counts = None
+    host = host_cpus()
     if not a.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        v, dt, n, counts = cpu_baseline(w, threads, a.cpu_stride, a.cpu_spp)
+        threads = host["threads_used"]
+        stride = a.cpu_stride or (1 if a.workload == "c1" else 2)
+        cspp = a.cpu_spp or min(spp, 64)
+        v, dt, n, counts = cpu_baseline(w, threads, stride, cspp)
         cpu = {"value": round(v, 4), "unit": "M paths/s", "cores": threads, "kind": "port",
-               "sample": f"FP32 oracle, 1/{a.cpu_stride ** 2} strided pixel subset x {a.cpu_spp} spp "
-                         f"= {n} paths in {dt:.1f} s"}
+               "sample": f"FP32 oracle, {'full frame' if stride == 1 else f'1/{stride ** 2} strided pixel subset'} "
+                         f"x {cspp} spp = {n} paths in {dt:.1f} s",
+               "host": host}
     roof = None
     if counts is not None:
         b_vox = B_VOX_PACKED if w["grids"][0].get("precision") == "packed" else B_VOX_FP32
@@ -273,24 +323,15 @@ def run_server(ctx, grids, env, reqs, budget=16 << 30, max_inflight=16, slice_pa
             "paths": paths}
 
 
-def serve_stream(ctx, grids, env, instance_sets, n_req, rate, seed, sizes, spps, max_depth, radius=2.5):
-    reqs = make_requests(n_req, rate, seed, instance_sets, sizes, spps, max_depth, radius)
-    res = run_server(ctx, grids, env, reqs, slice_paths=1 << 23)
-    res.update(arrival_rate_per_s=rate, sizes=list(sizes), spps=list(spps))
-    return res
-
-
-def bench_serving(a) -> int:
+def serve_c5(ctx, a, n_req: int) -> dict:
     """Render server under a seeded Poisson stream: device-resident LOD grids of the
     256^3 anisotropic field at 128^3, 256^3 and 512^3 plus the 64^3 value-noise
     grid; resolution uniform over {256^2, 512^2, 1024^2}, spp over {16, 64, 256},
     orbit camera and LOD chosen per request.  The same request set is first run
     closed-loop (all submitted at once) to measure the server's capacity; the
     Poisson run's offered load is stated against it."""
-    import renderer2_amd as r2
     from renderer2_amd import scenes
 
-    ctx = r2.Context(0)
     grids = []
     for n in (128, 256, 512):
         w = scenes.make_workload("c2", n=n)
@@ -302,15 +343,29 @@ def bench_serving(a) -> int:
     inst_sets = [[dict(grid=k, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))] for k in range(4)]
     kw = dict(instance_sets=inst_sets, sizes=(256, 512, 1024), spps=(16, 64, 256), max_depth=16)
     ctx.render(dict(make_requests(1, None, 0, **kw)[0][1], spp=1), grids, inst_sets[0], env)  # warm-up
-    cap = run_server(ctx, grids, env, make_requests(a.requests, None, 11, **kw))
-    mean_paths = cap["paths"] / a.requests
+    cap = run_server(ctx, grids, env, make_requests(n_req, None, 11, **kw))
+    mean_paths = cap["paths"] / n_req
     capacity_rps = cap["throughput_mpaths_s"] * 1e6 / mean_paths
-    res = run_server(ctx, grids, env, make_requests(a.requests, a.rate, 11, **kw))
+    res = run_server(ctx, grids, env, make_requests(n_req, a.rate, 11, **kw))
     offered = a.rate * mean_paths / 1e6
     res.update(arrival_rate_per_s=a.rate, offered_mpaths_s=round(offered, 2),
                capacity_mpaths_s=cap["throughput_mpaths_s"], capacity_req_s=round(capacity_rps, 2),
                utilisation=round(offered / cap["throughput_mpaths_s"], 3), closed_loop_p50_ms=cap["p50_ms"],
-               closed_loop_p99_ms=cap["p99_ms"], sizes=[256, 512, 1024], spps=[16, 64, 256])
+               closed_loop_p99_ms=cap["p99_ms"], sizes=[256, 512, 1024], spps=[16, 64, 256],
+               grids=["c2 field 128^3", "c2 field 256^3", "c2 field 512^3", "c1 field 64^3"], max_depth=16,
+               workload="seeded Poisson stream, resolution uniform over {256,512,1024}^2, spp over {16,64,256}, "
+                        "orbit camera and LOD uniform per request")
+    for g in grids:
+        g.free()
+    env.free()
+    return res
+
+
+def bench_serving(a) -> int:
+    import renderer2_amd as r2
+
+    ctx = r2.Context(0)
+    res = serve_c5(ctx, a, a.requests)
     line = {
         "metric": "render server: M paths/s with p50/p99 request latency under a Poisson stream",
         "value": res["throughput_mpaths_s"],

@@ -147,6 +147,20 @@ typedef struct {
  * out_on_device: 1 if out_rgb is a device pointer. */
 int r2_render(r2_ctx* ctx, const r2_render_params* p, float* out_rgb, int32_t out_on_device);
 
+/* Progress callback of r2_render_progressive: called after each completed sample
+ * chunk with the samples per pixel done so far; a nonzero return stops the frame. */
+typedef int (*r2_progress_fn)(void* user, int32_t samples_done, int32_t samples_total);
+
+/* r2_render with a callback between sample chunks: rt::Raytracer::renderFrame's
+ * keyCallback, polled after each spp launch, which returns the frame accumulated so
+ * far when it asks to stop (raytracer/raytracer.cu:161-190).  The frame runs as one
+ * launch per sample chunk (a chunk is ceil(spp W H / 2^24) samples of every pixel),
+ * with two launches in flight.  Completed: R2_OK and the same frame as r2_render,
+ * bit for bit.  Stopped: R2_ERR_CANCELLED, and out_rgb holds the sum over the
+ * completed samples of L / spp, as the reference's framebuffer does. */
+int r2_render_progressive(r2_ctx* ctx, const r2_render_params* p, float* out_rgb, int32_t out_on_device,
+                          r2_progress_fn progress, void* user);
+
 /* Parity probe: one path per (pixel, sample) pair; out: n*3 (L / spp);
  * counters: n*8 u32 lookup counters (voxel records, majorants, brick densities,
  * env lookups, DDA steps, free-flight steps, scatter events, unused) or NULL. */
@@ -353,6 +367,7 @@ typedef struct {
   float fov, cam_pos[3], cam_look_at[3], cam_up[3];
   int32_t n_media, n_meshes, n_surfaces, n_filter_objects;
   int32_t max_depth;
+  int32_t n_render_objects;    /* objects whose mask has the RENDER bit */
 } r2_config_summary_t;
 int r2_config_summary(const char* config_path, r2_config_summary_t* out, char* err, int32_t err_len);
 

@@ -118,6 +118,7 @@ def lib() -> C.CDLL:
             g("r2o_camera_view_to_world").argtypes = [C.POINTER(RenderParams), vp]
             g("r2o_camera_ray").argtypes = [C.POINTER(RenderParams), C.c_int, C.c_int, C.c_uint32, vp]
             g("r2o_scene_info").argtypes = [vp, C.c_int, vp]
+            g("r2o_instance_info").argtypes = [vp, C.c_int, vp]
             g("r2o_filtered_data").argtypes = [vp, C.c_int, vp, vp]
             g("r2o_decode_record").argtypes = [vp, vp]
             g("r2o_dda_cells").argtypes = [vp, C.c_int, vp, R, C.c_int, C.c_int, vp]
@@ -303,6 +304,16 @@ class Scene:
         m["bb_max"] = out[58:61].copy()
         m["ratio"] = out[61]
         return m
+
+    def instance_info(self, k: int = 0) -> dict:
+        """Instance k: model_to_world / world_to_model as 3x4 [R | t] matrices,
+        world bbox and filterRenderRatio."""
+        out = np.zeros(31, self.R)
+        self._fn("r2o_instance_info")(self.h, k, _p(out))
+        m2w = out[0:12].reshape(4, 3).T.astype(np.float64)  # column major -> rows
+        w2m = out[12:24].reshape(4, 3).T.astype(np.float64)
+        return {"model_to_world": m2w, "world_to_model": w2m, "bb_min": out[24:27].copy(),
+                "bb_max": out[27:30].copy(), "ratio": float(out[30])}
 
     def camera_view_to_world(self, cfg: dict) -> np.ndarray:
         p = self.params(cfg)

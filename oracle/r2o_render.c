@@ -1386,6 +1386,19 @@ void SFX(r2o_scene_info)(const scene_t* s, int medium, R* out) {
   out[k++] = s->inst[0].bb_max.x; out[k++] = s->inst[0].bb_max.y; out[k++] = s->inst[0].bb_max.z;
   out[k++] = s->inst[0].ratio;
 }
+/* instance k: model -> world and world -> model 4x3 (column major, 12 each), world
+ * bbox min and max, filterRenderRatio (31 values) */
+void SFX(r2o_instance_info)(const scene_t* s, int k, R* out) {
+  const instance_t* in = &s->inst[k];
+  int q = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) out[q++] = in->model_to_world.c[i][j];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) out[q++] = in->world_to_model.c[i][j];
+  out[q++] = in->bb_min.x; out[q++] = in->bb_min.y; out[q++] = in->bb_min.z;
+  out[q++] = in->bb_max.x; out[q++] = in->bb_max.y; out[q++] = in->bb_max.z;
+  out[q++] = in->ratio;
+}
 /* trilinear record at normalised p: density, S[6], diffuse, specular, normal, ior (17 values) */
 void SFX(r2o_filtered_data)(const scene_t* s, int medium, const R* p3, R* out17) {
   const rec_t r = filtered_data(&s->media[medium], V(p3[0], p3[1], p3[2]), NULL);

@@ -161,6 +161,19 @@ class Context:
         self._check(capi.lib().r2_render(self.h, C.byref(p), out.ctypes.data, 0))
         return out
 
+    def render_progressive(self, cfg: dict, grids, instances, env, callback):
+        """Render with callback(samples_done, samples_total) -> bool after each sample
+        chunk; True stops the frame.  Returns (frame, completed): a stopped frame holds
+        the sum of L / spp over the completed samples (renderFrame's keyCallback)."""
+        p, keep = self.render_params(cfg, grids, instances, env)
+        out = np.zeros((p.height, p.width, 3), np.float32)
+        fn = capi.PROGRESS_FN(lambda _u, done, total: 1 if callback(int(done), int(total)) else 0)
+        rc = capi.lib().r2_render_progressive(self.h, C.byref(p), out.ctypes.data, 0, C.cast(fn, C.c_void_p), None)
+        if rc == capi.R2_ERR_CANCELLED:
+            return out, False
+        self._check(rc)
+        return out, True
+
     def render_device(self, cfg: dict, grids, instances, env, out_dev_ptr: int):
         """Render into a caller-owned device buffer (inputs and output stay in HBM)."""
         p, keep = self.render_params(cfg, grids, instances, env)

@@ -25,6 +25,7 @@ R2_ERR = {
     -7: "R2_ERR_BUSY",
     -8: "R2_ERR_DEADLINE",
 }
+R2_ERR_CANCELLED = -6
 R2_PREC_FP32 = 0
 R2_PREC_PACKED = 1
 R2_IMAGE_PNG = 0
@@ -145,7 +146,7 @@ class ConfigSummary(C.Structure):
         ("sigma_t", C.c_float), ("estimation_iterations", C.c_int32), ("alpha", C.c_float),
         ("clip_rays", C.c_int32), ("fov", C.c_float), ("cam_pos", f3), ("cam_look_at", f3), ("cam_up", f3),
         ("n_media", C.c_int32), ("n_meshes", C.c_int32), ("n_surfaces", C.c_int32), ("n_filter_objects", C.c_int32),
-        ("max_depth", C.c_int32),
+        ("max_depth", C.c_int32), ("n_render_objects", C.c_int32),
     ]
 
 
@@ -187,6 +188,8 @@ SIGNATURES = [
     ("r2_env_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     ("r2_env_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("r2_render", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_int32]),
+    ("r2_render_progressive", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_int32,
+                                        C.c_void_p, C.c_void_p]),
     ("r2_render_paths", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_int32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
     ("r2_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
@@ -217,6 +220,10 @@ SIGNATURES = [
     ("r2_synth_mesh", C.c_int, [C.c_int32, C.c_int32, C.c_float, C.c_uint32, C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
+
+
+# r2_progress_fn: int (*)(void* user, int32_t samples_done, int32_t samples_total)
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, C.c_int32)
 
 
 class R2Error(RuntimeError):

@@ -160,6 +160,7 @@ int r2_create(r2_ctx** out, int device) {
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->main_mark, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void**)&c->d_scene, sizeof(DScene)) != hipSuccess || hipMalloc((void**)&c->d_counter, 64) != hipSuccess) {
     r2_destroy(c);
     return R2_ERR_HIP;
@@ -181,6 +182,7 @@ int r2_destroy(r2_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->side_done) (void)hipEventDestroy(c->side_done);
+  if (c->main_mark) (void)hipEventDestroy(c->main_mark);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -381,7 +383,8 @@ int r2_env_free(r2_ctx* ctx, r2_env* e) {
   return R2_OK;
 }
 
-int r2_render(r2_ctx* ctx, const r2_render_params* p, float* out, int32_t out_on_device) {
+static int render_common(r2_ctx* ctx, const r2_render_params* p, float* out, int32_t out_on_device,
+                         const RenderOpts& o) {
   if (!ctx || !p || !out) return R2_ERR_INVALID;
   const auto t0 = std::chrono::steady_clock::now();
   (void)hipSetDevice(ctx->device);
@@ -395,14 +398,27 @@ int r2_render(r2_ctx* ctx, const r2_render_params* p, float* out, int32_t out_on
     R2_HIP(ctx, hipMalloc(&tmp, bytes));
     d_out = (float*)tmp;
   }
-  rc = launch_render(ctx, s, d_out, (p->flags & R2_FLAG_FP64_ACCUM) != 0);
-  if (rc == R2_OK && !out_on_device) {
+  rc = launch_render_ex(ctx, s, d_out, (p->flags & R2_FLAG_FP64_ACCUM) != 0, o);
+  if ((rc == R2_OK || rc == R2_ERR_CANCELLED) && !out_on_device) {
     hipError_t e = hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = hip_fail(ctx, e, "hipMemcpy(framebuffer)");
   }
   if (tmp) (void)hipFree(tmp);
   ctx->stats.last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
+}
+
+int r2_render(r2_ctx* ctx, const r2_render_params* p, float* out, int32_t out_on_device) {
+  return render_common(ctx, p, out, out_on_device, RenderOpts{});
+}
+
+int r2_render_progressive(r2_ctx* ctx, const r2_render_params* p, float* out, int32_t out_on_device,
+                          r2_progress_fn progress, void* user) {
+  if (!progress) return R2_ERR_INVALID;
+  RenderOpts o;
+  o.progress = progress;
+  o.user = user;
+  return render_common(ctx, p, out, out_on_device, o);
 }
 
 int r2_render_paths(r2_ctx* ctx, const r2_render_params* p, int32_t n, const int32_t* pix, const int32_t* smp,

@@ -1082,23 +1082,32 @@ extern "C" int r2_filter_lod(r2_ctx* ctx, const r2_mesh* m, const r2_filter_para
   }
   // the no-hit fill of all records streams on the side stream while the binning
   // (latency-bound) runs on the main one; the main stream waits for it before
-  // the filter writes the active voxels
+  // the filter writes the active voxels.  The fill itself first waits for the
+  // work already queued on the main stream, which may still read the output.
+  R2F_HIP(hipEventRecord(ctx->main_mark, ctx->stream), "filter: mark main stream");
+  R2F_HIP(hipStreamWaitEvent(ctx->side, ctx->main_mark, 0), "filter: order fill after main stream");
   hipLaunchKernelGGL(k_fill_empty, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->side, rec, nvox);
-  R2F_HIP(hipGetLastError(), "filter: k_fill_empty");
-  R2F_HIP(hipEventRecord(ctx->side_done, ctx->side), "filter: record fill");
-  rc = bin_triangles(ctx, m, c, &B);
-  if (rc) {
+  // from here on the fill may be in flight: every exit joins the side stream
+  auto fail_joined = [&](int code) {
     (void)hipStreamSynchronize(ctx->side);
-    return rc;
-  }
-  R2F_HIP(hipStreamWaitEvent(ctx->stream, ctx->side_done, 0), "filter: join fill");
+    return code;
+  };
+  hipError_t he = hipGetLastError();
+  if (he == hipSuccess) he = hipEventRecord(ctx->side_done, ctx->side);
+  if (he != hipSuccess) return fail_joined(r2host::hip_fail(ctx, he, "filter: k_fill_empty"));
+  rc = bin_triangles(ctx, m, c, &B);
+  if (rc) return fail_joined(rc);
+  he = hipStreamWaitEvent(ctx->stream, ctx->side_done, 0);
+  if (he != hipSuccess) return fail_joined(r2host::hip_fail(ctx, he, "filter: join fill"));
   float kms = 0.f;
   rc = launch_filter(ctx, m, c, B, nullptr, B.n_act, true, rec, &kms);
-  if (rc) return rc;
+  if (rc) return fail_joined(rc);
   if (!out_on_device) {
-    R2F_HIP(hipMemcpyAsync(out_records, rec, 32ull * nvox, hipMemcpyDeviceToHost, ctx->stream), "filter: d2h records");
+    he = hipMemcpyAsync(out_records, rec, 32ull * nvox, hipMemcpyDeviceToHost, ctx->stream);
+    if (he != hipSuccess) return fail_joined(r2host::hip_fail(ctx, he, "filter: d2h records"));
   }
-  R2F_HIP(hipStreamSynchronize(ctx->stream), "filter: sync");
+  he = hipStreamSynchronize(ctx->stream);
+  if (he != hipSuccess) return fail_joined(r2host::hip_fail(ctx, he, "filter: sync"));
   if (grid_out) *grid_out = g;
   const auto t1 = std::chrono::steady_clock::now();
   ctx->fstats.active_voxels = B.n_act;

@@ -113,14 +113,18 @@ bool str(const Value* v, std::string* out) {
   return true;
 }
 
-// getMask (raytracer/include/scene/sceneobject_mask.hpp)
+// getMask (raytracer/src/scene/sceneobject_mask.cpp:4-17), ORed over the array as
+// config_loader.cu:34-41 does: "NONE" adds nothing, "FILTER" and "RENDER" their
+// bit, and any other string (e.g. "ALL") both bits.  No "Mask" key: `dflt`.
 unsigned mask_of(const Value* m, unsigned dflt) {
   if (!m || !m->is_array()) return dflt;
   unsigned r = 0;
   for (const Value& v : m->arr) {
     if (!v.is_string()) continue;
+    if (v.str == "NONE") continue;
     if (v.str == "RENDER") r |= kMaskRender;
     else if (v.str == "FILTER") r |= kMaskFilter;
+    else r |= kMaskRender | kMaskFilter;
   }
   return r;
 }
@@ -917,6 +921,7 @@ extern "C" int r2_config_summary(const char* config_path, r2_config_summary_t* o
       else if (o.type == "Mesh") ++out->n_meshes;
       else ++out->n_surfaces;
       if (o.mask & kMaskFilter) ++out->n_filter_objects;
+      if (o.mask & kMaskRender) ++out->n_render_objects;
     }
     out->max_depth = c.max_depth;
     return R2_OK;

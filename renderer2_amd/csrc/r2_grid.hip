@@ -346,9 +346,10 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   R2_HIP(ctx, hipMemcpyAsync(flags.data(), d_leaf_flag, nleaf_total * 4, hipMemcpyDeviceToHost, st));
   R2_HIP(ctx, hipMemcpyAsync(bbox, d_bbox, sizeof(bbox), hipMemcpyDeviceToHost, st));
   R2_HIP(ctx, hipMemcpyAsync(mm, d_max, 8, hipMemcpyDeviceToHost, st));
+  R2_HIP(ctx, hipStreamSynchronize(st));
+  // read the copied extrema only after the stream has finished the copies
   const uint32_t maxbits = mm[0];
   const uint32_t minbits = (mm[1] & 0x80000000u) ? (mm[1] & 0x7FFFFFFFu) : ~mm[1];
-  R2_HIP(ctx, hipStreamSynchronize(st));
   std::vector<int32_t> table(nleaf_total);
   int64_t nleaves = 0;
   for (size_t i = 0; i < nleaf_total; ++i) table[i] = flags[i] ? (int32_t)(nleaves++) : -1;

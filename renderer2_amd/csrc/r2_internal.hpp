@@ -14,6 +14,8 @@ struct r2_ctx {
   // joined back through side_done
   hipStream_t side = nullptr;
   hipEvent_t side_done = nullptr;
+  // marks the main stream's queued work, for the side stream to wait on
+  hipEvent_t main_mark = nullptr;
   std::string err;
   r2_stats stats{};
   // reusable device scratch for renders
@@ -81,6 +83,15 @@ int dda_cells(r2_ctx* ctx, int n, const float* rays, float sigma_t, int collide_
               int32_t* counts);
 // render kernels (r2_render.hip)
 int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum);
+struct RenderOpts {
+  // called after each sample chunk's launch (one launch per chunk); nonzero stops the frame
+  r2_progress_fn progress = nullptr;
+  void* user = nullptr;
+  // caller-provided chunk-sum buffer (no allocation); nullptr: the context's scratch
+  void* partial = nullptr;
+  size_t partial_bytes = 0;
+};
+int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum, const RenderOpts& o);
 int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
                  uint32_t* d_cnt);
 // one request's share of a batched launch (render server)

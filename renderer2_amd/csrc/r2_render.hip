@@ -100,29 +100,98 @@ int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& ite
   return R2_OK;
 }
 
-int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64) {
+// The frame's work items are numbered chunk-major (item = chunk * npix + pixel
+// tile order), so the items of sample chunk c are the range [c npix, (c + 1) npix).
+// A render runs them as one launch, or (with a progress callback) one launch per
+// sample chunk, like the reference's one launch per sample: a launch takes the
+// items [first, total) by starting the shared item counter at `first`.  Each item
+// writes its own chunk sum, so the frame is the same bit for bit either way.
+static int launch_items(r2_ctx* ctx, int prec, bool single, bool fp64, uint64_t first, uint64_t total, int chunk,
+                        void* partial) {
+  hipStream_t st = ctx->stream;
+  // the counter's low word = first (< 2^32: items of a launch fit 32 bits), high word 0
+  R2_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)ctx->d_counter, (int)(uint32_t)first, 1, st));
+  R2_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)(ctx->d_counter + 1), 0, 1, st));
+  const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total - first + 255) / 256);
+  launch_sm(ctx, prec, blocks, total, chunk, partial, fp64, single, false, nullptr, nullptr, nullptr, nullptr);
+  R2_HIP(ctx, hipGetLastError());
+  return R2_OK;
+}
+
+int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64, const RenderOpts& o) {
   using namespace r2;
   const uint64_t npix = (uint64_t)scene.W * scene.H;
   int chunk = 1, nchunks = 1;
   render_chunking(scene, &chunk, &nchunks);
-  const uint64_t total = (uint64_t)nchunks * npix;
   const size_t acc_bytes = fp64 ? 8 : 4;
-  const size_t part_bytes = (size_t)nchunks * npix * 3 * acc_bytes;
-  int rc = ensure_scratch(ctx, part_bytes);
-  if (rc) return rc;
+  void* partial = nullptr;
+  if (o.partial) {
+    // caller-provided chunk-sum buffer: fewer, longer items when the frame's
+    // default chunking does not fit it
+    const uint64_t cap_items = o.partial_bytes / (3 * acc_bytes);
+    if (cap_items < npix) return fail(ctx, R2_ERR_UNSUPPORTED, "render: frame larger than the chunk-sum buffer");
+    if ((uint64_t)nchunks * npix > cap_items) {
+      const uint64_t max_chunks = cap_items / npix;
+      chunk = (int)((scene.spp + max_chunks - 1) / max_chunks);
+      nchunks = (scene.spp + chunk - 1) / chunk;
+    }
+    partial = o.partial;
+  } else {
+    const size_t part_bytes = (size_t)nchunks * npix * 3 * acc_bytes;
+    int rc = ensure_scratch(ctx, part_bytes);
+    if (rc) return rc;
+    partial = ctx->scratch;
+  }
+  const uint64_t total = (uint64_t)nchunks * npix;
   hipStream_t st = ctx->stream;
-  R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(DScene), hipMemcpyHostToDevice, st));
-  R2_HIP(ctx, hipMemsetAsync(ctx->d_counter, 0, 8, st));
-  const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total + 255) / 256);
   const bool single = scene.n_inst == 1;
   const int prec = scene_precision(scene);
   if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
+  R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(DScene), hipMemcpyHostToDevice, st));
   R2_HIP(ctx, hipEventRecord(ctx->ev0, st));
-  launch_sm(ctx, prec, blocks, total, chunk, ctx->scratch, fp64, single, false, nullptr, nullptr, nullptr, nullptr);
-  R2_HIP(ctx, hipGetLastError());
+  int done_chunks = nchunks;
+  int launches = 1;
+  if (!o.progress) {
+    int rc = launch_items(ctx, prec, single, fp64, 0, total, chunk, partial);
+    if (rc) return rc;
+  } else {
+    // one launch per sample chunk, with up to two in flight; the callback runs
+    // after each one completes and may stop the frame
+    launches = nchunks;
+    std::vector<hipEvent_t> evs(2, nullptr);
+    for (auto& e : evs) R2_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    struct Ev {
+      std::vector<hipEvent_t>& v;
+      ~Ev() { for (auto e : v) if (e) (void)hipEventDestroy(e); }
+    } guard{evs};
+    int queued = 0, completed = 0, rc = R2_OK;
+    bool stop = false;
+    while (completed < nchunks) {
+      while (!stop && queued < nchunks && queued - completed < 2) {
+        rc = launch_items(ctx, prec, single, fp64, (uint64_t)queued * npix, (uint64_t)(queued + 1) * npix, chunk,
+                          partial);
+        if (rc) break;
+        R2_HIP(ctx, hipEventRecord(evs[queued & 1], st));
+        ++queued;
+      }
+      if (rc) break;
+      if (completed == queued) break;
+      R2_HIP(ctx, hipEventSynchronize(evs[completed & 1]));
+      ++completed;
+      if (!stop && o.progress(o.user, std::min(scene.spp, completed * chunk), scene.spp) != 0) stop = true;
+    }
+    if (rc) {
+      (void)hipStreamSynchronize(st);
+      return rc;
+    }
+    done_chunks = completed;
+    launches = completed;
+  }
   R2_HIP(ctx, hipEventRecord(ctx->ev1, st));
   const uint32_t n3 = (uint32_t)(npix * 3);
-  r2k_fp32::launch_reduce_k(st, ctx->scratch, fp64, nchunks, n3, d_out);
+  // the frame of the completed sample chunks: sum over paths of L / spp, as the
+  // reference's framebuffer holds it when renderFrame's callback stops the loop
+  r2k_fp32::launch_reduce_k(st, partial, fp64, done_chunks, n3, d_out);
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipStreamSynchronize(st));
   float ms = 0.f;
@@ -142,9 +211,14 @@ int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64)
     R2_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(r2k_fp32::g_sm_stats), z, sizeof(z)));
   }
 #endif
-  ctx->stats.last_paths = (int64_t)(npix * scene.spp);
-  ctx->stats.last_launches = 2;
+  ctx->stats.last_paths = (int64_t)std::min<uint64_t>(npix * scene.spp, npix * (uint64_t)done_chunks * chunk);
+  ctx->stats.last_launches = launches + 1;
+  if (done_chunks < nchunks) return fail(ctx, R2_ERR_CANCELLED, "render: stopped by the progress callback");
   return R2_OK;
+}
+
+int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64) {
+  return launch_render_ex(ctx, scene, d_out, fp64, RenderOpts{});
 }
 
 int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,

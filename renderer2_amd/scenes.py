@@ -26,9 +26,10 @@ WORKLOADS = {
     # 256^3 anisotropic fibre SGGX volume in [0,8], albedo 0.9, procedural sky + sun, 1024^2, 64 spp, depth 16
     "c2": dict(kind=2, n=256, scale=8.0, octaves=4, albedo=0.9, env="sky", env_w=512, env_h=256,
                width=1024, height=1024, spp=64, max_depth=16, camera="close"),
-    # the config-1 field at 512^3 (packed records), 1024^2, 256 spp
+    # the config-1 field at 512^3 (packed records), 1024^2, 256 spp, compensated
+    # FP64 framebuffer sums
     "c4": dict(kind=1, n=512, scale=4.0, octaves=3, albedo=0.8, env="white", env_w=32, env_h=16,
-               width=1024, height=1024, spp=256, max_depth=8, precision="packed"),
+               width=1024, height=1024, spp=256, max_depth=8, precision="packed", fp64_accum=True),
 }
 
 
@@ -127,7 +128,7 @@ def make_workload(name: str, seed: int = 1, **overrides) -> dict:
     env = synth_env(w["env"], w["env_w"], w["env_h"])
     cam = CAMERA_CLOSE if w.get("camera") == "close" else CAMERA
     render = dict(width=w["width"], height=w["height"], spp=w["spp"], max_depth=w["max_depth"], seed=seed,
-                  stream=0, use_brick_grid=True, **cam)
+                  stream=0, use_brick_grid=True, fp64_accum=bool(w.get("fp64_accum", False)), **cam)
     return dict(name=name, grids=[grid], instances=[dict(grid=0, pos=(0, 0, 0), orientation=(0, 1, 0),
                                                          scaling=(1, 1, 1))],
                 env=env, render=render, params=w)

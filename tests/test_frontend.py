@@ -88,3 +88,27 @@ def test_malformed_json_reports_line(tmp_path):
     cfg.write_text('{\n "General": {\n "FrameWidth": 10,,\n }\n}')
     rc, _, err = summary(cfg)
     assert rc < 0 and "line 3" in err
+
+
+@pytest.mark.parametrize("mask,filt,rend", [
+    (None, 0, 1),                    # no "Mask" key: RENDER (config_loader.cu:151-155)
+    (["ALL"], 1, 1),                 # getMask: any string other than NONE/FILTER/RENDER is ALL
+    (["NONE"], 0, 0),
+    (["SOMETHING"], 1, 1),
+    (["FILTER"], 1, 0),
+    (["RENDER"], 0, 1),
+    (["NONE", "FILTER"], 1, 0),      # the array is ORed (config_loader.cu:34-41)
+    (["FILTER", "RENDER"], 1, 1),
+])
+def test_mask_strings_follow_get_mask(tmp_path, mask, filt, rend):
+    """getMask (raytracer/src/scene/sceneobject_mask.cpp:4-17) ORed over the Mask array."""
+    cfg = layout(tmp_path)
+    sd = tmp_path / "raytracer" / "scenedescription.json"
+    obj = {"Type": "Medium", "Path": "p_0.2", "SigmaA": [1, 1, 1], "SigmaS": [1, 1, 1],
+           "Pos": [1, 0, 0], "Orientation": [0, 1, 0], "Scaling": [2, 2, 2]}
+    if mask is not None:
+        obj["Mask"] = mask
+    sd.write_text(json.dumps([obj]))
+    rc, s, err = summary(cfg)
+    assert rc == 0, err
+    assert (s.n_filter_objects, s.n_render_objects) == (filt, rend)

@@ -165,3 +165,27 @@ def test_argument_errors(ctx, c1):
     with pytest.raises(r2.R2Error) as e:
         ctx.render(c1["render"], grids * 5, [dict(grid=k) for k in range(5)], env)
     assert e.value.code == -4  # at most 4 medium instances
+
+
+def test_progressive_render_matches_and_stops(ctx):
+    """r2_render_progressive: run to completion it gives r2_render's frame bit for bit;
+    stopped by its callback after the first chunk it returns R2_ERR_CANCELLED with the
+    partial sum of L / spp over the completed sample chunks (renderFrame's
+    keyCallback, raytracer.cu:185-188)."""
+    w = scenes.make_workload("c1", width=64, height=64, spp=16)
+    grids = [ctx.upload_grid(g) for g in w["grids"]]
+    env = ctx.upload_env(w["env"])
+    cfg = w["render"]
+    want = ctx.render(cfg, grids, w["instances"], env)
+    seen = []
+    full, done = ctx.render_progressive(cfg, grids, w["instances"], env, lambda d, t: seen.append((d, t)) and False)
+    assert done and np.array_equal(full, want)
+    assert seen[-1] == (16, 16) and [d for d, _ in seen] == sorted(d for d, _ in seen)
+    part, done = ctx.render_progressive(cfg, grids, w["instances"], env, lambda d, t: True)
+    assert not done
+    # the stop takes effect after the launches already in flight (at most two)
+    k = ctx.stats()["paths"] // (64 * 64)
+    assert 1 <= k <= 2 and k < 16
+    ref = sum(ctx.render(dict(cfg, spp=1, sample_begin=s), grids, w["instances"], env) for s in range(k)) / 16.0
+    assert np.allclose(part, ref, rtol=1e-5, atol=1e-7)
+    assert part.sum() < full.sum()

@@ -1,11 +1,15 @@
 """Parity at the benchmark workloads' full sizes, through checks that do not
 depend on the oracle rendering the whole frame.
 
+  * 64^3 isotropic volume, 256x256: the full GPU frame at 4096 spp (256
+    launches of the workload's 16 spp) against the committed full-frame FP64
+    oracle image (4096 spp): relMSE <= 1e-3.
   * 256^3 anisotropic volume, 1024x1024: the GPU frame at 2048 spp (32 launches
     of 64 spp) against the committed FP64 oracle image (4096 spp, every 8th
     pixel in x and y): relMSE <= 1e-3, the bar of the render metric.
-  * 512^3 isotropic volume in packed records, 1024x1024: 4096 spp (16 launches
-    of 256 spp) against its FP64 oracle image: relMSE <= 1e-3.
+  * 512^3 isotropic volume in packed records, 1024x1024, compensated FP64
+    framebuffer sums: 4096 spp (16 launches of 256 spp) against its FP64 oracle
+    image: relMSE <= 1e-3.
   * 2.4 M-triangle mesh filtered to its 512^3 LOD: the dense output of
     r2_filter_lod bit-exact with the filter oracle on 10,000 uniformly random
     voxels and 10,000 random voxels that the filter found occupied.
@@ -45,6 +49,12 @@ def _converged(ctx, name, launches, precision=None):
     return relmse(acc[::st, ::st] / launches, z["image"])
 
 
+def test_c1_full_frame_converges(ctx):
+    e = _converged(ctx, "c1", 256)
+    print(f"64^3 isotropic, 256^2, 4096 spp: relMSE vs FP64 oracle (full frame) = {e:.3e}")
+    assert e <= 1e-3
+
+
 def test_c2_full_frame_converges(ctx):
     e = _converged(ctx, "c2", 32)
     print(f"256^3 aniso, 1024^2, 2048 spp: relMSE vs FP64 oracle = {e:.3e}")
@@ -52,6 +62,7 @@ def test_c2_full_frame_converges(ctx):
 
 
 def test_c4_packed_full_frame_converges(ctx):
+    assert scenes.WORKLOADS["c4"]["fp64_accum"]
     e = _converged(ctx, "c4", 16, precision="packed")
     print(f"512^3 packed, 1024^2, 4096 spp: relMSE vs FP64 oracle = {e:.3e}")
     assert e <= 1e-3

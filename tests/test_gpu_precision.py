@@ -104,11 +104,32 @@ def test_packed_frame_converges_to_oracle_fp64(ctx, wl, size, spp_gpu, spp_ref):
 
 
 def test_fp64_accumulation(ctx):
+    """The framebuffer sums against sums of the same paths taken on the host.  A
+    48^2 x 64 spp frame has one sample per work item, so its chunk sums are the
+    paths' own values L / 64.  Path (pixel, k) of that frame is path (pixel, 0) of
+    a 1-spp render that starts at sample k (the random stream is keyed by the
+    sample index), and L / 64 is that frame's value scaled by a power of two.
+    The FP32 form adds the 64 values in sample order in FP32; the compensated
+    form adds them in FP64 and rounds once, i.e. gives the correctly rounded sum
+    but for rare near-ties."""
+    import math
+
     w = scenes.make_workload("c1", width=48, height=48, spp=64)
     grids = [ctx.upload_grid(g) for g in w["grids"]]
     env = ctx.upload_env(w["env"])
     a = ctx.render(w["render"], grids, w["instances"], env)
     b = ctx.render(dict(w["render"], fp64_accum=True), grids, w["instances"], env)
-    # same paths, sums in FP64 with Neumaier compensation: differences are FP32 rounding only
-    assert np.abs(a - b).max() <= 1e-5 * max(1.0, np.abs(b).max())
-    assert not np.array_equal(a, b) or np.all(a == b)
+    one = np.stack([ctx.render(dict(w["render"], spp=1, sample_begin=k), grids, w["instances"], env)
+                    for k in range(64)])
+    vals = (one / np.float32(64)).astype(np.float32)
+    seq = np.zeros_like(a)
+    for k in range(64):
+        seq = (seq + vals[k]).astype(np.float32)
+    assert np.array_equal(a, seq), float(np.abs(a - seq).max())
+    flat = vals.reshape(64, -1).astype(np.float64)
+    exact = np.array([math.fsum(flat[:, i]) for i in range(flat.shape[1])]).astype(np.float32).reshape(b.shape)
+    ulp = np.spacing(np.abs(exact).astype(np.float32))
+    assert np.all(np.abs(b - exact) <= ulp)
+    assert np.mean(b == exact) >= 0.999
+    # and the FP32 form's rounding is visible against it
+    assert not np.array_equal(a, b)
