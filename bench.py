@@ -281,24 +281,6 @@ def copy_calibration(mib: int) -> None:
     dst.free()
 
 
-def make_requests(n_req, rate, seed, instance_sets, sizes, spps, max_depth, radius=2.5):
-    """Seeded Poisson request stream: each request picks a resolution, an spp, an
-    instance set (LOD) and an orbit camera at random.  rate None: all at t = 0."""
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req)) if rate else np.zeros(n_req)
-    reqs = []
-    for k in range(n_req):
-        res = int(rng.choice(sizes))
-        spp = int(rng.choice(spps))
-        inst = instance_sets[int(rng.integers(len(instance_sets)))]
-        az, el = rng.uniform(0, 2 * np.pi), rng.uniform(-0.3, 0.6)
-        cam = (radius * np.cos(el) * np.sin(az), radius * np.sin(el), radius * np.cos(el) * np.cos(az))
-        cfg = dict(width=res, height=res, spp=spp, max_depth=max_depth, cam_pos=cam, cam_look_at=(0, 0, 0),
-                   cam_up=(0, 1, 0), fov_deg=40.0, seed=1000 + k, stream=0, use_brick_grid=True)
-        reqs.append((float(arrivals[k]), cfg, inst))
-    return reqs
-
-
 def run_server(ctx, grids, env, reqs, budget=16 << 30, max_inflight=16, slice_paths=1 << 24):
     """Submit each request at its arrival time, wait for all; latency is per request
     from submission to the frame in host memory (the server's own clock)."""
@@ -332,21 +314,14 @@ def serve_c5(ctx, a, n_req: int) -> dict:
     Poisson run's offered load is stated against it."""
     from renderer2_amd import scenes
 
-    grids = []
-    for n in (128, 256, 512):
-        w = scenes.make_workload("c2", n=n)
-        grids.append(ctx.upload_grid(w["grids"][0]))
-        del w
-    c1 = scenes.make_workload("c1")
-    grids.append(ctx.upload_grid(c1["grids"][0]))
+    grids = [ctx.upload_grid(g) for g in scenes.serving_grids()]
     env = ctx.upload_env(scenes.synth_env("sky", 512, 256))
-    inst_sets = [[dict(grid=k, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))] for k in range(4)]
-    kw = dict(instance_sets=inst_sets, sizes=(256, 512, 1024), spps=(16, 64, 256), max_depth=16)
-    ctx.render(dict(make_requests(1, None, 0, **kw)[0][1], spp=1), grids, inst_sets[0], env)  # warm-up
-    cap = run_server(ctx, grids, env, make_requests(n_req, None, 11, **kw))
+    warm = scenes.serving_requests(1, None, 0)[0]
+    ctx.render(dict(warm[1], spp=1), grids, warm[2], env)  # warm-up
+    cap = run_server(ctx, grids, env, scenes.serving_requests(n_req, None, 11))
     mean_paths = cap["paths"] / n_req
     capacity_rps = cap["throughput_mpaths_s"] * 1e6 / mean_paths
-    res = run_server(ctx, grids, env, make_requests(n_req, a.rate, 11, **kw))
+    res = run_server(ctx, grids, env, scenes.serving_requests(n_req, a.rate, 11))
     offered = a.rate * mean_paths / 1e6
     res.update(arrival_rate_per_s=a.rate, offered_mpaths_s=round(offered, 2),
                capacity_mpaths_s=cap["throughput_mpaths_s"], capacity_req_s=round(capacity_rps, 2),

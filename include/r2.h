@@ -161,6 +161,7 @@ typedef int (*r2_progress_fn)(void* user, int32_t samples_done, int32_t samples_
 int r2_render_progressive(r2_ctx* ctx, const r2_render_params* p, float* out_rgb, int32_t out_on_device,
                           r2_progress_fn progress, void* user);
 
+
 /* Parity probe: one path per (pixel, sample) pair; out: n*3 (L / spp);
  * counters: n*8 u32 lookup counters (voxel records, majorants, brick densities,
  * env lookups, DDA steps, free-flight steps, scatter events, unused) or NULL. */
@@ -242,7 +243,9 @@ int r2_filter_get_stats(const r2_ctx* ctx, r2_filter_stats* out);
 /* Device-resident render server with continuous batching (no reference
  * counterpart beyond the interactive preview loop, raytracer/pipeline.cu:41-78). */
 typedef struct {
-  int64_t hbm_budget_bytes;    /* admission budget for per-request device memory */
+  int64_t hbm_budget_bytes;    /* admission budget for per-request device memory: one buffer of this
+                                  size is allocated by r2_server_start and sub-allocated per request
+                                  (default 8 GiB; R2_ERR_OOM if it does not fit the device) */
   int32_t max_inflight;        /* max requests resident on the device at once */
   int32_t slice_paths;         /* paths per scheduling slice */
 } r2_server_config;

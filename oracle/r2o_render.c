@@ -226,6 +226,32 @@ static inline R uni(r2o_rng* s) {
   return (R)(x >> 8) * C(5.9604644775390625e-08);
 }
 
+/* The random streams of one path.  Philox4x32-10 keyed (seed, stream) with
+ * counter (pixel, sample, block, sub): every vertex of the path draws from
+ * streams of its own, sub = 4 k + role, where vertex k = 0 is the camera and
+ * k >= 1 the k-th scatter event:
+ *   role 0: the vertex's own draws, in the reference's order (the camera
+ *           jitter; at a scatter: direct()'s light sample, its phase
+ *           evaluation and phase sample, Li's continuation sample, Russian
+ *           roulette);
+ *   role 1: the transmittance walk of direct()'s light-sample shadow ray;
+ *   role 2: the transmittance walk of its phase-sample shadow ray;
+ *   role 3: the free-flight walk of the ray that leaves the vertex (bounding
+ *           box pass-throughs included) up to the next scatter event.
+ * The reference draws all of these from one cuRAND sequence per pixel
+ * (sampler.hpp:11-37, raytracer.cu:33-39).  cuRAND is not reproduced, so which
+ * uniform feeds which draw is this restatement's own choice; the estimator is
+ * the reference's.  Streams per ray let a GPU shade a scatter event once and
+ * then trace its three rays one after another. */
+typedef struct { uint32_t k0, k1, c0, c1; } path_key;
+static inline void stream_init(r2o_rng* r, const path_key* pk, uint32_t k, uint32_t role) {
+  r2o_rng_init(r, pk->k0, pk->k1, pk->c0, pk->c1, 4u * k + role);
+}
+/* cap on the segments of one intersectTr walk (device_scene_impl.hpp:93-114
+ * loops until a miss): a segment enters or leaves an instance box, so scenes of
+ * at most R2O_MAX_INST boxes never reach it */
+#define SHADOW_SEG_CAP 64
+
 /* ------------------------------------------------------------------ ray -- */
 typedef struct { v3 o, d; R tmax; int medium; } ray_t;
 
@@ -967,7 +993,7 @@ static int intersect_tr(const scene_t* s, const ray_t* ray, r2o_rng* rng, int us
   v3 p0 = ray->o;
   const v3 p1 = vadd(p0, sscale(ray->tmax, ray->d));
   int cur = ray->medium;
-  for (int it = 0; it < LOOP_CAP; ++it) {
+  for (int it = 0; it < SHADOW_SEG_CAP; ++it) {
     ray_t r = spawn_ray(p0, p1, cur);
     const hit_t h = scene_intersect(s, &r);
     if (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur)) *Tr *= inst_tr(s, r.medium, &r, rng, use_dda, cnt);
@@ -978,8 +1004,11 @@ static int intersect_tr(const scene_t* s, const ray_t* ray, r2o_rng* rng, int us
   return 1;
 }
 
-/* direct (path_integrator_impl.hpp:23-118), medium interaction only */
-static v3 direct(const scene_t* s, const mhit_t* mi, v3 wo, r2o_rng* rng, int use_dda, uint64_t* cnt) {
+/* direct (path_integrator_impl.hpp:23-118), medium interaction only, at scatter
+ * vertex k: the vertex's own draws come from `rng`, the two shadow walks from
+ * the path's role-1 and role-2 streams of vertex k */
+static v3 direct(const scene_t* s, const mhit_t* mi, v3 wo, r2o_rng* rng, const path_key* pk, uint32_t k, int use_dda,
+                 uint64_t* cnt) {
   v3 L = V(0, 0, 0);
   {
     v3 ldir;
@@ -991,7 +1020,9 @@ static v3 direct(const scene_t* s, const mhit_t* mi, v3 wo, r2o_rng* rng, int us
       ray_offset(&rl, mi->normal);
       rl = inst_move_to_voxel_border(s, mi->inst, &rl);
       R trs;
-      const int hit = intersect_tr(s, &rl, rng, use_dda, &trs, cnt);
+      r2o_rng walk;
+      stream_init(&walk, pk, k, 1);
+      const int hit = intersect_tr(s, &rl, &walk, use_dda, &trs, cnt);
       const R p = phase_p(mi->S, mi->normal, mi->ior, wo, ldir, rng);
       const R mis = (C(1) * light_pdf) / (C(1) * light_pdf + C(1) * p);
       const R tr = hit ? C(0) : trs;
@@ -1007,7 +1038,9 @@ static v3 direct(const scene_t* s, const mhit_t* mi, v3 wo, r2o_rng* rng, int us
       ray_offset(&rb, mi->normal);
       rb = inst_move_to_voxel_border(s, mi->inst, &rb);
       R trs;
-      const int hit = intersect_tr(s, &rb, rng, use_dda, &trs, cnt);
+      r2o_rng walk;
+      stream_init(&walk, pk, k, 2);
+      const int hit = intersect_tr(s, &rb, &walk, use_dda, &trs, cnt);
       R light_pdf;
       const v3 Le = env_le(s, wi, &light_pdf, cnt);
       const R mis = (C(1) * p) / (C(1) * p + C(1) * light_pdf);
@@ -1061,10 +1094,15 @@ static ray_t camera_ray(const camera_t* c, int px, int py, r2o_rng* rng) {
 static int any_zero(v3 a) { return a.x == C(0) || a.y == C(0) || a.z == C(0); }
 
 /* CPathIntegrator::Li (path_integrator_impl.hpp:120-247) for media-only scenes */
-static v3 Li(const scene_t* s, const camera_t* cam, int px, int py, int spp, int max_depth, int use_dda, r2o_rng* rng,
-             uint64_t* cnt) {
+static v3 Li(const scene_t* s, const camera_t* cam, int px, int py, int spp, int max_depth, int use_dda,
+             const path_key* pk, uint64_t* cnt) {
   v3 L = V(0, 0, 0), thr = V(C(1), C(1), C(1));
-  ray_t ray = camera_ray(cam, px, py, rng);
+  r2o_rng vtx, walk; /* the current vertex's own stream, the walk leaving it */
+  uint32_t k = 0;
+  stream_init(&vtx, pk, 0, 0);
+  ray_t ray = camera_ray(cam, px, py, &vtx);
+  stream_init(&walk, pk, 0, 3);
+  r2o_rng* rng = &walk;
   int iters = 0;
   for (int b = 0; b < max_depth; ++b) {
     if (++iters > 4 * max_depth + 64) break;
@@ -1087,13 +1125,16 @@ static v3 Li(const scene_t* s, const camera_t* cam, int px, int py, int spp, int
       if (any_zero(thr)) break;
     }
     if (mi.valid) {
+      ++k;
+      stream_init(&vtx, pk, k, 0);
       const v3 wo = vneg(ray.d);
-      L = vadd(L, vmul(thr, direct(s, &mi, wo, rng, use_dda, cnt)));
+      L = vadd(L, vmul(thr, direct(s, &mi, wo, &vtx, pk, k, use_dda, cnt)));
       v3 wi;
-      (void)phase_sample(mi.S, mi.normal, mi.ior, wo, &wi, rng);
+      (void)phase_sample(mi.S, mi.normal, mi.ior, wo, &wi, &vtx);
       ray_t nr = mkray(mi.pos, vnorm(wi), DEFAULT_TMAX, mi.inst);
       ray_offset(&nr, wi);
       ray = inst_move_to_voxel_border(s, mi.inst, &nr);
+      stream_init(&walk, pk, k, 3);
     } else {
       if (b == 0 && !si.hit) {
         R p;
@@ -1108,7 +1149,7 @@ static v3 Li(const scene_t* s, const camera_t* cam, int px, int py, int spp, int
     }
     if (b > 3) {
       const R p = C(1) - (thr.x + thr.y + thr.z) / C(3);
-      if (uni(rng) <= p) break;
+      if (uni(&vtx) <= p) break;
       thr = vdivs(thr, C(1) - p);
     }
   }
@@ -1320,9 +1361,8 @@ void SFX(r2o_render)(const scene_t* s, const r2o_render_params* p, R* out, uint6
       for (int x = 0; x < W; ++x) {
         R acc[3] = {C(0), C(0), C(0)};
         for (int sidx = 0; sidx < p->spp; ++sidx) {
-          r2o_rng rng;
-          r2o_rng_init(&rng, p->seed, p->stream, (uint32_t)(y * W + x), (uint32_t)(p->sample_begin + sidx), 0u);
-          const v3 L = Li(s, &cam, x, y, p->spp, p->max_depth, p->use_brick_grid, &rng, counters ? loc : NULL);
+          const path_key pk = {p->seed, p->stream, (uint32_t)(y * W + x), (uint32_t)(p->sample_begin + sidx)};
+          const v3 L = Li(s, &cam, x, y, p->spp, p->max_depth, p->use_brick_grid, &pk, counters ? loc : NULL);
           acc[0] += L.x; acc[1] += L.y; acc[2] += L.z;
         }
         R* o = out + ((size_t)y * W + x) * 3;
@@ -1347,11 +1387,10 @@ void SFX(r2o_render_paths)(const scene_t* s, const r2o_render_params* p, int n, 
 #pragma omp parallel for schedule(dynamic, 16) num_threads(p->threads > 0 ? p->threads : omp_get_max_threads())
   for (int i = 0; i < n; ++i) {
     const int x = pix[i] % W, y = pix[i] / W;
-    r2o_rng rng;
-    r2o_rng_init(&rng, p->seed, p->stream, (uint32_t)pix[i], (uint32_t)smp[i], 0u);
+    const path_key pk = {p->seed, p->stream, (uint32_t)pix[i], (uint32_t)smp[i]};
     uint64_t* c = cnt ? cnt + (size_t)i * R2O_NCNT : NULL;
     if (c) memset(c, 0, sizeof(uint64_t) * R2O_NCNT);
-    const v3 L = Li(s, &cam, x, y, p->spp, p->max_depth, p->use_brick_grid, &rng, c);
+    const v3 L = Li(s, &cam, x, y, p->spp, p->max_depth, p->use_brick_grid, &pk, c);
     out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
   }
 }
@@ -1542,9 +1581,11 @@ void SFX(r2o_direct_mean)(const scene_t* s, int inst, const R* pos, const R* S6,
   mi.ior = ior;
   double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
   for (int64_t i = 0; i < count; ++i) {
+    /* the first scatter vertex of a path keyed (key, 0) at pixel c0 + i, sample 0 */
+    const path_key pk = {key, 0, c0 + (uint32_t)i, 0};
     r2o_rng rng;
-    r2o_rng_init(&rng, key, 0, c0 + (uint32_t)i, 0, 0);
-    const v3 L = direct(s, &mi, V(wo[0], wo[1], wo[2]), &rng, use_dda, NULL);
+    stream_init(&rng, &pk, 1, 0);
+    const v3 L = direct(s, &mi, V(wo[0], wo[1], wo[2]), &rng, &pk, 1, use_dda, NULL);
     const double l[3] = {(double)L.x, (double)L.y, (double)L.z};
     for (int k = 0; k < 3; ++k) {
       sum[k] += l[k];

@@ -63,8 +63,14 @@ COMMON_FLAGS = [
 # Together these give about 27 % more paths per second.  Every other source keeps
 # each operation separately and correctly rounded, because the filter, grid
 # build and post-process are bit-exact with the oracle.
+# Without sinking of common instructions out of branch arms: LLVM's SimplifyCFG
+# otherwise merges the stores of a lane's state variables from the two arms of a
+# branch into one store through a selected address, which keeps that state in
+# scratch memory, reloaded at every scheduling step of the path program (a
+# code-layout choice only: no arithmetic changes).
 RENDER_SOURCES = {"r2_render.hip", "r2_render_packed.hip"}
-RENDER_FLAGS = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]
+RENDER_FLAGS = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
+                "-mllvm", "-simplifycfg-sink-common=false"]
 
 
 def _compile(src: Path, obj: Path) -> None:

@@ -1,85 +1,110 @@
-// Persistent path state machine for the volume integrator (gfx950).
+// Path integrator as a two-phase lane program for gfx950 wavefronts.
 //
-// CPathIntegrator::Li (path_integrator_impl.hpp:120-247) with direct()
-// (:23-118) and CDeviceScene::intersectTr (device_scene_impl.hpp:93-114) is
-// re-expressed as explicit per-lane states.  Every free-flight (sampleDDA /
-// sampleInternal) and transmittance (trDDA / trInternal) traversal of every
-// path runs through ONE shared step loop, so the lanes of a wavefront advance
-// their traversals together whatever shading stage each path is in; shading
-// events run in batches between traversal rounds.  A lane whose path ends takes
-// the next sample of its work item, or a new item: the wavefront ballots the
-// lanes that need work, takes that many items with one atomic, and hands them
-// out by prefix rank.  Random numbers are drawn in exactly the order of the
-// recursive formulation, so each path's result is independent of scheduling.
+// CPathIntegrator::Li (path_integrator_impl.hpp:120-247) with direct() (:23-118)
+// and CDeviceScene::intersectTr (device_scene_impl.hpp:93-114), re-expressed so
+// that a path alternates between two kinds of work:
+//   * SHADE at a vertex (the camera, or a scatter event): everything that does
+//     not walk through the volume.  The direct light of the previous vertex once
+//     its shadow walks are done; the collision record of the ray that arrived
+//     (trilinear voxel fetch, Fresnel-mixed albedo); direct()'s light sample and
+//     phase evaluation and its phase sample; Li's continuation sample and Russian
+//     roulette; the three rays spawned at the scatter point and moved to the
+//     voxel border.  A lane whose path ends accumulates it and starts its next
+//     sample in the same pass.
+//   * WALK of the vertex's rays, one after another: the light-sample shadow ray
+//     and the phase-sample shadow ray (intersectTr segment walks with DDA or
+//     ratio-tracking transmittance), then the continuation (Li's bounce loop up
+//     to the next collision: scene intersection, the entry-segment double
+//     tracking, bounding-box pass-throughs, free flight).
+// Each vertex and each of its rays draws from a Philox stream of its own
+// (counter word 3 = 4 k + role, the oracle's path streams), so the rays need not
+// run in the reference's interleaved order, and a path on the GPU follows the
+// FP32 oracle's path draw for draw until a decision falls within rounding of its
+// threshold.  A wavefront shades all its lanes that are at a vertex in one pass
+// and advances all its walking lanes one walk step per round, so it runs one of
+// two code blocks at a time instead of a mix of shading states.
 #pragma once
 #include "r2_volume.hpp"
 
 namespace r2 {
 
+// Where a lane waits.  A walking lane either starts a segment (scene
+// intersection, traversal set-up) or advances a traversal.  One variable holds
+// it: stores of different states in the arms of a branch then merge into one
+// value select, where stores to different fields could merge into a store
+// through a selected address and keep the lane's state out of registers.
 enum : int {
-  ST_IDLE = 0,    // no more work
-  ST_FETCH,       // needs a work item (handled wave-wide)
-  ST_TRAV,        // inside a traversal
-  ST_START,       // new sample of the current item
-  ST_BOUNCE,      // top of the bounce loop
-  ST_AFTER_SAMPLE,
-  ST_NOHIT,
-  ST_SCATTER,
-  ST_LIGHT_DONE,
-  ST_BRDF_SAMPLE,
-  ST_BRDF_DONE,
-  ST_DIRECT_DONE,
-  ST_SH_SEG,
-  ST_SH_NEXT,
-  ST_MOVE,        // spawn a ray from the scatter point and move it to the voxel border
-  ST_FINISH,
+  ST_IDLE = 0,  // no more work
+  ST_FETCH,     // needs a work item (handled wave-wide)
+  ST_SHADE,     // at a vertex
+  ST_SEG,       // walking: at the start of a segment
+  ST_TRAV,      // walking: inside a traversal
 };
-
+// roles of a vertex's streams; the rays are walked in this order
+enum : int { RL_VERTEX = 0, RL_LIGHT = 1, RL_PHASE = 2, RL_CONT = 3 };
+// how the ray that arrived at a vertex ended (set by the continuation walk)
+enum : int { CR_START = 0, CR_NONE, CR_END, CR_ESCAPED, CR_COLLIDED };
 enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO = 4 };
+// ColdLane::flags
+enum : uint32_t { FL_LIGHT = 1u, FL_PHASE = 2u, FL_CONT = 4u, FL_NEE = 8u, FL_RR = 16u };
+// cap on the segments of one intersectTr walk, which the reference loops until a
+// miss: a segment enters or leaves an instance box, so scenes of kMaxInstances
+// boxes never reach it (the oracle uses the same cap)
+constexpr int kShadowSegCap = 64;
 
-// Path state touched only at the start and end of a path and in shading events
-// lives in LDS, one slot per lane (51 dwords: an odd stride, so the 64 lanes of
-// a wavefront fall on distinct banks; 3 blocks of 256 lanes fit a CU's 160 KB);
-// the state the traversal loop touches stays in registers.  This keeps the
-// kernel within the register budget of three wavefronts per SIMD without
-// spilling to scratch.
+// Path state that is touched only in shading passes and at walk boundaries
+// lives in LDS, one slot per lane; the state the walk steps touch stays in
+// registers.  47 dwords: an odd stride, so the 64 lanes of a wavefront fall on
+// distinct banks, and three blocks of 256 lanes (141 KB) fit a CU's LDS.
 struct ColdLane {
   uint32_t pix;
-  int32_t sh_next, sh_seg;  // shadow ray walk: medium after the segment, segments walked
   int32_t s_cur, s_end;
   uint32_t item;
-  float ax, ay, az;     // FP32 item sum (same order as the reference's framebuffer +=)
-  v3 Ld, pdir, pLe;     // pending NEE sample (light direction / phase direction, radiance)
-  float ppdf;
-  v3 mi_pos, mi_n;      // scatter event
-  float S[6];
-  float mi_ior;
-  int32_t mi_inst;
-  v3 sh_p0, sh_p1, si_pos;
+  float ax, ay, az;  // FP32 item sum (the reference's framebuffer += order)
+  uint32_t ring[8];  // random words of the current stream (PathRng)
+  // rays of the current vertex waiting for their walk (the light-sample shadow
+  // ray, walked first, goes straight to the walk registers): the phase-sample
+  // shadow ray as segment end points and medium (intersectTr's p0, p1,
+  // currentMedium), the continuation as a ray
+  v3 lb0, lb1;
+  int32_t lb_med;
+  v3 c_o, c_d;
+  float c_tmax;
+  int32_t c_med;
+  // direct light of the vertex per strategy: Le * (mis * p), then times the
+  // shadow walk's transmittance over the strategy's pdf once the walk ends
+  v3 nee_a;
+  float pdf_a;
+  v3 nee_b;
+  float pdf_b;
+  float rr_div;    // 1 - q of the vertex's Russian roulette (FL_RR)
+  uint32_t flags;  // FL_*: rays of the vertex, direct light pending, roulette applied
+  v3 L, thr;       // radiance and throughput of the path (touched only when shading)
   uint32_t pad;
-  uint32_t ring[8];     // random words of the path (PathRng)
 };
-static_assert(sizeof(ColdLane) == 51 * 4, "ColdLane must keep an odd dword stride");
+static_assert(sizeof(ColdLane) == 47 * 4, "ColdLane must keep an odd dword stride");
 
-// The random stream of one path: Philox4x32-10 keyed (seed, stream), counter
-// (pixel, sample, block, 0); draw k is word k & 3 of block k >> 2, the same
-// stream as Rng.  Words are read from an 8-word ring in LDS that holds blocks
-// b and b + 1 (block b at words 4 (b & 1) ..).  top_up() computes the next block
-// ahead of use at points that the whole wavefront passes (the top of an event
-// pass, of a traversal round), so that after it every lane has at least five
-// words in hand; a draw computes a block itself only when its lane has run out.
-// A wavefront therefore runs about one block computation per pass or round,
-// where a lazily refilled buffer runs one at every draw site at which any of
-// its lanes happens to cross into a new block.
+// The random stream of one (path, vertex, role): Philox4x32-10 keyed (seed,
+// stream), counter (pixel, sample, block, 4 k + role); draw j is word j & 3 of
+// block j >> 2.  Words are read from an 8-word ring in LDS holding blocks b and
+// b + 1 (block b at words 4 (b & 1) ..).  top_up() computes blocks ahead of use at
+// points that all lanes of a pass or step reach together, so that after it every
+// lane has at least R2_RNG_AHEAD words in hand; a draw computes a block itself
+// only when its lane has run out.  init() computes nothing: the first top_up()
+// or draw does.
+#ifndef R2_RNG_AHEAD
+#define R2_RNG_AHEAD 5
+#endif
+static_assert(R2_RNG_AHEAD >= 1 && R2_RNG_AHEAD <= 5, "the ring holds two blocks");
 struct PathRng {
   const uint32_t k0, k1;  // launch-uniform: kept in scalar registers
-  uint32_t c0, c1;
+  uint32_t c0, c1, c3;
   uint32_t n;   // words drawn
-  uint32_t hi;  // blocks 0 .. hi - 1 computed; the ring holds the last two of them (block 0 alone after init)
+  uint32_t hi;  // blocks 0 .. hi - 1 computed; the ring holds the last two of them
   uint32_t (&ring)[8];
   __device__ PathRng(uint32_t (&r)[8], uint32_t seed, uint32_t stream) : k0(seed), k1(stream), ring(r) {}
   __device__ __forceinline__ void block(uint32_t b) {
-    uint32_t c[4] = {c0, c1, b, 0u};
+    uint32_t c[4] = {c0, c1, b, c3};
     philox4x32_10(c, k0, k1);
     const uint32_t o = (b & 1u) * 4u;
     ring[o] = c[0];
@@ -87,16 +112,16 @@ struct PathRng {
     ring[o + 2] = c[2];
     ring[o + 3] = c[3];
   }
-  __device__ __forceinline__ void init(uint32_t ctr0, uint32_t ctr1) {
+  __device__ __forceinline__ void init(uint32_t ctr0, uint32_t ctr1, uint32_t sub) {
     c0 = ctr0;
     c1 = ctr1;
+    c3 = sub;
     n = 0;
-    block(0);
-    hi = 1;
+    hi = 0;
   }
-  // make words n .. 4 (n / 4 + 2) - 1 available
+  // make at least the next R2_RNG_AHEAD words (at most 5) available
   __device__ __forceinline__ void top_up() {
-    const uint32_t want = (n >> 2) + 2u;
+    const uint32_t want = (n + (uint32_t)R2_RNG_AHEAD + 3u) >> 2;
     if (hi < want) {
       if (hi + 1u < want) block(want - 2u);
       block(want - 1u);
@@ -114,50 +139,44 @@ struct PathRng {
 };
 
 struct PathLane {
+  ColdLane& c;
   // work item (LDS)
   uint32_t& pix;
   int32_t &s_cur, &s_end;
   uint32_t& item;
   float &ax, &ay, &az;
   double dax, day, daz;  // FP64 item sum (compensated framebuffer form)
-  // path
   PathRng rng;
-  int stage, b, iters;
-  v3 L, thr;
-  v3& Ld;
+  int stage, role, cres;
+  int b, k, iters;
+  v3 &L, &thr;
+  // continuation walk: the ray of the bounce loop, and its last scene
+  // intersection.  Shadow walks keep intersectTr's p0 in ray.o, p1 in ray.d and
+  // currentMedium in ray.medium, the segment's hit flag in si_hit.
   Ray ray;
-  v3& si_pos;
+  v3 si_pos;
   int si_medium, si_hit;
-  // scatter event (LDS)
-  v3 &mi_pos, &mi_n;
-  float (&S)[6];
-  float& mi_ior;
-  int32_t& mi_inst;
-  // pending NEE sample (LDS)
-  v3 &pdir, &pLe;
-  float& ppdf;
-  // shadow ray walk (intersectTr); sh_ret is also the state that follows ST_MOVE
-  v3 &sh_p0, &sh_p1;
-  int32_t &sh_next, &sh_seg;
-  int sh_cur, sh_hit, sh_ret;
+  int sh_seg;
   float sh_Tr;
   // traversal
   int tk, tinst, tsteps, tres;
   v3 to, td, tri;
   float tt, ttau, tmip, ttmax, ttr;
 
-  __device__ PathLane(ColdLane& c, const DScene& s)
-      : pix(c.pix), s_cur(c.s_cur), s_end(c.s_end), item(c.item), ax(c.ax), ay(c.ay), az(c.az),
-        rng(c.ring, s.seed, s.stream),
-        Ld(c.Ld), si_pos(c.si_pos), mi_pos(c.mi_pos), mi_n(c.mi_n), S(c.S), mi_ior(c.mi_ior), mi_inst(c.mi_inst),
-        pdir(c.pdir), pLe(c.pLe), ppdf(c.ppdf), sh_p0(c.sh_p0), sh_p1(c.sh_p1), sh_next(c.sh_next),
-        sh_seg(c.sh_seg) {}
+  __device__ PathLane(ColdLane& cl, const DScene& s)
+      : c(cl), pix(cl.pix), s_cur(cl.s_cur), s_end(cl.s_end), item(cl.item), ax(cl.ax), ay(cl.ay), az(cl.az),
+        rng(cl.ring, s.seed, s.stream), L(cl.L), thr(cl.thr) {}
 };
 
-template <bool kSingle>
-__device__ __forceinline__ int lane_grid(const DScene& s, int inst) {
-  return kSingle ? s.inst[0].grid : s.inst[inst].grid;
-}
+// scatter record of a collision (CMediumInstance::sample's SMediumInteraction)
+struct Scatter {
+  v3 pos, n;
+  float S[6];
+  float ior;
+  int inst;
+};
+
+__device__ __forceinline__ bool any_zero(v3 a) { return a.x == 0.f || a.y == 0.f || a.z == 0.f; }
 
 // --------------------------------------------------------------- traversal --
 // Prologue of sampleDDA / sampleInternal / trDDA / trInternal on instance k.
@@ -265,9 +284,9 @@ __device__ __forceinline__ bool traversal_step(const DScene& s, PathLane& p, Cnt
 }
 
 // Collision tail of sampleDDA / sampleInternal + CMediumInstance::sample's
-// transform back to world space; returns the albedo, fills the scatter record.
+// transform back to world space: returns the albedo, fills the scatter record.
 template <bool kSingle, bool kCount>
-__device__ __forceinline__ v3 flight_collision(const DScene& s, PathLane& p, Cnt& cnt) {
+__device__ __forceinline__ v3 flight_collision(const DScene& s, const PathLane& p, Scatter& sc, Cnt& cnt) {
   const int ki = kSingle ? 0 : p.tinst;
   const DInstance& in = s.inst[ki];
   const DGrid& g = s.grid[in.grid];
@@ -287,12 +306,12 @@ __device__ __forceinline__ v3 flight_collision(const DScene& s, PathLane& p, Cnt
   }
   MHit h;
   const v3 alb = collision_albedo(g, fd, rw, h);
-  p.mi_inst = p.tinst;
-  p.mi_pos = mul_point(in.m2w, h.pos);
-  p.mi_n = normalize(mul_dir(in.m2w, h.normal));
+  sc.inst = p.tinst;
+  sc.pos = mul_point(in.m2w, h.pos);
+  sc.n = normalize(mul_dir(in.m2w, h.normal));
 #pragma unroll
-  for (int i = 0; i < 6; ++i) p.S[i] = h.S[i];
-  p.mi_ior = h.ior;
+  for (int i = 0; i < 6; ++i) sc.S[i] = h.S[i];
+  sc.ior = h.ior;
   count<kCount>(cnt, kCntScatter);
   return alb;
 }
@@ -302,223 +321,226 @@ __device__ __forceinline__ Ray lane_move_to_voxel_border(const DScene& s, int k,
   return inst_move_to_voxel_border(s, kSingle ? 0 : k, r);
 }
 
-__device__ __forceinline__ void start_shadow(PathLane& p, const Ray& r, int ret) {
-  p.sh_Tr = 1.f;
-  p.sh_p0 = r.o;
-  p.sh_p1 = r.o + r.tmax * r.d;
-  p.sh_cur = r.medium;
+// a ray spawned at a scatter point along dir, its origin offset along `off` (the
+// voxel normal for shadow rays, the unnormalised sample for the continuation),
+// moved to the voxel border (the tails of direct() and of Li's bounce)
+template <bool kSingle>
+__device__ __forceinline__ Ray spawn_at_scatter(const DScene& s, const Scatter& sc, v3 dir, v3 off) {
+  Ray r = mkray(sc.pos, dir, kDefaultTmax, sc.inst);
+  ray_offset(r, off);
+  return lane_move_to_voxel_border<kSingle>(s, sc.inst, r);
+}
+
+// -------------------------------------------------------------------- walk --
+// Begin the walk of the vertex's ray `role`: the light-sample shadow ray is in
+// the walk registers already, the others wait in the lane's LDS slots.
+__device__ __forceinline__ void walk_begin(PathLane& p, int role) {
+  p.role = role;
+  p.rng.init(p.rng.c0, p.rng.c1, 4u * (uint32_t)p.k + (uint32_t)role);
+  p.stage = ST_SEG;
+  if (role == RL_CONT) {
+    p.ray = mkray(p.c.c_o, p.c.c_d, p.c.c_tmax, p.c.c_med);
+    return;
+  }
+  if (role == RL_PHASE) {
+    p.ray.o = p.c.lb0;
+    p.ray.d = p.c.lb1;
+    p.ray.medium = p.c.lb_med;
+  }
   p.sh_seg = 0;
-  p.sh_ret = ret;
-  p.stage = ST_SH_SEG;
+  p.sh_Tr = 1.f;
+}
+// walk the vertex's next ray after `after`, or go back to shading
+__device__ __forceinline__ void walk_next(PathLane& p, int after) {
+  const uint32_t f = p.c.flags;
+  if (after < RL_LIGHT && (f & FL_LIGHT)) walk_begin(p, RL_LIGHT);
+  else if (after < RL_PHASE && (f & FL_PHASE)) walk_begin(p, RL_PHASE);
+  else if (after < RL_CONT && (f & FL_CONT)) walk_begin(p, RL_CONT);
+  else p.stage = ST_SHADE;
 }
 
-// One transition of a lane that is not traversing, not fetching and not idle.
-// `st` is the state to run (the lane's stage); passing a wave-uniform value lets
-// the switch compile to one uniform branch.
-template <bool kSingle, bool kCount>
-__device__ __forceinline__ void event_step(const DScene& s, PathLane& p, Cnt& cnt, int st) {
+// end of a shadow walk: the strategy's share of direct(),
+// ((Le * (mis * p)) * Tr) / pdf, with Tr = 0 when the walk hit a surface
+__device__ __forceinline__ void shadow_done(PathLane& p) {
+  const float tr = p.si_hit ? 0.f : p.sh_Tr;
+  if (p.role == RL_LIGHT) p.c.nee_a = (p.c.nee_a * tr) / p.c.pdf_a;
+  else p.c.nee_b = (p.c.nee_b * tr) / p.c.pdf_b;
+  walk_next(p, p.role);
+}
+
+// one intersectTr segment: scene intersection, then the transmittance traversal
+// of the segment if it lies inside a medium
+template <bool kSingle>
+__device__ __forceinline__ void shadow_segment(const DScene& s, PathLane& p) {
   const bool dda = (s.flags & 1u) != 0u;
-  switch (st) {
-    case ST_START: {
-      p.rng.init(p.pix, (uint32_t)(s.sample_begin + p.s_cur));
-      p.ray = camera_ray(s, (int)(p.pix % (uint32_t)s.W), (int)(p.pix / (uint32_t)s.W), p.rng);
-      p.L = mk(0.f, 0.f, 0.f);
-      p.thr = mk(1.f, 1.f, 1.f);
-      p.b = 0;
-      p.iters = 0;
-      p.stage = ST_BOUNCE;
-      break;
-    }
-    case ST_BOUNCE: {
-      // also runs the lanes in ST_SH_SEG: both intersect the scene and may start a
-      // traversal (Li's free flight, or the transmittance of one intersectTr
-      // segment), so a wavefront runs that code once for both
-      const bool sh = p.stage == ST_SH_SEG;
-      if (sh ? ++p.sh_seg > 64 : (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64)) {
-        if (sh) {
-          p.sh_hit = 1;
-          p.stage = p.sh_ret;
-        } else {
-          p.stage = ST_FINISH;
-        }
-        break;
-      }
-      Ray r = sh ? spawn_ray(p.sh_p0, p.sh_p1, p.sh_cur) : p.ray;
-      const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
-      Ray fr = r;
-      int next;
-      if (sh) {
-        p.sh_hit = h.hit;
-        p.sh_next = r.medium < 0 ? h.medium : -1;
-        p.sh_p0 = h.pos;
-        next = (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == p.sh_cur)) ? ST_TRAV : ST_SH_NEXT;
-      } else {
-        p.ray = r;
-        p.si_pos = h.pos;
-        p.si_medium = h.medium;
-        p.si_hit = h.hit;
-        p.tres = 0;
-        p.tk = 0;
-        // the free flight starts on the current ray (inside a medium) or on the ray
-        // entering the medium that was hit
-        next = ST_NOHIT;
-        if (h.medium >= 0) {
-          if (r.medium >= 0) {
-            next = h.medium != r.medium ? ST_FINISH : ST_TRAV;
-          } else {
-            Ray mr = mkray(h.pos, normalize(r.d), kDefaultTmax, h.medium);
-            ray_offset(mr, r.d);
-            const Hit se = scene_intersect(s, mr);
-            next = !se.hit ? ST_AFTER_SAMPLE : ((se.medium >= 0 && h.medium != se.medium) ? ST_FINISH : ST_TRAV);
-            fr = mr;
-          }
-        }
-      }
-      if (next == ST_TRAV)
-        start_traversal<kSingle>(s, p, fr.medium, fr,
-                                 sh ? (dda ? TK_TR_DDA : TK_TR_RATIO) : (dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA));
-      else
-        p.stage = next;
-      break;
-    }
-    case ST_AFTER_SAMPLE: {
-      const bool collided = p.tres != 0;
-      if (collided) p.thr = p.thr * flight_collision<kSingle, kCount>(s, p, cnt);
-      p.tres = 0;
-      if (p.thr.x == 0.f || p.thr.y == 0.f || p.thr.z == 0.f) {
-        p.stage = ST_FINISH;
-        break;
-      }
-      p.stage = collided ? ST_SCATTER : ST_NOHIT;
-      break;
-    }
-    case ST_NOHIT: {
-      // also runs the lanes in ST_BRDF_DONE: both look up the environment (the
-      // radiance of an escaped camera ray, or the light along direct()'s phase
-      // sample), so a wavefront runs that lookup once for both
-      const bool brdf = p.stage == ST_BRDF_DONE;
-      const bool escaped = !brdf && p.b == 0 && !p.si_hit;
-      float lpdf = 0.f;
-      v3 Le = mk(0.f, 0.f, 0.f);
-      if (brdf || escaped) Le = env_le<kCount>(s.env, brdf ? p.pdir : p.ray.d, lpdf, cnt);
-      if (brdf) {
-        const float ph = p.ppdf;
-        const float mis = (1.f * ph) / (1.f * ph + 1.f * lpdf);
-        const float tr = p.sh_hit ? 0.f : p.sh_Tr;
-        p.Ld = p.Ld + ((Le * (mis * ph)) * tr) / ph;
-        p.stage = ST_DIRECT_DONE;
-        break;
-      }
-      if (escaped) p.L = p.L + Le * p.thr;
-      if (!p.si_hit) {
-        p.stage = ST_FINISH;
-        break;
-      }
-      Ray nr = mkray(p.si_pos, normalize(p.ray.d), kDefaultTmax, p.ray.medium < 0 ? p.si_medium : -1);
-      ray_offset(nr, p.ray.d);
-      p.ray = nr;
-      p.stage = ST_BOUNCE;
-      break;
-    }
-    case ST_SCATTER: {
-      p.Ld = mk(0.f, 0.f, 0.f);
-      v3 ldir;
-      float lpdf = 0.f;
-      const v3 Le = env_sample<kCount>(s.env, p.rng, ldir, lpdf, cnt);
-      if (lpdf > 0.f) {
-        p.pdir = normalize(ldir);
-        p.pLe = Le;
-        p.ppdf = lpdf;
-        p.sh_ret = ST_LIGHT_DONE;  // shadow ray toward the light sample (ST_MOVE)
-        p.stage = ST_MOVE;
-      } else {
-        p.stage = ST_BRDF_SAMPLE;
-      }
-      break;
-    }
-    case ST_LIGHT_DONE: {
-      const v3 wo = -p.ray.d;
-      const float ph = phase_p(p.S, p.mi_n, p.mi_ior, wo, p.pdir, p.rng);
-      const float mis = (1.f * p.ppdf) / (1.f * p.ppdf + 1.f * ph);
-      const float tr = p.sh_hit ? 0.f : p.sh_Tr;
-      p.Ld = p.Ld + ((p.pLe * (mis * ph)) * tr) / p.ppdf;
-      p.stage = ST_BRDF_SAMPLE;
-      break;
-    }
-    case ST_BRDF_SAMPLE: {
-      // also runs the lanes in ST_DIRECT_DONE: both start with a phase sample at
-      // the scatter point (direct()'s phase-sampled light direction, then Li's
-      // continuation direction), so a wavefront runs that code once for both
-      const bool cont = p.stage == ST_DIRECT_DONE;
-      if (cont) p.L = p.L + p.thr * p.Ld;
-      const v3 wo = -p.ray.d;
-      v3 wi;
-      const float ph = phase_sample(p.S, p.mi_n, p.mi_ior, wo, wi, p.rng);
-      if (!cont) {
-        if (ph > 0.f) {
-          p.pdir = normalize(wi);
-          p.ppdf = ph;
-          p.sh_ret = ST_BRDF_DONE;  // shadow ray along the phase sample (ST_MOVE)
-          p.stage = ST_MOVE;
-        } else {
-          p.stage = ST_DIRECT_DONE;
-        }
-        break;
-      }
-      p.pdir = normalize(wi);
-      p.Ld = wi;  // the continuation ray is offset along the unnormalised sample
-      if (p.b > 3) {
-        const float q = 1.f - (p.thr.x + p.thr.y + p.thr.z) / 3.f;
-        if (p.rng.uni() <= q) {
-          p.stage = ST_FINISH;
-          break;
-        }
-        p.thr = p.thr / (1.f - q);
-      }
-      ++p.b;
-      p.sh_ret = ST_BOUNCE;
-      p.stage = ST_MOVE;
-      break;
-    }
-    case ST_MOVE: {
-      // the ray from the scatter point along pdir, offset along the normal (shadow
-      // rays) or along the unnormalised direction sample (continuation), moved to
-      // the voxel border: the tails of direct() and of Li's bounce
-      const bool cont = p.sh_ret == ST_BOUNCE;
-      Ray r = mkray(p.mi_pos, p.pdir, kDefaultTmax, p.mi_inst);
-      ray_offset(r, cont ? p.Ld : p.mi_n);
-      r = lane_move_to_voxel_border<kSingle>(s, p.mi_inst, r);
-      if (cont) {
-        p.ray = r;
-        p.stage = ST_BOUNCE;
-      } else {
-        start_shadow(p, r, p.sh_ret);
-      }
-      break;
-    }
-    case ST_SH_NEXT: {
-      if (!p.sh_hit) {
-        p.stage = p.sh_ret;
-        break;
-      }
-      p.sh_cur = p.sh_next;
-      p.stage = ST_SH_SEG;
-      break;
-    }
-    default:
-      break;
+  if (++p.sh_seg > kShadowSegCap) {
+    p.si_hit = 1;
+    shadow_done(p);
+    return;
   }
+  const int cur = p.ray.medium;
+  Ray r = spawn_ray(p.ray.o, p.ray.d, cur);
+  const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit
+  p.si_hit = h.hit;
+  p.ray.o = h.pos;
+  p.ray.medium = r.medium < 0 ? h.medium : -1;
+  if (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur)) {
+    start_traversal<kSingle>(s, p, r.medium, r, dda ? TK_TR_DDA : TK_TR_RATIO);
+    return;
+  }
+  if (!h.hit) shadow_done(p);
 }
 
-// Called when a traversal ended: hand the result to the owning stage.
-__device__ __forceinline__ void end_traversal(PathLane& p) {
-  if (p.tk == TK_TR_DDA || p.tk == TK_TR_RATIO) {
-    p.sh_Tr *= p.ttr;
-    p.stage = ST_SH_NEXT;
+__device__ __forceinline__ void cont_done(PathLane& p, int cr) {
+  p.cres = cr;
+  p.stage = ST_SHADE;  // the continuation is the vertex's last ray
+}
+// the bounce loop's no-scatter branch: the environment for an escaped camera
+// ray (looked up by the next shading pass), the end of the path on a miss,
+// else a pass-through: the ray continues from the box it hit without a bounce
+__device__ __forceinline__ void cont_nohit(PathLane& p) {
+  if (!p.si_hit) {
+    cont_done(p, p.b == 0 ? CR_ESCAPED : CR_END);
+    return;
+  }
+  Ray nr = mkray(p.si_pos, normalize(p.ray.d), kDefaultTmax, p.ray.medium < 0 ? p.si_medium : -1);
+  ray_offset(nr, p.ray.d);
+  p.ray = nr;
+  p.stage = ST_SEG;
+}
+// top of Li's bounce loop: depth and iteration caps, scene intersection, and
+// the start of the free flight on the current ray (inside a medium) or on the
+// ray entering the medium that was hit (the entry segment is sampled again from
+// inside on the pass-through that follows a miss: the reference's double
+// tracking, path_integrator_impl.hpp:133-160,188-191)
+template <bool kSingle>
+__device__ __forceinline__ void cont_segment(const DScene& s, PathLane& p) {
+  const bool dda = (s.flags & 1u) != 0u;
+  if (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64) {
+    cont_done(p, CR_END);
+    return;
+  }
+  Ray r = p.ray;
+  const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
+  p.ray = r;
+  p.si_pos = h.pos;
+  p.si_medium = h.medium;
+  p.si_hit = h.hit;
+  if (h.medium < 0) {
+    cont_nohit(p);
+    return;
+  }
+  if (r.medium >= 0) {
+    if (h.medium != r.medium) cont_done(p, CR_END);
+    else start_traversal<kSingle>(s, p, r.medium, r, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
+    return;
+  }
+  Ray mr = mkray(h.pos, normalize(r.d), kDefaultTmax, h.medium);
+  ray_offset(mr, r.d);
+  const Hit se = scene_intersect(s, mr);
+  if (!se.hit) cont_nohit(p);
+  else if (se.medium >= 0 && h.medium != se.medium) cont_done(p, CR_END);
+  else start_traversal<kSingle>(s, p, mr.medium, mr, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
+}
+
+// A lane at the start of a segment (ST_SEG): the continuation's bounce
+// top or one intersectTr segment.  Either starts a traversal or ends the ray.
+template <bool kSingle>
+__device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
+  if (p.role == RL_CONT) cont_segment<kSingle>(s, p);
+  else shadow_segment<kSingle>(s, p);
+}
+
+// A lane inside a traversal (ST_TRAV): one DDA / tracking step; at the end
+// of the traversal the continuation has collided (next vertex) or leaves the
+// medium box, and a shadow ray's segment ends at a box face.
+template <bool kSingle, bool kCount>
+__device__ __forceinline__ void trav_step(const DScene& s, PathLane& p, Cnt& cnt) {
+  if (!traversal_step<kSingle, kCount>(s, p, cnt)) return;
+  if (p.role == RL_CONT) {
+    if (p.tres) cont_done(p, CR_COLLIDED);
+    else cont_nohit(p);
   } else {
-    p.stage = ST_AFTER_SAMPLE;
+    p.sh_Tr *= p.ttr;  // intersectTr continues from the box face
+    p.stage = ST_SEG;
   }
 }
 
+// ------------------------------------------------------------------- shade --
+// The scatter vertex k (k >= 1) of a path at a collision: direct()'s two
+// strategies up to their shadow walks, Li's continuation and Russian roulette,
+// in the reference's order of draws from the vertex's own stream.
+template <bool kSingle, bool kCount>
+__device__ __forceinline__ void shade_scatter(const DScene& s, PathLane& p, const Scatter& sc, Cnt& cnt) {
+  ++p.k;
+  p.rng.init(p.rng.c0, p.rng.c1, 4u * (uint32_t)p.k);
+  p.rng.top_up();
+  uint32_t flags = FL_NEE;
+  const v3 wo = -p.ray.d;
+  // direct(): light sample, its shadow ray and the phase function toward it
+  v3 ldir;
+  float lpdf = 0.f;
+  const v3 Le = env_sample<kCount>(s.env, p.rng, ldir, lpdf, cnt);
+  if (lpdf > 0.f) {
+    const v3 dl = normalize(ldir);
+    const Ray r = spawn_at_scatter<kSingle>(s, sc, dl, sc.n);
+    p.ray.o = r.o;  // walked first: straight to the walk registers (wo is taken)
+    p.ray.d = r.o + r.tmax * r.d;
+    p.ray.medium = r.medium;
+    const float ph = phase_p(sc.S, sc.n, sc.ior, wo, dl, p.rng);
+    const float mis = (1.f * lpdf) / (1.f * lpdf + 1.f * ph);
+    p.c.nee_a = Le * (mis * ph);
+    p.c.pdf_a = lpdf;
+    flags |= FL_LIGHT;
+  }
+  p.rng.top_up();
+  // direct(): phase sample, its shadow ray and the environment along it
+  {
+    v3 wi;
+    const float ph = phase_sample(sc.S, sc.n, sc.ior, wo, wi, p.rng);
+    if (ph > 0.f) {
+      const v3 dw = normalize(wi);
+      const Ray r = spawn_at_scatter<kSingle>(s, sc, dw, sc.n);
+      p.c.lb0 = r.o;
+      p.c.lb1 = r.o + r.tmax * r.d;
+      p.c.lb_med = r.medium;
+      float epdf = 0.f;
+      const v3 Le2 = env_le<kCount>(s.env, dw, epdf, cnt);
+      const float mis = (1.f * ph) / (1.f * ph + 1.f * epdf);
+      p.c.nee_b = Le2 * (mis * ph);
+      p.c.pdf_b = ph;
+      flags |= FL_PHASE;
+    }
+  }
+  p.rng.top_up();
+  // Li: continuation direction; its ray is offset along the unnormalised sample
+  {
+    v3 wi;
+    (void)phase_sample(sc.S, sc.n, sc.ior, wo, wi, p.rng);
+    const Ray r = spawn_at_scatter<kSingle>(s, sc, normalize(wi), wi);
+    p.c.c_o = r.o;
+    p.c.c_d = r.d;
+    p.c.c_tmax = r.tmax;
+    p.c.c_med = r.medium;
+    flags |= FL_CONT;
+  }
+  // Russian roulette (the throughput's division waits until the vertex's
+  // direct light, which uses the throughput before it, has been added)
+  if (p.b > 3) {
+    const float q = 1.f - (p.thr.x + p.thr.y + p.thr.z) / 3.f;
+    if (p.rng.uni() <= q) {
+      flags &= ~FL_CONT;
+    } else {
+      p.c.rr_div = 1.f - q;
+      flags |= FL_RR;
+    }
+  }
+  ++p.b;
+  // the bounce loop's depth test: a continuation past max_depth is never walked
+  if (!(p.b < s.max_depth)) flags &= ~FL_CONT;
+  p.c.flags = flags;
+  p.cres = CR_NONE;
+  walk_next(p, RL_VERTEX);
+}
 
 }  // namespace r2

@@ -201,12 +201,11 @@ int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp
   {
     unsigned long long st_host[64];
     R2_HIP(ctx, hipMemcpyFromSymbol(st_host, HIP_SYMBOL(r2k_fp32::g_sm_stats), sizeof(st_host)));
-    fprintf(stderr, "[sm-stats] paths %llu outer %llu ev_iters %llu trav_rounds %llu trav_lanes %llu\n",
-            (unsigned long long)(npix * scene.spp), st_host[0], st_host[1], st_host[2], st_host[3]);
-    fprintf(stderr, "[sm-stats] wave cycles: events %llu traversal %llu\n", st_host[4], st_host[5]);
-    for (int k = ST_START; k <= ST_FINISH; ++k)
-      fprintf(stderr, "[sm-stats] state %d: execs %llu lanes %llu cycles %llu\n", k, st_host[16 + 2 * k],
-              st_host[17 + 2 * k], st_host[48 + k]);
+    fprintf(stderr,
+            "[sm-stats] paths %llu steps %llu | shade %llu passes %llu lanes %llu cyc | seg %llu passes %llu lanes "
+            "%llu cyc | trav %llu steps %llu lanes %llu cyc\n",
+            (unsigned long long)(npix * scene.spp), st_host[0], st_host[1], st_host[2], st_host[5], st_host[3],
+            st_host[4], st_host[6], st_host[7], st_host[8], st_host[9]);
     const unsigned long long z[64] = {0};
     R2_HIP(ctx, hipMemcpyToSymbol(HIP_SYMBOL(r2k_fp32::g_sm_stats), z, sizeof(z)));
   }

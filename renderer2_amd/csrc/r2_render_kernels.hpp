@@ -31,27 +31,15 @@ __device__ __forceinline__ void item_pixel(uint32_t q, int W, int H, int& x, int
   }
 }
 
-// Persistent path state machine (see r2_pathsm.hpp).  Render mode: item =
+// Persistent two-phase path program (see r2_pathsm.hpp).  Render mode: item =
 // (sample chunk, pixel), output = chunk sums.  Probe mode (probe_pix != null):
 // item = one (pixel, sample) path, output = its radiance and lookup counters.
-#ifndef R2_EV_ITERS
-#define R2_EV_ITERS 12    // event passes per scheduling round
-#endif
-#ifndef R2_TR_MIN_LANES
-#define R2_TR_MIN_LANES 16  // leave the traversal round when fewer lanes are still traversing
-#endif
-#ifndef R2_RNG_TRAV_TOPUP
-#define R2_RNG_TRAV_TOPUP 1  // top up the random words of traversing lanes every traversal round
-#endif
-#ifndef R2_TR_MIN_ROUNDS
-#define R2_TR_MIN_ROUNDS 8
-#endif
 #ifdef R2_SM_STATS
 // debug-build scheduling statistics, kept per wavefront in LDS and added to the
-// global table once when the wavefront exits: [0] outer iterations, [1] event
-// iterations, [2] traversal rounds, [3] lanes active in traversal rounds,
-// [4] / [5] wave cycles in event passes / traversal rounds, [16+2s] executions
-// of state s, [17+2s] lanes in those executions, [48+s] wave cycles in state s
+// global table once when the wavefront exits: [0] scheduling steps, [1] / [2]
+// shading passes and their lanes, [3] / [4] segment passes and their lanes,
+// [7] / [8] traversal steps and their lanes, [5] / [6] / [9] wave cycles in
+// shading / segment / traversal steps
 __device__ unsigned long long g_sm_stats[64];
 __device__ __forceinline__ unsigned long long* wave_stats() {
   __shared__ unsigned long long st[4][64];
@@ -67,20 +55,18 @@ __device__ __forceinline__ void stat_add(int i, uint64_t v) {
   if ((threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) wave_stats()[i] += v;
 }
 #define SM_STAT(i, v) stat_add((i), (uint64_t)(v))
-__device__ __forceinline__ void stat_cycles(int i, uint64_t c) { stat_add(i, c); }
-__device__ __forceinline__ void stat_state(int st) {
-  stat_add(16 + 2 * st, 1);
-  stat_add(17 + 2 * st, __popcll(__ballot(true)));
-}
+#define SM_CLOCK() clock64()
 #else
 #define SM_STAT(i, v) do {} while (0)
-__device__ __forceinline__ void stat_state(int) {}
+#define SM_CLOCK() 0ull
 __device__ __forceinline__ void stats_begin() {}
 __device__ __forceinline__ void stats_end() {}
 #endif
-// end of one path: radiance, NaN guard, accumulation into the item or probe output
+
+// end of one path: NaN guard, radiance / spp, accumulation into the item (or the
+// probe output); the lane starts the item's next sample, or fetches a new item
 template <typename Acc>
-__device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, int chunk,
+__device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix,
                                             Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
                                             float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
   v3 L = p.L;
@@ -110,7 +96,7 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
   }
   ++p.s_cur;
   if (p.s_cur < p.s_end) {
-    p.stage = ST_START;
+    p.cres = CR_START;
   } else {
     const uint32_t c = (uint32_t)(p.item / npix);
     Acc* o = partial + ((size_t)c * npix + p.pix) * 3;
@@ -127,93 +113,168 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
   }
 }
 
-// Run state kSt for the lanes that are in it, or in kSt2 when the code of kSt
-// also serves kSt2 (a divergent branch the wave skips when no lane is).
-template <bool kSingle, bool kCount, int kSt, int kSt2 = -1>
-__device__ __forceinline__ void event_if(const DScene& s, PathLane& p, Cnt& cnt) {
-  if (p.stage == kSt || p.stage == kSt2) {
-#ifdef R2_SM_STATS
-    stat_state(kSt);
-    const uint64_t t0 = clock64();
-    event_step<kSingle, kCount>(s, p, cnt, kSt);
-    stat_cycles(48 + kSt, clock64() - t0);
-#else
-    event_step<kSingle, kCount>(s, p, cnt, kSt);
-#endif
+// One shading pass of a lane at a vertex (ST_SHADE): the direct light of the
+// previous scatter vertex, then the end of the ray that arrived (a collision
+// becomes the next scatter vertex; an escaped camera ray sees the environment;
+// anything else ends the path), and for a path that ends the start of the
+// item's next sample (vertex 0: the camera ray).
+template <bool kSingle, bool kCount, typename Acc>
+__device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, Acc* __restrict__ partial,
+                                      const int32_t* __restrict__ probe_pix, float* __restrict__ probe_out,
+                                      uint32_t* __restrict__ probe_cnt) {
+  const uint32_t fl = p.c.flags;
+  if (fl & FL_NEE) {
+    v3 Ld = mk(0.f, 0.f, 0.f);
+    if (fl & FL_LIGHT) Ld = Ld + p.c.nee_a;
+    if (fl & FL_PHASE) Ld = Ld + p.c.nee_b;
+    p.L = p.L + p.thr * Ld;
+    if (fl & FL_RR) p.thr = p.thr / p.c.rr_div;
+    p.c.flags = 0u;
   }
+  bool scatter = false;
+  Scatter sc;
+  if (p.cres == CR_COLLIDED) {
+    p.thr = p.thr * flight_collision<kSingle, kCount>(s, p, sc, cnt);
+    scatter = !any_zero(p.thr);
+  } else if (p.cres == CR_ESCAPED) {
+    float lpdf = 0.f;
+    const v3 Le = env_le<kCount>(s.env, p.ray.d, lpdf, cnt);
+    p.L = p.L + Le * p.thr;
+  }
+  if (scatter) {
+    shade_scatter<kSingle, kCount>(s, p, sc, cnt);
+    return;
+  }
+  if (p.cres != CR_START) {
+    finish_path<Acc>(s, p, cnt, npix, partial, probe_pix, probe_out, probe_cnt);
+    if (p.stage == ST_FETCH) return;
+  }
+  // vertex 0 of the item's next sample: the camera ray
+  p.rng.init(p.pix, (uint32_t)(s.sample_begin + p.s_cur), 0u);
+  const Ray r = camera_ray(s, (int)(p.pix % (uint32_t)s.W), (int)(p.pix / (uint32_t)s.W), p.rng);
+  p.c.c_o = r.o;
+  p.c.c_d = r.d;
+  p.c.c_tmax = r.tmax;
+  p.c.c_med = r.medium;
+  p.L = mk(0.f, 0.f, 0.f);
+  p.thr = mk(1.f, 1.f, 1.f);
+  p.b = 0;
+  p.iters = 0;
+  p.k = 0;
+  p.c.flags = FL_CONT;
+  p.cres = CR_NONE;
+  walk_begin(p, RL_CONT);
 }
 
-// One scheduling round of a wavefront: shading events, batched across the
-// wavefront, then traversal rounds in which every traversing lane advances one
-// DDA / tracking step.
-template <bool kSingle, bool kCount, typename Acc>
-__device__ __forceinline__ void run_round(const DScene& s, PathLane& p, Cnt& cnt, int lane, uint32_t npix, int chunk,
-                                          Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
-                                          float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
-  // ---- shading events, batched across the wavefront
-#ifdef R2_SM_STATS
-  const uint64_t te0 = clock64();
+// Work fetch of the standalone kernel: one atomic per wavefront, items handed
+// out by prefix rank; a lane that gets an item starts its first path at the next
+// shading pass.
+struct FrameFetch {
+  uint64_t* counter;
+  uint64_t total;
+  int chunk;
+  const int32_t* probe_pix;
+  const int32_t* probe_smp;
+  __device__ __forceinline__ void operator()(const DScene& s, PathLane& p, int lane, uint32_t npix) const {
+    const uint64_t need = __ballot(p.stage == ST_FETCH);
+    if (!need) return;
+    const int n = __popcll(need);
+    const int leader = __ffsll((unsigned long long)need) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)n);
+    base = __shfl(base, leader);
+    if (p.stage != ST_FETCH) return;
+    const uint64_t item = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+    if (item >= total) {
+      p.stage = ST_IDLE;
+      return;
+    }
+    p.item = item;
+    p.ax = p.ay = p.az = 0.f;
+    p.dax = p.day = p.daz = 0.0;
+    if (probe_pix) {
+      p.pix = (uint32_t)probe_pix[item];
+      p.s_cur = probe_smp[item];
+      p.s_end = p.s_cur + 1;
+    } else {
+      // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
+      const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
+      int x, y;
+      item_pixel(q, s.W, s.H, x, y);
+      p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+      p.s_cur = (int)c * chunk;
+      p.s_end = min(s.spp, p.s_cur + chunk);
+    }
+    p.c.flags = 0u;
+    p.cres = CR_START;
+    p.stage = ST_SHADE;
+  }
+};
+
+// One scheduling step of a wavefront.  Its lanes wait at one of three blocks of
+// code: shading (a vertex, or a new item to fetch), the start of a walk segment,
+// or a traversal step.  The wavefront runs one block per step for the lanes
+// waiting at it, chosen from ballots so that the costly blocks run with many
+// lanes: shading once at least R2_SHADE_MIN lanes wait for it (or nothing else
+// is left), segment starts once at least R2_SEG_MIN lanes, or R2_SEG_RATIO times
+// as many lanes as are in traversals, wait for them, else a traversal step.  Returns false when every
+// lane is idle.
+#ifndef R2_SHADE_MIN
+#define R2_SHADE_MIN 48
 #endif
-  // pass-through: the states run in the forward order of the path's state graph,
-  // so within one pass a lane advances through every state that follows its own
-  // in that order (e.g. SH_NEXT -> BRDF_DONE -> DIRECT_DONE -> MOVE -> BOUNCE ->
-  // a new traversal), and each state's code runs at most once per pass for the
-  // wave; states whose code shares its expensive part run as one block (BOUNCE /
-  // SH_SEG: scene intersection + traversal set-up; NOHIT / BRDF_DONE: environment
-  // lookup; BRDF_SAMPLE / DIRECT_DONE: phase sample; ST_MOVE: the voxel-border
-  // move of every ray spawned at a scatter point)
-  for (int k = 0; k < R2_EV_ITERS; ++k) {
-    if (!__any(p.stage >= ST_START)) break;
+#ifndef R2_SEG_MIN
+#define R2_SEG_MIN 32
+#endif
+#ifndef R2_SEG_RATIO
+#define R2_SEG_RATIO 2
+#endif
+template <bool kSingle, bool kCount, typename Acc, typename Fetch>
+__device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cnt, int lane, uint32_t npix,
+                                           Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
+                                           float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt,
+                                           const Fetch& fetch) {
+  const bool at_shade = p.stage == ST_SHADE || p.stage == ST_FETCH;
+  const bool at_seg = p.stage == ST_SEG;
+  const bool at_trav = p.stage == ST_TRAV;
+  const int nh = __popcll(__ballot(at_shade)), ns = __popcll(__ballot(at_seg)), nt = __popcll(__ballot(at_trav));
+  if (nh + ns + nt == 0) return false;
+#ifdef R2_SM_STATS
+  const uint64_t t0 = SM_CLOCK();
+#endif
+  if (nh >= R2_SHADE_MIN || ns + nt == 0) {
+    fetch(s, p, lane, npix);
     SM_STAT(1, 1);
-    // random words for this pass (a lane in ST_START starts a new stream)
-    if (p.stage > ST_START && p.stage != ST_FINISH) p.rng.top_up();
-    if (p.stage == ST_FINISH) {
-      stat_state(ST_FINISH);
+    SM_STAT(2, __popcll(__ballot(p.stage == ST_SHADE)));
+    if (p.stage == ST_SHADE) shade<kSingle, kCount, Acc>(s, p, cnt, npix, partial, probe_pix, probe_out, probe_cnt);
 #ifdef R2_SM_STATS
-      const uint64_t t0 = clock64();
+    SM_STAT(5, SM_CLOCK() - t0);
 #endif
-      finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
-#ifdef R2_SM_STATS
-      stat_cycles(48 + ST_FINISH, clock64() - t0);
-#endif
+  } else if (ns >= R2_SEG_MIN || ns >= R2_SEG_RATIO * nt) {
+    SM_STAT(3, 1);
+    SM_STAT(4, ns);
+    if (at_seg) {
+      p.rng.top_up();
+      seg_step<kSingle>(s, p);
     }
-    event_if<kSingle, kCount, ST_START>(s, p, cnt);
-    event_if<kSingle, kCount, ST_SH_NEXT>(s, p, cnt);
-    event_if<kSingle, kCount, ST_AFTER_SAMPLE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_NOHIT, ST_BRDF_DONE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_LIGHT_DONE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_SCATTER>(s, p, cnt);
-    event_if<kSingle, kCount, ST_BRDF_SAMPLE, ST_DIRECT_DONE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_MOVE>(s, p, cnt);
-    event_if<kSingle, kCount, ST_BOUNCE, ST_SH_SEG>(s, p, cnt);
-  }
 #ifdef R2_SM_STATS
-  SM_STAT(4, clock64() - te0);
+    SM_STAT(6, SM_CLOCK() - t0);
 #endif
-  // ---- traversal rounds: every traversing lane advances one DDA / tracking step
-#ifdef R2_SM_STATS
-  const uint64_t tt0 = clock64();
-#endif
-  for (int k = 0; k < 256; ++k) {
-    const uint64_t tm = __ballot(p.stage == ST_TRAV);
-    if (tm == 0) break;
-    if (k >= R2_TR_MIN_ROUNDS && __popcll(tm) < R2_TR_MIN_LANES) break;
-    SM_STAT(2, 1);
-    SM_STAT(3, __popcll(tm));
-#if R2_RNG_TRAV_TOPUP
-    if (p.stage == ST_TRAV) p.rng.top_up();
-#endif
-    if (p.stage == ST_TRAV) {
-      if (traversal_step<kSingle, kCount>(s, p, cnt)) end_traversal(p);
+  } else {
+    SM_STAT(7, 1);
+    SM_STAT(8, nt);
+    if (at_trav) {
+      p.rng.top_up();
+      trav_step<kSingle, kCount>(s, p, cnt);
     }
-  }
 #ifdef R2_SM_STATS
-  SM_STAT(5, clock64() - tt0);
+    SM_STAT(9, SM_CLOCK() - t0);
 #endif
+  }
+  return true;
 }
 
 #ifndef R2_SM_WAVES
-#define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for (measured best)
+#define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for
 #endif
 template <bool kSingle, bool kCount, typename Acc>
 __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __restrict__ sp, uint64_t* __restrict__ counter,
@@ -228,47 +289,15 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
   PathLane p(cold[threadIdx.x], s);
   p.stage = ST_FETCH;
   p.item = 0;
+  p.cres = CR_START;
   Cnt cnt;
 #pragma unroll
   for (int k = 0; k < kNumCnt; ++k) cnt.c[k] = 0;
+  const FrameFetch fetch{counter, total, chunk, probe_pix, probe_smp};
   stats_begin();
-  for (int guard = 0; guard < (1 << 24); ++guard) {
-    // ---- work fetch: one atomic per wavefront, items handed out by prefix rank
-    const uint64_t need = __ballot(p.stage == ST_FETCH);
-    if (need) {
-      const int n = __popcll(need);
-      const int leader = __ffsll((unsigned long long)need) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)n);
-      base = __shfl(base, leader);
-      if (p.stage == ST_FETCH) {
-        const uint64_t item = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-        if (item < total) {
-          p.item = item;
-          p.ax = p.ay = p.az = 0.f;
-          p.dax = p.day = p.daz = 0.0;
-          if (probe_pix) {
-            p.pix = (uint32_t)probe_pix[item];
-            p.s_cur = probe_smp[item];
-            p.s_end = p.s_cur + 1;
-          } else {
-            // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
-            const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
-            int x, y;
-            item_pixel(q, s.W, s.H, x, y);
-            p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-            p.s_cur = (int)c * chunk;
-            p.s_end = min(s.spp, p.s_cur + chunk);
-          }
-          p.stage = ST_START;
-        } else {
-          p.stage = ST_IDLE;
-        }
-      }
-    }
-    if (__ballot(p.stage != ST_IDLE) == 0) break;
+  for (uint32_t guard = 0; guard < (1u << 30); ++guard) {
     SM_STAT(0, 1);
-    run_round<kSingle, kCount, Acc>(s, p, cnt, lane, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
+    if (!sched_step<kSingle, kCount, Acc>(s, p, cnt, lane, npix, partial, probe_pix, probe_out, probe_cnt, fetch)) break;
   }
   stats_end();
 }
@@ -278,6 +307,39 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
 // serves one entry at a time (the scene stays wave-uniform); when the entry is
 // exhausted and all its lanes are idle it moves to the next entry.  Per item the
 // computation is the one of k_paths_sm, so results do not depend on the mix.
+struct BatchFetch {
+  const BatchEntry* e;
+  uint64_t* counter;
+  __device__ __forceinline__ void operator()(const DScene& s, PathLane& p, int lane, uint32_t npix) const {
+    const uint64_t need = __ballot(p.stage == ST_FETCH);
+    if (!need) return;
+    const int n = __popcll(need);
+    const int leader = __ffsll((unsigned long long)need) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)n);
+    base = __shfl(base, leader);
+    if (p.stage != ST_FETCH) return;
+    const uint64_t local = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+    if (local >= e->count) {
+      p.stage = ST_IDLE;
+      return;
+    }
+    const uint64_t item = e->first + local;
+    p.item = item;
+    p.ax = p.ay = p.az = 0.f;
+    p.dax = p.day = p.daz = 0.0;
+    // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
+    const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
+    int x, y;
+    item_pixel(q, s.W, s.H, x, y);
+    p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+    p.s_cur = (int)c * e->chunk;
+    p.s_end = min(s.spp, p.s_cur + e->chunk);
+    p.c.flags = 0u;
+    p.cres = CR_START;
+    p.stage = ST_SHADE;
+  }
+};
 
 template <bool kSingle, typename Acc>
 __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEntry* __restrict__ ents, int n_ent,
@@ -297,41 +359,14 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
     const BatchEntry& E = (&kconst(ents))[e];
     const DScene& s = kconst(E.scene);
     const uint32_t npix = (uint32_t)s.W * (uint32_t)s.H;
-    const int chunk = E.chunk;
     Acc* partial = (Acc*)E.partial;
     PathLane p(cold[threadIdx.x], s);
     p.stage = ST_FETCH;
     p.item = 0;
-    for (int guard = 0; guard < (1 << 24); ++guard) {
-      const uint64_t need = __ballot(p.stage == ST_FETCH);
-      if (need) {
-        const int n = __popcll(need);
-        const int leader = __ffsll((unsigned long long)need) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd((unsigned long long*)&counters[e], (unsigned long long)n);
-        base = __shfl(base, leader);
-        if (p.stage == ST_FETCH) {
-          const uint64_t local = base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-          if (local < E.count) {
-            const uint64_t item = E.first + local;
-            p.item = item;
-            p.ax = p.ay = p.az = 0.f;
-            p.dax = p.day = p.daz = 0.0;
-            // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
-            const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
-            int x, y;
-            item_pixel(q, s.W, s.H, x, y);
-            p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-            p.s_cur = (int)c * chunk;
-            p.s_end = min(s.spp, p.s_cur + chunk);
-            p.stage = ST_START;
-          } else {
-            p.stage = ST_IDLE;
-          }
-        }
-      }
-      if (__ballot(p.stage != ST_IDLE) == 0) break;
-      run_round<kSingle, false, Acc>(s, p, cnt, lane, npix, chunk, partial, nullptr, nullptr, nullptr);
+    p.cres = CR_START;
+    const BatchFetch fetch{&E, &counters[e]};
+    for (uint32_t guard = 0; guard < (1u << 30); ++guard) {
+      if (!sched_step<kSingle, false, Acc>(s, p, cnt, lane, npix, partial, nullptr, nullptr, nullptr, fetch)) break;
     }
   }
   stats_end();

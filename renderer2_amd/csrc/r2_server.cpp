@@ -16,7 +16,11 @@
 //     not idle while the host retires one and plans the next;
 //   * reduces a request's chunk sums when all its items are done (on a second
 //     stream, after the slice that finished it), copies the frame to the
-//     caller's buffer and frees its memory, which admits the next.
+//     caller's buffer and releases its memory, which admits the next.
+// Request memory is carved out of one device buffer of the HBM budget,
+// allocated when the server starts: admission and release are host-side
+// bookkeeping, so the scheduler loop never calls hipMalloc or hipFree (hipFree
+// would wait for the whole device and drain the slices queued on the stream).
 // A request is split into the same work items (sample chunk x pixel) and the
 // same chunk-ordered reduction as a standalone r2_render, so its result is the
 // same alone and in any mix.
@@ -65,6 +69,48 @@ struct SrvReq {
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// First-fit allocator over one device buffer: 256-byte aligned ranges, the free
+// list kept by offset and merged with its neighbours on release.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+  void reset(char* b, size_t n) {
+    base = b;
+    size = n;
+    free_.clear();
+    if (n) free_[0] = n;
+  }
+  char* alloc(size_t bytes) {
+    bytes = align256(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > bytes) free_[off + bytes] = len - bytes;
+      return base + off;
+    }
+    return nullptr;
+  }
+  void release(char* p, size_t bytes) {
+    bytes = align256(bytes);
+    size_t off = (size_t)(p - base);
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + bytes == next->first) {
+      bytes += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += bytes;
+        return;
+      }
+    }
+    free_[off] = bytes;
+  }
+};
+
 }  // namespace
 
 struct r2_server {
@@ -88,13 +134,16 @@ struct r2_server {
   void* d_ents = nullptr;       // two halves: one per slice in flight
   uint64_t* d_counters = nullptr;
   hipStream_t copy_stream = nullptr;  // chunk reduction and frame copy-out
+  void* d_pool = nullptr;             // the HBM budget, sub-allocated per request
+  Arena arena;
 };
 
 namespace {
 
 void finish_locked(r2_server* s, SrvReq& r, ReqState st, int rc) {
   if (r.d_mem) {
-    (void)hipFree(r.d_mem);
+    // no slice in flight and no copy-out uses the range any more
+    s->arena.release((char*)r.d_mem, r.bytes);
     r.d_mem = nullptr;
     s->inflight_bytes -= (int64_t)r.bytes;
   }
@@ -167,14 +216,14 @@ void admit_locked(r2_server* s) {
       continue;
     }
     if (s->inflight_bytes + (int64_t)r.bytes > s->cfg.hbm_budget_bytes) break;
-    void* mem = nullptr;
-    if (hipMalloc(&mem, r.bytes) != hipSuccess) {
+    void* mem = s->arena.alloc(r.bytes);
+    if (!mem) {
       if (s->active.empty()) {  // nothing to wait for: fail the request
         s->queue.erase(qi);
         finish_locked(s, r, RQ_FAILED, R2_ERR_OOM);
         continue;
       }
-      break;
+      break;  // fragmented: wait for a release
     }
     s->queue.erase(qi);
     r.d_mem = mem;
@@ -232,8 +281,11 @@ void scheduler(r2_server* s) {
       SrvReq& r = *s->reqs[id];
       if (r.cancel && r.inflight == 0) drop_locked(s, r);
     }
-    // plan and launch a slice while fewer than two are in flight: an equal share
-    // of paths per resident request with items left, in whole 64-item tiles
+    // plan and launch a slice while fewer than two are in flight.  Half of the
+    // slice's paths are shared equally by the resident requests with items left,
+    // the other half goes to the requests with the least work left first
+    // (shortest remaining work first), so that small requests finish within a
+    // few slices while large ones keep progressing; whole 64-item tiles
     bool launched = false;
     if (flight.size() < 2) {
       std::vector<r2host::BatchSlice> slice;
@@ -243,14 +295,25 @@ void scheduler(r2_server* s) {
         SrvReq& r = *s->reqs[id];
         if (!r.cancel && r.next_item < r.total_items) cand.push_back(id);
       }
+      auto left_paths = [&](int64_t id) {
+        const SrvReq& r = *s->reqs[id];
+        return (r.total_items - r.next_item) * (uint64_t)r.chunk;
+      };
+      std::stable_sort(cand.begin(), cand.end(), [&](int64_t a, int64_t b) { return left_paths(a) < left_paths(b); });
       const int n_act = std::min<int>((int)cand.size(), r2host::kMaxBatchEntries / 2);
       if (n_act > 0) {
-        const uint64_t share = std::max<uint64_t>(1, (uint64_t)s->cfg.slice_paths / (uint64_t)n_act);
+        const uint64_t budget = (uint64_t)s->cfg.slice_paths;
+        const uint64_t share = std::max<uint64_t>(1, budget / (2 * (uint64_t)n_act));
+        uint64_t extra = budget / 2;  // handed out in order of least work left
         for (int i = 0; i < n_act; ++i) {
           SrvReq& r = *s->reqs[cand[i]];
+          const uint64_t left = r.total_items - r.next_item;
           uint64_t items = std::max<uint64_t>(64 * 64, share / (uint64_t)r.chunk);
+          const uint64_t more = std::min<uint64_t>(extra / (uint64_t)r.chunk, left > items ? left - items : 0);
+          items += more;
+          extra -= more * (uint64_t)r.chunk;
           items = (items + 63) & ~(uint64_t)63;
-          items = std::min<uint64_t>(items, r.total_items - r.next_item);
+          items = std::min<uint64_t>(items, left);
           slice.push_back(r2host::BatchSlice{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.fp64,
                                              r.packed});
           sl.took.push_back({r.id, items});
@@ -324,12 +387,16 @@ extern "C" int r2_server_start(r2_ctx* ctx, const r2_server_config* cfg, r2_serv
     return r2host::fail(ctx, R2_ERR_HIP, "server: stream");
   }
   if (hipMalloc(&s->d_ents, r2host::batch_entry_bytes() * r2host::kMaxBatchEntries) != hipSuccess ||
-      hipMalloc((void**)&s->d_counters, sizeof(uint64_t) * r2host::kMaxBatchEntries) != hipSuccess) {
+      hipMalloc((void**)&s->d_counters, sizeof(uint64_t) * r2host::kMaxBatchEntries) != hipSuccess ||
+      hipMalloc(&s->d_pool, (size_t)s->cfg.hbm_budget_bytes) != hipSuccess) {
+    (void)hipGetLastError();
     if (s->d_ents) (void)hipFree(s->d_ents);
+    if (s->d_counters) (void)hipFree(s->d_counters);
     (void)hipStreamDestroy(s->copy_stream);
     (void)hipStreamDestroy(s->shadow.stream);
-    return r2host::fail(ctx, R2_ERR_OOM, "server: device tables");
+    return r2host::fail(ctx, R2_ERR_OOM, "server: device tables or the HBM budget do not fit the device");
   }
+  s->arena.reset((char*)s->d_pool, (size_t)s->cfg.hbm_budget_bytes);
   r2_server* raw = s.release();
   raw->th = std::thread(scheduler, raw);
   *out = raw;
@@ -478,6 +545,7 @@ extern "C" int r2_server_stop(r2_server* s) {
   }
   (void)hipFree(s->d_ents);
   (void)hipFree(s->d_counters);
+  (void)hipFree(s->d_pool);
   (void)hipStreamDestroy(s->copy_stream);
   (void)hipStreamDestroy(s->shadow.stream);
   delete s;

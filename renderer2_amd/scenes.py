@@ -148,3 +148,41 @@ def filtered_grid(fgrid, voxel_size: float, albedo: float = 0.9, records=None, r
     else:
         g["records"] = records
     return g
+
+
+# serving workload: device-resident LOD grids of the 256^3 anisotropic field at
+# 128^3, 256^3 and 512^3 plus the 64^3 value-noise grid, the procedural sky + sun
+# environment; requests of resolution {256, 512, 1024}^2, spp {16, 64, 256},
+# depth 16, an orbit camera and one LOD grid each
+SERVING_SIZES = (256, 512, 1024)
+SERVING_SPPS = (16, 64, 256)
+SERVING_DEPTH = 16
+
+
+def serving_grids() -> list:
+    """Host descriptions of the serving workload's four grids (128^3, 256^3 and
+    512^3 fibre field, 64^3 value noise), in that order."""
+    out = [make_workload("c2", n=n)["grids"][0] for n in (128, 256, 512)]
+    out.append(make_workload("c1")["grids"][0])
+    return out
+
+
+def serving_requests(n_req: int, rate: float | None, seed: int, n_grids: int = 4, sizes=SERVING_SIZES,
+                     spps=SERVING_SPPS, max_depth: int = SERVING_DEPTH, radius: float = 2.5) -> list:
+    """Seeded Poisson request stream: (arrival time s, render config, instances) per
+    request, each with a resolution, an spp, a grid (LOD) and an orbit camera drawn
+    at random.  rate None: all arrive at t = 0."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req)) if rate else np.zeros(n_req)
+    reqs = []
+    for k in range(n_req):
+        res = int(rng.choice(sizes))
+        spp = int(rng.choice(spps))
+        g = int(rng.integers(n_grids))
+        az, el = rng.uniform(0, 2 * np.pi), rng.uniform(-0.3, 0.6)
+        cam = (radius * np.cos(el) * np.sin(az), radius * np.sin(el), radius * np.cos(el) * np.cos(az))
+        cfg = dict(width=res, height=res, spp=spp, max_depth=max_depth, cam_pos=cam, cam_look_at=(0, 0, 0),
+                   cam_up=(0, 1, 0), fov_deg=40.0, seed=1000 + k, stream=0, use_brick_grid=True)
+        inst = [dict(grid=g, pos=(0, 0, 0), orientation=(0, 1, 0), scaling=(1, 1, 1))]
+        reqs.append((float(arrivals[k]), cfg, inst))
+    return reqs

@@ -162,10 +162,13 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
     assert np.isfinite(hdr).all() and hdr.mean() > 0
     _, by = O.postprocess(np.ascontiguousarray(hdr), 2.0)
     assert np.array_equal(png, by[::-1])
-    # output.jpg: the same bytes as a quality-100 JPEG (decoded by libjpeg through PIL)
+    # output.jpg: the same bytes as a quality-100 JPEG (decoded by libjpeg through PIL),
+    # no further from them than libjpeg's own quality-100 encoding of the same bytes
     from PIL import Image
+    from test_image import libjpeg_q100_error
     jpg = np.asarray(Image.open(tmp_path / "output.jpg").convert("RGB")).astype(np.int64)
-    assert jpg.shape == png.shape and np.abs(jpg - png.astype(np.int64)).max() <= 3
+    lib = libjpeg_q100_error(png)
+    assert jpg.shape == png.shape and np.abs(jpg - png.astype(np.int64)).max() <= max(lib.max(), 3)
     # the same scene rendered from the .nvdb files alone (the reference's own filter
     # output format): the .r2grid files removed, filtering off -> the same image
     for f in (tmp_path / "filtering" / "blob").glob("*.r2grid"):

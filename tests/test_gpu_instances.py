@@ -72,14 +72,28 @@ def test_multi_instance_paths_match_oracle_fp32(ctx, multi, use_brick):
     assert same_cnt.mean() >= 0.95
 
 
+def _blocks(x, W, H, k=4):
+    """Mean over k x k pixel blocks: keeps a bias, divides independent noise by k^2."""
+    x = np.asarray(x, np.float64).reshape(H, W, 3)
+    return x.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+
+
 def test_multi_instance_frame_converges_to_oracle_fp64(ctx):
-    w = multi_instance_workload(width=32, height=32, spp=4096)
+    """The sun (1e4:1) makes single pixels noisy: two independent FP64 oracle
+    frames of this scene at 2048 spp differ by relMSE 2.0e-2.  At 16384 GPU spp
+    against 8192 oracle spp the noise floor measured with the FP32 oracle in
+    place of the GPU is 3.5e-3 per pixel and 2.3e-4 to 3.1e-4 on 4x4 block means
+    (three seeds), so a bias of the multi-instance path above 1e-3 fails."""
+    W = H = 32
+    w = multi_instance_workload(width=W, height=H, spp=16384)
     grids, env = _upload(ctx, w)
     img = ctx.render(dict(w["render"], seed=101), grids, w["instances"], env)
-    ref = O.Scene(w["grids"], w["instances"], w["env"], "f64").render(dict(w["render"], spp=2048, seed=202))
+    ref = O.Scene(w["grids"], w["instances"], w["env"], "f64").render(dict(w["render"], spp=8192, seed=202))
     e = relmse(img, ref)
-    print(f"4 instances 32^2: relMSE(GPU 4096 spp, oracle FP64 2048 spp) = {e:.3e}")
-    assert e <= 1e-3
+    eb = relmse(_blocks(img, W, H), _blocks(ref, W, H))
+    print(f"4 instances 32^2: relMSE(GPU 16384 spp, oracle FP64 8192 spp) = {e:.3e} per pixel, {eb:.3e} on 4x4 blocks")
+    assert e <= 1e-2
+    assert eb <= 1e-3
 
 
 def test_multi_instance_server_matches_standalone(ctx, multi):

@@ -7,9 +7,11 @@ in this image, so the byte streams are not compared with stb's.  What these test
 pin, with PIL (libjpeg / zlib) as the independent decoder:
 
   * PNG: the decoded pixels equal the frame flipped top to bottom, bit for bit;
-  * JPEG: the decoded pixels are within the error of a lossless-quantiser JPEG
-    at quality 100 (max 3 of 255), and within a PSNR that only a correct DCT,
-    zig-zag order and Huffman coding reach at lower qualities;
+  * JPEG: at quality 100 the decoded pixels are no further from the frame than
+    libjpeg's own quality-100 4:4:4 encoding of the same pixels is (the colour
+    conversion and DCT rounding of both sides: a max error of 3 or 4 of 255 on
+    most images), and at lower qualities within a PSNR that only a correct
+    DCT, zig-zag order and Huffman coding reach;
   * the quantisation tables equal the ones libjpeg's own encoder writes at
     the same quality (the IJG scaling of the T.81 Annex K tables), and the
     chroma sampling is 4:4:4 above quality 90 and 4:2:0 at 90 and below;
@@ -29,6 +31,17 @@ from renderer2_amd import capi
 
 def _decode(path):
     return np.asarray(Image.open(path).convert("RGB")).astype(np.int64)
+
+
+def libjpeg_q100_error(rgb: np.ndarray) -> np.ndarray:
+    """|decoded - original| of libjpeg's own quality-100, 4:4:4 JPEG of `rgb` (rows
+    top to bottom): the error a correct quality-100 encoder is held to."""
+    import io
+
+    buf = io.BytesIO()
+    Image.fromarray(np.ascontiguousarray(rgb, np.uint8)).save(buf, format="JPEG", quality=100, subsampling=0)
+    buf.seek(0)
+    return np.abs(np.asarray(Image.open(buf).convert("RGB")).astype(np.int64) - rgb.astype(np.int64))
 
 
 def _psnr(a, b):
@@ -65,7 +78,8 @@ def test_jpg_quality_100(tmp_path, h, w):
         assert JpegImagePlugin.get_sampling(im) == 0  # 4:4:4
         assert all(v == 1 for t in im.quantization.values() for v in t)
         err = np.abs(_decode(p) - np.flipud(img).astype(np.int64))
-        assert err.max() <= 3 and err.mean() <= 0.7, (err.max(), err.mean())
+        lib = libjpeg_q100_error(np.flipud(img))
+        assert err.max() <= max(lib.max(), 3) and err.mean() <= lib.mean() + 0.1, (err.max(), err.mean(), lib.max())
 
 
 @pytest.mark.parametrize("q", [1, 10, 25, 50, 75, 90, 91, 95, 100])
