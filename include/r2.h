@@ -161,6 +161,18 @@ typedef int (*r2_progress_fn)(void* user, int32_t samples_done, int32_t samples_
 int r2_render_progressive(r2_ctx* ctx, const r2_render_params* p, float* out_rgb, int32_t out_on_device,
                           r2_progress_fn progress, void* user);
 
+/* r2_render with no device allocation.  No counterpart in the reference: this
+ * entry point is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * out_rgb must be device memory (W*H*3 floats).  The frame's chunk sums go to a
+ * fixed 64 MiB device array that the library holds on each GPU it runs on until
+ * the process ends, instead of the context's scratch.  Frames up to 5592405
+ * pixels (2796202 with R2_FLAG_FP64_ACCUM) are accepted, larger ones return
+ * R2_ERR_UNSUPPORTED; when the default chunking does not fit, items take more
+ * samples each.  The result equals r2_render's bit for bit whenever r2_render's
+ * chunk sums fit in 64 MiB.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (from several contexts or threads) race. */
+int r2_render_noalloc(r2_ctx* ctx, const r2_render_params* p, float* out_rgb);
 
 /* Parity probe: one path per (pixel, sample) pair; out: n*3 (L / spp);
  * counters: n*8 u32 lookup counters (voxel records, majorants, brick densities,

@@ -179,6 +179,12 @@ class Context:
         p, keep = self.render_params(cfg, grids, instances, env)
         self._check(capi.lib().r2_render(self.h, C.byref(p), out_dev_ptr, 1))
 
+    def render_noalloc(self, cfg: dict, grids, instances, env, out_dev_ptr: int):
+        """r2_render_noalloc: render into a caller-owned device buffer with no device
+        allocation (chunk sums in the library's fixed 64 MiB per-GPU array)."""
+        p, keep = self.render_params(cfg, grids, instances, env)
+        self._check(capi.lib().r2_render_noalloc(self.h, C.byref(p), out_dev_ptr))
+
     def render_paths(self, cfg: dict, grids, instances, env, pix, smp, counters: bool = False):
         p, keep = self.render_params(cfg, grids, instances, env)
         pix = np.ascontiguousarray(pix, np.int32)

@@ -190,6 +190,7 @@ SIGNATURES = [
     ("r2_render", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_int32]),
     ("r2_render_progressive", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_int32,
                                         C.c_void_p, C.c_void_p]),
+    ("r2_render_noalloc", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p]),
     ("r2_render_paths", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_int32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
     ("r2_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),

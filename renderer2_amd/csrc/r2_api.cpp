@@ -165,6 +165,10 @@ int r2_create(r2_ctx** out, int device) {
     r2_destroy(c);
     return R2_ERR_HIP;
   }
+  // the address of r2_render_noalloc's scratch array on this device (kept per
+  // device; a failure is reported by that entry point, the context stays usable)
+  (void)noalloc_scratch_init(c);
+  c->err.clear();
   *out = c;
   return R2_OK;
 }
@@ -419,6 +423,37 @@ int r2_render_progressive(r2_ctx* ctx, const r2_render_params* p, float* out, in
   o.progress = progress;
   o.user = user;
   return render_common(ctx, p, out, out_on_device, o);
+}
+
+int r2_render_noalloc(r2_ctx* ctx, const r2_render_params* p, float* out) {
+  if (!ctx || !p || !out) return R2_ERR_INVALID;
+  const auto t0 = std::chrono::steady_clock::now();
+  (void)hipSetDevice(ctx->device);
+  void* scr = noalloc_scratch(ctx);
+  if (!scr) return fail(ctx, R2_ERR_HIP, "render_noalloc: the scratch array has no address on this device");
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, out) != hipSuccess ||
+      (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)) {
+    (void)hipGetLastError();
+    return fail(ctx, R2_ERR_INVALID, "render_noalloc: out_rgb must be device memory");
+  }
+  DScene s;
+  int rc = build_scene(ctx, p, &s);
+  if (rc) return rc;
+  const bool fp64 = (p->flags & R2_FLAG_FP64_ACCUM) != 0;
+  const uint64_t per_chunk = (uint64_t)p->width * p->height * 3 * (fp64 ? 8 : 4);
+  if (per_chunk > kNoAllocScratchBytes)
+    return fail(ctx, R2_ERR_UNSUPPORTED,
+                "render_noalloc: one sample chunk of a " + std::to_string(p->width) + "x" + std::to_string(p->height) +
+                    " frame needs " + std::to_string(per_chunk) + " bytes of chunk sums, the scratch holds " +
+                    std::to_string(kNoAllocScratchBytes) + " (at most " +
+                    std::to_string(kNoAllocScratchBytes / (3 * (fp64 ? 8 : 4))) + " pixels)");
+  RenderOpts o;
+  o.partial = scr;
+  o.partial_bytes = kNoAllocScratchBytes;
+  rc = launch_render_ex(ctx, s, out, fp64, o);
+  ctx->stats.last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
 }
 
 int r2_render_paths(r2_ctx* ctx, const r2_render_params* p, int32_t n, const int32_t* pix, const int32_t* smp,

@@ -94,6 +94,13 @@ struct RenderOpts {
 int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum, const RenderOpts& o);
 int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
                  uint32_t* d_cnt);
+// chunk-sum scratch of r2_render_noalloc: a file-scope device array of
+// kNoAllocScratchBytes in r2_render.hip, one copy per GPU for the process
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+// looks up the array's address on the current device once (called by r2_create)
+int noalloc_scratch_init(r2_ctx* ctx);
+// that address for ctx's device, or nullptr when the look-up failed
+void* noalloc_scratch(const r2_ctx* ctx);
 // one request's share of a batched launch (render server)
 struct BatchSlice {
   const r2::DScene* d_scene;

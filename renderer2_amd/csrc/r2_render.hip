@@ -10,6 +10,7 @@
 #define R2_PREC_MODE 0
 #define R2K_NS r2k_fp32
 #include "r2_render_kernels.hpp"
+#include <mutex>
 
 using namespace r2;
 
@@ -218,6 +219,38 @@ int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp
 
 int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64) {
   return launch_render_ex(ctx, scene, d_out, fp64, RenderOpts{});
+}
+
+// Chunk sums of r2_render_noalloc, for callers that cannot allocate device
+// memory.  Each GPU the library runs on holds its own copy of this array until
+// the process ends.  In host code the array's name is a host-side stand-in,
+// not a device address: host code uses only the per-device address that
+// hipGetSymbolAddress returns (kept in g_scratch_dev).  Every item of a frame
+// writes its own chunk sum before the reduction reads it, so the array needs no
+// clearing between calls; calls that overlap on one GPU share it and race.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kScratchDevices] = {};
+
+int noalloc_scratch_init(r2_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kScratchDevices)
+    return fail(ctx, R2_ERR_UNSUPPORTED, "render_noalloc: device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[ctx->device]) return R2_OK;
+  void* ptr = nullptr;
+  // the calling thread has ctx->device current (r2_create)
+  const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+  g_scratch_dev[ctx->device] = ptr;
+  return R2_OK;
+}
+
+void* noalloc_scratch(const r2_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_dev[ctx->device];
 }
 
 int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
