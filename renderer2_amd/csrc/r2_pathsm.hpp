@@ -368,29 +368,6 @@ __device__ __forceinline__ void shadow_done(PathLane& p) {
   walk_next(p, p.role);
 }
 
-// one intersectTr segment: scene intersection, then the transmittance traversal
-// of the segment if it lies inside a medium
-template <bool kSingle>
-__device__ __forceinline__ void shadow_segment(const DScene& s, PathLane& p) {
-  const bool dda = (s.flags & 1u) != 0u;
-  if (++p.sh_seg > kShadowSegCap) {
-    p.si_hit = 1;
-    shadow_done(p);
-    return;
-  }
-  const int cur = p.ray.medium;
-  Ray r = spawn_ray(p.ray.o, p.ray.d, cur);
-  const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit
-  p.si_hit = h.hit;
-  p.ray.o = h.pos;
-  p.ray.medium = r.medium < 0 ? h.medium : -1;
-  if (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur)) {
-    start_traversal<kSingle>(s, p, r.medium, r, dda ? TK_TR_DDA : TK_TR_RATIO);
-    return;
-  }
-  if (!h.hit) shadow_done(p);
-}
-
 __device__ __forceinline__ void cont_done(PathLane& p, int cr) {
   p.cres = cr;
   p.stage = ST_SHADE;  // the continuation is the vertex's last ray
@@ -408,47 +385,74 @@ __device__ __forceinline__ void cont_nohit(PathLane& p) {
   p.ray = nr;
   p.stage = ST_SEG;
 }
-// top of Li's bounce loop: depth and iteration caps, scene intersection, and
-// the start of the free flight on the current ray (inside a medium) or on the
-// ray entering the medium that was hit (the entry segment is sampled again from
+// A lane at the start of a segment (ST_SEG).  For the continuation this is the
+// top of Li's bounce loop: depth and iteration caps, scene intersection, and the
+// start of the free flight on the current ray (inside a medium) or on the ray
+// entering the medium that was hit (the entry segment is sampled again from
 // inside on the pass-through that follows a miss: the reference's double
-// tracking, path_integrator_impl.hpp:133-160,188-191)
-template <bool kSingle>
-__device__ __forceinline__ void cont_segment(const DScene& s, PathLane& p) {
-  const bool dda = (s.flags & 1u) != 0u;
-  if (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64) {
-    cont_done(p, CR_END);
-    return;
-  }
-  Ray r = p.ray;
-  const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
-  p.ray = r;
-  p.si_pos = h.pos;
-  p.si_medium = h.medium;
-  p.si_hit = h.hit;
-  if (h.medium < 0) {
-    cont_nohit(p);
-    return;
-  }
-  if (r.medium >= 0) {
-    if (h.medium != r.medium) cont_done(p, CR_END);
-    else start_traversal<kSingle>(s, p, r.medium, r, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
-    return;
-  }
-  Ray mr = mkray(h.pos, normalize(r.d), kDefaultTmax, h.medium);
-  ray_offset(mr, r.d);
-  const Hit se = scene_intersect(s, mr);
-  if (!se.hit) cont_nohit(p);
-  else if (se.medium >= 0 && h.medium != se.medium) cont_done(p, CR_END);
-  else start_traversal<kSingle>(s, p, mr.medium, mr, dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA);
-}
-
-// A lane at the start of a segment (ST_SEG): the continuation's bounce
-// top or one intersectTr segment.  Either starts a traversal or ends the ray.
+// tracking, path_integrator_impl.hpp:133-160,188-191).  For a shadow ray it is
+// one intersectTr segment: spawnRay(p0, p1, currentMedium), scene intersection,
+// then the transmittance traversal of the segment if it lies inside a medium.
+// Both roles share one scene intersection and one traversal set-up, so a pass
+// whose lanes hold both kinds of ray runs each once.
 template <bool kSingle>
 __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
-  if (p.role == RL_CONT) cont_segment<kSingle>(s, p);
-  else shadow_segment<kSingle>(s, p);
+  const bool dda = (s.flags & 1u) != 0u;
+  const bool cont = p.role == RL_CONT;
+  if (cont ? (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64) : ++p.sh_seg > kShadowSegCap) {
+    if (cont) {
+      cont_done(p, CR_END);
+    } else {
+      p.si_hit = 1;
+      shadow_done(p);
+    }
+    return;
+  }
+  const int cur = p.ray.medium;
+  Ray r = cont ? p.ray : spawn_ray(p.ray.o, p.ray.d, cur);
+  const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
+  Ray tr = r;  // the ray of the traversal to start
+  int kind;
+  if (cont) {
+    p.ray = r;
+    p.si_pos = h.pos;
+    p.si_medium = h.medium;
+    p.si_hit = h.hit;
+    if (h.medium < 0) {
+      cont_nohit(p);
+      return;
+    }
+    if (r.medium >= 0) {
+      if (h.medium != r.medium) {
+        cont_done(p, CR_END);
+        return;
+      }
+    } else {
+      Ray mr = mkray(h.pos, normalize(r.d), kDefaultTmax, h.medium);
+      ray_offset(mr, r.d);
+      const Hit se = scene_intersect(s, mr);
+      if (!se.hit) {
+        cont_nohit(p);
+        return;
+      }
+      if (se.medium >= 0 && h.medium != se.medium) {
+        cont_done(p, CR_END);
+        return;
+      }
+      tr = mr;
+    }
+    kind = dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA;
+  } else {
+    p.si_hit = h.hit;
+    p.ray.o = h.pos;
+    p.ray.medium = r.medium < 0 ? h.medium : -1;
+    if (!(h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur))) {
+      if (!h.hit) shadow_done(p);
+      return;
+    }
+    kind = dda ? TK_TR_DDA : TK_TR_RATIO;
+  }
+  start_traversal<kSingle>(s, p, tr.medium, tr, kind);
 }
 
 // A lane inside a traversal (ST_TRAV): one DDA / tracking step; at the end
