@@ -31,7 +31,7 @@ extern "C" {
 #define R2_ERR_BUSY (-7)        /* server queue full */
 #define R2_ERR_DEADLINE (-8)    /* server request not finished by its deadline (dropped) */
 
-#define R2_VERSION 1
+#define R2_VERSION 2
 
 /* Grid record precision (r2_grid_desc.precision) */
 #define R2_PREC_FP32 0   /* 32-byte SFilteredDataCompact voxel records (filtered_data.hpp:33-61) */
@@ -193,6 +193,14 @@ int r2_get_stats(const r2_ctx* ctx, r2_stats* out);
  * (raytracer/include/mesh/mesh.hpp:18-59, raytracer/src/mesh/mesh.cu:140-168)
  * with per-LOD candidate lists: each voxel is tested against the triangles
  * within reach of its rays. */
+/* RGBA float texture of a mesh material (CTexture: 8-bit images divided by 255,
+ * raytracer/src/texture/texture.cu:29-99), sampled like tex2D<float4> with linear
+ * filtering, wrap addressing and normalised coordinates (texture.cu:166-189). */
+typedef struct {
+  int32_t width, height;
+  const float* rgba;           /* height rows of width RGBA texels, row 0 at v = 0 */
+} r2_texture_desc;
+
 typedef struct {
   int32_t n_vertices, n_triangles;
   const float* positions;      /* n_vertices * 3, model space */
@@ -205,6 +213,17 @@ typedef struct {
   const float* mat_roughness;  /* n_materials (specularRoughness, material.cu:146-148) */
   const float* mat_ior;        /* n_materials */
   float pos[3], orientation[3], scaling[3];  /* scene object transform */
+  /* Textured materials (CMaterial's albedo, glossy, normal and alpha textures,
+   * raytracer/include/material/material.hpp:54-57,111-156).  uvs: n_vertices * 2
+   * texture coordinates, or NULL (every hit then has tc = (0, 0) and no alpha
+   * test or normal map applies, as for a mesh without texture coordinates in
+   * shaders.optix.cu:48-152).  mat_textures: n_materials * 4 indices into
+   * textures (diffuse colour, glossy colour, normal map, alpha mask; -1 = the
+   * constant colour / none), or NULL. */
+  const float* uvs;
+  int32_t n_textures;
+  const r2_texture_desc* textures;
+  const int32_t* mat_textures;
 } r2_mesh_desc;
 int r2_mesh_upload(r2_ctx* ctx, const r2_mesh_desc* desc, r2_mesh** out);
 int r2_mesh_free(r2_ctx* ctx, r2_mesh* mesh);

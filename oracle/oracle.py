@@ -56,6 +56,10 @@ class Bricks(C.Structure):
     ]
 
 
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("rgba", C.c_void_p)]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [
         ("n_vertices", C.c_int), ("n_triangles", C.c_int),
@@ -64,7 +68,29 @@ class MeshDesc(C.Structure):
         ("mat_diffuse", C.c_void_p), ("mat_specular", C.c_void_p),
         ("mat_roughness", C.c_void_p), ("mat_ior", C.c_void_p),
         ("pos", C.c_float * 3), ("orientation", C.c_float * 3), ("scaling", C.c_float * 3),
+        ("uvs", C.c_void_p), ("n_textures", C.c_int), ("textures", C.c_void_p), ("mat_textures", C.c_void_p),
     ]
+
+
+def mesh_textures(mesh: dict, keep: dict, tex_struct):
+    """(uvs pointer, n_textures, textures array, mat_textures pointer) of a mesh dict
+    with optional uvs (V,2), textures [list of (H,W,4) float32 RGBA] and
+    mat_textures (M,4) int32 (diffuse, glossy, normal map, alpha; -1 none)."""
+    uv = mesh.get("uvs")
+    keep["uvs"] = None if uv is None else np.ascontiguousarray(uv, np.float32)
+    texs = [np.ascontiguousarray(t, np.float32) for t in mesh.get("textures", [])]
+    keep["tex_data"] = texs
+    arr = (tex_struct * max(1, len(texs)))()
+    for i, t in enumerate(texs):
+        assert t.ndim == 3 and t.shape[2] == 4
+        arr[i].height, arr[i].width = t.shape[0], t.shape[1]
+        arr[i].rgba = t.ctypes.data
+    keep["tex_arr"] = arr
+    mt = mesh.get("mat_textures")
+    keep["mat_textures"] = None if mt is None else np.ascontiguousarray(mt, np.int32).reshape(-1, 4)
+    return (None if keep["uvs"] is None else keep["uvs"].ctypes.data, len(texs),
+            C.cast(arr, C.c_void_p).value if texs else None,
+            None if keep["mat_textures"] is None else keep["mat_textures"].ctypes.data)
 
 
 class FilterGrid(C.Structure):
@@ -155,6 +181,7 @@ def lib() -> C.CDLL:
         L.r2o_filter_probe.argtypes = [vp, C.POINTER(FilterGrid), C.POINTER(FilterParams), C.c_int64, vp, vp, vp, vp,
                                        vp, vp]
         L.r2o_filter_closest_hit.argtypes = [vp, vp, vp, C.c_float, vp]
+        L.r2o_tex_fetch.argtypes = [vp, C.c_float, C.c_float, vp]
         L.r2o_det_expf.argtypes = [C.c_float]
         L.r2o_det_expf.restype = C.c_float
         L.r2o_det_logf.argtypes = [C.c_float]
@@ -399,7 +426,8 @@ class MeshFilter:
 
     mesh: dict with positions (V,3) f32, normals (V,3) f32, indices (T,3) i32,
     material (T,) i32 or None, mat_diffuse (M,3), mat_specular (M,3),
-    mat_roughness (M,), mat_ior (M,), pos/orientation/scaling (3,).
+    mat_roughness (M,), mat_ior (M,), pos/orientation/scaling (3,); optional
+    uvs (V,2), textures (list of (H,W,4) RGBA float32) and mat_textures (M,4).
     """
 
     def __init__(self, mesh: dict):
@@ -418,6 +446,7 @@ class MeshFilter:
         d.n_materials = self._keep["mat_ior"].shape[0]
         for k, dflt in (("pos", (0, 0, 0)), ("orientation", (0, 1, 0)), ("scaling", (1, 1, 1))):
             setattr(d, k, (C.c_float * 3)(*[float(v) for v in mesh.get(k, dflt)]))
+        d.uvs, d.n_textures, d.textures, d.mat_textures = mesh_textures(mesh, self._keep, Texture)
         self.desc = d
         self._h = lib().r2o_filter_new(C.byref(d))
         if not self._h:
@@ -473,6 +502,15 @@ class MeshFilter:
         out = np.zeros(3, np.float32)
         tri = lib().r2o_filter_closest_hit(self._h, _p(o), _p(d), C.c_float(tmax), _p(out))
         return tri, out
+
+
+def tex_fetch(rgba: np.ndarray, x: float, y: float) -> np.ndarray:
+    """The filter's texture fetch of (H, W, 4) RGBA float32 texels at (x, y)."""
+    t = np.ascontiguousarray(rgba, np.float32)
+    desc = Texture(t.shape[1], t.shape[0], t.ctypes.data)
+    out = np.zeros(4, np.float32)
+    lib().r2o_tex_fetch(C.byref(desc), C.c_float(x), C.c_float(y), _p(out))
+    return out
 
 
 def grid_dict(g: FilterGrid) -> dict:

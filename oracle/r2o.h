@@ -95,6 +95,13 @@ r2o_scene_f64* r2o_scene_new_f64(const r2o_scene_desc* d);
 void r2o_scene_free_f64(r2o_scene_f64* s);
 
 /* ---- mesh -> SGGX filter (r2o_filter.c) ---- */
+/* RGBA float texture of a material (CTexture, texture.cu:29-99): height rows of
+ * width texels, row 0 at v = 0 */
+typedef struct {
+  int width, height;
+  const float* rgba;
+} r2o_texture;
+
 typedef struct {
   int n_vertices, n_triangles;
   const float* positions;   /* 3 per vertex, model space */
@@ -107,6 +114,10 @@ typedef struct {
   const float* mat_roughness; /* 1 per material: SGGX sigma of buildS */
   const float* mat_ior;       /* 1 per material */
   float pos[3], orientation[3], scaling[3];
+  const float* uvs;             /* 2 per vertex (texture coordinates), or NULL */
+  int n_textures;
+  const r2o_texture* textures;
+  const int32_t* mat_textures;  /* 4 per material: diffuse, glossy, normal map, alpha mask (-1: none), or NULL */
 } r2o_mesh_desc;
 
 typedef struct {
@@ -131,6 +142,7 @@ int r2o_filter_run(const r2o_filter_ctx* c, const r2o_filter_grid* g, const r2o_
 int r2o_filter_probe(const r2o_filter_ctx* c, const r2o_filter_grid* g, const r2o_filter_params* p, int64_t vid,
                      uint8_t* rec, float* o, float* d, float* tmax, int32_t* tri, float* t);
 int r2o_filter_closest_hit(const r2o_filter_ctx* c, const float o[3], const float d[3], float tmax, float out[3]);
+void r2o_tex_fetch(const r2o_texture* t, float x, float y, float out[4]);
 float r2o_det_expf(float x);
 float r2o_det_logf(float x);
 float r2o_det_powf(float x, float y);

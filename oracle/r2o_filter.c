@@ -371,7 +371,64 @@ static int box_hit(f3 mn, f3 mx, f3 o, f3 inv, float tmax) {
   return 1;
 }
 
-/* closest hit in (0, tmax]: lexicographic minimum of (t, triangle index) */
+/* ------------------------------------------------------------ textures --- */
+/* CTexture::operator() (texture.hpp:55-58): tex2D<float4> with linear filtering,
+ * wrap addressing and normalised coordinates (texture.cu:166-189), texel centres
+ * at (i + 0.5) / width; full-precision weights (CUDA's are 8-bit fractions).
+ * The blend is two nested linear interpolations, so that a constant texture
+ * returns its value exactly, as tex2D's fixed-point weights (which sum to one)
+ * do: the normal map's zero test depends on it. */
+static void tex_fetch(const r2o_texture* t, float x, float y, float out[4]) {
+  const int W = t->width, H = t->height;
+  x = x - floorf(x);
+  y = y - floorf(y);
+  const float tx = x * (float)W - 0.5f, ty = y * (float)H - 0.5f;
+  const float fx = floorf(tx), fy = floorf(ty);
+  const float a = tx - fx, b = ty - fy;
+  int i0 = (int)fx, j0 = (int)fy;
+  int i1 = i0 + 1, j1 = j0 + 1;
+  i0 = ((i0 % W) + W) % W; i1 = ((i1 % W) + W) % W;
+  j0 = ((j0 % H) + H) % H; j1 = ((j1 % H) + H) % H;
+  const float* p00 = t->rgba + ((size_t)j0 * W + i0) * 4;
+  const float* p10 = t->rgba + ((size_t)j0 * W + i1) * 4;
+  const float* p01 = t->rgba + ((size_t)j1 * W + i0) * 4;
+  const float* p11 = t->rgba + ((size_t)j1 * W + i1) * 4;
+  for (int c = 0; c < 4; ++c) {
+    const float top = p00[c] + a * (p10[c] - p00[c]);
+    const float bot = p01[c] + a * (p11[c] - p01[c]);
+    out[c] = top + b * (bot - top);
+  }
+}
+enum { TEX_DIFFUSE = 0, TEX_GLOSSY = 1, TEX_NORMAL = 2, TEX_ALPHA = 3 };
+/* texture of a material slot, or NULL (CMaterial's m_albedoTexture, m_glossyTexture,
+ * m_normalTexture, m_alphaTexture, material.hpp:54-57) */
+static const r2o_texture* mat_tex(const r2o_mesh_desc* md, int mat, int slot) {
+  if (!md->mat_textures) return NULL;
+  const int k = md->mat_textures[4 * mat + slot];
+  return (k >= 0 && k < md->n_textures) ? &md->textures[k] : NULL;
+}
+static int tri_material(const r2o_mesh_desc* md, int t) { return md->material ? md->material[t] : 0; }
+/* barycentric texture coordinate: (1 - b1 - b2) TC0 + b1 TC1 + b2 TC2 (shaders.optix.cu:61-66) */
+static void tri_tc(const r2o_mesh_desc* md, int t, float b1, float b2, float tc[2]) {
+  const int32_t* id = md->indices + 3 * t;
+  const float* U = md->uvs;
+  const float w0 = 1.f - b1 - b2;
+  for (int c = 0; c < 2; ++c) tc[c] = (w0 * U[2 * id[0] + c] + b1 * U[2 * id[1] + c]) + b2 * U[2 * id[2] + c];
+}
+/* __anyhit__mesh (shaders.optix.cu:132-152) with CMaterial::opaque (material.hpp:130-137):
+ * a hit on a mesh with texture coordinates is ignored where the material's alpha
+ * mask is not > 0 */
+static int hit_opaque(const r2o_mesh_desc* md, int t, float b1, float b2) {
+  if (!md->uvs) return 1;
+  const r2o_texture* at = mat_tex(md, tri_material(md, t), TEX_ALPHA);
+  if (!at) return 1;
+  float tc[2], v[4];
+  tri_tc(md, t, b1, b2, tc);
+  tex_fetch(at, tc[0], tc[1], v);
+  return v[0] > 0.f;
+}
+
+/* closest opaque hit in (0, tmax]: lexicographic minimum of (t, triangle index) */
 static int closest_hit(const r2o_filter_ctx* b, f3 o, f3 d, float tmax, int* tri, float* tb1, float* tb2, float* tt) {
   const wt_ray w = wt_setup(d);
   const f3 inv = F3(1.f / d.x, 1.f / d.y, 1.f / d.z);
@@ -387,7 +444,7 @@ static int closest_hit(const r2o_filter_ctx* b, f3 o, f3 d, float tmax, int* tri
       for (int i = nd->a; i < nd->a + nd->count; ++i) {
         const int t = b->order[i];
         float th, b1, b2;
-        if (wt_hit(&w, o, b->p0[t], b->p1[t], b->p2[t], tmax, &th, &b1, &b2)) {
+        if (wt_hit(&w, o, b->p0[t], b->p1[t], b->p2[t], tmax, &th, &b1, &b2) && hit_opaque(b->mesh, t, b1, b2)) {
           if (best < 0 || th < bt || (th == bt && t < best)) {
             bt = th; best = t; *tb1 = b1; *tb2 = b2;
           }
@@ -483,6 +540,65 @@ static void fconst_init(fconst* c, const r2o_filter_grid* g) {
 
 typedef struct { float* o; float* d; float* tmax; int32_t* tri; float* t; } fprobe;
 
+/* __closesthit__ch for a mesh (shaders.optix.cu:48-114): the smooth normal from the
+ * barycentrics; with texture coordinates, the hit's tc and, where the material has a
+ * normal map and the triangle a tangent frame, CMaterial::normalmap(fromTBN(T, -B, N),
+ * tc) (material.hpp:143-156, objects.hpp:60-67); then modelToWorld and normalize.
+ * Without a normal map normalmap returns N, so the frame is only built with one. */
+static f3 hit_normal(const r2o_filter_ctx* b, int tri, float b1, float b2, int mat, float tc[2]) {
+  const r2o_mesh_desc* md = b->mesh;
+  const int32_t* idx = md->indices + 3 * tri;
+  const float* Nn = md->normals;
+  const f3 N0 = F3(Nn[3 * idx[0]], Nn[3 * idx[0] + 1], Nn[3 * idx[0] + 2]);
+  const f3 N1 = F3(Nn[3 * idx[1]], Nn[3 * idx[1] + 1], Nn[3 * idx[1] + 2]);
+  const f3 N2 = F3(Nn[3 * idx[2]], Nn[3 * idx[2] + 1], Nn[3 * idx[2] + 2]);
+  const float w0 = 1.f - b1 - b2;
+  const f3 gn = nrm(add(add(sscl(w0, N0), sscl(b1, N1)), sscl(b2, N2)));
+  f3 n = gn;
+  if (md->uvs) {
+    tri_tc(md, tri, b1, b2, tc);
+    const r2o_texture* nt = mat_tex(md, mat, TEX_NORMAL);
+    if (nt) {
+      const float* P = md->positions;
+      const float* U = md->uvs;
+      const f3 P0 = F3(P[3 * idx[0]], P[3 * idx[0] + 1], P[3 * idx[0] + 2]);
+      const f3 P1 = F3(P[3 * idx[1]], P[3 * idx[1] + 1], P[3 * idx[1] + 2]);
+      const f3 P2 = F3(P[3 * idx[2]], P[3 * idx[2] + 1], P[3 * idx[2] + 2]);
+      const f3 Ph = add(add(sscl(w0, P0), sscl(b1, P1)), sscl(b2, P2));
+      const f3 e1 = sub(P1, Ph), e2 = sub(P2, Ph);
+      const float d1x = U[2 * idx[1]] - tc[0], d1y = U[2 * idx[1] + 1] - tc[1];
+      const float d2x = U[2 * idx[2]] - tc[0], d2y = U[2 * idx[2] + 1] - tc[1];
+      const float den = d1x * d2y - d2x * d1y;
+      if (den != 0.f) {
+        const float f = 1.f / den;
+        const f3 T = F3(f * (d2y * e1.x - d1y * e2.x), f * (d2y * e1.y - d1y * e2.y), f * (d2y * e1.z - d1y * e2.z));
+        const f3 B = F3(f * (-d2x * e1.x + d1x * e2.x), f * (-d2x * e1.y + d1x * e2.y), f * (-d2x * e1.z + d1x * e2.z));
+        const int tz = T.x == 0.f && T.y == 0.f && T.z == 0.f, bz = B.x == 0.f && B.y == 0.f && B.z == 0.f;
+        if (!tz && !bz) {
+          const f3 Tn = nrm(T), Bn = nrm(F3(-B.x, -B.y, -B.z));
+          float v[4];
+          tex_fetch(nt, tc[0], tc[1], v);
+          const f3 m = F3(v[0] * 2.f - 1.f, v[1] * 2.f - 1.f, v[2] * 2.f - 1.f);
+          if (!(m.x == 0.f && m.y == 0.f && m.z == 0.f)) {
+            const f3 mn = nrm(m);
+            n = add(add(sscl(mn.x, Tn), sscl(mn.y, Bn)), sscl(mn.z, gn));
+          }
+        }
+      }
+    }
+  }
+  return nrm(xdir(&b->m2w, n));
+}
+
+/* CMaterial::diffuse / glossy (material.hpp:111-127): the texture at tc, else the constant */
+static f3 mat_color(const r2o_mesh_desc* md, int mat, int slot, const float* constant, const float tc[2]) {
+  const r2o_texture* t = mat_tex(md, mat, slot);
+  if (!t) return F3(constant[3 * mat], constant[3 * mat + 1], constant[3 * mat + 2]);
+  float v[4];
+  tex_fetch(t, tc[0], tc[1], v);
+  return F3(v[0], v[1], v[2]);
+}
+
 /* CMeshFilter::run for one voxel id; returns the hit count */
 static uint32_t filter_voxel(const r2o_filter_ctx* b, const fconst* fc, const r2o_filter_params* fp, int64_t vid,
                              uint8_t* o8, const fprobe* pr) {
@@ -533,16 +649,9 @@ static uint32_t filter_voxel(const r2o_filter_ctx* b, const fconst* fc, const r2
     if (hit) {
       rayTs[hits] = cur_tmax;
       ++hits;
-      /* closest-hit: smooth normal from the barycentrics, modelToWorld, normalize */
-      const int32_t* idx = md->indices + 3 * tri;
-      const float* Nn = md->normals;
-      const f3 N0 = F3(Nn[3 * idx[0]], Nn[3 * idx[0] + 1], Nn[3 * idx[0] + 2]);
-      const f3 N1 = F3(Nn[3 * idx[1]], Nn[3 * idx[1] + 1], Nn[3 * idx[1] + 2]);
-      const f3 N2 = F3(Nn[3 * idx[2]], Nn[3 * idx[2] + 1], Nn[3 * idx[2] + 2]);
-      const float w0 = 1.f - b1 - b2;
-      f3 n = nrm(add(add(sscl(w0, N0), sscl(b1, N1)), sscl(b2, N2)));
-      n = nrm(xdir(&b->m2w, n));
-      const int mat = md->material ? md->material[tri] : 0;
+      const int mat = tri_material(md, tri);
+      float tc[2] = {0.f, 0.f};
+      const f3 n = hit_normal(b, tri, b1, b2, mat, tc);
       const float sg = md->mat_roughness[mat];
       /* buildS(n, sigma) = n n^T + sigma (I - n n^T) */
       const float xx = n.x * n.x, xy = n.x * n.y, xz = n.x * n.z, yy = n.y * n.y, yz = n.y * n.z, zz = n.z * n.z;
@@ -552,8 +661,8 @@ static uint32_t filter_voxel(const r2o_filter_ctx* b, const fconst* fc, const r2
       S[3] += xy + sg * (-xy);
       S[4] += xz + sg * (-xz);
       S[5] += yz + sg * (-yz);
-      dif = add(dif, F3(md->mat_diffuse[3 * mat], md->mat_diffuse[3 * mat + 1], md->mat_diffuse[3 * mat + 2]));
-      spec = add(spec, F3(md->mat_specular[3 * mat], md->mat_specular[3 * mat + 1], md->mat_specular[3 * mat + 2]));
+      dif = add(dif, mat_color(md, mat, TEX_DIFFUSE, md->mat_diffuse, tc));
+      spec = add(spec, mat_color(md, mat, TEX_GLOSSY, md->mat_specular, tc));
       dist += cur_tmax;
       nsum = add(nsum, n);
       ior += md->mat_ior[mat];
@@ -609,6 +718,9 @@ int r2o_filter_probe(const r2o_filter_ctx* b, const r2o_filter_grid* g, const r2
   const fprobe pr = {o, d, tmax, tri, t};
   return (int)filter_voxel(b, &fc, fp, vid, rec, &pr);
 }
+
+/* probe: one texture fetch (for tests of the texture restatement) */
+void r2o_tex_fetch(const r2o_texture* t, float x, float y, float out[4]) { tex_fetch(t, x, y, out); }
 
 /* probe: closest hit of one world-space ray (for tests of the triangle query) */
 int r2o_filter_closest_hit(const r2o_filter_ctx* b, const float o[3], const float d[3], float tmax, float out[3]) {

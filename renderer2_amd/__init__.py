@@ -225,6 +225,21 @@ class Context:
         d.pos[:] = [float(v) for v in mesh.get("pos", (0, 0, 0))]
         d.orientation[:] = [float(v) for v in mesh.get("orientation", (0, 1, 0))]
         d.scaling[:] = [float(v) for v in mesh.get("scaling", (1, 1, 1))]
+        # textured materials: uvs (V,2), textures [(H,W,4) RGBA float32], mat_textures (M,4)
+        uv = mesh.get("uvs")
+        keep["uvs"] = None if uv is None else np.ascontiguousarray(uv, np.float32)
+        keep["textures"] = [np.ascontiguousarray(t, np.float32) for t in mesh.get("textures", [])]
+        tex = (capi.TextureDesc * max(1, len(keep["textures"])))()
+        for i, t in enumerate(keep["textures"]):
+            if t.ndim != 3 or t.shape[2] != 4:
+                raise ValueError("textures are (height, width, 4) RGBA arrays")
+            tex[i].height, tex[i].width, tex[i].rgba = t.shape[0], t.shape[1], t.ctypes.data
+        mt = mesh.get("mat_textures")
+        keep["mat_textures"] = None if mt is None else np.ascontiguousarray(mt, np.int32).reshape(-1, 4)
+        d.uvs = None if keep["uvs"] is None else keep["uvs"].ctypes.data
+        d.n_textures = len(keep["textures"])
+        d.textures = C.cast(tex, C.c_void_p).value if keep["textures"] else None
+        d.mat_textures = None if keep["mat_textures"] is None else keep["mat_textures"].ctypes.data
         h = C.c_void_p()
         self._check(capi.lib().r2_mesh_upload(self.h, C.byref(d), C.byref(h)))
         return Mesh(self, h)

@@ -94,6 +94,10 @@ class Stats(C.Structure):
     ]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgba", C.c_void_p)]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [
         ("n_vertices", C.c_int32), ("n_triangles", C.c_int32),
@@ -102,6 +106,7 @@ class MeshDesc(C.Structure):
         ("mat_diffuse", C.c_void_p), ("mat_specular", C.c_void_p),
         ("mat_roughness", C.c_void_p), ("mat_ior", C.c_void_p),
         ("pos", f3), ("orientation", f3), ("scaling", f3),
+        ("uvs", C.c_void_p), ("n_textures", C.c_int32), ("textures", C.c_void_p), ("mat_textures", C.c_void_p),
     ]
 
 

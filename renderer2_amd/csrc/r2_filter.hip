@@ -41,6 +41,16 @@ struct r2_mesh {
   float* nrm = nullptr;    // 3 per vertex, model space
   int32_t* mat = nullptr;  // material id per triangle
   float* matf = nullptr;   // 8 per material: diffuse rgb, specular rgb, roughness, ior
+  // textured materials: texture coordinates, model-space positions (tangent
+  // frames of normal-mapped hits), an RGBA texel pool with (first texel, width,
+  // height) per texture, and 4 texture indices per material (diffuse, glossy,
+  // normal map, alpha mask; -1 none)
+  float2* uv = nullptr;
+  float* pos = nullptr;
+  float4* texels = nullptr;
+  int4* tex = nullptr;
+  int4* mtex = nullptr;
+  bool any_alpha = false, any_normal = false;
   std::vector<void*> allocs;
 };
 
@@ -63,6 +73,14 @@ struct FConst {
   float sigma_t, alpha;
   uint32_t seed, lod;
   int n_materials;
+  // textured materials: uv null without texture coordinates, mtex null without
+  // material textures
+  const float2* uv;
+  const float* pos;
+  const float4* texels;
+  const int4* tex;
+  const int4* mtex;
+  int alpha_on, normal_on;  // some material has an alpha mask / a normal map
 };
 
 __device__ __forceinline__ float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
@@ -475,8 +493,102 @@ __device__ __forceinline__ bool wt_hit(const WRay& w, v3 o, v3 v0, v3 v1, v3 v2,
   return true;
 }
 
+// ------------------------------------------------------------ textures ---
+// CTexture::operator() (texture.hpp:55-58): tex2D<float4> with linear filtering,
+// wrap addressing and normalised coordinates (texture.cu:166-189), texel centres
+// at (i + 0.5) / width; full-precision weights (CUDA's are 8-bit fractions).
+// The blend is two nested linear interpolations, so that a constant texture
+// returns its value exactly, as tex2D's fixed-point weights (which sum to one)
+// do: the normal map's zero test depends on it.  info = (first texel, width,
+// height).
+__device__ __forceinline__ float4 tex_fetch(const float4* __restrict__ texels, int4 info, float x, float y) {
+  const int W = info.y, H = info.z;
+  x = x - floorf(x);
+  y = y - floorf(y);
+  const float tx = x * (float)W - 0.5f, ty = y * (float)H - 0.5f;
+  const float fx = floorf(tx), fy = floorf(ty);
+  const float a = tx - fx, b = ty - fy;
+  int i0 = (int)fx, j0 = (int)fy;
+  int i1 = i0 + 1, j1 = j0 + 1;
+  i0 = ((i0 % W) + W) % W;
+  i1 = ((i1 % W) + W) % W;
+  j0 = ((j0 % H) + H) % H;
+  j1 = ((j1 % H) + H) % H;
+  const float4* t = texels + info.x;
+  const float4 p00 = t[j0 * W + i0], p10 = t[j0 * W + i1], p01 = t[j1 * W + i0], p11 = t[j1 * W + i1];
+  auto blend = [a, b](float v00, float v10, float v01, float v11) {
+    const float top = v00 + a * (v10 - v00);
+    const float bot = v01 + a * (v11 - v01);
+    return top + b * (bot - top);
+  };
+  return make_float4(blend(p00.x, p10.x, p01.x, p11.x), blend(p00.y, p10.y, p01.y, p11.y),
+                     blend(p00.z, p10.z, p01.z, p11.z), blend(p00.w, p10.w, p01.w, p11.w));
+}
+// barycentric texture coordinate: (1 - b1 - b2) TC0 + b1 TC1 + b2 TC2 (shaders.optix.cu:61-66)
+__device__ __forceinline__ float2 tri_tc(const FConst& c, const int32_t* __restrict__ tidx, int t, float b1,
+                                         float b2) {
+  const float2 U0 = c.uv[tidx[3 * t]], U1 = c.uv[tidx[3 * t + 1]], U2 = c.uv[tidx[3 * t + 2]];
+  const float w0 = 1.f - b1 - b2;
+  return make_float2((w0 * U0.x + b1 * U1.x) + b2 * U2.x, (w0 * U0.y + b1 * U1.y) + b2 * U2.y);
+}
+// __anyhit__mesh (shaders.optix.cu:132-152) with CMaterial::opaque (material.hpp:130-137):
+// a hit on a textured mesh is ignored where the material's alpha mask is not > 0
+__device__ __forceinline__ bool hit_opaque(const FConst& c, const int32_t* __restrict__ tidx,
+                                           const int32_t* __restrict__ tmat, int t, float b1, float b2) {
+  int m = tmat ? tmat[t] : 0;
+  m = (m >= 0 && m < c.n_materials) ? m : 0;
+  const int at = c.mtex[m].w;
+  if (at < 0) return true;
+  const float2 tc = tri_tc(c, tidx, t, b1, b2);
+  return tex_fetch(c.texels, c.tex[at], tc.x, tc.y).x > 0.f;
+}
+// __closesthit__ch for a mesh (shaders.optix.cu:48-114): the smooth normal; where
+// the material has a normal map and the triangle a tangent frame,
+// CMaterial::normalmap(fromTBN(T, -B, N), tc) (material.hpp:143-156,
+// objects.hpp:60-67); then modelToWorld and normalize
+__device__ __forceinline__ v3 hit_normal(const FConst& c, const int32_t* __restrict__ tidx,
+                                         const float* __restrict__ nrm, int t, float b1, float b2, int nt, float2 tc) {
+  const int i0 = tidx[3 * t], i1 = tidx[3 * t + 1], i2 = tidx[3 * t + 2];
+  const v3 N0 = mk(nrm[3 * i0], nrm[3 * i0 + 1], nrm[3 * i0 + 2]);
+  const v3 N1 = mk(nrm[3 * i1], nrm[3 * i1 + 1], nrm[3 * i1 + 2]);
+  const v3 N2 = mk(nrm[3 * i2], nrm[3 * i2 + 1], nrm[3 * i2 + 2]);
+  const float w0 = 1.f - b1 - b2;
+  const v3 gn = normalize((w0 * N0 + b1 * N1) + b2 * N2);
+  v3 n = gn;
+  if (nt >= 0) {
+    const float* P = c.pos;
+    const v3 P0 = mk(P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]);
+    const v3 P1 = mk(P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]);
+    const v3 P2 = mk(P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]);
+    const v3 Ph = (w0 * P0 + b1 * P1) + b2 * P2;
+    const v3 e1 = P1 - Ph, e2 = P2 - Ph;
+    const float2 U1 = c.uv[i1], U2 = c.uv[i2];
+    const float d1x = U1.x - tc.x, d1y = U1.y - tc.y, d2x = U2.x - tc.x, d2y = U2.y - tc.y;
+    const float den = d1x * d2y - d2x * d1y;
+    if (den != 0.f) {
+      const float f = 1.f / den;
+      const v3 T = mk(f * (d2y * e1.x - d1y * e2.x), f * (d2y * e1.y - d1y * e2.y), f * (d2y * e1.z - d1y * e2.z));
+      const v3 B = mk(f * (-d2x * e1.x + d1x * e2.x), f * (-d2x * e1.y + d1x * e2.y), f * (-d2x * e1.z + d1x * e2.z));
+      const bool tz = T.x == 0.f && T.y == 0.f && T.z == 0.f, bz = B.x == 0.f && B.y == 0.f && B.z == 0.f;
+      if (!tz && !bz) {
+        const v3 Tn = normalize(T), Bn = normalize(mk(-B.x, -B.y, -B.z));
+        const float4 v = tex_fetch(c.texels, c.tex[nt], tc.x, tc.y);
+        const v3 m = mk(v.x * 2.f - 1.f, v.y * 2.f - 1.f, v.z * 2.f - 1.f);
+        if (!(m.x == 0.f && m.y == 0.f && m.z == 0.f)) {
+          const v3 mn = normalize(m);
+          n = (mn.x * Tn + mn.y * Bn) + mn.z * gn;
+        }
+      }
+    }
+  }
+  return normalize(mul_dir(c.mesh_m2w, n));
+}
+
 // one wave per work voxel.  direct: work[w] indexes act/off/cnt; otherwise work
 // holds voxel ids, looked up in the sorted active list, and records go to out[w].
+// kTex: the mesh has texture coordinates or material textures (alpha test,
+// textured colours, normal maps); without, the kernel carries none of that code.
+template <bool kTex>
 __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __restrict__ work, uint32_t n_work,
                                                 int direct, const uint32_t* __restrict__ act,
                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt,
@@ -550,7 +662,8 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
       const uint32_t t = cand[beg + k];
       const v3 p0 = f4v(tri[3 * t]), p1 = f4v(tri[3 * t + 1]), p2 = f4v(tri[3 * t + 2]);
       float th, b1, b2;
-      if (live && wt_hit(wr, o, p0, p1, p2, cur_tmax, &th, &b1, &b2)) {
+      if (live && wt_hit(wr, o, p0, p1, p2, cur_tmax, &th, &b1, &b2) &&
+          (!kTex || !c.alpha_on || hit_opaque(c, tidx, tmat, (int)t, b1, b2))) {
         if (best < 0 || th < bt || (th == bt && (int)t < best)) {
           best = (int)t;
           bt = th;
@@ -564,17 +677,15 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
 #pragma unroll
     for (int q = 0; q < kVals; ++q) val[q] = 0.f;
     if (hit) {
-      // closest-hit: barycentric smooth normal, modelToWorld, normalize (shaders.optix.cu:48-58,106-114)
-      const int i0 = tidx[3 * best], i1 = tidx[3 * best + 1], i2 = tidx[3 * best + 2];
-      const v3 N0 = mk(nrm[3 * i0], nrm[3 * i0 + 1], nrm[3 * i0 + 2]);
-      const v3 N1 = mk(nrm[3 * i1], nrm[3 * i1 + 1], nrm[3 * i1 + 2]);
-      const v3 N2 = mk(nrm[3 * i2], nrm[3 * i2 + 1], nrm[3 * i2 + 2]);
-      const float w0 = 1.f - bb1 - bb2;
-      v3 n = normalize((w0 * N0 + bb1 * N1) + bb2 * N2);
-      n = normalize(mul_dir(c.mesh_m2w, n));
+      // closest-hit: the hit's texture coordinate (0 without texture coordinates) and
+      // normal (shaders.optix.cu:48-114)
       int m = tmat ? tmat[best] : 0;
       m = (m >= 0 && m < c.n_materials) ? m : 0;
       const float* mf = matf + 8 * m;
+      const int4 mt = kTex && c.mtex ? c.mtex[m] : make_int4(-1, -1, -1, -1);
+      float2 tc = make_float2(0.f, 0.f);
+      if (kTex && c.uv) tc = tri_tc(c, tidx, best, bb1, bb2);
+      const v3 n = hit_normal(c, tidx, nrm, best, bb1, bb2, kTex && c.normal_on ? mt.z : -1, tc);
       const float sg = mf[6];
       // buildS(n, sigma) = n n^T + sigma (I - n n^T)  (sggx_phase_function.hpp:95-105)
       const float xx = n.x * n.x, xy = n.x * n.y, xz = n.x * n.z, yy = n.y * n.y, yz = n.y * n.z, zz = n.z * n.z;
@@ -584,12 +695,16 @@ __global__ __launch_bounds__(256) void k_filter(FConst c, const uint32_t* __rest
       val[3] = xy + sg * (-xy);
       val[4] = xz + sg * (-xz);
       val[5] = yz + sg * (-yz);
-      val[6] = mf[0];
-      val[7] = mf[1];
-      val[8] = mf[2];
-      val[9] = mf[3];
-      val[10] = mf[4];
-      val[11] = mf[5];
+      // CMaterial::diffuse / glossy (material.hpp:111-127): the texture at tc, else the constant
+      float4 dc = make_float4(mf[0], mf[1], mf[2], 0.f), gc = make_float4(mf[3], mf[4], mf[5], 0.f);
+      if (kTex && mt.x >= 0) dc = tex_fetch(c.texels, c.tex[mt.x], tc.x, tc.y);
+      if (kTex && mt.y >= 0) gc = tex_fetch(c.texels, c.tex[mt.y], tc.x, tc.y);
+      val[6] = dc.x;
+      val[7] = dc.y;
+      val[8] = dc.z;
+      val[9] = gc.x;
+      val[10] = gc.y;
+      val[11] = gc.z;
       val[12] = n.x;
       val[13] = n.y;
       val[14] = n.z;
@@ -788,6 +903,13 @@ FConst make_consts(const r2_mesh* m, const r2_filter_grid& g, const r2_filter_pa
   c.seed = p.seed;
   c.lod = (uint32_t)p.lod;
   c.n_materials = m->n_materials;
+  c.uv = m->uv;
+  c.pos = m->pos;
+  c.texels = m->texels;
+  c.tex = m->tex;
+  c.mtex = m->mtex;
+  c.alpha_on = m->uv && m->any_alpha ? 1 : 0;
+  c.normal_on = m->uv && m->any_normal ? 1 : 0;
   return c;
 }
 
@@ -921,8 +1043,12 @@ int launch_filter(r2_ctx* ctx, const r2_mesh* m, const FConst& c, const Binning&
   const size_t lds = sizeof(float) * kWaves * (kVals * kRow + ts_cap);
   const unsigned blocks = (n_work + kWaves - 1) / kWaves;
   R2F_HIP(hipEventRecord(ctx->ev0, ctx->stream), "filter: event");
-  hipLaunchKernelGGL(k_filter, dim3(blocks), dim3(256), lds, ctx->stream, c, work, n_work, direct ? 1 : 0, B.act,
-                     B.off, B.cnt, B.n_act, B.cand, m->tri, m->idx, m->nrm, m->mat, m->matf, out, ts_cap);
+  if (c.uv || c.mtex)
+    hipLaunchKernelGGL(k_filter<true>, dim3(blocks), dim3(256), lds, ctx->stream, c, work, n_work, direct ? 1 : 0,
+                       B.act, B.off, B.cnt, B.n_act, B.cand, m->tri, m->idx, m->nrm, m->mat, m->matf, out, ts_cap);
+  else
+    hipLaunchKernelGGL(k_filter<false>, dim3(blocks), dim3(256), lds, ctx->stream, c, work, n_work, direct ? 1 : 0,
+                       B.act, B.off, B.cnt, B.n_act, B.cand, m->tri, m->idx, m->nrm, m->mat, m->matf, out, ts_cap);
   R2F_HIP(hipGetLastError(), "filter: k_filter");
   R2F_HIP(hipEventRecord(ctx->ev1, ctx->stream), "filter: event");
   R2F_HIP(hipEventSynchronize(ctx->ev1), "filter: sync");
@@ -956,7 +1082,26 @@ extern "C" int r2_mesh_upload(r2_ctx* ctx, const r2_mesh_desc* d, r2_mesh** out)
     for (int64_t t = 0; t < d->n_triangles; ++t)
       if (d->material[t] < 0 || d->material[t] >= d->n_materials)
         return r2host::fail(ctx, R2_ERR_INVALID, "mesh: material id out of range");
+  if (d->n_textures < 0 || (d->n_textures > 0 && !d->textures))
+    return r2host::fail(ctx, R2_ERR_INVALID, "mesh: missing texture table");
+  int64_t n_texels = 0;
+  for (int i = 0; i < d->n_textures; ++i) {
+    const r2_texture_desc& t = d->textures[i];
+    if (t.width <= 0 || t.height <= 0 || !t.rgba) return r2host::fail(ctx, R2_ERR_INVALID, "mesh: empty texture");
+    n_texels += (int64_t)t.width * t.height;
+  }
+  if (n_texels >= (1ll << 31)) return r2host::fail(ctx, R2_ERR_UNSUPPORTED, "mesh: more than 2^31 texels");
+  bool any_alpha = false, any_normal = false;
+  if (d->mat_textures)
+    for (int64_t i = 0; i < 4ll * d->n_materials; ++i) {
+      if (d->mat_textures[i] < -1 || d->mat_textures[i] >= d->n_textures)
+        return r2host::fail(ctx, R2_ERR_INVALID, "mesh: texture index out of range");
+      any_normal = any_normal || (i % 4 == 2 && d->mat_textures[i] >= 0);
+      any_alpha = any_alpha || (i % 4 == 3 && d->mat_textures[i] >= 0);
+    }
   r2_mesh* m = new r2_mesh();
+  m->any_alpha = any_alpha;
+  m->any_normal = any_normal;
   m->n_vertices = d->n_vertices;
   m->n_triangles = d->n_triangles;
   m->n_materials = d->n_materials;
@@ -1023,13 +1168,48 @@ extern "C" int r2_mesh_upload(r2_ctx* ctx, const r2_mesh_desc* d, r2_mesh** out)
   }
   if ((e = hipMemcpyAsync(m->matf, matf.data(), sizeof(float) * matf.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
     return bail(e, "mesh: copy");
+  // textured materials: texture coordinates, and the model-space positions that
+  // normal-mapped hits build their tangent frames from
+  const bool keep_pos = d->uvs && any_normal;
+  if (d->uvs) {
+    if ((e = dalloc((void**)&m->uv, sizeof(float2) * nv)) != hipSuccess) return bail(e, "mesh: alloc");
+    if ((e = hipMemcpyAsync(m->uv, d->uvs, sizeof(float2) * nv, hipMemcpyHostToDevice, s)) != hipSuccess)
+      return bail(e, "mesh: copy");
+  }
+  std::vector<int4> tex_tab((size_t)std::max(d->n_textures, 1));
+  if (d->n_textures > 0) {
+    if ((e = dalloc((void**)&m->texels, sizeof(float4) * (size_t)n_texels)) != hipSuccess) return bail(e, "mesh: alloc");
+    if ((e = dalloc((void**)&m->tex, sizeof(int4) * (size_t)d->n_textures)) != hipSuccess) return bail(e, "mesh: alloc");
+    int64_t first = 0;
+    for (int i = 0; i < d->n_textures; ++i) {
+      const r2_texture_desc& t = d->textures[i];
+      const size_t n = (size_t)t.width * t.height;
+      if ((e = hipMemcpyAsync(m->texels + first, t.rgba, sizeof(float4) * n, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return bail(e, "mesh: copy");
+      tex_tab[i] = make_int4((int)first, t.width, t.height, 0);
+      first += (int64_t)n;
+    }
+    if ((e = hipMemcpyAsync(m->tex, tex_tab.data(), sizeof(int4) * d->n_textures, hipMemcpyHostToDevice, s)) !=
+        hipSuccess)
+      return bail(e, "mesh: copy");
+  }
+  if (d->mat_textures) {
+    if ((e = dalloc((void**)&m->mtex, sizeof(int4) * (size_t)d->n_materials)) != hipSuccess) return bail(e, "mesh: alloc");
+    if ((e = hipMemcpyAsync(m->mtex, d->mat_textures, sizeof(int4) * d->n_materials, hipMemcpyHostToDevice, s)) !=
+        hipSuccess)
+      return bail(e, "mesh: copy");
+  }
   hipLaunchKernelGGL(k_world_tris, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, dpos, m->idx, m->m2w,
                      (int)nt, m->tri);
   if ((e = hipGetLastError()) != hipSuccess) return bail(e, "mesh: k_world_tris");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return bail(e, "mesh: sync");
-  // positions are only needed for the world-space triangles
-  (void)hipFree(dpos);
-  m->allocs.erase(std::find(m->allocs.begin(), m->allocs.end(), (void*)dpos));
+  if (keep_pos) {
+    m->pos = dpos;
+  } else {
+    // positions are only needed for the world-space triangles
+    (void)hipFree(dpos);
+    m->allocs.erase(std::find(m->allocs.begin(), m->allocs.end(), (void*)dpos));
+  }
   *out = m;
   return R2_OK;
 }

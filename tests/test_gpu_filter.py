@@ -158,3 +158,60 @@ def test_filtered_lod_render_converges_to_oracle_fp64(ctx):
     e = float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
     print(f"filtered 32^3 LOD, 32^2: relMSE(GPU 8192 spp, oracle FP64 4096 spp) = {e:.3e}")
     assert ref.max() > 0 and e <= 1e-3
+
+
+def _textured_foliage(n_quads=90, seed=11, with_uvs=True):
+    """Randomly placed and oriented quads with three materials: a textured diffuse
+    colour with a disk-shaped alpha mask and a bumpy normal map; a textured glossy
+    colour with a striped alpha mask; constant colours."""
+    from test_oracle_filter import quad_mesh
+    from test_oracle_textures import quad_uvs
+
+    rng = np.random.default_rng(seed)
+    quads = []
+    for _ in range(n_quads):
+        n = rng.normal(size=3)
+        quads.append((rng.uniform(-0.8, 0.8, 3), n / np.linalg.norm(n), float(rng.uniform(0.15, 0.45))))
+    m = quad_mesh(quads, material=rng.integers(0, 3, n_quads))
+    m["mat_diffuse"] = np.array([[0.5, 0.25, 0.75], [0.9, 0.1, 0.2], [0.3, 0.6, 0.1]], np.float32)
+    m["mat_specular"] = np.array([[0.04, 0.5, 1.0], [0.3, 0.3, 0.3], [0.1, 0.2, 0.3]], np.float32)
+    m["mat_roughness"] = np.array([0.2, 0.7, 0.4], np.float32)
+    m["mat_ior"] = np.array([1.5, 1.25, 1.33], np.float32)
+    yy, xx = np.mgrid[0:32, 0:32]
+    disk = ((xx - 15.5) ** 2 + (yy - 15.5) ** 2 < 13.0 ** 2).astype(np.float32)
+    stripes = ((xx // 4) % 2 == 0).astype(np.float32)
+    j, i = np.mgrid[0:16, 0:16].astype(np.float32)
+    bumps = np.stack([0.5 + 0.3 * np.sin(i / 2.5), 0.5 + 0.3 * np.cos(j / 3.0), 0.8 + 0.1 * np.sin((i + j) / 4.0)], -1)
+    textures = [
+        rng.uniform(0.05, 0.95, (16, 16, 4)).astype(np.float32),                          # 0 diffuse
+        np.dstack([disk, disk, disk, np.ones_like(disk)]),                                # 1 alpha disk
+        np.dstack([bumps, np.ones((16, 16), np.float32)]).astype(np.float32),             # 2 normal map
+        rng.uniform(0.0, 0.6, (8, 8, 4)).astype(np.float32),                              # 3 glossy
+        np.dstack([stripes, stripes, stripes, np.ones_like(stripes)]),                    # 4 alpha stripes
+    ]
+    m["textures"] = textures
+    m["mat_textures"] = np.array([[0, -1, 2, 1], [-1, 3, -1, 4], [-1, -1, -1, -1]], np.int32)
+    if with_uvs:
+        m["uvs"] = quad_uvs(n_quads)
+    return m
+
+
+@pytest.mark.parametrize("with_uvs", [True, False])
+def test_filter_textured_materials_bit_exact(ctx, with_uvs):
+    """Alpha-masked, textured and normal-mapped foliage (shaders.optix.cu:48-152,
+    material.hpp:111-156): GPU records equal the oracle's bit for bit.  Without
+    texture coordinates every hit reads its textures at tc = (0, 0) and no alpha
+    test or normal map applies."""
+    m = _textured_foliage(with_uvs=with_uvs)
+    gm = ctx.upload_mesh(m)
+    of = O.MeshFilter(m)
+    for n, lod in ((24, 0), (12, 1)):
+        vs = scenes.finest_voxel_size(m, n)
+        rec, g = ctx.filter_lod(gm, r2.filter_params(vs, 64, 8, 10.0, 0.1, True, seed=9, lod=lod))
+        og = of.setup(vs)
+        ref = of.run(og, samples=64, iterations=8, sigma_t=10.0, alpha=0.1, clip_rays=True, seed=9,
+                     lod=lod).reshape(rec.shape)
+        d = O.decode_records(ref.reshape(-1, 32))
+        print(f"textured foliage uvs={with_uvs} {n}^3: {int((d['density'] > 0).sum())} voxels with hits", flush=True)
+        assert (d["density"] > 0).sum() > 50
+        assert np.array_equal(rec, ref), f"{n}^3: " + _diff_report(rec, ref)
