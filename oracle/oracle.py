@@ -30,11 +30,42 @@ class InstanceDesc(C.Structure):
     _fields_ = [("grid", C.c_int), ("pos", C.c_float * 3), ("orientation", C.c_float * 3), ("scaling", C.c_float * 3)]
 
 
+MAX_SURF = 8
+SHAPES = {"circle": 0, "sphere": 1}
+
+
+class SurfaceDesc(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int), ("pos", C.c_float * 3), ("radius", C.c_float), ("normal", C.c_float * 3),
+        ("diffuse", C.c_float * 3), ("diffuse_roughness", C.c_float), ("specular", C.c_float * 3),
+        ("alpha_x", C.c_float), ("alpha_y", C.c_float), ("eta_i", C.c_float), ("eta_t", C.c_float),
+    ]
+
+
+def surface_desc(sf: dict) -> SurfaceDesc:
+    """Surface dict (shape, pos, radius, normal, diffuse, diffuse_roughness,
+    specular, alpha_x, alpha_y, eta_i, eta_t) -> SurfaceDesc."""
+    d = SurfaceDesc()
+    d.shape = SHAPES[sf.get("shape", "circle")]
+    d.pos[:] = list(sf.get("pos", (0, 0, 0)))
+    d.radius = sf.get("radius", 1.0)
+    d.normal[:] = list(sf.get("normal", (0, 1, 0)))
+    d.diffuse[:] = list(sf.get("diffuse", (0.5, 0.5, 0.5)))
+    d.diffuse_roughness = sf.get("diffuse_roughness", 0.0)
+    d.specular[:] = list(sf.get("specular", (0, 0, 0)))
+    d.alpha_x = sf.get("alpha_x", 0.5)
+    d.alpha_y = sf.get("alpha_y", 0.5)
+    d.eta_i = sf.get("eta_i", 1.0)
+    d.eta_t = sf.get("eta_t", 1.5)
+    return d
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("n_grids", C.c_int), ("grids", GridDesc * MAX_INST),
         ("n_instances", C.c_int), ("instances", InstanceDesc * MAX_INST),
         ("env_rgb", C.c_void_p), ("env_w", C.c_int), ("env_h", C.c_int),
+        ("n_surfaces", C.c_int), ("surfaces", SurfaceDesc * MAX_SURF),
     ]
 
 
@@ -173,6 +204,14 @@ def lib() -> C.CDLL:
             g("r2o_direct_mean").argtypes = [vp, C.c_int, vp, vp, vp, R, vp, C.c_int, C.c_uint32, C.c_uint32,
                                              C.c_int64, vp, vp]
             g("r2o_phase_p_mean").argtypes = [vp, vp, R, vp, vp, C.c_int64, C.c_uint32, C.c_uint32, C.c_int, vp]
+            g("r2o_bsdf_eval").argtypes = [C.POINTER(SurfaceDesc), vp, vp, vp]
+            g("r2o_ggx_eval").argtypes = [C.POINTER(SurfaceDesc), vp, vp, vp, vp]
+            g("r2o_ggx_sample_h").argtypes = [C.POINTER(SurfaceDesc), vp, R, R, vp]
+            g("r2o_bsdf_sample").argtypes = [C.POINTER(SurfaceDesc), vp, C.c_uint32, C.c_uint32, vp]
+            g("r2o_fresnel_ie").argtypes = [R, R, R]
+            g("r2o_fresnel_ie").restype = R
+            g("r2o_surface_hit").argtypes = [C.POINTER(SurfaceDesc), vp, vp]
+            g("r2o_surface_hit").restype = C.c_int
         L.r2o_filter_setup.argtypes = [C.POINTER(MeshDesc), C.c_float, C.POINTER(FilterGrid)]
         L.r2o_filter_new.argtypes = [C.POINTER(MeshDesc)]
         L.r2o_filter_new.restype = vp
@@ -244,8 +283,9 @@ class Brickgrid:
 class Scene:
     """Oracle scene: grids (records) + instances + env map, FP32 or FP64."""
 
-    def __init__(self, grids, instances, env_rgb: np.ndarray, precision: str = "f64"):
+    def __init__(self, grids, instances, env_rgb: np.ndarray, precision: str = "f64", surfaces=()):
         assert precision in ("f32", "f64")
+        assert len(surfaces) <= MAX_SURF
         self.sfx = precision
         self.R = np.float32 if precision == "f32" else np.float64
         self._keep = []
@@ -273,6 +313,9 @@ class Scene:
         self._keep.append(env)
         d.env_h, d.env_w = env.shape[0], env.shape[1]
         d.env_rgb = _p(env)
+        d.n_surfaces = len(surfaces)
+        for j, sf in enumerate(surfaces):
+            d.surfaces[j] = surface_desc(sf)
         self.desc = d
         self.h = self._fn("r2o_scene_new")(C.byref(d))
 
@@ -411,6 +454,59 @@ class Scene:
         cells = np.zeros((max_cells, 4), np.int32)
         n = self._fn("r2o_dda_cells")(self.h, medium, _p(r), sigma_t, collide_every, max_cells, _p(cells))
         return cells[:n].copy()
+
+
+class Surface:
+    """One analytic surface with its material, for the BSDF and shape probes."""
+
+    def __init__(self, precision: str = "f64", **sf):
+        self.sfx = precision
+        self.R = np.float32 if precision == "f32" else np.float64
+        self.desc = surface_desc(sf)
+
+    def _fn(self, name):
+        return getattr(lib(), f"{name}_{self.sfx}")
+
+    def _a(self, x):
+        return np.ascontiguousarray(x, self.R)
+
+    def eval(self, wo, wi) -> dict:
+        """Material f (rgb) and mixture pdf, both lobes' f and pdf, Fresnel weight."""
+        out = np.zeros(9, self.R)
+        o, i = self._a(wo), self._a(wi)
+        self._fn("r2o_bsdf_eval")(C.byref(self.desc), _p(o), _p(i), _p(out))
+        return {"f": out[0:3].copy(), "pdf": out[3], "on_f": out[4], "on_pdf": out[5], "mf_f": out[6],
+                "mf_pdf": out[7], "weight": out[8]}
+
+    def ggx(self, wo, wi, h) -> dict:
+        out = np.zeros(4, self.R)
+        o, i, hh = self._a(wo), self._a(wi), self._a(h)
+        self._fn("r2o_ggx_eval")(C.byref(self.desc), _p(o), _p(i), _p(hh), _p(out))
+        return {"D": out[0], "G1": out[1], "G": out[2], "pdf": out[3]}
+
+    def sample_h(self, wo, u1: float, u2: float) -> np.ndarray:
+        out = np.zeros(3, self.R)
+        o = self._a(wo)
+        self._fn("r2o_ggx_sample_h")(C.byref(self.desc), _p(o), u1, u2, _p(out))
+        return out
+
+    def sample(self, wo, key: int, c0: int):
+        """(wi, f rgb, pdf) of CMaterial::sampleF on stream (key, c0)."""
+        out = np.zeros(7, self.R)
+        o = self._a(wo)
+        self._fn("r2o_bsdf_sample")(C.byref(self.desc), _p(o), key, c0, _p(out))
+        return out[0:3].copy(), out[3:6].copy(), out[6]
+
+    def hit(self, ray7):
+        """(hit, t, normal) of the shape test against ray (o, d, tmax)."""
+        r = self._a(ray7)
+        out = np.zeros(4, self.R)
+        ok = self._fn("r2o_surface_hit")(C.byref(self.desc), _p(r), _p(out))
+        return bool(ok), float(out[0]), out[1:4].copy()
+
+
+def fresnel_ie(cos_i: float, eta_i: float, eta_t: float, precision: str = "f64") -> float:
+    return fn("r2o_fresnel_ie", precision)(cos_i, eta_i, eta_t)
 
 
 def fn(name: str, precision: str = "f64"):

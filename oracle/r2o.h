@@ -65,6 +65,18 @@ typedef struct {
   float pos[3], orientation[3], scaling[3];
 } r2o_instance_desc;
 
+/* analytic surface with its material: CCircle / Sphere(worldPos, radius, normal)
+ * and CMaterial(diffuse, specular, COrenNayarBRDF(roughness),
+ * CMicrofacetBRDF(alphaX, alphaY, etaI, etaT)) (config_loader.cu:123-215,
+ * sceneobject.cu:28-48) */
+#define R2O_MAX_SURFACES 8
+enum { R2O_SHAPE_CIRCLE = 0, R2O_SHAPE_SPHERE = 1 };
+typedef struct {
+  int shape;
+  float pos[3], radius, normal[3];
+  float diffuse[3], diffuse_roughness, specular[3], alpha_x, alpha_y, eta_i, eta_t;
+} r2o_surface_desc;
+
 typedef struct {
   int n_grids;
   r2o_grid_desc grids[R2O_MAX_INSTANCES];
@@ -72,6 +84,8 @@ typedef struct {
   r2o_instance_desc instances[R2O_MAX_INSTANCES];
   const float* env_rgb;     /* env_h rows of env_w RGB float texels (row 0 = theta 0) */
   int env_w, env_h;
+  int n_surfaces;
+  r2o_surface_desc surfaces[R2O_MAX_SURFACES];
 } r2o_scene_desc;
 
 typedef struct {

@@ -398,11 +398,24 @@ typedef struct {
   R ratio;           /* filterRenderRatio */
 } instance_t;
 
+/* CCircle / Sphere with the CMaterial of CHostSceneobject (sceneobject.cu:28-48) */
+typedef struct {
+  int shape;
+  v3 pos, normal;
+  R radius;
+  m43 model_to_world, world_to_model; /* translate(pos) * getRotation(normal) (shape.hpp:49-57) */
+  v3 diffuse, specular;
+  R on_a, on_b;                       /* COrenNayarBRDF(roughness) (oren_nayar_brdf.hpp:31-35) */
+  R alpha_x, alpha_y, eta_i, eta_t;   /* CMicrofacetBRDF */
+} surface_t;
+
 struct SFX(r2o_scene) {
   int n_media;
   medium_t media[R2O_MAX_INSTANCES];
   int n_inst;
   instance_t inst[R2O_MAX_INSTANCES];
+  int n_surf;
+  surface_t surf[R2O_MAX_SURFACES];
   /* env map */
   int env_w, env_h;
   R* tex;       /* env_h * env_w * 3 */
@@ -475,12 +488,12 @@ static R step_dda(v3 pos, v3 inv_dir, int mip) {
   return (myz < tx) ? myz : tx;
 }
 
-/* CFresnel::evaluate (fresnel.hpp:28-54), etaI = 1, etaT = ior */
-static R fresnel(R cos_i, R eta_t_in) {
+/* CFresnel::evaluate (fresnel.hpp:28-54) */
+static R fresnel_ie(R cos_i, R eta_i_in, R eta_t_in) {
   cos_i = cos_i < C(-1) ? C(-1) : (cos_i > C(1) ? C(1) : cos_i);
-  R eta_i = C(1), eta_t = eta_t_in;
+  R eta_i = eta_i_in, eta_t = eta_t_in;
   if (!(cos_i > C(0))) {
-    eta_i = eta_t_in; eta_t = C(1);
+    eta_i = eta_t_in; eta_t = eta_i_in;
     cos_i = RABS(cos_i);
   }
   const R sin_i = RSQRT(gmax(C(0), C(1) - cos_i * cos_i));
@@ -491,6 +504,8 @@ static R fresnel(R cos_i, R eta_t_in) {
   const R rper = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
   return (rpar * rpar + rper * rper) / C(2);
 }
+/* the SGGX phase's Fresnel: etaI = 1, etaT = voxel ior */
+static R fresnel(R cos_i, R eta_t_in) { return fresnel_ie(cos_i, C(1), eta_t_in); }
 
 /* Scattering event recorded by a medium sample (SInteraction of a medium hit). */
 typedef struct {
@@ -897,15 +912,252 @@ static v3 env_le(const scene_t* s, v3 d, R* pdf, uint64_t* cnt) {
   return env_tex(s, x, y);
 }
 
+/* ------------------------------------------------------ surface BSDFs --- */
+/* The reference evaluates M_PI, M_1_PI and a few literals in double inside
+ * float code; DMUL keeps those products in double and rounds once, as the
+ * float build does. */
+#define DPI 3.14159265358979323846
+#define D1PI 0.318309886183790671538
+static inline R dmul(double c, R x) { return (R)(c * (double)x); }
+
+/* brdf_functions.hpp:7-55 and sameHemisphere (functions.hpp:58-60) */
+static inline R cos2_theta(v3 w) { return w.z * w.z; }
+static inline R sin2_theta(v3 w) { return gmax(C(0), C(1) - cos2_theta(w)); }
+static inline R sin_theta(v3 w) { return RSQRT(sin2_theta(w)); }
+static inline R tan_theta(v3 w) { return sin_theta(w) / w.z; }
+static inline R tan2_theta(v3 w) { return sin2_theta(w) / cos2_theta(w); }
+static inline R clamp11(R x) { return x < C(-1) ? C(-1) : (x > C(1) ? C(1) : x); }
+static inline R cos_phi(v3 w) {
+  const R st = sin_theta(w);
+  return st == C(0) ? C(1) : clamp11(w.x / st);
+}
+static inline R sin_phi(v3 w) {
+  const R st = sin_theta(w);
+  return st == C(0) ? C(0) : clamp11(w.y / st);
+}
+static inline int same_hemisphere(v3 a, v3 b) { return a.z * b.z > C(0); }
+
+/* COrenNayarBRDF::f (oren_nayar_brdf.hpp:42-67) */
+static R on_f(const surface_t* m, v3 wo, v3 wi) {
+  const R st_i = sin_theta(wi), st_o = sin_theta(wo);
+  R max_cos = C(0);
+  if ((double)st_i > 1.0e-4 && (double)st_o > 1.0e-4) {
+    const R sp_i = sin_phi(wi), cp_i = cos_phi(wi), sp_o = sin_phi(wo), cp_o = cos_phi(wo);
+    max_cos = gmax(C(0), cp_i * cp_o + sp_i * sp_o);
+  }
+  R sin_a, tan_b;
+  if (RABS(wi.z) > RABS(wo.z)) {
+    sin_a = st_o;
+    tan_b = st_i / RABS(wi.z);
+  } else {
+    sin_a = st_i;
+    tan_b = st_o / RABS(wo.z);
+  }
+  return dmul(D1PI, m->on_a + m->on_b * max_cos * sin_a * tan_b);
+}
+/* COrenNayarBRDF::pdf (oren_nayar_brdf.hpp:78-80), cosineHemispherePdf (sampler.hpp:82-84) */
+static R on_pdf(v3 wo, v3 wi) { return same_hemisphere(wo, wi) ? dmul(D1PI, RABS(wi.z)) : C(0); }
+/* COrenNayarBRDF::sampleF (oren_nayar_brdf.hpp:69-76) */
+static R on_sample(const surface_t* m, v3 wo, v3* wi, r2o_rng* rng, R* pdf) {
+  *wi = cosine_hemisphere(rng);
+  if (wo.z < C(0)) wi->z = -wi->z;
+  *pdf = on_pdf(wo, *wi);
+  return on_f(m, wo, *wi);
+}
+
+/* CMicrofacetDistribution, Trowbridge-Reitz (microfacet_distribution.hpp:40-140) */
+static R ggx_D(const surface_t* m, v3 h) {
+  const R t2 = tan2_theta(h);
+  if (isinf(t2)) return C(0);
+  const R c2 = cos2_theta(h);
+  const R c4 = c2 * c2;
+  const R cp = cos_phi(h), sp = sin_phi(h);
+  const R e = (cp * cp / (m->alpha_x * m->alpha_x) + sp * sp / (m->alpha_y * m->alpha_y)) * t2;
+  const R e1 = C(1) + e;
+  return (R)(1.0 / ((((DPI * (double)m->alpha_x) * (double)m->alpha_y) * (double)c4) * (double)e1 * (double)e1));
+}
+static R ggx_lambda(const surface_t* m, v3 w) {
+  const R at = RABS(tan_theta(w));
+  if (isinf(at)) return C(0);
+  const R cp = cos_phi(w), sp = sin_phi(w);
+  const R alpha = RSQRT(cp * cp * m->alpha_x * m->alpha_x + sp * sp * m->alpha_y * m->alpha_y);
+  const R a2t2 = (alpha * at) * (alpha * at);
+  return (C(-1) + RSQRT(C(1) + a2t2)) / C(2);
+}
+static R ggx_G(const surface_t* m, v3 wo, v3 wi) { return C(1) / (C(1) + ggx_lambda(m, wo) + ggx_lambda(m, wi)); }
+static R ggx_G1(const surface_t* m, v3 w) { return C(1) / (C(1) + ggx_lambda(m, w)); }
+/* sample11: visible-slope sampling of the unit-roughness distribution
+ * (Heitz and d'Eon 2014, with pbrt-v3's rational fit of the y slope) */
+static void ggx_sample11(R ct, R u1, R u2, R* sx, R* sy) {
+  if ((double)ct > 0.9999) {
+    const R r = RSQRT(u1 / (C(1) - u1));
+    const R phi = dmul(6.28318530718, u2);
+    *sx = r * RCOS(phi);
+    *sy = r * RSIN(phi);
+    return;
+  }
+  const R st = RSQRT(gmax(C(0), C(1) - ct * ct));
+  const R tt = st / ct;
+  const R a = C(1) / tt;
+  const R g1 = C(2) / (C(1) + RSQRT(C(1) + C(1) / (a * a)));
+  const R A = C(2) * u1 / g1 - C(1);
+  R tmp = C(1) / (A * A - C(1));
+  if ((double)tmp > 1e10) tmp = C(1e10);
+  const R B = tt;
+  const R D = RSQRT(gmax(B * B * tmp * tmp - (A * A - B * B) * tmp, C(0)));
+  const R x1 = B * tmp - D, x2 = B * tmp + D;
+  *sx = (A < C(0) || x2 > C(1) / tt) ? x1 : x2;
+  R S;
+  if (u2 > C(0.5)) {
+    S = C(1);
+    u2 = C(2) * (u2 - C(0.5));
+  } else {
+    S = C(-1);
+    u2 = C(2) * (C(0.5) - u2);
+  }
+  const R z = (u2 * (u2 * (u2 * C(0.27385f) - C(0.73369f)) + C(0.46341f))) /
+              (u2 * (u2 * (u2 * C(0.093073f) + C(0.309420f)) - C(1)) + C(0.597999f));
+  *sy = S * z * RSQRT(C(1) + *sx * *sx);
+}
+static v3 ggx_sample_wi(const surface_t* m, v3 wi, R u1, R u2) {
+  const v3 ws = vnorm(V(m->alpha_x * wi.x, m->alpha_y * wi.y, wi.z));
+  R sx, sy;
+  ggx_sample11(ws.z, u1, u2, &sx, &sy);
+  const R cp = cos_phi(ws), sp = sin_phi(ws);
+  const R tmp = cp * sx - sp * sy;
+  sy = sp * sx + cp * sy;
+  sx = tmp;
+  sx = m->alpha_x * sx;
+  sy = m->alpha_y * sy;
+  return vnorm(V(-sx, -sy, C(1)));
+}
+static v3 ggx_sample_h(const surface_t* m, v3 wo, R u1, R u2) {
+  const int flip = wo.z < C(0);
+  v3 h = ggx_sample_wi(m, flip ? vneg(wo) : wo, u1, u2);
+  if (flip) h = vneg(h);
+  return h;
+}
+static R ggx_pdf(const surface_t* m, v3 wo, v3 wh) {
+  return ggx_D(m, wh) * ggx_G1(m, wo) * RABS(vdot(wo, wh)) / RABS(wo.z);
+}
+
+/* CMicrofacetBRDF (microfacet_brdf.hpp:57-100) */
+static R mf_f(const surface_t* m, v3 wo, v3 wi) {
+  const R co = RABS(wo.z), ci = RABS(wi.z);
+  v3 h = vadd(wi, wo);
+  if (ci == C(0) || co == C(0)) return C(0);
+  if (h.x == C(0) && h.y == C(0) && h.z == C(0)) return C(0);
+  h = vnorm(h);
+  /* faceforward(h, +z) */
+  const v3 hf = h.z < C(0) ? vneg(h) : h;
+  const R F = fresnel_ie(vdot(wi, hf), m->eta_i, m->eta_t);
+  return ggx_D(m, h) * ggx_G(m, wo, wi) * F / (C(4) * ci * co);
+}
+/* returns f; leaves *wi and *pdf untouched on the early exits, as the reference does */
+static R mf_sample(const surface_t* m, v3 wo, v3* wi, r2o_rng* rng, R* pdf) {
+  if (wo.z == C(0)) return C(0);
+  const R u1 = uni(rng), u2 = uni(rng);
+  const v3 h = ggx_sample_h(m, wo, u1, u2);
+  const R dh = vdot(wo, h);
+  if (dh < C(0)) return C(0);
+  /* glm::reflect(-wo, h) = I - N * dot(N, I) * 2 */
+  const v3 I = vneg(wo);
+  const R d = vdot(h, I);
+  *wi = V(I.x - h.x * d * C(2), I.y - h.y * d * C(2), I.z - h.z * d * C(2));
+  if (!same_hemisphere(wo, *wi)) return C(0);
+  *pdf = ggx_pdf(m, wo, h) / (C(4) * dh);
+  return mf_f(m, wo, *wi);
+}
+static R mf_pdf(const surface_t* m, v3 wo, v3 wi) {
+  if (!same_hemisphere(wo, wi)) return C(0);
+  const v3 h = vnorm(vadd(wo, wi));
+  return ggx_pdf(m, wo, h) / (C(4) * vdot(wo, h));
+}
+
+/* CMaterial::f / sampleF / pdf (material.hpp:67-100): the two lobes mixed by
+ * the Fresnel weight of |cos theta_o| */
+static v3 mat_f(const surface_t* m, v3 wo, v3 wi) {
+  const R fd = on_f(m, wo, wi), fg = mf_f(m, wo, wi);
+  const R w = fresnel_ie(RABS(wo.z), m->eta_i, m->eta_t);
+  return vadd(vscale(vscale(m->diffuse, fd), C(1) - w), vscale(vscale(m->specular, fg), w));
+}
+static v3 mat_sample(const surface_t* m, v3 wo, v3* wi, r2o_rng* rng, R* pdf) {
+  const R p = fresnel_ie(RABS(wo.z), m->eta_i, m->eta_t);
+  if (uni(rng) < p) return vscale(m->specular, mf_sample(m, wo, wi, rng, pdf));
+  return vscale(m->diffuse, on_sample(m, wo, wi, rng, pdf));
+}
+static R mat_pdf(const surface_t* m, v3 wo, v3 wi) {
+  const R w = fresnel_ie(RABS(wo.z), m->eta_i, m->eta_t);
+  return on_pdf(wo, wi) * (C(1) - w) + mf_pdf(m, wo, wi) * w;
+}
+
+/* CCoordinateFrame::fromNormal (objects.hpp:28-40): tangent-to-world columns
+ * (T, B, N) and its glm::inverse */
+typedef struct { m33 t2w, w2t; } sframe_t;
+static sframe_t surface_frame(v3 n) {
+  v3 T, B;
+  frame_basic(n, &T, &B);
+  sframe_t f;
+  f.t2w.c[0][0] = T.x; f.t2w.c[0][1] = T.y; f.t2w.c[0][2] = T.z;
+  f.t2w.c[1][0] = B.x; f.t2w.c[1][1] = B.y; f.t2w.c[1][2] = B.z;
+  f.t2w.c[2][0] = n.x; f.t2w.c[2][1] = n.y; f.t2w.c[2][2] = n.z;
+  f.w2t = m33_inverse(&f.t2w);
+  return f;
+}
+
 /* -------------------------------------------------------------- scene ---- */
-typedef struct { int hit; R t; v3 pos; int medium; } hit_t;
+typedef struct { int hit; R t; v3 pos; int medium; int surf; v3 normal; } hit_t;
+
+/* CCircle::intersect (circle.hpp:88-113): the disk in the plane through pos
+ * with normal m_normal; t in (0, tmax] */
+static int circle_hit(const surface_t* sf, const ray_t* ray, R tmax, R* t_out, v3* n_out) {
+  const R den = vdot(ray->d, sf->normal);
+  if (den == C(0)) return 0;
+  const R t = vdot(vsub(sf->pos, ray->o), sf->normal) / den;
+  if (!(t > C(0) && t <= tmax)) return 0;
+  const v3 ip = vadd(ray->o, sscale(t, ray->d));
+  if (!(vlen(vsub(ip, sf->pos)) < sf->radius)) return 0;
+  *t_out = t;
+  *n_out = sf->normal;
+  return 1;
+}
+/* Sphere::intersect (sphere.hpp:33-90): quadratic in model space (the ray
+ * through CRay::transform, ray.hpp:41-47), t compared against the world ray's
+ * tmax; the far root when the origin is inside */
+static int sphere_hit(const surface_t* sf, const ray_t* ray, R tmax, R* t_out, v3* n_out) {
+  const v3 om = m43_point(&sf->world_to_model, ray->o);
+  const v3 dm = vnorm(m43_dir(&sf->world_to_model, ray->d));
+  const R a = vdot(dm, dm);
+  const R b = C(2) * vdot(dm, om);
+  const R c = vdot(om, om) - sf->radius * sf->radius;
+  const R disc = b * b - C(4) * a * c;
+  R t;
+  if (disc == C(0)) {
+    t = (-b + RSQRT(disc)) / (C(2) * a);
+    if (!(t > C(0) && t <= tmax)) return 0;
+  } else if (disc > C(0)) {
+    const R sq = RSQRT(disc);
+    const R den = C(1) / (C(2) * a);
+    const R t1 = (-b + sq) * den, t2 = (-b - sq) * den;
+    const R mn = gmin(t1, t2), mx = gmax(t1, t2);
+    if (!(mx > C(0))) return 0;
+    if (mn > C(0) && mn <= tmax) t = mn;
+    else if (mn <= C(0) && mx <= tmax) t = mx;
+    else return 0;
+  } else {
+    return 0;
+  }
+  *t_out = t;
+  *n_out = vnorm(m43_dir(&sf->model_to_world, vadd(om, sscale(t, dm))));
+  return 1;
+}
 
 /* __intersection__volume (shaders.optix.cu:192-236) with nanovdb::Ray::intersects
  * (Williams et al. 2005 slab test, NanoVDB Ray.h) and BBox::isInside; the
  * closest accepted hit over all instances wins (OptiX closest-hit). */
 static hit_t scene_intersect(const scene_t* s, ray_t* ray) {
   hit_t h;
-  h.hit = 0; h.t = C(0); h.medium = -1; h.pos = V(0, 0, 0);
+  h.hit = 0; h.t = C(0); h.medium = -1; h.surf = -1; h.pos = V(0, 0, 0); h.normal = V(0, 0, 0);
   R best = ray->tmax;
   for (int k = 0; k < s->n_inst; ++k) {
     const instance_t* in = &s->inst[k];
@@ -940,7 +1192,19 @@ static hit_t scene_intersect(const scene_t* s, ray_t* ray) {
     if (h.hit && !(t < h.t)) continue;
     h.hit = 1; h.t = t; h.medium = k;
   }
+  /* __intersection__surface (shaders.optix.cu:170-189): the shape reports t
+   * and its normal; the closest-hit program normalises the normal (:117) */
+  for (int j = 0; j < s->n_surf; ++j) {
+    const surface_t* sf = &s->surf[j];
+    R t;
+    v3 n;
+    const R tmax = h.hit ? h.t : best;
+    const int ok = sf->shape == R2O_SHAPE_CIRCLE ? circle_hit(sf, ray, tmax, &t, &n) : sphere_hit(sf, ray, tmax, &t, &n);
+    if (!ok || (h.hit && !(t < h.t))) continue;
+    h.hit = 1; h.t = t; h.medium = -1; h.surf = j; h.normal = n;
+  }
   if (h.hit) {
+    if (h.surf >= 0) h.normal = vnorm(h.normal);
     h.pos = V(ray->o.x + h.t * ray->d.x, ray->o.y + h.t * ray->d.y, ray->o.z + h.t * ray->d.z);
     ray->tmax = h.t; /* CDeviceScene::intersect mutates m_t_max (device_scene_impl.hpp:28-30) */
   }
@@ -996,6 +1260,9 @@ static int intersect_tr(const scene_t* s, const ray_t* ray, r2o_rng* rng, int us
   for (int it = 0; it < SHADOW_SEG_CAP; ++it) {
     ray_t r = spawn_ray(p0, p1, cur);
     const hit_t h = scene_intersect(s, &r);
+    /* a surface ends the walk; its (zero) emission zeroes the shadow term, so
+     * the transmittance up to it is not tracked */
+    if (h.hit && h.surf >= 0) return 1;
     if (h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur)) *Tr *= inst_tr(s, r.medium, &r, rng, use_dda, cnt);
     if (!h.hit) return 0;
     p0 = h.pos;
@@ -1051,6 +1318,58 @@ static v3 direct(const scene_t* s, const mhit_t* mi, v3 wo, r2o_rng* rng, const 
   return L;
 }
 
+/* direct (path_integrator_impl.hpp:23-118) at a surface vertex k: light
+ * sample and BSDF sample in the tangent frame of the shading normal, the
+ * shadow walks start in the medium the incoming ray travelled in */
+static v3 direct_surface(const scene_t* s, const surface_t* sf, const hit_t* si, int cur_medium, v3 wo, r2o_rng* rng,
+                         const path_key* pk, uint32_t k, int use_dda, uint64_t* cnt) {
+  v3 L = V(0, 0, 0);
+  const sframe_t fr = surface_frame(si->normal);
+  const v3 wot = m33_mul(&fr.w2t, wo);
+  {
+    v3 ldir;
+    R light_pdf = C(0);
+    const v3 Le = env_sample(s, rng, &ldir, &light_pdf, cnt);
+    if (light_pdf > C(0)) {
+      ldir = vnorm(ldir);
+      const v3 lt = vnorm(m33_mul(&fr.w2t, ldir));
+      ray_t rl = mkray(si->pos, ldir, DEFAULT_TMAX, cur_medium);
+      ray_offset(&rl, si->normal);
+      R trs;
+      r2o_rng walk;
+      stream_init(&walk, pk, k, 1);
+      const int hit = intersect_tr(s, &rl, &walk, use_dda, &trs, cnt);
+      const v3 f = vscale(mat_f(sf, wot, lt), RABS(vdot(si->normal, rl.d)));
+      const R sp = mat_pdf(sf, wot, lt);
+      const R mis = (C(1) * light_pdf) / (C(1) * light_pdf + C(1) * sp);
+      const R tr = hit ? C(0) : trs;
+      L = vadd(L, vdivs(vscale(vmul(vscale(f, mis), Le), tr), light_pdf));
+    }
+  }
+  {
+    R sp = C(0);
+    v3 wit = V(0, 0, 0);
+    v3 f = mat_sample(sf, wot, &wit, rng, &sp);
+    v3 wi = vnorm(m33_mul(&fr.t2w, wit));
+    f = vscale(f, RABS(vdot(si->normal, wi)));
+    if (sp > C(0)) {
+      wi = vnorm(wi);
+      ray_t rb = mkray(si->pos, wi, DEFAULT_TMAX, cur_medium);
+      ray_offset(&rb, si->normal);
+      R trs;
+      r2o_rng walk;
+      stream_init(&walk, pk, k, 2);
+      const int hit = intersect_tr(s, &rb, &walk, use_dda, &trs, cnt);
+      R light_pdf;
+      const v3 Le = env_le(s, wi, &light_pdf, cnt);
+      const R mis = (C(1) * sp) / (C(1) * sp + C(1) * light_pdf);
+      const R tr = hit ? C(0) : trs;
+      L = vadd(L, vdivs(vscale(vmul(vscale(f, mis), Le), tr), sp));
+    }
+  }
+  return L;
+}
+
 /* --------------------------------------------------------------- camera -- */
 typedef struct { m43 view_to_world; R pixel_size, near; int W, H; } camera_t;
 
@@ -1093,7 +1412,7 @@ static ray_t camera_ray(const camera_t* c, int px, int py, r2o_rng* rng) {
 /* ---------------------------------------------------------- integrator --- */
 static int any_zero(v3 a) { return a.x == C(0) || a.y == C(0) || a.z == C(0); }
 
-/* CPathIntegrator::Li (path_integrator_impl.hpp:120-247) for media-only scenes */
+/* CPathIntegrator::Li (path_integrator_impl.hpp:120-247): media and analytic surfaces */
 static v3 Li(const scene_t* s, const camera_t* cam, int px, int py, int spp, int max_depth, int use_dda,
              const path_key* pk, uint64_t* cnt) {
   v3 L = V(0, 0, 0), thr = V(C(1), C(1), C(1));
@@ -1141,11 +1460,30 @@ static v3 Li(const scene_t* s, const camera_t* cam, int px, int py, int spp, int
         L = vadd(L, vmul(env_le(s, ray.d, &p, cnt), thr));
       }
       if (!si.hit) break;
-      ray_t nr = mkray(si.pos, vnorm(ray.d), DEFAULT_TMAX, ray.medium < 0 ? si.medium : -1);
-      ray_offset(&nr, ray.d);
+      if (si.surf < 0) {
+        ray_t nr = mkray(si.pos, vnorm(ray.d), DEFAULT_TMAX, ray.medium < 0 ? si.medium : -1);
+        ray_offset(&nr, ray.d);
+        ray = nr;
+        --b;
+        continue;
+      }
+      /* surface vertex (path_integrator_impl.hpp:194-223) */
+      const surface_t* sf = &s->surf[si.surf];
+      ++k;
+      stream_init(&vtx, pk, k, 0);
+      L = vadd(L, vmul(direct_surface(s, sf, &si, ray.medium, vneg(ray.d), &vtx, pk, k, use_dda, cnt), thr));
+      const sframe_t fr = surface_frame(si.normal);
+      const v3 wt = vnorm(m33_mul(&fr.w2t, ray.d)); /* ray.transform(worldToTangent) */
+      v3 wi = V(0, 0, 0);
+      R pdf = C(0);
+      const v3 f = mat_sample(sf, vneg(wt), &wi, &vtx, &pdf);
+      if (pdf == C(0)) break;
+      ray_t nr = mkray(si.pos, vnorm(m33_mul(&fr.t2w, wi)), DEFAULT_TMAX, ray.medium);
+      ray_offset(&nr, si.normal);
+      const R cosine = RABS(vdot(si.normal, nr.d));
+      thr = vmul(thr, vdivs(vscale(f, cosine), pdf));
       ray = nr;
-      --b;
-      continue;
+      stream_init(&walk, pk, k, 3);
     }
     if (b > 3) {
       const R p = C(1) - (thr.x + thr.y + thr.z) / C(3);
@@ -1273,6 +1611,30 @@ static void medium_init(medium_t* m, const r2o_grid_desc* g) {
   m->voxel_size_filtering = g->voxel_size_filtering;
 }
 
+/* CShape(worldPos, normal) (shape.hpp:49-57) and CHostSceneobject's material
+ * (sceneobject.cu:28-39) */
+static void surface_init(surface_t* sf, const r2o_surface_desc* sd) {
+  sf->shape = sd->shape;
+  sf->pos = V(sd->pos[0], sd->pos[1], sd->pos[2]);
+  sf->normal = V(sd->normal[0], sd->normal[1], sd->normal[2]);
+  sf->radius = sd->radius;
+  m44 T = m44_identity();
+  T.c[3][0] = sd->pos[0]; T.c[3][1] = sd->pos[1]; T.c[3][2] = sd->pos[2];
+  const m44 Rm = get_rotation(sf->normal);
+  const m44 M = m44_mul(&T, &Rm);
+  sf->model_to_world = m43_from44(&M);
+  const m44 M4 = m44_from43(&sf->model_to_world);
+  const m44 Mi = m44_inverse(&M4);
+  sf->world_to_model = m43_from44(&Mi);
+  sf->diffuse = V(sd->diffuse[0], sd->diffuse[1], sd->diffuse[2]);
+  sf->specular = V(sd->specular[0], sd->specular[1], sd->specular[2]);
+  const R sig2 = (R)sd->diffuse_roughness * (R)sd->diffuse_roughness;
+  sf->on_a = C(1) - (sig2 / (C(2) * (sig2 + C(0.33f))));
+  sf->on_b = C(0.45f) * sig2 / (sig2 + C(0.09f));
+  sf->alpha_x = sd->alpha_x; sf->alpha_y = sd->alpha_y;
+  sf->eta_i = sd->eta_i; sf->eta_t = sd->eta_t;
+}
+
 scene_t* SFX(r2o_scene_new)(const r2o_scene_desc* d) {
   scene_t* s = (scene_t*)calloc(1, sizeof(scene_t));
   s->n_media = d->n_grids;
@@ -1303,6 +1665,8 @@ scene_t* SFX(r2o_scene_new)(const r2o_scene_desc* d) {
     const R vsr = (in->bb_max.x - in->bb_min.x) / (R)m->size[0];
     in->ratio = m->voxel_size_filtering / vsr;
   }
+  s->n_surf = d->n_surfaces;
+  for (int j = 0; j < d->n_surfaces; ++j) surface_init(&s->surf[j], &d->surfaces[j]);
   /* env map + CDistribution2D (environmentmap.cu:11-29, distribution_2d.cu:7-14) */
   const int W = d->env_w, H = d->env_h;
   s->env_w = W; s->env_h = H;
@@ -1611,4 +1975,66 @@ void SFX(r2o_phase_p_mean)(const R* S6, const R* n, R ior, const R* wo, const R*
     }
     out[j] = acc / n_c;
   }
+}
+
+/* ---- surface probes (BSDF lobes, shapes) ---- */
+/* out: mat_f rgb, mat_pdf, on_f, on_pdf, mf_f, mf_pdf, Fresnel weight of |wo.z| */
+void SFX(r2o_bsdf_eval)(const r2o_surface_desc* sd, const R* wo, const R* wi, R* out9) {
+  surface_t sf;
+  surface_init(&sf, sd);
+  const v3 o = V(wo[0], wo[1], wo[2]), i = V(wi[0], wi[1], wi[2]);
+  const v3 f = mat_f(&sf, o, i);
+  out9[0] = f.x; out9[1] = f.y; out9[2] = f.z;
+  out9[3] = mat_pdf(&sf, o, i);
+  out9[4] = on_f(&sf, o, i);
+  out9[5] = on_pdf(o, i);
+  out9[6] = mf_f(&sf, o, i);
+  out9[7] = mf_pdf(&sf, o, i);
+  out9[8] = fresnel_ie(RABS(o.z), sf.eta_i, sf.eta_t);
+}
+/* out: D(h), G1(wo), G(wo, wi), VNDF pdf(wo, h) */
+void SFX(r2o_ggx_eval)(const r2o_surface_desc* sd, const R* wo, const R* wi, const R* h, R* out4) {
+  surface_t sf;
+  surface_init(&sf, sd);
+  const v3 o = V(wo[0], wo[1], wo[2]), i = V(wi[0], wi[1], wi[2]), hh = V(h[0], h[1], h[2]);
+  out4[0] = ggx_D(&sf, hh);
+  out4[1] = ggx_G1(&sf, o);
+  out4[2] = ggx_G(&sf, o, i);
+  out4[3] = ggx_pdf(&sf, o, hh);
+}
+void SFX(r2o_ggx_sample_h)(const r2o_surface_desc* sd, const R* wo, R u1, R u2, R* out3) {
+  surface_t sf;
+  surface_init(&sf, sd);
+  const v3 h = ggx_sample_h(&sf, V(wo[0], wo[1], wo[2]), u1, u2);
+  out3[0] = h.x; out3[1] = h.y; out3[2] = h.z;
+}
+/* CMaterial::sampleF with its own stream; out: wi, f rgb, pdf (pdf 0 and wi 0
+ * when the sampled lobe rejected the draw) */
+void SFX(r2o_bsdf_sample)(const r2o_surface_desc* sd, const R* wo, uint32_t key, uint32_t c0, R* out7) {
+  surface_t sf;
+  surface_init(&sf, sd);
+  r2o_rng rng;
+  r2o_rng_init(&rng, key, 0, c0, 0, 0);
+  v3 wi = V(0, 0, 0);
+  R pdf = C(0);
+  const v3 f = mat_sample(&sf, V(wo[0], wo[1], wo[2]), &wi, &rng, &pdf);
+  out7[0] = wi.x; out7[1] = wi.y; out7[2] = wi.z;
+  out7[3] = f.x; out7[4] = f.y; out7[5] = f.z;
+  out7[6] = pdf;
+}
+R SFX(r2o_fresnel_ie)(R cos_i, R eta_i, R eta_t) { return fresnel_ie(cos_i, eta_i, eta_t); }
+/* shape test of one surface against a ray (o, d, tmax); out: t, normal
+ * (normalised as the closest-hit program does); returns the hit flag */
+int SFX(r2o_surface_hit)(const r2o_surface_desc* sd, const R* ray7, R* out4) {
+  surface_t sf;
+  surface_init(&sf, sd);
+  const ray_t r = mkray(V(ray7[0], ray7[1], ray7[2]), V(ray7[3], ray7[4], ray7[5]), ray7[6], -1);
+  R t;
+  v3 n;
+  const int ok = sf.shape == R2O_SHAPE_CIRCLE ? circle_hit(&sf, &r, r.tmax, &t, &n) : sphere_hit(&sf, &r, r.tmax, &t, &n);
+  if (ok) {
+    n = vnorm(n);
+    out4[0] = t; out4[1] = n.x; out4[2] = n.y; out4[3] = n.z;
+  }
+  return ok;
 }
