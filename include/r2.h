@@ -31,7 +31,7 @@ extern "C" {
 #define R2_ERR_BUSY (-7)        /* server queue full */
 #define R2_ERR_DEADLINE (-8)    /* server request not finished by its deadline (dropped) */
 
-#define R2_VERSION 2
+#define R2_VERSION 3
 
 /* Grid record precision (r2_grid_desc.precision) */
 #define R2_PREC_FP32 0   /* 32-byte SFilteredDataCompact voxel records (filtered_data.hpp:33-61) */
@@ -128,6 +128,25 @@ typedef struct {
   float pos[3], orientation[3], scaling[3];   /* CHostSceneobject(medium, pos, orientation, scaling) */
 } r2_instance;
 
+/* Analytic surface with its material: CHostSceneobject(CCircle / Sphere(pos, radius,
+ * normal), diffuseReflection, diffuseRoughness, specularReflection, alphaX, alphaY,
+ * etaI, etaT) as the config loader's addCircle / addSphere build it
+ * (raytracer/src/backend/config_loader.cu:122-215, raytracer/src/scene/sceneobject.cu:28-48):
+ * an Oren-Nayar lobe of roughness diffuse_roughness (raytracer/include/material/oren_nayar_brdf.hpp:31-35)
+ * and a GGX microfacet lobe (microfacet_brdf.hpp:42-45) mixed by the Fresnel weight
+ * of (eta_i, eta_t) (material.hpp:67-100).  A circle is the disk of that radius in
+ * the plane through pos with that normal (circle.hpp:88-113); a sphere is solved
+ * in the frame translate(pos) * getRotation(normal) (shape.hpp:49-57, sphere.hpp:33-90). */
+#define R2_SHAPE_CIRCLE 0
+#define R2_SHAPE_SPHERE 1
+#define R2_MAX_SURFACES 8
+typedef struct {
+  int32_t shape;                                          /* R2_SHAPE_* */
+  float pos[3], radius, normal[3];
+  float diffuse[3], diffuse_roughness, specular[3];
+  float alpha_x, alpha_y, eta_i, eta_t;
+} r2_surface;
+
 typedef struct {
   int32_t width, height, spp, max_depth;
   float cam_pos[3], cam_look_at[3], cam_up[3], fov_deg;  /* CCamera (raytracer/src/camera/camera.cu:20-32) */
@@ -137,6 +156,8 @@ typedef struct {
   const r2_env* env;
   uint32_t flags;                                         /* R2_FLAG_* */
   int32_t sample_begin;                                   /* first sample index */
+  int32_t n_surfaces;                                     /* 0 .. R2_MAX_SURFACES */
+  const r2_surface* surfaces;                             /* (since version 3) */
 } r2_render_params;
 
 /* Render one frame: linear HDR RGB, W*H*3 floats, row 0 = bottom (launch index y = 0).
@@ -335,6 +356,7 @@ typedef struct {
   int32_t width, height, spp;
   double render_ms, mpaths_per_s;
   int32_t media_rendered, objects_skipped;
+  int32_t surfaces_rendered;    /* Circle / Sphere objects rendered (since version 3) */
 } r2_pipeline_report;
 int r2_pipeline_run(const char* config_path, const char* out_dir, int32_t device, r2_pipeline_report* report,
                     char* err, int32_t err_len);

@@ -145,6 +145,22 @@ class Context:
             arr[k].scaling[:] = list(inst.get("scaling", (1, 1, 1)))
         p.n_instances = len(instances)
         p.instances = arr
+        surfs = cfg.get("surfaces", ())
+        sarr = (capi.Surface * max(1, len(surfs)))()
+        for j, sf in enumerate(surfs):
+            sarr[j].shape = {"circle": capi.R2_SHAPE_CIRCLE, "sphere": capi.R2_SHAPE_SPHERE}[sf.get("shape", "circle")]
+            sarr[j].pos[:] = list(sf.get("pos", (0, 0, 0)))
+            sarr[j].radius = sf.get("radius", 1.0)
+            sarr[j].normal[:] = list(sf.get("normal", (0, 1, 0)))
+            sarr[j].diffuse[:] = list(sf.get("diffuse", (0.5, 0.5, 0.5)))
+            sarr[j].diffuse_roughness = sf.get("diffuse_roughness", 0.0)
+            sarr[j].specular[:] = list(sf.get("specular", (0, 0, 0)))
+            sarr[j].alpha_x = sf.get("alpha_x", 0.5)
+            sarr[j].alpha_y = sf.get("alpha_y", 0.5)
+            sarr[j].eta_i = sf.get("eta_i", 1.0)
+            sarr[j].eta_t = sf.get("eta_t", 1.5)
+        p.n_surfaces = len(surfs)
+        p.surfaces = sarr
         p.env = env.h
         f = flags
         if cfg.get("use_brick_grid", True):
@@ -153,7 +169,7 @@ class Context:
             f |= capi.R2_FLAG_FP64_ACCUM
         p.flags = f
         p.sample_begin = cfg.get("sample_begin", 0)
-        return p, arr
+        return p, (arr, sarr)
 
     def render(self, cfg: dict, grids, instances, env) -> np.ndarray:
         p, keep = self.render_params(cfg, grids, instances, env)

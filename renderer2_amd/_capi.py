@@ -75,6 +75,18 @@ class Instance(C.Structure):
     _fields_ = [("grid", C.c_void_p), ("pos", f3), ("orientation", f3), ("scaling", f3)]
 
 
+R2_SHAPE_CIRCLE, R2_SHAPE_SPHERE = 0, 1
+R2_MAX_SURFACES = 8
+
+
+class Surface(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32), ("pos", f3), ("radius", C.c_float), ("normal", f3),
+        ("diffuse", f3), ("diffuse_roughness", C.c_float), ("specular", f3),
+        ("alpha_x", C.c_float), ("alpha_y", C.c_float), ("eta_i", C.c_float), ("eta_t", C.c_float),
+    ]
+
+
 class RenderParams(C.Structure):
     _fields_ = [
         ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
@@ -84,6 +96,7 @@ class RenderParams(C.Structure):
         ("env", C.c_void_p),
         ("flags", C.c_uint32),
         ("sample_begin", C.c_int32),
+        ("n_surfaces", C.c_int32), ("surfaces", C.POINTER(Surface)),
     ]
 
 
@@ -140,6 +153,7 @@ class PipelineReport(C.Structure):
         ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
         ("render_ms", C.c_double), ("mpaths_per_s", C.c_double),
         ("media_rendered", C.c_int32), ("objects_skipped", C.c_int32),
+        ("surfaces_rendered", C.c_int32),
     ]
 
 

@@ -136,6 +136,36 @@ int build_scene(r2_ctx* ctx, const r2_render_params* p, DScene* out) {
     const float vsr = (in.bb_max[0] - in.bb_min[0]) / (float)gi.size[0];
     in.ratio = id.grid->voxel_size_filtering / vsr;
   }
+  // analytic surfaces (CHostSceneobject(shape, material ...), sceneobject.cu:28-48)
+  if (p->n_surfaces < 0 || p->n_surfaces > kMaxSurfaces)
+    return fail(ctx, R2_ERR_UNSUPPORTED, "render: at most 8 surfaces");
+  if (p->n_surfaces > 0 && !p->surfaces) return fail(ctx, R2_ERR_INVALID, "render: surfaces missing");
+  s.n_surf = p->n_surfaces;
+  for (int j = 0; j < p->n_surfaces; ++j) {
+    const r2_surface& sd = p->surfaces[j];
+    if (sd.shape != R2_SHAPE_CIRCLE && sd.shape != R2_SHAPE_SPHERE)
+      return fail(ctx, R2_ERR_INVALID, "render: unknown surface shape");
+    DSurface& sf = s.surf[j];
+    sf.shape = sd.shape;
+    for (int a = 0; a < 3; ++a) {
+      sf.pos[a] = sd.pos[a];
+      sf.normal[a] = sd.normal[a];
+      sf.diffuse[a] = sd.diffuse[a];
+      sf.specular[a] = sd.specular[a];
+    }
+    sf.radius = sd.radius;
+    // CShape (shape.hpp:49-57): translate(worldPos) * getRotation(normal), and its inverse
+    m44 T = identity44();
+    T.c[3][0] = sd.pos[0]; T.c[3][1] = sd.pos[1]; T.c[3][2] = sd.pos[2];
+    sf.m2w = to43(mul44(T, get_rotation(mk(sd.normal[0], sd.normal[1], sd.normal[2]))));
+    sf.w2m = inverse43(sf.m2w);
+    // COrenNayarBRDF(roughness) (oren_nayar_brdf.hpp:31-35)
+    const float sig2 = sd.diffuse_roughness * sd.diffuse_roughness;
+    sf.on_a = 1.f - (sig2 / (2.f * (sig2 + 0.33f)));
+    sf.on_b = 0.45f * sig2 / (sig2 + 0.09f);
+    sf.alpha_x = sd.alpha_x; sf.alpha_y = sd.alpha_y;
+    sf.eta_i = sd.eta_i; sf.eta_t = sd.eta_t;
+  }
   return R2_OK;
 }
 

@@ -17,9 +17,9 @@
 // Radiance .hdr environment maps (RGBE, flat or run-length scanlines), and this
 // build's grid files "<Path>.r2grid" (dense SFilteredDataCompact records with the
 // index -> model transform) or, where none exists, the reference's NanoVDB file
-// "<Path>.nvdb"; filtering writes both, and the .brickgrid beside them.  Surface objects (Circle, Sphere, meshes with the
-// RENDER mask) are parsed but not rendered: the surface path is not part of this
-// library.
+// "<Path>.nvdb"; filtering writes both, and the .brickgrid beside them.  Circle and Sphere objects render as analytic
+// surfaces with their material (r2_surface); meshes with the RENDER mask are parsed but not rendered (no triangle
+// intersection in the render path).
 #include "r2_internal.hpp"
 #include "r2_json.hpp"
 #include <algorithm>
@@ -50,6 +50,7 @@ struct SceneObject {
   float g = 0.f;
   float pos[3] = {0, 0, 0}, orientation[3] = {0, 1, 0}, scaling[3] = {1, 1, 1};
   unsigned mask = 1;  // RENDER
+  r2_surface surf{};  // Circle / Sphere: shape and material
 };
 
 enum : unsigned { kMaskRender = 1u, kMaskFilter = 2u };
@@ -197,6 +198,20 @@ bool add_object(const Value& o, Config* c) {
         !num(o.find("EtaI"), &ei) || !num(o.find("EtaT"), &et))
       return false;
     so.mask = mask_of(o.find("Mask"), kMaskRender);
+    r2_surface& sf = so.surf;
+    sf.shape = type == "Circle" ? R2_SHAPE_CIRCLE : R2_SHAPE_SPHERE;
+    for (int a = 0; a < 3; ++a) {
+      sf.pos[a] = so.pos[a];
+      sf.normal[a] = n[a];
+      sf.diffuse[a] = d[a];
+      sf.specular[a] = sp[a];
+    }
+    sf.radius = (float)r;  // radius.get<float>(): 3.402823466e+38 (the forest ground) stays finite
+    sf.diffuse_roughness = (float)dr;
+    sf.alpha_x = (float)ax;
+    sf.alpha_y = (float)ay;
+    sf.eta_i = (float)ei;
+    sf.eta_t = (float)et;
   } else if (type == "Medium") {
     if (!str(o.find("Path"), &so.path) || !vec3(o.find("SigmaA"), so.sigma_a) || !vec3(o.find("SigmaS"), so.sigma_s) ||
         !vec3(o.find("Pos"), so.pos) || !vec3(o.find("Orientation"), so.orientation) ||
@@ -743,9 +758,15 @@ std::vector<float> load_env(const Config& c, int* w, int* h) {
 void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeline_report* rep) {
   std::vector<r2_grid*> grids;
   std::vector<r2_instance> inst;
+  std::vector<r2_surface> surfs;
   std::map<std::string, r2_grid*> cache;  // CAssetManager::loadMedium reuses media by path
   for (const SceneObject& o : c.objects) {
     if (!(o.mask & kMaskRender)) continue;
+    if (o.type == "Circle" || o.type == "Sphere") {
+      if ((int)surfs.size() >= R2_MAX_SURFACES) throw std::runtime_error("too many surfaces in the scene");
+      surfs.push_back(o.surf);
+      continue;
+    }
     if (o.type != "Medium") {
       fprintf(stderr, "[WARNING]: %s objects are not rendered by the volumetric path; skipped\n", o.type.c_str());
       ++rep->objects_skipped;
@@ -823,6 +844,8 @@ void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeli
   p.seed = c.seed;
   p.n_instances = (int32_t)inst.size();
   p.instances = inst.data();
+  p.n_surfaces = (int32_t)surfs.size();
+  p.surfaces = surfs.data();
   p.env = env;
   p.flags = c.use_brick_grid ? R2_FLAG_BRICK_GRID : 0u;
   std::vector<float> hdr((size_t)p.width * p.height * 3);
@@ -833,6 +856,7 @@ void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeli
   rep->height = p.height;
   rep->spp = p.spp;
   rep->media_rendered = (int32_t)inst.size();
+  rep->surfaces_rendered = (int32_t)surfs.size();
   rep->mpaths_per_s = (double)p.width * p.height * p.spp / (rep->render_ms * 1e-3) / 1e6;
   std::vector<uint8_t> bytes(hdr.size());
   check(ctx, r2_postprocess(ctx, hdr.data(), p.width, p.height, c.gamma, nullptr, bytes.data(), 0), "postprocess");

@@ -62,10 +62,10 @@ struct BatchEntry {
 #define R2K_DECLARE(NS)                                                                                              \
   namespace NS {                                                                                                     \
   void launch_sm(hipStream_t st, int blocks, const r2::DScene* d_scene, uint64_t* counter, uint64_t total, int chunk, \
-                 void* partial, bool fp64, bool single, bool count, const int32_t* pp, const int32_t* ps, float* po, \
-                 uint32_t* pc);                                                                                      \
+                 void* partial, bool fp64, bool single, bool surf, bool count, const int32_t* pp, const int32_t* ps, \
+                 float* po, uint32_t* pc);                                                                           \
   void launch_batch_group(hipStream_t st, int blocks, const void* ents, int n, uint64_t* counters, bool single,      \
-                          bool fp64);                                                                                \
+                          bool surf, bool fp64);                                                                     \
   void launch_reduce_k(hipStream_t st, const void* partial, bool fp64, int nchunks, uint32_t n3, float* out);        \
   }
 R2K_DECLARE(r2k_fp32)
@@ -107,7 +107,7 @@ struct BatchSlice {
   void* d_partial;
   uint64_t first, count;
   int chunk;
-  bool single, fp64, packed;
+  bool single, surf, fp64, packed;
 };
 constexpr int kMaxBatchEntries = 256;
 void render_chunking(const r2::DScene& scene, int* chunk, int* nchunks);

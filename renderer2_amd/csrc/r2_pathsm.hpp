@@ -43,10 +43,10 @@ enum : int {
 // roles of a vertex's streams; the rays are walked in this order
 enum : int { RL_VERTEX = 0, RL_LIGHT = 1, RL_PHASE = 2, RL_CONT = 3 };
 // how the ray that arrived at a vertex ended (set by the continuation walk)
-enum : int { CR_START = 0, CR_NONE, CR_END, CR_ESCAPED, CR_COLLIDED };
+enum : int { CR_START = 0, CR_NONE, CR_END, CR_ESCAPED, CR_COLLIDED, CR_SURFACE };
 enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO = 4 };
-// ColdLane::flags
-enum : uint32_t { FL_LIGHT = 1u, FL_PHASE = 2u, FL_CONT = 4u, FL_NEE = 8u, FL_RR = 16u };
+// ColdLane::flags (FL_PREMUL: the direct light already carries the throughput)
+enum : uint32_t { FL_LIGHT = 1u, FL_PHASE = 2u, FL_CONT = 4u, FL_NEE = 8u, FL_RR = 16u, FL_PREMUL = 32u };
 // cap on the segments of one intersectTr walk, which the reference loops until a
 // miss: a segment enters or leaves an instance box, so scenes of kMaxInstances
 // boxes never reach it (the oracle uses the same cap)
@@ -151,8 +151,9 @@ struct PathLane {
   int b, k, iters;
   v3 &L, &thr;
   // continuation walk: the ray of the bounce loop, and its last scene
-  // intersection.  Shadow walks keep intersectTr's p0 in ray.o, p1 in ray.d and
-  // currentMedium in ray.medium, the segment's hit flag in si_hit.
+  // intersection (a surface hit keeps the surface in si_medium and its normal in
+  // ray.o until it is shaded).  Shadow walks keep intersectTr's p0 in ray.o, p1
+  // in ray.d and currentMedium in ray.medium, the segment's hit flag in si_hit.
   Ray ray;
   v3 si_pos;
   int si_medium, si_hit;
@@ -395,7 +396,7 @@ __device__ __forceinline__ void cont_nohit(PathLane& p) {
 // then the transmittance traversal of the segment if it lies inside a medium.
 // Both roles share one scene intersection and one traversal set-up, so a pass
 // whose lanes hold both kinds of ray runs each once.
-template <bool kSingle>
+template <bool kSingle, bool kSurf>
 __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
   const bool dda = (s.flags & 1u) != 0u;
   const bool cont = p.role == RL_CONT;
@@ -410,7 +411,7 @@ __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
   }
   const int cur = p.ray.medium;
   Ray r = cont ? p.ray : spawn_ray(p.ray.o, p.ray.d, cur);
-  const Hit h = scene_intersect(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
+  const Hit h = scene_intersect<kSurf>(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
   Ray tr = r;  // the ray of the traversal to start
   int kind;
   if (cont) {
@@ -419,6 +420,12 @@ __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
     p.si_medium = h.medium;
     p.si_hit = h.hit;
     if (h.medium < 0) {
+      if (kSurf && h.surf >= 0) {  // a surface vertex (path_integrator_impl.hpp:194-223)
+        p.si_medium = h.surf;
+        p.ray.o = h.normal;
+        cont_done(p, CR_SURFACE);
+        return;
+      }
       cont_nohit(p);
       return;
     }
@@ -430,7 +437,7 @@ __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
     } else {
       Ray mr = mkray(h.pos, normalize(r.d), kDefaultTmax, h.medium);
       ray_offset(mr, r.d);
-      const Hit se = scene_intersect(s, mr);
+      const Hit se = scene_intersect<kSurf>(s, mr);
       if (!se.hit) {
         cont_nohit(p);
         return;
@@ -444,6 +451,10 @@ __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
     kind = dda ? TK_FLIGHT_DDA : TK_FLIGHT_DELTA;
   } else {
     p.si_hit = h.hit;
+    if (kSurf && h.surf >= 0) {  // intersectTr stops at a surface, whose zero emission zeroes the term
+      shadow_done(p);
+      return;
+    }
     p.ray.o = h.pos;
     p.ray.medium = r.medium < 0 ? h.medium : -1;
     if (!(h.hit && r.medium >= 0 && (h.medium < 0 || h.medium == cur))) {
@@ -541,6 +552,98 @@ __device__ __forceinline__ void shade_scatter(const DScene& s, PathLane& p, cons
   }
   ++p.b;
   // the bounce loop's depth test: a continuation past max_depth is never walked
+  if (!(p.b < s.max_depth)) flags &= ~FL_CONT;
+  p.c.flags = flags;
+  p.cres = CR_NONE;
+  walk_next(p, RL_VERTEX);
+}
+
+// The surface vertex k (k >= 1) of a path whose continuation hit surface
+// p.si_medium at p.si_pos with normal p.ray.o: direct()'s light sample and BSDF
+// sample in the tangent frame of the normal, their shadow rays starting in the
+// medium the incoming ray travelled in, Li's BSDF continuation, its throughput
+// and Russian roulette, in the reference's order of draws from the vertex's own
+// stream.  The throughput is updated here, so the direct light of the vertex
+// is stored already multiplied by the throughput it arrived with (FL_PREMUL).
+template <bool kCount>
+__device__ __forceinline__ void shade_surface(const DScene& s, PathLane& p, Cnt& cnt) {
+  const DSurface& sf = s.surf[p.si_medium];
+  const v3 n = p.ray.o, pos = p.si_pos, rd = p.ray.d;
+  const int cur = p.ray.medium;
+  const v3 thr0 = p.thr;
+  ++p.k;
+  p.rng.init(p.rng.c0, p.rng.c1, 4u * (uint32_t)p.k);
+  p.rng.top_up();
+  uint32_t flags = FL_NEE | FL_PREMUL;
+  const SFrame fr = surface_frame(n);
+  const v3 wot = mat3_mul(fr.w2t, -rd);
+  {
+    v3 ldir;
+    float lpdf = 0.f;
+    const v3 Le = env_sample<kCount>(s.env, p.rng, ldir, lpdf, cnt);
+    if (lpdf > 0.f) {
+      const v3 dl = normalize(ldir);
+      const v3 lt = normalize(mat3_mul(fr.w2t, dl));
+      Ray r = mkray(pos, dl, kDefaultTmax, cur);
+      ray_offset(r, n);
+      p.ray.o = r.o;  // walked first: straight to the walk registers
+      p.ray.d = r.o + r.tmax * r.d;
+      p.ray.medium = r.medium;
+      const v3 f = mat_f(sf, wot, lt) * fabsf(dot(n, r.d));
+      const float sp = mat_pdf(sf, wot, lt);
+      const float mis = (1.f * lpdf) / (1.f * lpdf + 1.f * sp);
+      p.c.nee_a = thr0 * ((f * mis) * Le);
+      p.c.pdf_a = lpdf;
+      flags |= FL_LIGHT;
+    }
+  }
+  p.rng.top_up();
+  {
+    float sp = 0.f;
+    v3 wit = mk(0.f, 0.f, 0.f);
+    v3 f = mat_sample(sf, wot, wit, p.rng, sp);
+    v3 wi = normalize(mat3_mul(fr.t2w, wit));
+    f = f * fabsf(dot(n, wi));
+    if (sp > 0.f) {
+      wi = normalize(wi);
+      Ray r = mkray(pos, wi, kDefaultTmax, cur);
+      ray_offset(r, n);
+      p.c.lb0 = r.o;
+      p.c.lb1 = r.o + r.tmax * r.d;
+      p.c.lb_med = r.medium;
+      float epdf = 0.f;
+      const v3 Le2 = env_le<kCount>(s.env, wi, epdf, cnt);
+      const float mis = (1.f * sp) / (1.f * sp + 1.f * epdf);
+      p.c.nee_b = thr0 * ((f * mis) * Le2);
+      p.c.pdf_b = sp;
+      flags |= FL_PHASE;
+    }
+  }
+  p.rng.top_up();
+  // Li: the BSDF continuation (ray.transform(worldToTangent) of the incoming ray)
+  {
+    const v3 wt = normalize(mat3_mul(fr.w2t, rd));
+    v3 wi = mk(0.f, 0.f, 0.f);
+    float pdf = 0.f;
+    const v3 f = mat_sample(sf, -wt, wi, p.rng, pdf);
+    if (pdf != 0.f) {
+      Ray r = mkray(pos, normalize(mat3_mul(fr.t2w, wi)), kDefaultTmax, cur);
+      ray_offset(r, n);
+      const float cosine = fabsf(dot(n, r.d));
+      p.thr = p.thr * ((f * cosine) / pdf);
+      p.c.c_o = r.o;
+      p.c.c_d = r.d;
+      p.c.c_tmax = r.tmax;
+      p.c.c_med = r.medium;
+      flags |= FL_CONT;
+      if (p.b > 3) {
+        const float q = 1.f - (p.thr.x + p.thr.y + p.thr.z) / 3.f;
+        if (p.rng.uni() <= q) flags &= ~FL_CONT;
+        else p.thr = p.thr / (1.f - q);
+      }
+    }
+  }
+  ++p.b;
   if (!(p.b < s.max_depth)) flags &= ~FL_CONT;
   p.c.flags = flags;
   p.cres = CR_NONE;

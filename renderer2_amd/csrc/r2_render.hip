@@ -39,13 +39,13 @@ static int scene_precision(const r2::DScene& s) {
 }
 
 static void launch_sm(r2_ctx* ctx, int prec, int blocks, uint64_t total, int chunk, void* partial, bool fp64, bool single,
-                      bool count, const int32_t* pp, const int32_t* ps, float* po, uint32_t* pc) {
+                      bool surf, bool count, const int32_t* pp, const int32_t* ps, float* po, uint32_t* pc) {
   if (prec == 1)
     r2k_packed::launch_sm(ctx->stream, blocks, ctx->d_scene, (uint64_t*)ctx->d_counter, total, chunk, partial, fp64,
-                          single, count, pp, ps, po, pc);
+                          single, surf, count, pp, ps, po, pc);
   else
     r2k_fp32::launch_sm(ctx->stream, blocks, ctx->d_scene, (uint64_t*)ctx->d_counter, total, chunk, partial, fp64,
-                        single, count, pp, ps, po, pc);
+                        single, surf, count, pp, ps, po, pc);
 }
 
 size_t batch_entry_bytes() { return sizeof(r2::BatchEntry); }
@@ -70,15 +70,17 @@ int launch_reduce(hipStream_t st, const void* partial, bool fp64, int nchunks, u
 int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& items, void* d_ents, uint64_t* d_counters) {
   using namespace r2;
   if (items.empty()) return R2_OK;
-  // one launch per (single instance, accumulator type) group
+  // one launch per (single instance, surfaces, accumulator type, record precision) group
   std::vector<BatchEntry> host(items.size());
   int off = 0;
-  for (int g = 0; g < 8; ++g) {
-    const bool single = (g & 1) != 0, fp64 = (g & 2) != 0, packed = (g & 4) != 0;
+  for (int g = 0; g < 16; ++g) {
+    const bool single = (g & 1) != 0, fp64 = (g & 2) != 0, packed = (g & 4) != 0, surf = (g & 8) != 0;
+    if (single && surf) continue;  // surface scenes run the general-instance form, whatever their instance count
     const int first = off;
     uint64_t total = 0;
     for (const BatchSlice& b : items) {
-      if (b.single != single || b.fp64 != fp64 || b.packed != packed || b.count == 0) continue;
+      if (b.surf != surf || (!surf && b.single != single) || b.fp64 != fp64 || b.packed != packed || b.count == 0)
+        continue;
       BatchEntry& e = host[off++];
       e.scene = b.d_scene;
       e.partial = b.d_partial;
@@ -94,8 +96,8 @@ int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& ite
     R2_HIP(ctx, hipMemcpyAsync(de, &host[first], sizeof(BatchEntry) * n, hipMemcpyHostToDevice, st));
     R2_HIP(ctx, hipMemsetAsync(d_counters + first, 0, sizeof(uint64_t) * n, st));
     const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total + 255) / 256);
-    if (packed) r2k_packed::launch_batch_group(st, blocks, de, n, d_counters + first, single, fp64);
-    else r2k_fp32::launch_batch_group(st, blocks, de, n, d_counters + first, single, fp64);
+    if (packed) r2k_packed::launch_batch_group(st, blocks, de, n, d_counters + first, single, surf, fp64);
+    else r2k_fp32::launch_batch_group(st, blocks, de, n, d_counters + first, single, surf, fp64);
     R2_HIP(ctx, hipGetLastError());
   }
   return R2_OK;
@@ -107,14 +109,14 @@ int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& ite
 // sample chunk, like the reference's one launch per sample: a launch takes the
 // items [first, total) by starting the shared item counter at `first`.  Each item
 // writes its own chunk sum, so the frame is the same bit for bit either way.
-static int launch_items(r2_ctx* ctx, int prec, bool single, bool fp64, uint64_t first, uint64_t total, int chunk,
-                        void* partial) {
+static int launch_items(r2_ctx* ctx, int prec, bool single, bool surf, bool fp64, uint64_t first, uint64_t total,
+                        int chunk, void* partial) {
   hipStream_t st = ctx->stream;
   // the counter's low word = first (< 2^32: items of a launch fit 32 bits), high word 0
   R2_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)ctx->d_counter, (int)(uint32_t)first, 1, st));
   R2_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)(ctx->d_counter + 1), 0, 1, st));
   const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, (total - first + 255) / 256);
-  launch_sm(ctx, prec, blocks, total, chunk, partial, fp64, single, false, nullptr, nullptr, nullptr, nullptr);
+  launch_sm(ctx, prec, blocks, total, chunk, partial, fp64, single, surf, false, nullptr, nullptr, nullptr, nullptr);
   R2_HIP(ctx, hipGetLastError());
   return R2_OK;
 }
@@ -145,7 +147,7 @@ int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp
   }
   const uint64_t total = (uint64_t)nchunks * npix;
   hipStream_t st = ctx->stream;
-  const bool single = scene.n_inst == 1;
+  const bool single = scene.n_inst == 1, surf = scene.n_surf > 0;
   const int prec = scene_precision(scene);
   if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
   R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(DScene), hipMemcpyHostToDevice, st));
@@ -153,7 +155,7 @@ int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp
   int done_chunks = nchunks;
   int launches = 1;
   if (!o.progress) {
-    int rc = launch_items(ctx, prec, single, fp64, 0, total, chunk, partial);
+    int rc = launch_items(ctx, prec, single, surf, fp64, 0, total, chunk, partial);
     if (rc) return rc;
   } else {
     // one launch per sample chunk, with up to two in flight; the callback runs
@@ -169,8 +171,8 @@ int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp
     bool stop = false;
     while (completed < nchunks) {
       while (!stop && queued < nchunks && queued - completed < 2) {
-        rc = launch_items(ctx, prec, single, fp64, (uint64_t)queued * npix, (uint64_t)(queued + 1) * npix, chunk,
-                          partial);
+        rc = launch_items(ctx, prec, single, surf, fp64, (uint64_t)queued * npix, (uint64_t)(queued + 1) * npix,
+                          chunk, partial);
         if (rc) break;
         R2_HIP(ctx, hipEventRecord(evs[queued & 1], st));
         ++queued;
@@ -261,7 +263,8 @@ int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_p
   const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, ((uint64_t)n + 255) / 256);
   const int prec = scene_precision(scene);
   if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
-  launch_sm(ctx, prec, blocks, (uint64_t)n, 1, nullptr, false, scene.n_inst == 1, true, d_pix, d_smp, d_out, d_cnt);
+  launch_sm(ctx, prec, blocks, (uint64_t)n, 1, nullptr, false, scene.n_inst == 1, scene.n_surf > 0, true, d_pix, d_smp,
+            d_out, d_cnt);
   R2_HIP(ctx, hipGetLastError());
   R2_HIP(ctx, hipStreamSynchronize(st));
   return R2_OK;

@@ -114,11 +114,11 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
 }
 
 // One shading pass of a lane at a vertex (ST_SHADE): the direct light of the
-// previous scatter vertex, then the end of the ray that arrived (a collision
-// becomes the next scatter vertex; an escaped camera ray sees the environment;
-// anything else ends the path), and for a path that ends the start of the
-// item's next sample (vertex 0: the camera ray).
-template <bool kSingle, bool kCount, typename Acc>
+// previous vertex, then the end of the ray that arrived (a collision becomes the
+// next scatter vertex, a surface hit the next surface vertex; an escaped camera
+// ray sees the environment; anything else ends the path), and for a path that
+// ends the start of the item's next sample (vertex 0: the camera ray).
+template <bool kSingle, bool kSurf, bool kCount, typename Acc>
 __device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, Acc* __restrict__ partial,
                                       const int32_t* __restrict__ probe_pix, float* __restrict__ probe_out,
                                       uint32_t* __restrict__ probe_cnt) {
@@ -127,7 +127,7 @@ __device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, ui
     v3 Ld = mk(0.f, 0.f, 0.f);
     if (fl & FL_LIGHT) Ld = Ld + p.c.nee_a;
     if (fl & FL_PHASE) Ld = Ld + p.c.nee_b;
-    p.L = p.L + p.thr * Ld;
+    p.L = p.L + ((kSurf && (fl & FL_PREMUL)) ? Ld : p.thr * Ld);
     if (fl & FL_RR) p.thr = p.thr / p.c.rr_div;
     p.c.flags = 0u;
   }
@@ -143,6 +143,10 @@ __device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, ui
   }
   if (scatter) {
     shade_scatter<kSingle, kCount>(s, p, sc, cnt);
+    return;
+  }
+  if (kSurf && p.cres == CR_SURFACE) {
+    shade_surface<kCount>(s, p, cnt);
     return;
   }
   if (p.cres != CR_START) {
@@ -228,7 +232,7 @@ struct FrameFetch {
 #ifndef R2_SEG_RATIO
 #define R2_SEG_RATIO 2
 #endif
-template <bool kSingle, bool kCount, typename Acc, typename Fetch>
+template <bool kSingle, bool kSurf, bool kCount, typename Acc, typename Fetch>
 __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cnt, int lane, uint32_t npix,
                                            Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
                                            float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt,
@@ -245,7 +249,8 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
     fetch(s, p, lane, npix);
     SM_STAT(1, 1);
     SM_STAT(2, __popcll(__ballot(p.stage == ST_SHADE)));
-    if (p.stage == ST_SHADE) shade<kSingle, kCount, Acc>(s, p, cnt, npix, partial, probe_pix, probe_out, probe_cnt);
+    if (p.stage == ST_SHADE)
+      shade<kSingle, kSurf, kCount, Acc>(s, p, cnt, npix, partial, probe_pix, probe_out, probe_cnt);
 #ifdef R2_SM_STATS
     SM_STAT(5, SM_CLOCK() - t0);
 #endif
@@ -254,7 +259,7 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
     SM_STAT(4, ns);
     if (at_seg) {
       p.rng.top_up();
-      seg_step<kSingle>(s, p);
+      seg_step<kSingle, kSurf>(s, p);
     }
 #ifdef R2_SM_STATS
     SM_STAT(6, SM_CLOCK() - t0);
@@ -276,7 +281,7 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
 #ifndef R2_SM_WAVES
 #define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for
 #endif
-template <bool kSingle, bool kCount, typename Acc>
+template <bool kSingle, bool kSurf, bool kCount, typename Acc>
 __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __restrict__ sp, uint64_t* __restrict__ counter,
                                                   uint64_t total, int chunk, Acc* __restrict__ partial,
                                                   const int32_t* __restrict__ probe_pix,
@@ -297,7 +302,8 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
   stats_begin();
   for (uint32_t guard = 0; guard < (1u << 30); ++guard) {
     SM_STAT(0, 1);
-    if (!sched_step<kSingle, kCount, Acc>(s, p, cnt, lane, npix, partial, probe_pix, probe_out, probe_cnt, fetch)) break;
+    if (!sched_step<kSingle, kSurf, kCount, Acc>(s, p, cnt, lane, npix, partial, probe_pix, probe_out, probe_cnt, fetch))
+      break;
   }
   stats_end();
 }
@@ -341,7 +347,7 @@ struct BatchFetch {
   }
 };
 
-template <bool kSingle, typename Acc>
+template <bool kSingle, bool kSurf, typename Acc>
 __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEntry* __restrict__ ents, int n_ent,
                                                                   uint64_t* __restrict__ counters) {
   const int lane = threadIdx.x & 63;
@@ -366,7 +372,8 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
     p.cres = CR_START;
     const BatchFetch fetch{&E, &counters[e]};
     for (uint32_t guard = 0; guard < (1u << 30); ++guard) {
-      if (!sched_step<kSingle, false, Acc>(s, p, cnt, lane, npix, partial, nullptr, nullptr, nullptr, fetch)) break;
+      if (!sched_step<kSingle, kSurf, false, Acc>(s, p, cnt, lane, npix, partial, nullptr, nullptr, nullptr, fetch))
+        break;
     }
   }
   stats_end();
@@ -396,37 +403,46 @@ __global__ void k_reduce(const Acc* __restrict__ partial, int nchunks, uint32_t 
 
 
 
-// host-side launchers of this instantiation
+// host-side launchers of this instantiation.  Scenes with analytic surfaces run
+// the general-instance form with the surface code; the others the forms without.
 void launch_sm(hipStream_t st, int blocks, const DScene* d_scene, uint64_t* counter, uint64_t total, int chunk,
-               void* partial, bool fp64, bool single, bool count, const int32_t* pp, const int32_t* ps, float* po,
-               uint32_t* pc) {
-#define R2K_LAUNCH(S, C, A)                                                                                        \
-  hipLaunchKernelGGL((k_paths_sm<S, C, A>), dim3(blocks), dim3(256), 0, st, d_scene, counter, total, chunk, (A*)partial, \
-                     pp, ps, po, pc)
-  if (count) {
-    if (single) R2K_LAUNCH(true, true, float);
-    else R2K_LAUNCH(false, true, float);
+               void* partial, bool fp64, bool single, bool surf, bool count, const int32_t* pp, const int32_t* ps,
+               float* po, uint32_t* pc) {
+#define R2K_LAUNCH(S, F, C, A)                                                                                  \
+  hipLaunchKernelGGL((k_paths_sm<S, F, C, A>), dim3(blocks), dim3(256), 0, st, d_scene, counter, total, chunk, \
+                     (A*)partial, pp, ps, po, pc)
+  if (surf) {
+    if (count) R2K_LAUNCH(false, true, true, float);
+    else if (fp64) R2K_LAUNCH(false, true, false, double);
+    else R2K_LAUNCH(false, true, false, float);
+  } else if (count) {
+    if (single) R2K_LAUNCH(true, false, true, float);
+    else R2K_LAUNCH(false, false, true, float);
   } else if (fp64) {
-    if (single) R2K_LAUNCH(true, false, double);
-    else R2K_LAUNCH(false, false, double);
+    if (single) R2K_LAUNCH(true, false, false, double);
+    else R2K_LAUNCH(false, false, false, double);
   } else {
-    if (single) R2K_LAUNCH(true, false, float);
-    else R2K_LAUNCH(false, false, float);
+    if (single) R2K_LAUNCH(true, false, false, float);
+    else R2K_LAUNCH(false, false, false, float);
   }
 #undef R2K_LAUNCH
 }
 
 void launch_batch_group(hipStream_t st, int blocks, const void* ents, int n, uint64_t* counters, bool single,
-                        bool fp64) {
+                        bool surf, bool fp64) {
   const BatchEntry* de = (const BatchEntry*)ents;
-  if (single && fp64)
-    hipLaunchKernelGGL((k_paths_batch<true, double>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+  if (surf && fp64)
+    hipLaunchKernelGGL((k_paths_batch<false, true, double>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+  else if (surf)
+    hipLaunchKernelGGL((k_paths_batch<false, true, float>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+  else if (single && fp64)
+    hipLaunchKernelGGL((k_paths_batch<true, false, double>), dim3(blocks), dim3(256), 0, st, de, n, counters);
   else if (single)
-    hipLaunchKernelGGL((k_paths_batch<true, float>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+    hipLaunchKernelGGL((k_paths_batch<true, false, float>), dim3(blocks), dim3(256), 0, st, de, n, counters);
   else if (fp64)
-    hipLaunchKernelGGL((k_paths_batch<false, double>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+    hipLaunchKernelGGL((k_paths_batch<false, false, double>), dim3(blocks), dim3(256), 0, st, de, n, counters);
   else
-    hipLaunchKernelGGL((k_paths_batch<false, float>), dim3(blocks), dim3(256), 0, st, de, n, counters);
+    hipLaunchKernelGGL((k_paths_batch<false, false, float>), dim3(blocks), dim3(256), 0, st, de, n, counters);
 }
 
 void launch_reduce_k(hipStream_t st, const void* partial, bool fp64, int nchunks, uint32_t n3, float* out) {

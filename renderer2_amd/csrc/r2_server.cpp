@@ -47,10 +47,11 @@ struct SrvReq {
   int64_t id = 0;
   r2_render_params params{};
   std::vector<r2_instance> inst;
+  std::vector<r2_surface> surfs;
   float* out = nullptr;
   r2::DScene scene{};
   int chunk = 1, nchunks = 1;
-  bool fp64 = false, single = true, packed = false;
+  bool fp64 = false, single = true, surf = false, packed = false;
   uint64_t total_items = 0, next_item = 0;  // items in all, items handed to slices
   uint64_t done_items = 0;                   // items of completed slices
   int inflight = 0;                          // slices on the GPU that hold items of this request
@@ -314,8 +315,8 @@ void scheduler(r2_server* s) {
           extra -= more * (uint64_t)r.chunk;
           items = (items + 63) & ~(uint64_t)63;
           items = std::min<uint64_t>(items, left);
-          slice.push_back(r2host::BatchSlice{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.fp64,
-                                             r.packed});
+          slice.push_back(r2host::BatchSlice{r.d_scene, r.d_partial, r.next_item, items, r.chunk, r.single, r.surf,
+                                             r.fp64, r.packed});
           sl.took.push_back({r.id, items});
           r.next_item += items;
           ++r.inflight;
@@ -413,6 +414,11 @@ extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
     return R2_ERR_INVALID;
   r->inst.assign(r->params.instances, r->params.instances + r->params.n_instances);
   r->params.instances = r->inst.data();
+  if (r->params.n_surfaces < 0 || r->params.n_surfaces > r2::kMaxSurfaces ||
+      (r->params.n_surfaces > 0 && !r->params.surfaces))
+    return R2_ERR_INVALID;
+  r->surfs.assign(r->params.surfaces, r->params.surfaces + r->params.n_surfaces);
+  r->params.surfaces = r->surfs.data();
   r->out = req->out_rgb;
   {
     // scene assembly on the caller's thread (validates the request)
@@ -423,7 +429,9 @@ extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
   }
   r2host::render_chunking(r->scene, &r->chunk, &r->nchunks);
   r->fp64 = (r->params.flags & R2_FLAG_FP64_ACCUM) != 0;
-  r->single = r->scene.n_inst == 1;
+  // scenes with surfaces run the general-instance form of the batched kernel
+  r->surf = r->scene.n_surf > 0;
+  r->single = r->scene.n_inst == 1 && !r->surf;
   {
     int prec = -2;
     for (int i = 0; i < r->scene.n_inst; ++i) {

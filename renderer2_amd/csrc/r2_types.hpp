@@ -67,10 +67,27 @@ struct DEnv {
   float marg_int;
 };
 
+// An analytic surface (CCircle / Sphere) with its CMaterial: shape 0 circle, 1
+// sphere; transforms translate(pos) * getRotation(normal) and its inverse; the
+// Oren-Nayar A, B of the diffuse roughness; the microfacet alphas and the
+// Fresnel indices of the mixture weight.
+constexpr int kMaxSurfaces = 8;
+struct DSurface {
+  int32_t shape;
+  float pos[3], normal[3];
+  float radius;
+  m43 m2w, w2m;
+  float diffuse[3], specular[3];
+  float on_a, on_b;
+  float alpha_x, alpha_y, eta_i, eta_t;
+};
+
 struct DScene {
   int32_t n_inst;
   DInstance inst[kMaxInstances];
   DGrid grid[kMaxInstances];
+  int32_t n_surf;
+  DSurface surf[kMaxSurfaces];
   DEnv env;
   m43 view_to_world;
   float pixel_size, near_dist;

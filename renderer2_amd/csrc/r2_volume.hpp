@@ -691,17 +691,26 @@ __device__ __forceinline__ v3 env_le(const DEnv& e, v3 d, float& pdf, Cnt& cnt) 
 }
 
 // ---------------------------------------------------------------- scene --
+}  // namespace r2
+#include "r2_surface.hpp"
+namespace r2 {
+
 struct Hit {
   int hit;
   float t;
   v3 pos;
   int medium;
+  int surf;   // surface index, -1 for a box or a miss
+  v3 normal;  // a surface's normalised normal
 };
 // __intersection__volume (shaders.optix.cu:192-236) with nanovdb::Ray::intersects
-// (Williams et al. 2005 slab test) and BBox::isInside; closest accepted hit wins.
+// (Williams et al. 2005 slab test) and BBox::isInside, then (kSurf) the analytic
+// surfaces through __intersection__surface (:170-189); closest accepted hit wins.
+template <bool kSurf = false>
 __device__ __forceinline__ Hit scene_intersect(const DScene& s, Ray& ray) {
   Hit h;
   h.hit = 0; h.t = 0.f; h.medium = -1; h.pos = mk(0.f, 0.f, 0.f);
+  h.surf = -1; h.normal = mk(0.f, 0.f, 0.f);
   const float best = ray.tmax;
   const float o[3] = {ray.o.x, ray.o.y, ray.o.z};
   const float inv[3] = {1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z};
@@ -736,6 +745,15 @@ __device__ __forceinline__ Hit scene_intersect(const DScene& s, Ray& ray) {
     if (!(t >= 0.f && t <= best)) continue;
     if (h.hit && !(t < h.t)) continue;
     h.hit = 1; h.t = t; h.medium = k;
+  }
+  if constexpr (kSurf) {
+    for (int j = 0; j < s.n_surf; ++j) {
+      float t;
+      v3 n;
+      if (!surface_hit(s.surf[j], ray, h.hit ? h.t : best, t, n) || (h.hit && !(t < h.t))) continue;
+      h.hit = 1; h.t = t; h.medium = -1; h.surf = j; h.normal = n;
+    }
+    if (h.surf >= 0) h.normal = normalize(h.normal);
   }
   if (h.hit) {
     h.pos = mk(ray.o.x + h.t * ray.d.x, ray.o.y + h.t * ray.d.y, ray.o.z + h.t * ray.d.z);

@@ -134,6 +134,29 @@ def make_workload(name: str, seed: int = 1, **overrides) -> dict:
                 env=env, render=render, params=w)
 
 
+# The ground of the reference's forest scenes: a circle of radius FLT_MAX through
+# the origin with a dark rough diffuse lobe under a rough glossy one
+# (raytracer/src/filtering/common_functions.py:94-106)
+GROUND = dict(shape="circle", pos=(0.0, 0.0, 0.0), radius=3.402823466e38, normal=(0.0, 1.0, 0.0),
+              diffuse=(0.07, 0.07, 0.01), diffuse_roughness=0.999, specular=(0.9, 0.9, 0.9), alpha_x=0.99,
+              alpha_y=0.99, eta_i=1.00029, eta_t=1.2)
+
+
+def mixed_workload(name: str = "c1", ground_y: float = -0.4, seed: int = 1, **overrides) -> dict:
+    """A volume workload standing on the forest ground, with a glossy sphere half
+    inside the volume's box: the ground cuts the box's bottom, so paths meet
+    surfaces both outside and inside the medium.  The camera looks down at the
+    scene from above the ground."""
+    w = make_workload(name, seed=seed, **overrides)
+    ground = dict(GROUND, pos=(0.0, ground_y, 0.0))
+    sphere = dict(shape="sphere", pos=(0.45, -0.15, 0.35), radius=0.25, normal=(0.0, 1.0, 0.0),
+                  diffuse=(0.6, 0.3, 0.1), diffuse_roughness=0.5, specular=(0.9, 0.9, 0.9), alpha_x=0.2,
+                  alpha_y=0.35, eta_i=1.0, eta_t=1.5)
+    w["render"].update(surfaces=[ground, sphere], cam_pos=(0.0, 0.7, 2.3), cam_look_at=(0.0, -0.15, 0.0))
+    w["name"] = name + "+surfaces"
+    return w
+
+
 def filtered_grid(fgrid, voxel_size: float, albedo: float = 0.9, records=None, records_dev: int | None = None,
                   precision: str = "fp32") -> dict:
     """Grid description of one filtered LOD (r2_filter_grid): the index -> model map

@@ -34,7 +34,7 @@ def test_signature_table_covers_the_header():
 
 
 def test_version():
-    assert capi.lib().r2_version() == 2
+    assert capi.lib().r2_version() == 3
 
 
 def test_null_arguments_are_rejected():
