@@ -182,3 +182,50 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
     assert rc == 0, err.value.decode()
     assert rep.lods_filtered == 0 and rep.media_rendered == 1
     assert np.array_equal(read_pfm(out2 / "output.pfm"), hdr)
+
+
+def test_pipeline_renders_surfaces(ctx, tmp_path):
+    """A scene description with the forest scenes' ground Circle and a Sphere
+    beside a Medium read from a NanoVDB file: the front end renders both surfaces
+    with their materials, and its frame equals r2_render of the same scene."""
+    w = scenes.mixed_workload("c1", width=64, height=48, spp=16, max_depth=8)
+    g = w["grids"][0]
+    vs = 1.0 / 64
+    path = tmp_path / "blob" / ("filtered_mesh_%f" % vs)
+    path.parent.mkdir()
+    r2.nvdb_write(str(path) + ".nvdb", g["records"], np.array(g["map_mat"], np.float32),
+                  np.array(g["map_vec"], np.float32))
+    rc_ = w["render"]
+    cfg = {
+        "General": {"FrameWidth": 64, "FrameHeight": 48, "ChannelsPerPixel": 3, "Samples": 16, "Gamma": 2.0,
+                    "UseBrickGrid": True, "DebugMode": "NONE", "MaxDepth": 8, "Seed": 3},
+        "Filtering": {"Active": False, "MinVoxelSize": vs, "LODs": 1, "SamplesPerVoxel": 32, "Debug": False,
+                      "DebugSamples": 1, "SigmaT": 10.0, "EstimationIterations": 4, "Alpha": 0.1, "ClipRays": True},
+        "Camera": {"Fov": rc_["fov_deg"], "Pos": list(rc_["cam_pos"]), "LookAt": list(rc_["cam_look_at"]),
+                   "Up": list(rc_["cam_up"])},
+        "Scene": {"Envmap": "synth:white", "Scenedescription": "scene.json"},
+    }
+    shape_key = {"circle": "Circle", "sphere": "Sphere"}
+    scene = [{"Type": "Medium", "Path": "blob/filtered_mesh_%f" % vs, "SigmaA": list(g["sigma_a"]),
+              "SigmaS": list(g["sigma_s"]), "Pos": [0, 0, 0], "Orientation": [0, 1, 0], "Scaling": [1, 1, 1]}]
+    for sf in rc_["surfaces"]:
+        scene.append({"Type": shape_key[sf["shape"]], "Pos": list(sf["pos"]), "Radius": sf["radius"],
+                      "Normal": list(sf["normal"]), "DiffuseReflection": list(sf["diffuse"]),
+                      "DiffuseRoughness": sf["diffuse_roughness"], "SpecularReflection": list(sf["specular"]),
+                      "AlphaX": sf["alpha_x"], "AlphaY": sf["alpha_y"], "EtaI": sf["eta_i"], "EtaT": sf["eta_t"]})
+    (tmp_path / "config.json").write_text("// surfaces\n" + json.dumps(cfg))
+    (tmp_path / "scene.json").write_text(json.dumps(scene))
+    rep = capi.PipelineReport()
+    err = C.create_string_buffer(1024)
+    rc = capi.lib().r2_pipeline_run(str(tmp_path / "config.json").encode(), str(tmp_path).encode(), 0, C.byref(rep),
+                                    err, 1024)
+    assert rc == 0, err.value.decode()
+    assert rep.media_rendered == 1 and rep.surfaces_rendered == 2 and rep.objects_skipped == 0
+    hdr = read_pfm(tmp_path / "output.pfm")
+    grid = ctx.upload_grid(dict(g, voxel_size_filtering=float(np.float32(vs))))
+    env = ctx.upload_env(scenes.synth_env("white", 32, 16))
+    ref = ctx.render(dict(rc_, width=64, height=48, spp=16, max_depth=8, seed=3, stream=0), [grid], w["instances"],
+                     env)
+    assert np.array_equal(hdr, ref)
+    # the ground is visible: the bottom rows see the dark ground, not the white sky
+    assert hdr[:4].mean() < 0.5 * hdr[-4:].mean()
