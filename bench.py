@@ -79,7 +79,7 @@ def cpu_baseline(w, threads: int, stride: int, spp: int):
     pix = (ys * W + xs).reshape(-1).astype(np.int32)
     P = np.repeat(pix, spp)
     S = np.tile(np.arange(spp, dtype=np.int32), len(pix))
-    sc = O.Scene(w["grids"], w["instances"], w["env"], "f32")
+    sc = O.Scene(w["grids"], w["instances"], w["env"], "f32", surfaces=cfg.get("surfaces", ()))
     t0 = time.perf_counter()
     _, cnt = sc.render_paths(cfg, P, S, threads=threads, counters=True)
     dt = time.perf_counter() - t0
@@ -94,6 +94,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--no-serving", action="store_true", help="skip the render-server latency probe")
+    ap.add_argument("--surfaces", action="store_true",
+                    help="render workloads: the volume on the forest scenes' ground circle with a glossy sphere")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=None,
                     help="pixel stride of the CPU baseline sample (default: 1 for c1, 2 otherwise)")
@@ -131,7 +133,8 @@ def main(argv=None) -> int:
     import renderer2_amd as r2
     from renderer2_amd import _hip, scenes
 
-    w = scenes.make_workload(a.workload, seed=1, **({"precision": a.precision} if a.precision else {}))
+    kw = {"precision": a.precision} if a.precision else {}
+    w = scenes.mixed_workload(a.workload, **kw) if a.surfaces else scenes.make_workload(a.workload, seed=1, **kw)
     cfg = dict(w["render"], use_brick_grid=not a.no_brick_grid)
     W, H, spp = cfg["width"], cfg["height"], cfg["spp"]
     ctx = r2.Context(0)
@@ -162,7 +165,7 @@ def main(argv=None) -> int:
     # frame is already divided by its own spp, so the frame is their mean
     quality = None
     gold = GOLDEN / f"{a.workload}_oracle_fp64.npz"
-    if gold.exists():
+    if gold.exists() and not a.surfaces:
         z = np.load(gold, allow_pickle=False)
         st = int(z["stride"])
         ref = z["image"].astype(np.float64)
@@ -216,7 +219,7 @@ def main(argv=None) -> int:
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "bytes_per_path": round(b_path, 1), "kernel": "k_paths_sm", "kernel_ms": round(avg_kernel_ms, 3)}
-        if a.workload == "c2" and w["grids"][0].get("precision", "fp32") == "fp32":
+        if a.workload == "c2" and w["grids"][0].get("precision", "fp32") == "fp32" and not a.surfaces:
             roof["traffic"], roof["traffic_source"] = profile_traffic("k_paths_sm_c2_256cube_1024sq_64spp")
     line = {
         "metric": METRIC,
@@ -235,7 +238,8 @@ def main(argv=None) -> int:
                                 "c2": "256^3 anisotropic SGGX fibre volume, procedural sky+sun env, "
                                       "1024x1024, 64 spp per step, depth 16",
                                 "c4": "512^3 isotropic value-noise volume, 1024x1024, 256 spp per step, depth 8"}
-                               [a.workload],
+                               [a.workload] + (" on the forest ground circle with a glossy sphere"
+                                               if a.surfaces else ""),
                    "precision": w["grids"][0].get("precision", "fp32"),
                    "grid": w["params"]["n"], "width": W, "height": H, "spp_per_step": spp,
                    "max_depth": cfg["max_depth"], "brick_grid": cfg["use_brick_grid"],
