@@ -123,6 +123,13 @@ int r2_env_upload(r2_ctx* ctx, const float* rgb, int32_t w, int32_t h, r2_env** 
 int r2_env_free(r2_ctx* ctx, r2_env* env);
 
 /* -------------------------------------------------------------- render -- */
+/* A render takes up to R2_MAX_INSTANCES instances of at most R2_MAX_GRIDS
+ * distinct grids (the reference's instance acceleration structure over its medium
+ * objects, raytracer/src/scene/scene.cu:188-256).  Beyond 8 instances the boxes
+ * are found through a hierarchy built per render, with the same closest hit as a
+ * scan in instance order. */
+#define R2_MAX_INSTANCES (1 << 20)
+#define R2_MAX_GRIDS 16
 typedef struct {
   const r2_grid* grid;
   float pos[3], orientation[3], scaling[3];   /* CHostSceneobject(medium, pos, orientation, scaling) */
@@ -194,6 +201,15 @@ int r2_render_progressive(r2_ctx* ctx, const r2_render_params* p, float* out_rgb
  * chunk sums fit in 64 MiB.  All calls on one GPU share the array: calls that
  * overlap on one GPU (from several contexts or threads) race. */
 int r2_render_noalloc(r2_ctx* ctx, const r2_render_params* p, float* out_rgb);
+/* (r2_render_noalloc also takes the scene's instance table from the context's
+ * preallocated scene buffer: scenes of more than 4096 instances return
+ * R2_ERR_UNSUPPORTED there.) */
+
+/* Parity probe: CDeviceScene::intersect (raytracer/include/scene/device_scene_impl.hpp:12-31, with
+ * __intersection__volume / __intersection__surface, shaders.optix.cu:170-236) of n rays
+ * (origin xyz, direction xyz, tmax) against the instance boxes and surfaces of p's scene
+ * (the camera and frame fields are not used).  out: n x (hit, t, instance or -1, surface or -1). */
+int r2_scene_intersect(r2_ctx* ctx, const r2_render_params* p, int32_t n, const float* rays7, float* out4);
 
 /* Parity probe: one path per (pixel, sample) pair; out: n*3 (L / spp);
  * counters: n*8 u32 lookup counters (voxel records, majorants, brick densities,

@@ -11,7 +11,7 @@ from pathlib import Path
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libr2oracle.so"
-MAX_INST = 8
+MAX_GRIDS = 16
 NCNT = 7
 CNT_NAMES = ["vox", "maj", "brick", "env", "dda", "flight", "scatter"]
 
@@ -62,8 +62,8 @@ def surface_desc(sf: dict) -> SurfaceDesc:
 
 class SceneDesc(C.Structure):
     _fields_ = [
-        ("n_grids", C.c_int), ("grids", GridDesc * MAX_INST),
-        ("n_instances", C.c_int), ("instances", InstanceDesc * MAX_INST),
+        ("n_grids", C.c_int), ("grids", GridDesc * MAX_GRIDS),
+        ("n_instances", C.c_int), ("instances", C.POINTER(InstanceDesc)),
         ("env_rgb", C.c_void_p), ("env_w", C.c_int), ("env_h", C.c_int),
         ("n_surfaces", C.c_int), ("surfaces", SurfaceDesc * MAX_SURF),
     ]
@@ -302,9 +302,13 @@ class Scene:
             gd.sigma_a[:] = list(g["sigma_a"])
             gd.sigma_s[:] = list(g["sigma_s"])
             gd.voxel_size_filtering = g["voxel_size_filtering"]
+        assert len(grids) <= MAX_GRIDS
         d.n_instances = len(instances)
+        iarr = (InstanceDesc * max(1, len(instances)))()
+        self._keep.append(iarr)
+        d.instances = iarr
         for k, inst in enumerate(instances):
-            idd = d.instances[k]
+            idd = iarr[k]
             idd.grid = inst.get("grid", 0)
             idd.pos[:] = list(inst.get("pos", (0, 0, 0)))
             idd.orientation[:] = list(inst.get("orientation", (0, 1, 0)))
@@ -441,8 +445,9 @@ class Scene:
         return out
 
     def intersect(self, ray7):
+        """(hit, [t, pos xyz, instance, surface]) of the scene intersection."""
         r = np.ascontiguousarray(ray7, self.R)
-        out = np.zeros(5, self.R)
+        out = np.zeros(6, self.R)
         hit = self._fn("r2o_intersect")(self.h, _p(r), _p(out))
         return hit, out
 

@@ -49,7 +49,8 @@ float r2o_brick_majorant(const r2o_bricks* b, int bx, int by, int bz, int mip);
 float r2o_brick_density(const r2o_bricks* b, int ix, int iy, int iz);
 
 /* ---- scene description shared by the FP32 / FP64 instantiations ---- */
-#define R2O_MAX_INSTANCES 8
+/* distinct grids of a scene; instances (any number) refer to them by index */
+#define R2O_MAX_GRIDS 16
 
 typedef struct {
   int nx, ny, nz;
@@ -79,9 +80,9 @@ typedef struct {
 
 typedef struct {
   int n_grids;
-  r2o_grid_desc grids[R2O_MAX_INSTANCES];
+  r2o_grid_desc grids[R2O_MAX_GRIDS];
   int n_instances;
-  r2o_instance_desc instances[R2O_MAX_INSTANCES];
+  const r2o_instance_desc* instances;  /* n_instances entries */
   const float* env_rgb;     /* env_h rows of env_w RGB float texels (row 0 = theta 0) */
   int env_w, env_h;
   int n_surfaces;

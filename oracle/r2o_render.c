@@ -247,10 +247,13 @@ typedef struct { uint32_t k0, k1, c0, c1; } path_key;
 static inline void stream_init(r2o_rng* r, const path_key* pk, uint32_t k, uint32_t role) {
   r2o_rng_init(r, pk->k0, pk->k1, pk->c0, pk->c1, 4u * k + role);
 }
-/* cap on the segments of one intersectTr walk (device_scene_impl.hpp:93-114
- * loops until a miss): a segment enters or leaves an instance box, so scenes of
- * at most R2O_MAX_INST boxes never reach it */
-#define SHADOW_SEG_CAP 64
+/* intersectTr (device_scene_impl.hpp:93-114) loops until a miss, and Li's
+ * bounce loop does not count box pass-throughs; both are capped here: 64
+ * segments (iterations beyond 4 max_depth) plus 4 per instance beyond the
+ * fourth.  A segment or pass-through enters or leaves an instance box, so a
+ * walk that crosses every box once stays below the cap. */
+static inline int inst_extra(int n_inst) { return n_inst > 4 ? 4 * (n_inst - 4) : 0; }
+#define SHADOW_SEG_CAP(s) (64 + inst_extra((s)->n_inst))
 
 /* ------------------------------------------------------------------ ray -- */
 typedef struct { v3 o, d; R tmax; int medium; } ray_t;
@@ -411,9 +414,9 @@ typedef struct {
 
 struct SFX(r2o_scene) {
   int n_media;
-  medium_t media[R2O_MAX_INSTANCES];
+  medium_t media[R2O_MAX_GRIDS];
   int n_inst;
-  instance_t inst[R2O_MAX_INSTANCES];
+  instance_t* inst; /* n_inst */
   int n_surf;
   surface_t surf[R2O_MAX_SURFACES];
   /* env map */
@@ -1257,7 +1260,7 @@ static int intersect_tr(const scene_t* s, const ray_t* ray, r2o_rng* rng, int us
   v3 p0 = ray->o;
   const v3 p1 = vadd(p0, sscale(ray->tmax, ray->d));
   int cur = ray->medium;
-  for (int it = 0; it < SHADOW_SEG_CAP; ++it) {
+  for (int it = 0; it < SHADOW_SEG_CAP(s); ++it) {
     ray_t r = spawn_ray(p0, p1, cur);
     const hit_t h = scene_intersect(s, &r);
     /* a surface ends the walk; its (zero) emission zeroes the shadow term, so
@@ -1424,7 +1427,7 @@ static v3 Li(const scene_t* s, const camera_t* cam, int px, int py, int spp, int
   r2o_rng* rng = &walk;
   int iters = 0;
   for (int b = 0; b < max_depth; ++b) {
-    if (++iters > 4 * max_depth + 64) break;
+    if (++iters > 4 * max_depth + 64 + inst_extra(s->n_inst)) break;
     const hit_t si = scene_intersect(s, &ray);
     mhit_t mi;
     mi.valid = 0;
@@ -1640,6 +1643,7 @@ scene_t* SFX(r2o_scene_new)(const r2o_scene_desc* d) {
   s->n_media = d->n_grids;
   for (int i = 0; i < d->n_grids; ++i) medium_init(&s->media[i], &d->grids[i]);
   s->n_inst = d->n_instances;
+  s->inst = (instance_t*)calloc(d->n_instances > 0 ? (size_t)d->n_instances : 1, sizeof(instance_t));
   for (int k = 0; k < d->n_instances; ++k) {
     const r2o_instance_desc* id = &d->instances[k];
     instance_t* in = &s->inst[k];
@@ -1706,6 +1710,7 @@ void SFX(r2o_scene_free)(scene_t* s) {
     r2o_brick_free(&s->media[i].bricks);
   }
   free(s->tex); free(s->row_func); free(s->row_cdf); free(s->row_int); free(s->marg_func); free(s->marg_cdf);
+  free(s->inst);
   free(s);
 }
 
@@ -1925,10 +1930,12 @@ void SFX(r2o_move_to_voxel_border)(const scene_t* s, const R* ray7, R* out7) {
   out7[0] = m.o.x; out7[1] = m.o.y; out7[2] = m.o.z; out7[3] = m.d.x; out7[4] = m.d.y; out7[5] = m.d.z; out7[6] = m.tmax;
 }
 /* closest-hit query: returns hit flag; out: t, pos(3), medium */
-int SFX(r2o_intersect)(const scene_t* s, const R* ray7, R* out5) {
+/* out: t, hit position, instance (-1: none), surface (-1: none) */
+int SFX(r2o_intersect)(const scene_t* s, const R* ray7, R* out6) {
   ray_t r = mkray(V(ray7[0], ray7[1], ray7[2]), V(ray7[3], ray7[4], ray7[5]), ray7[6], -1);
   const hit_t h = scene_intersect(s, &r);
-  out5[0] = h.t; out5[1] = h.pos.x; out5[2] = h.pos.y; out5[3] = h.pos.z; out5[4] = (R)h.medium;
+  out6[0] = h.t; out6[1] = h.pos.x; out6[2] = h.pos.y; out6[3] = h.pos.z; out6[4] = (R)h.medium;
+  out6[5] = (R)h.surf;
   return h.hit;
 }
 /* direct() at a medium interaction of instance `inst` at world position pos with

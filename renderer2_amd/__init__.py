@@ -212,6 +212,15 @@ class Context:
                                                out.ctypes.data, cnt.ctypes.data if counters else None))
         return (out, cnt[:, :7]) if counters else out
 
+    def scene_intersect(self, cfg: dict, grids, instances, env, rays7) -> np.ndarray:
+        """r2_scene_intersect (parity probe): n x (hit, t, instance, surface) of rays
+        (origin, direction, tmax) against the scene's instance boxes and surfaces."""
+        p, keep = self.render_params(cfg, grids, instances, env)
+        r = np.ascontiguousarray(rays7, np.float32)
+        out = np.zeros((len(r), 4), np.float32)
+        self._check(capi.lib().r2_scene_intersect(self.h, C.byref(p), len(r), r.ctypes.data, out.ctypes.data))
+        return out
+
     def dda_cells(self, rays7, sigma_t: float = 1.0, collide_every: int = 0, max_cells: int = 4096):
         """Majorant-DDA walks (parity probe): for each ray (origin, direction, tmax in
         voxel space) the (brick x, y, z, mip) of every majorant lookup."""

@@ -212,6 +212,7 @@ SIGNATURES = [
     ("r2_render_noalloc", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p]),
     ("r2_render_paths", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_int32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
+    ("r2_scene_intersect", C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("r2_mesh_upload", C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("r2_mesh_free", C.c_int, [C.c_void_p, C.c_void_p]),

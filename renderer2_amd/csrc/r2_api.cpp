@@ -84,14 +84,130 @@ m44 get_rotation(v3 n) {
   return r;
 }
 
-int build_scene(r2_ctx* ctx, const r2_render_params* p, DScene* out) {
+// ------------------------------------------------------ instance hierarchy --
+// Median-split hierarchy over the instances' world boxes (the stand-in for the
+// OptiX instance acceleration structure, scene.cu:188-256): nodes in depth-first
+// order with the index of the node after each subtree, leaves of at most 4
+// instances.  The split is on the longest axis of the box centres, instances
+// ordered by centre then index, so the build is deterministic.
+static void bvh_build(const std::vector<DInstance>& inst, std::vector<DBvhNode>& nodes, std::vector<int32_t>& idx) {
+  const int n = (int)inst.size();
+  idx.resize(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  nodes.clear();
+  nodes.reserve(2 * (size_t)n);
+  struct Build {
+    const std::vector<DInstance>& inst;
+    std::vector<DBvhNode>& nodes;
+    std::vector<int32_t>& idx;
+    void run(int lo, int hi) {
+      const int me = (int)nodes.size();
+      nodes.push_back(DBvhNode{});
+      DBvhNode nd{};
+      float clo[3], chi[3];
+      for (int a = 0; a < 3; ++a) {
+        nd.lo[a] = clo[a] = INFINITY;
+        nd.hi[a] = chi[a] = -INFINITY;
+      }
+      for (int i = lo; i < hi; ++i) {
+        const DInstance& in = inst[idx[i]];
+        for (int a = 0; a < 3; ++a) {
+          nd.lo[a] = std::min(nd.lo[a], in.bb_min[a]);
+          nd.hi[a] = std::max(nd.hi[a], in.bb_max[a]);
+          const float c = 0.5f * (in.bb_min[a] + in.bb_max[a]);
+          clo[a] = std::min(clo[a], c);
+          chi[a] = std::max(chi[a], c);
+        }
+      }
+      if (hi - lo <= 4) {
+        nd.leaf = (lo << 3) | (hi - lo);
+        nd.skip = me + 1;
+        nodes[me] = nd;
+        return;
+      }
+      int ax = 0;
+      for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+      const int mid = (lo + hi) / 2;
+      std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int32_t x, int32_t y) {
+        const float cx = inst[x].bb_min[ax] + inst[x].bb_max[ax], cy = inst[y].bb_min[ax] + inst[y].bb_max[ax];
+        return cx < cy || (cx == cy && x < y);
+      });
+      nd.leaf = -1;
+      run(lo, mid);
+      run(mid, hi);
+      nd.skip = (int)nodes.size();
+      nodes[me] = nd;
+    }
+  } b{inst, nodes, idx};
+  if (n > 0) b.run(0, n);
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t HostScene::device_bytes() const {
+  return align256(sizeof(DScene)) + align256(inst.size() * sizeof(DInstance)) +
+         align256(bvh.size() * sizeof(DBvhNode)) + align256(bvh_idx.size() * sizeof(int32_t));
+}
+
+void HostScene::pack(void* dev, std::vector<unsigned char>& out) const {
+  out.assign(device_bytes(), 0);
+  const size_t o_inst = align256(sizeof(DScene));
+  const size_t o_bvh = o_inst + align256(inst.size() * sizeof(DInstance));
+  const size_t o_idx = o_bvh + align256(bvh.size() * sizeof(DBvhNode));
+  DScene d = s;
+  char* base = (char*)dev;
+  d.inst = (const DInstance*)(base + o_inst);
+  d.bvh = bvh.empty() ? nullptr : (const DBvhNode*)(base + o_bvh);
+  d.bvh_idx = bvh_idx.empty() ? nullptr : (const int32_t*)(base + o_idx);
+  memcpy(out.data(), &d, sizeof(DScene));
+  if (!inst.empty()) memcpy(out.data() + o_inst, inst.data(), inst.size() * sizeof(DInstance));
+  if (!bvh.empty()) memcpy(out.data() + o_bvh, bvh.data(), bvh.size() * sizeof(DBvhNode));
+  if (!bvh_idx.empty()) memcpy(out.data() + o_idx, bvh_idx.data(), bvh_idx.size() * sizeof(int32_t));
+}
+
+size_t scene_bytes_for(int n_inst) {
+  HostScene h;
+  h.inst.resize(n_inst);
+  if (n_inst > kLinearInstances) {
+    h.bvh.resize(2 * (size_t)n_inst);
+    h.bvh_idx.resize(n_inst);
+  }
+  return h.device_bytes();
+}
+
+int upload_scene(r2_ctx* ctx, const HostScene& hs, bool may_alloc) {
+  const size_t need = hs.device_bytes();
+  if (need > ctx->scene_cap) {
+    if (!may_alloc)
+      return fail(ctx, R2_ERR_UNSUPPORTED,
+                  "render: the scene's " + std::to_string(hs.inst.size()) + " instances exceed the context's " +
+                      "preallocated scene table (" + std::to_string(kCtxSceneInstances) + " instances)");
+    R2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->d_scene);
+    ctx->d_scene = nullptr;
+    ctx->scene_cap = 0;
+    R2_HIP(ctx, hipMalloc((void**)&ctx->d_scene, need));
+    ctx->scene_cap = need;
+  }
+  // the image stays in the context until the next upload, past the stream's copy
+  hs.pack(ctx->d_scene, ctx->scene_img);
+  R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, ctx->scene_img.data(), need, hipMemcpyHostToDevice, ctx->stream));
+  return R2_OK;
+}
+
+int build_scene(r2_ctx* ctx, const r2_render_params* p, HostScene* hs) {
   if (!p || p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth < 0)
     return fail(ctx, R2_ERR_INVALID, "render: bad frame size / spp / max_depth");
   if ((int64_t)p->width * p->height > (1ll << 28)) return fail(ctx, R2_ERR_UNSUPPORTED, "render: frame too large");
   if (p->n_instances < 0 || p->n_instances > kMaxInstances)
-    return fail(ctx, R2_ERR_UNSUPPORTED, "render: at most 4 medium instances");
+    return fail(ctx, R2_ERR_UNSUPPORTED, "render: at most " + std::to_string(kMaxInstances) + " medium instances");
+  if (p->n_instances > 0 && !p->instances) return fail(ctx, R2_ERR_INVALID, "render: instances missing");
   if (!p->env) return fail(ctx, R2_ERR_INVALID, "render: an environment map is required (config Scene.Envmap)");
-  DScene& s = *out;
+  hs->inst.clear();
+  hs->bvh.clear();
+  hs->bvh_idx.clear();
+  DScene& s = hs->s;
   memset(&s, 0, sizeof(s));
   s.W = p->width; s.H = p->height; s.spp = p->spp; s.max_depth = p->max_depth;
   s.seed = p->seed; s.stream = p->stream; s.sample_begin = p->sample_begin; s.flags = p->flags;
@@ -112,14 +228,25 @@ int build_scene(r2_ctx* ctx, const r2_render_params* p, DScene* out) {
   M.c[0][2] = -f.x; M.c[1][2] = -f.y; M.c[2][2] = -f.z;
   M.c[3][0] = -dot(sv, eye); M.c[3][1] = -dot(u, eye); M.c[3][2] = dot(f, eye);
   s.view_to_world = to43(inverse44(to44(to43(M))));
-  // instances (CHostSceneobject(medium, pos, orientation, scaling), sceneobject.cu:50-65)
+  // instances (CHostSceneobject(medium, pos, orientation, scaling), sceneobject.cu:50-65);
+  // instances of one grid share its device view
   s.n_inst = p->n_instances;
+  hs->inst.resize(p->n_instances);
+  hs->precision = -2;
   for (int k = 0; k < p->n_instances; ++k) {
     const r2_instance& id = p->instances[k];
     if (!id.grid) return fail(ctx, R2_ERR_INVALID, "render: instance without grid");
-    s.grid[k] = id.grid->dev;
-    DInstance& in = s.inst[k];
-    in.grid = k;
+    int gidx = 0;
+    while (gidx < s.n_grid && s.grid[gidx].leaf_table != id.grid->dev.leaf_table) ++gidx;
+    if (gidx == s.n_grid) {
+      if (s.n_grid == kMaxGrids)
+        return fail(ctx, R2_ERR_UNSUPPORTED, "render: at most " + std::to_string(kMaxGrids) + " distinct grids");
+      s.grid[s.n_grid++] = id.grid->dev;
+      const int q = id.grid->dev.precision == 1 ? 1 : 0;
+      hs->precision = hs->precision == -2 ? q : (hs->precision == q ? q : -1);
+    }
+    DInstance& in = hs->inst[k];
+    in.grid = gidx;
     m44 T = identity44();
     T.c[3][0] = id.pos[0]; T.c[3][1] = id.pos[1]; T.c[3][2] = id.pos[2];
     const m44 R = get_rotation(mk(id.orientation[0], id.orientation[1], id.orientation[2]));
@@ -135,6 +262,11 @@ int build_scene(r2_ctx* ctx, const r2_render_params* p, DScene* out) {
     // getFilterRenderRatio (sceneobject.cu:108-114)
     const float vsr = (in.bb_max[0] - in.bb_min[0]) / (float)gi.size[0];
     in.ratio = id.grid->voxel_size_filtering / vsr;
+  }
+  if (hs->precision == -2) hs->precision = 0;
+  if (p->n_instances > kLinearInstances) {
+    bvh_build(hs->inst, hs->bvh, hs->bvh_idx);
+    s.n_bvh = (int32_t)hs->bvh.size();
   }
   // analytic surfaces (CHostSceneobject(shape, material ...), sceneobject.cu:28-48)
   if (p->n_surfaces < 0 || p->n_surfaces > kMaxSurfaces)
@@ -191,10 +323,12 @@ int r2_create(r2_ctx** out, int device) {
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->main_mark, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc((void**)&c->d_scene, sizeof(DScene)) != hipSuccess || hipMalloc((void**)&c->d_counter, 64) != hipSuccess) {
+      hipMalloc((void**)&c->d_scene, scene_bytes_for(kCtxSceneInstances)) != hipSuccess ||
+      hipMalloc((void**)&c->d_counter, 64) != hipSuccess) {
     r2_destroy(c);
     return R2_ERR_HIP;
   }
+  c->scene_cap = scene_bytes_for(kCtxSceneInstances);
   // the address of r2_render_noalloc's scratch array on this device (kept per
   // device; a failure is reported by that entry point, the context stays usable)
   (void)noalloc_scratch_init(c);
@@ -306,6 +440,16 @@ int r2_grid_sample(r2_ctx* ctx, const r2_grid* g, int32_t n, const float* pos, f
   if (!ctx || !g || n < 0 || (n > 0 && (!pos || !out))) return R2_ERR_INVALID;
   if (n == 0) return R2_OK;
   return grid_sample(ctx, g, n, pos, out);
+}
+
+int r2_scene_intersect(r2_ctx* ctx, const r2_render_params* p, int32_t n, const float* rays7, float* out4) {
+  if (!ctx || !p || n < 0 || (n > 0 && (!rays7 || !out4))) return R2_ERR_INVALID;
+  if (n == 0) return R2_OK;
+  (void)hipSetDevice(ctx->device);
+  HostScene s;
+  const int rc = build_scene(ctx, p, &s);
+  if (rc) return rc;
+  return scene_intersect_probe(ctx, s, n, rays7, out4);
 }
 
 int r2_dda_cells(r2_ctx* ctx, int32_t n, const float* rays7, float sigma_t, int32_t collide_every, int32_t max_cells,
@@ -422,7 +566,7 @@ static int render_common(r2_ctx* ctx, const r2_render_params* p, float* out, int
   if (!ctx || !p || !out) return R2_ERR_INVALID;
   const auto t0 = std::chrono::steady_clock::now();
   (void)hipSetDevice(ctx->device);
-  DScene s;
+  HostScene s;
   int rc = build_scene(ctx, p, &s);
   if (rc) return rc;
   const size_t bytes = (size_t)p->width * p->height * 3 * sizeof(float);
@@ -467,7 +611,7 @@ int r2_render_noalloc(r2_ctx* ctx, const r2_render_params* p, float* out) {
     (void)hipGetLastError();
     return fail(ctx, R2_ERR_INVALID, "render_noalloc: out_rgb must be device memory");
   }
-  DScene s;
+  HostScene s;
   int rc = build_scene(ctx, p, &s);
   if (rc) return rc;
   const bool fp64 = (p->flags & R2_FLAG_FP64_ACCUM) != 0;
@@ -481,6 +625,7 @@ int r2_render_noalloc(r2_ctx* ctx, const r2_render_params* p, float* out) {
   RenderOpts o;
   o.partial = scr;
   o.partial_bytes = kNoAllocScratchBytes;
+  o.scene_alloc = false;
   rc = launch_render_ex(ctx, s, out, fp64, o);
   ctx->stats.last_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
@@ -491,7 +636,7 @@ int r2_render_paths(r2_ctx* ctx, const r2_render_params* p, int32_t n, const int
   if (!ctx || !p || n < 0 || !pix || !smp || !out) return R2_ERR_INVALID;
   if (n == 0) return R2_OK;
   (void)hipSetDevice(ctx->device);
-  DScene s;
+  HostScene s;
   int rc = build_scene(ctx, p, &s);
   if (rc) return rc;
   for (int i = 0; i < n; ++i)

@@ -558,6 +558,39 @@ int grid_sample(r2_ctx* ctx, const r2_grid* g, int n, const float* pos, float* o
   return R2_OK;
 }
 
+// scene-intersection probe: CDeviceScene::intersect of each ray against the
+// scene's instance boxes (linear scan or hierarchy) and surfaces
+__global__ void k_scene_intersect(const r2::DScene* __restrict__ sp, const float* __restrict__ rays, int n,
+                                  float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const r2::DScene& s = r2::kconst(sp);
+  const float* q = rays + 7 * (size_t)i;
+  r2::Ray r = r2::mkray(r2::mk(q[0], q[1], q[2]), r2::mk(q[3], q[4], q[5]), q[6], -1);
+  const r2::Hit h = r2::scene_intersect<true, false>(s, r);
+  float* o = out + 4 * (size_t)i;
+  o[0] = (float)h.hit;
+  o[1] = h.t;
+  o[2] = (float)h.medium;
+  o[3] = (float)h.surf;
+}
+
+int scene_intersect_probe(r2_ctx* ctx, const HostScene& hs, int n, const float* rays7, float* out4) {
+  hipStream_t st = ctx->stream;
+  int rc = upload_scene(ctx, hs, true);
+  if (rc) return rc;
+  void *dr = nullptr, *dout = nullptr;
+  R2_HIP(ctx, pool_get(ctx, 0, (size_t)n * 7 * 4, &dr));
+  R2_HIP(ctx, pool_get(ctx, 1, (size_t)n * 4 * 4, &dout));
+  R2_HIP(ctx, hipMemcpyAsync(dr, rays7, (size_t)n * 7 * 4, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_scene_intersect, dim3((n + 255) / 256), dim3(256), 0, st, ctx->d_scene, (const float*)dr, n,
+                     (float*)dout);
+  R2_HIP(ctx, hipGetLastError());
+  R2_HIP(ctx, hipMemcpyAsync(out4, dout, (size_t)n * 4 * 4, hipMemcpyDeviceToHost, st));
+  R2_HIP(ctx, hipStreamSynchronize(st));
+  return R2_OK;
+}
+
 int dda_cells(r2_ctx* ctx, int n, const float* rays, float sigma_t, int collide_every, int max_cells, int32_t* cells,
               int32_t* counts) {
   void *dr = nullptr, *dc = nullptr, *dn = nullptr;

@@ -21,7 +21,11 @@ struct r2_ctx {
   // reusable device scratch for renders
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // the scene image (scene, instance table, hierarchy) of the last render, and
+  // its host copy, kept until the next upload
   r2::DScene* d_scene = nullptr;
+  size_t scene_cap = 0;
+  std::vector<unsigned char> scene_img;
   uint32_t* d_counter = nullptr;
   r2_filter_stats fstats{};
   // grow-only device buffers reused across calls (filter binning scratch)
@@ -81,8 +85,29 @@ int grid_lookup(r2_ctx* ctx, const r2_grid* g, int n, const int32_t* coords, uin
 int grid_sample(r2_ctx* ctx, const r2_grid* g, int n, const float* pos, float* out);
 int dda_cells(r2_ctx* ctx, int n, const float* rays, float sigma_t, int collide_every, int max_cells, int32_t* cells,
               int32_t* counts);
+// A render's scene as the host assembles it: the device scene and the tables that
+// follow it in device memory (the instances, and their hierarchy when there are
+// more than kLinearInstances), and the record precision of its grids (0 FP32,
+// 1 packed, -1 mixed).
+struct HostScene {
+  r2::DScene s{};
+  std::vector<r2::DInstance> inst;
+  std::vector<r2::DBvhNode> bvh;
+  std::vector<int32_t> bvh_idx;
+  int precision = 0;
+  size_t device_bytes() const;
+  // the device image for base address `dev`: the scene with its table pointers
+  // set into the image, then the tables
+  void pack(void* dev, std::vector<unsigned char>& out) const;
+};
+// instances a context's preallocated scene image holds (r2_render_noalloc's limit)
+constexpr int kCtxSceneInstances = 4096;
+size_t scene_bytes_for(int n_inst);
+// copy the scene image to ctx->d_scene on ctx->stream, growing the buffer when
+// may_alloc (else R2_ERR_UNSUPPORTED beyond kCtxSceneInstances)
+int upload_scene(r2_ctx* ctx, const HostScene& hs, bool may_alloc);
 // render kernels (r2_render.hip)
-int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum);
+int launch_render(r2_ctx* ctx, const HostScene& scene, float* d_out, bool fp64_accum);
 struct RenderOpts {
   // called after each sample chunk's launch (one launch per chunk); nonzero stops the frame
   r2_progress_fn progress = nullptr;
@@ -90,9 +115,11 @@ struct RenderOpts {
   // caller-provided chunk-sum buffer (no allocation); nullptr: the context's scratch
   void* partial = nullptr;
   size_t partial_bytes = 0;
+  // false: the scene image must fit the context's preallocated buffer
+  bool scene_alloc = true;
 };
-int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64_accum, const RenderOpts& o);
-int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
+int launch_render_ex(r2_ctx* ctx, const HostScene& scene, float* d_out, bool fp64_accum, const RenderOpts& o);
+int launch_paths(r2_ctx* ctx, const HostScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
                  uint32_t* d_cnt);
 // chunk-sum scratch of r2_render_noalloc: a file-scope device array of
 // kNoAllocScratchBytes in r2_render.hip, one copy per GPU for the process
@@ -116,7 +143,9 @@ int launch_reduce(hipStream_t st, const void* partial, bool fp64, int nchunks, u
 int launch_batch(r2_ctx* ctx, hipStream_t st, const std::vector<BatchSlice>& items, void* d_ents, uint64_t* d_counters);
 size_t batch_entry_bytes();
 // scene assembly (r2_api.cpp)
-int build_scene(r2_ctx* ctx, const r2_render_params* p, r2::DScene* out);
+int build_scene(r2_ctx* ctx, const r2_render_params* p, HostScene* out);
+// scene-intersection probe (r2_grid.hip): n x (hit, t, instance, surface)
+int scene_intersect_probe(r2_ctx* ctx, const HostScene& hs, int n, const float* rays7, float* out4);
 int ensure_scratch(r2_ctx* ctx, size_t bytes);
 // device buffer of at least `bytes` in pool slot `slot` (contents undefined)
 hipError_t pool_get(r2_ctx* ctx, int slot, size_t bytes, void** out);

@@ -26,6 +26,14 @@ __device__ __forceinline__ const T& kconst(const T* p) {
   asm volatile("" : "+s"(v));
   return *(const T*)(const __attribute__((address_space(4))) T*)v;
 }
+// element i of such a table: the base is wave-uniform, the index may differ per
+// lane (then the load is a vector load of the constant address space)
+template <typename T>
+__device__ __forceinline__ const T& kconst_at(const T* p, int i) {
+  uint64_t v = (uint64_t)(uintptr_t)p;
+  asm volatile("" : "+s"(v));
+  return ((const T*)(const __attribute__((address_space(4))) T*)v)[i];
+}
 
 struct v3 { float x, y, z; };
 R2_HD v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }

@@ -47,10 +47,12 @@ enum : int { CR_START = 0, CR_NONE, CR_END, CR_ESCAPED, CR_COLLIDED, CR_SURFACE 
 enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO = 4 };
 // ColdLane::flags (FL_PREMUL: the direct light already carries the throughput)
 enum : uint32_t { FL_LIGHT = 1u, FL_PHASE = 2u, FL_CONT = 4u, FL_NEE = 8u, FL_RR = 16u, FL_PREMUL = 32u };
-// cap on the segments of one intersectTr walk, which the reference loops until a
-// miss: a segment enters or leaves an instance box, so scenes of kMaxInstances
-// boxes never reach it (the oracle uses the same cap)
-constexpr int kShadowSegCap = 64;
+// caps on the segments of one intersectTr walk and on the bounce loop's
+// iterations, which the reference loops until a miss and does not count box
+// pass-throughs: 64 (beyond 4 max_depth iterations) plus 4 per instance beyond
+// the fourth; a segment or pass-through enters or leaves an instance box, so a
+// walk that crosses every box once stays below them (the oracle uses the same caps)
+__device__ __forceinline__ int inst_extra(const DScene& s) { return s.n_inst > 4 ? 4 * (s.n_inst - 4) : 0; }
 
 // Path state that is touched only in shading passes and at walk boundaries
 // lives in LDS, one slot per lane; the state the walk steps touch stays in
@@ -184,7 +186,7 @@ __device__ __forceinline__ bool any_zero(v3 a) { return a.x == 0.f || a.y == 0.f
 template <bool kSingle>
 __device__ __forceinline__ void start_traversal(const DScene& s, PathLane& p, int k, const Ray& rw, int kind) {
   const int ki = kSingle ? 0 : k;
-  const DInstance& in = s.inst[ki];
+  const DInstance& in = inst_at(s, ki);
   const DGrid& g = s.grid[in.grid];
   const Ray rm = ray_transform2(rw, in.w2m);
   p.tinst = k;
@@ -215,7 +217,7 @@ __device__ __forceinline__ void start_traversal(const DScene& s, PathLane& p, in
 template <bool kSingle, bool kCount>
 __device__ __forceinline__ bool traversal_step(const DScene& s, PathLane& p, Cnt& cnt) {
   const int ki = kSingle ? 0 : p.tinst;
-  const DInstance& in = s.inst[ki];
+  const DInstance& in = inst_at(s, ki);
   const DGrid& g = s.grid[in.grid];
   const float ratio = in.ratio;
   if (p.tk == TK_FLIGHT_DDA || p.tk == TK_TR_DDA) {
@@ -289,7 +291,7 @@ __device__ __forceinline__ bool traversal_step(const DScene& s, PathLane& p, Cnt
 template <bool kSingle, bool kCount>
 __device__ __forceinline__ v3 flight_collision(const DScene& s, const PathLane& p, Scatter& sc, Cnt& cnt) {
   const int ki = kSingle ? 0 : p.tinst;
-  const DInstance& in = s.inst[ki];
+  const DInstance& in = inst_at(s, ki);
   const DGrid& g = s.grid[in.grid];
   Rec fd;
   Ray rw;
@@ -400,7 +402,8 @@ template <bool kSingle, bool kSurf>
 __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
   const bool dda = (s.flags & 1u) != 0u;
   const bool cont = p.role == RL_CONT;
-  if (cont ? (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64) : ++p.sh_seg > kShadowSegCap) {
+  if (cont ? (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64 + inst_extra(s))
+           : ++p.sh_seg > 64 + inst_extra(s)) {
     if (cont) {
       cont_done(p, CR_END);
     } else {
@@ -411,7 +414,7 @@ __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
   }
   const int cur = p.ray.medium;
   Ray r = cont ? p.ray : spawn_ray(p.ray.o, p.ray.d, cur);
-  const Hit h = scene_intersect<kSurf>(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
+  const Hit h = scene_intersect<kSurf, kSingle>(s, r);  // shortens r.tmax to the hit, as the reference's intersect does
   Ray tr = r;  // the ray of the traversal to start
   int kind;
   if (cont) {
@@ -437,7 +440,7 @@ __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
     } else {
       Ray mr = mkray(h.pos, normalize(r.d), kDefaultTmax, h.medium);
       ray_offset(mr, r.d);
-      const Hit se = scene_intersect<kSurf>(s, mr);
+      const Hit se = scene_intersect<kSurf, kSingle>(s, mr);
       if (!se.hit) {
         cont_nohit(p);
         return;

@@ -27,17 +27,6 @@ static int num_cus(r2_ctx* ctx) {
   return g_num_cus;
 }
 
-// record precision of a scene: 0 FP32, 1 packed; -1 when its instances mix both
-static int scene_precision(const r2::DScene& s) {
-  int p = -2;
-  for (int i = 0; i < s.n_inst; ++i) {
-    const int q = s.grid[s.inst[i].grid].precision == 1 ? 1 : 0;
-    if (p == -2) p = q;
-    else if (p != q) return -1;
-  }
-  return p < 0 ? 0 : p;
-}
-
 static void launch_sm(r2_ctx* ctx, int prec, int blocks, uint64_t total, int chunk, void* partial, bool fp64, bool single,
                       bool surf, bool count, const int32_t* pp, const int32_t* ps, float* po, uint32_t* pc) {
   if (prec == 1)
@@ -121,8 +110,9 @@ static int launch_items(r2_ctx* ctx, int prec, bool single, bool surf, bool fp64
   return R2_OK;
 }
 
-int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64, const RenderOpts& o) {
+int launch_render_ex(r2_ctx* ctx, const HostScene& hs, float* d_out, bool fp64, const RenderOpts& o) {
   using namespace r2;
+  const DScene& scene = hs.s;
   const uint64_t npix = (uint64_t)scene.W * scene.H;
   int chunk = 1, nchunks = 1;
   render_chunking(scene, &chunk, &nchunks);
@@ -148,9 +138,12 @@ int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp
   const uint64_t total = (uint64_t)nchunks * npix;
   hipStream_t st = ctx->stream;
   const bool single = scene.n_inst == 1, surf = scene.n_surf > 0;
-  const int prec = scene_precision(scene);
+  const int prec = hs.precision;
   if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
-  R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(DScene), hipMemcpyHostToDevice, st));
+  {
+    const int rc = upload_scene(ctx, hs, o.scene_alloc);
+    if (rc) return rc;
+  }
   R2_HIP(ctx, hipEventRecord(ctx->ev0, st));
   int done_chunks = nchunks;
   int launches = 1;
@@ -219,7 +212,7 @@ int launch_render_ex(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp
   return R2_OK;
 }
 
-int launch_render(r2_ctx* ctx, const r2::DScene& scene, float* d_out, bool fp64) {
+int launch_render(r2_ctx* ctx, const HostScene& scene, float* d_out, bool fp64) {
   return launch_render_ex(ctx, scene, d_out, fp64, RenderOpts{});
 }
 
@@ -255,14 +248,18 @@ void* noalloc_scratch(const r2_ctx* ctx) {
   return g_scratch_dev[ctx->device];
 }
 
-int launch_paths(r2_ctx* ctx, const r2::DScene& scene, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
+int launch_paths(r2_ctx* ctx, const HostScene& hs, int n, const int32_t* d_pix, const int32_t* d_smp, float* d_out,
                  uint32_t* d_cnt) {
   hipStream_t st = ctx->stream;
-  R2_HIP(ctx, hipMemcpyAsync(ctx->d_scene, &scene, sizeof(r2::DScene), hipMemcpyHostToDevice, st));
+  const r2::DScene& scene = hs.s;
+  const int prec = hs.precision;
+  if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
+  {
+    const int rc = upload_scene(ctx, hs, true);
+    if (rc) return rc;
+  }
   R2_HIP(ctx, hipMemsetAsync(ctx->d_counter, 0, 8, st));
   const int blocks = (int)std::min<uint64_t>((uint64_t)num_cus(ctx) * 8, ((uint64_t)n + 255) / 256);
-  const int prec = scene_precision(scene);
-  if (prec < 0) return fail(ctx, R2_ERR_UNSUPPORTED, "render: instances mix FP32 and packed grids");
   launch_sm(ctx, prec, blocks, (uint64_t)n, 1, nullptr, false, scene.n_inst == 1, scene.n_surf > 0, true, d_pix, d_smp,
             d_out, d_cnt);
   R2_HIP(ctx, hipGetLastError());

@@ -49,7 +49,9 @@ struct SrvReq {
   std::vector<r2_instance> inst;
   std::vector<r2_surface> surfs;
   float* out = nullptr;
-  r2::DScene scene{};
+  r2host::HostScene hs;             // the request's scene, instance table and hierarchy
+  const r2::DScene& scene = hs.s;
+  std::vector<unsigned char> scene_img;  // host image of the device copy (kept until the request ends)
   int chunk = 1, nchunks = 1;
   bool fp64 = false, single = true, surf = false, packed = false;
   uint64_t total_items = 0, next_item = 0;  // items in all, items handed to slices
@@ -229,10 +231,12 @@ void admit_locked(r2_server* s) {
     s->queue.erase(qi);
     r.d_mem = mem;
     r.d_scene = (r2::DScene*)mem;
-    r.d_partial = (char*)mem + align256(sizeof(r2::DScene));
+    r.d_partial = (char*)mem + align256(r.hs.device_bytes());
     const size_t part = (size_t)r.nchunks * r.scene.W * r.scene.H * 3 * (r.fp64 ? 8 : 4);
     r.d_frame = (float*)((char*)r.d_partial + align256(part));
-    if (hipMemcpyAsync(r.d_scene, &r.scene, sizeof(r2::DScene), hipMemcpyHostToDevice, s->shadow.stream) != hipSuccess) {
+    r.hs.pack(r.d_scene, r.scene_img);
+    if (hipMemcpyAsync(r.d_scene, r.scene_img.data(), r.scene_img.size(), hipMemcpyHostToDevice, s->shadow.stream) !=
+        hipSuccess) {
       finish_locked(s, r, RQ_FAILED, R2_ERR_HIP);
       continue;
     }
@@ -424,7 +428,7 @@ extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
     // scene assembly on the caller's thread (validates the request)
     r2_ctx probe;
     probe.device = s->ctx->device;
-    const int rc = r2host::build_scene(&probe, &r->params, &r->scene);
+    const int rc = r2host::build_scene(&probe, &r->params, &r->hs);
     if (rc != R2_OK) return rc;
   }
   r2host::render_chunking(r->scene, &r->chunk, &r->nchunks);
@@ -432,18 +436,11 @@ extern "C" int64_t r2_server_submit(r2_server* s, const r2_request* req) {
   // scenes with surfaces run the general-instance form of the batched kernel
   r->surf = r->scene.n_surf > 0;
   r->single = r->scene.n_inst == 1 && !r->surf;
-  {
-    int prec = -2;
-    for (int i = 0; i < r->scene.n_inst; ++i) {
-      const int q = r->scene.grid[r->scene.inst[i].grid].precision == 1 ? 1 : 0;
-      if (prec == -2) prec = q;
-      else if (prec != q) return R2_ERR_UNSUPPORTED;  // instances mix FP32 and packed grids
-    }
-    r->packed = prec == 1;
-  }
+  if (r->hs.precision < 0) return R2_ERR_UNSUPPORTED;  // instances mix FP32 and packed grids
+  r->packed = r->hs.precision == 1;
   const uint64_t npix = (uint64_t)r->scene.W * r->scene.H;
   r->total_items = (uint64_t)r->nchunks * npix;
-  r->bytes = align256(sizeof(r2::DScene)) + align256((size_t)r->nchunks * npix * 3 * (r->fp64 ? 8 : 4)) +
+  r->bytes = align256(r->hs.device_bytes()) + align256((size_t)r->nchunks * npix * 3 * (r->fp64 ? 8 : 4)) +
              align256((size_t)npix * 3 * 4);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stopping) return R2_ERR_CANCELLED;

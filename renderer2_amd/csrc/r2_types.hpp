@@ -4,7 +4,13 @@
 
 namespace r2 {
 
-constexpr int kMaxInstances = 4;
+// A scene holds up to kMaxGrids distinct grids and any number (up to
+// kMaxInstances) of instances of them.  Up to kLinearInstances instances are
+// tested by a linear scan; more go through a bounding-volume hierarchy over the
+// instances' world boxes, which gives the same closest hit.
+constexpr int kMaxGrids = 16;
+constexpr int kMaxInstances = 1 << 20;
+constexpr int kLinearInstances = 8;
 constexpr int kLeafDim = 8;            // NanoVDB leaf node: 8^3 voxels
 constexpr int kLeafVoxels = 512;
 constexpr float kDefaultTmax = 1e12f;  // CRay::DEFAULT_TMAX (ray.hpp:12)
@@ -82,10 +88,24 @@ struct DSurface {
   float alpha_x, alpha_y, eta_i, eta_t;
 };
 
+// Node of the instance hierarchy, in depth-first order: the box of its
+// subtree, the index of the node after the subtree, and for a leaf
+// (first << 3 | count) into DScene::bvh_idx (count 1..4); -1 for an inner node.
+struct DBvhNode {
+  float lo[3];
+  int32_t skip;
+  float hi[3];
+  int32_t leaf;
+};
+
 struct DScene {
   int32_t n_inst;
-  DInstance inst[kMaxInstances];
-  DGrid grid[kMaxInstances];
+  const DInstance* inst;     // n_inst instances (device memory, after the scene)
+  int32_t n_bvh;             // 0: linear scan
+  const DBvhNode* bvh;
+  const int32_t* bvh_idx;
+  int32_t n_grid;
+  DGrid grid[kMaxGrids];
   int32_t n_surf;
   DSurface surf[kMaxSurfaces];
   DEnv env;

@@ -703,10 +703,75 @@ struct Hit {
   int surf;   // surface index, -1 for a box or a miss
   v3 normal;  // a surface's normalised normal
 };
-// __intersection__volume (shaders.optix.cu:192-236) with nanovdb::Ray::intersects
-// (Williams et al. 2005 slab test) and BBox::isInside, then (kSurf) the analytic
-// surfaces through __intersection__surface (:170-189); closest accepted hit wins.
-template <bool kSurf = false>
+// instance k of the scene (the table follows the scene in device memory)
+__device__ __forceinline__ const DInstance& inst_at(const DScene& s, int k) { return kconst_at(s.inst, k); }
+
+// The slab test of nanovdb::Ray::intersects (Williams et al. 2005) over the
+// world box [bmin, bmax] in the reference's order of comparisons, and
+// BBox::isInside: the t that __intersection__volume reports (the exit when the
+// origin is inside, else the entry), or false.
+__device__ __forceinline__ bool instance_box_t(const float* bmin, const float* bmax, const float o[3],
+                                               const float inv[3], int sg0, int sg1, int sg2, float& t_out) {
+  const float span1 = kDefaultTmax;
+  float t0 = ((sg0 ? bmax : bmin)[0] - o[0]) * inv[0];
+  float t2 = ((sg1 ? bmin : bmax)[1] - o[1]) * inv[1];
+  if (t0 > t2) return false;
+  float t1 = ((sg0 ? bmin : bmax)[0] - o[0]) * inv[0];
+  float t3 = ((sg1 ? bmax : bmin)[1] - o[1]) * inv[1];
+  if (t3 > t1) return false;
+  if (t3 > t0) t0 = t3;
+  if (t2 < t1) t1 = t2;
+  t3 = ((sg2 ? bmax : bmin)[2] - o[2]) * inv[2];
+  if (t3 > t1) return false;
+  t2 = ((sg2 ? bmin : bmax)[2] - o[2]) * inv[2];
+  if (t0 > t2) return false;
+  if (t3 > t0) t0 = t3;
+  if (span1 < t0) return false;
+  if (t2 < t1) t1 = t2;
+  if (0.f > t1) return false;
+  if (0.f > t0) t0 = 0.f;
+  if (span1 < t1) t1 = span1;
+  if (!(t1 < kDefaultTmax)) return false;
+  const bool inside =
+      o[0] > bmin[0] && o[1] > bmin[1] && o[2] > bmin[2] && o[0] < bmax[0] && o[1] < bmax[1] && o[2] < bmax[2];
+  t_out = inside ? t1 : t0;
+  return true;
+}
+// Culling test of a hierarchy node: the same rejections as instance_box_t on
+// the node's box, which contains its instances' boxes.  Each rejection compares
+// an entry against an exit, and a contained box has a later (or equal) entry
+// and an earlier (or equal) exit under the same rounded arithmetic, so a node
+// is rejected only when all its instances are (comparisons with NaN never
+// reject).  The node is also skipped when its entry lies beyond `cur`: every
+// instance inside then reports a larger t.
+__device__ __forceinline__ bool node_may_hit(const DBvhNode& nd, const float o[3], const float inv[3], int sg0,
+                                             int sg1, int sg2, float cur) {
+  const float* bmin = nd.lo;
+  const float* bmax = nd.hi;
+  float t0 = ((sg0 ? bmax : bmin)[0] - o[0]) * inv[0];
+  float t2 = ((sg1 ? bmin : bmax)[1] - o[1]) * inv[1];
+  if (t0 > t2) return false;
+  float t1 = ((sg0 ? bmin : bmax)[0] - o[0]) * inv[0];
+  float t3 = ((sg1 ? bmax : bmin)[1] - o[1]) * inv[1];
+  if (t3 > t1) return false;
+  if (t3 > t0) t0 = t3;
+  if (t2 < t1) t1 = t2;
+  t3 = ((sg2 ? bmax : bmin)[2] - o[2]) * inv[2];
+  if (t3 > t1) return false;
+  t2 = ((sg2 ? bmin : bmax)[2] - o[2]) * inv[2];
+  if (t0 > t2) return false;
+  if (t3 > t0) t0 = t3;
+  if (kDefaultTmax < t0) return false;
+  if (t2 < t1) t1 = t2;
+  if (0.f > t1) return false;
+  return !(t0 > cur);
+}
+
+// __intersection__volume (shaders.optix.cu:192-236) over the instance boxes, then
+// (kSurf) the analytic surfaces through __intersection__surface (:170-189);
+// closest accepted hit wins, the lowest instance index among equal t (the order
+// of a linear scan, which the hierarchy walk reproduces).
+template <bool kSurf = false, bool kSingle = false>
 __device__ __forceinline__ Hit scene_intersect(const DScene& s, Ray& ray) {
   Hit h;
   h.hit = 0; h.t = 0.f; h.medium = -1; h.pos = mk(0.f, 0.f, 0.f);
@@ -715,36 +780,37 @@ __device__ __forceinline__ Hit scene_intersect(const DScene& s, Ray& ray) {
   const float o[3] = {ray.o.x, ray.o.y, ray.o.z};
   const float inv[3] = {1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z};
   const int sg0 = inv[0] < 0.f, sg1 = inv[1] < 0.f, sg2 = inv[2] < 0.f;
-  for (int k = 0; k < s.n_inst; ++k) {
-    const DInstance& in = s.inst[k];
-    const float* bmin = in.bb_min;
-    const float* bmax = in.bb_max;
-    const float span1 = kDefaultTmax;
-    float t0 = ((sg0 ? bmax : bmin)[0] - o[0]) * inv[0];
-    float t2 = ((sg1 ? bmin : bmax)[1] - o[1]) * inv[1];
-    if (t0 > t2) continue;
-    float t1 = ((sg0 ? bmin : bmax)[0] - o[0]) * inv[0];
-    float t3 = ((sg1 ? bmax : bmin)[1] - o[1]) * inv[1];
-    if (t3 > t1) continue;
-    if (t3 > t0) t0 = t3;
-    if (t2 < t1) t1 = t2;
-    t3 = ((sg2 ? bmax : bmin)[2] - o[2]) * inv[2];
-    if (t3 > t1) continue;
-    t2 = ((sg2 ? bmin : bmax)[2] - o[2]) * inv[2];
-    if (t0 > t2) continue;
-    if (t3 > t0) t0 = t3;
-    if (span1 < t0) continue;
-    if (t2 < t1) t1 = t2;
-    if (0.f > t1) continue;
-    if (0.f > t0) t0 = 0.f;
-    if (span1 < t1) t1 = span1;
-    if (!(t1 < kDefaultTmax)) continue;
-    const bool inside = o[0] > bmin[0] && o[1] > bmin[1] && o[2] > bmin[2] && o[0] < bmax[0] && o[1] < bmax[1] &&
-                        o[2] < bmax[2];
-    const float t = inside ? t1 : t0;
-    if (!(t >= 0.f && t <= best)) continue;
-    if (h.hit && !(t < h.t)) continue;
-    h.hit = 1; h.t = t; h.medium = k;
+  if (kSingle || s.n_bvh == 0) {
+    const int n = kSingle ? 1 : s.n_inst;
+    for (int k = 0; k < n; ++k) {
+      const DInstance& in = inst_at(s, k);
+      float t;
+      if (!instance_box_t(in.bb_min, in.bb_max, o, inv, sg0, sg1, sg2, t)) continue;
+      if (!(t >= 0.f && t <= best)) continue;
+      if (h.hit && !(t < h.t)) continue;
+      h.hit = 1; h.t = t; h.medium = k;
+    }
+  } else {
+    for (int i = 0; i < s.n_bvh;) {
+      const DBvhNode nd = kconst_at(s.bvh, i);
+      if (!node_may_hit(nd, o, inv, sg0, sg1, sg2, h.hit ? h.t : best)) {
+        i = nd.skip;
+        continue;
+      }
+      if (nd.leaf >= 0) {
+        const int first = nd.leaf >> 3, cnt = nd.leaf & 7;
+        for (int j = 0; j < cnt; ++j) {
+          const int k = kconst_at(s.bvh_idx, first + j);
+          const DInstance& in = inst_at(s, k);
+          float t;
+          if (!instance_box_t(in.bb_min, in.bb_max, o, inv, sg0, sg1, sg2, t)) continue;
+          if (!(t >= 0.f && t <= best)) continue;
+          if (h.hit && !(t < h.t || (t == h.t && k < h.medium))) continue;
+          h.hit = 1; h.t = t; h.medium = k;
+        }
+      }
+      ++i;
+    }
   }
   if constexpr (kSurf) {
     for (int j = 0; j < s.n_surf; ++j) {
@@ -764,7 +830,7 @@ __device__ __forceinline__ Hit scene_intersect(const DScene& s, Ray& ray) {
 
 // CMediumInstance::moveToVoxelBorder (medium_instance_impl.hpp:38-41)
 __device__ __forceinline__ Ray inst_move_to_voxel_border(const DScene& s, int k, const Ray& ray) {
-  const DInstance& in = s.inst[k];
+  const DInstance& in = inst_at(s, k);
   const Ray rm = ray_transform(ray, in.w2m);
   const Ray moved = medium_move_to_voxel_border(s.grid[in.grid], rm);
   Ray r = ray_transform(moved, in.m2w);

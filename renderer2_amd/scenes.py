@@ -157,6 +157,37 @@ def mixed_workload(name: str = "c1", ground_y: float = -0.4, seed: int = 1, **ov
     return w
 
 
+def forest_workload(n_trees: int = 256, seed: int = 3, ground: bool = True, width: int = 48, height: int = 48,
+                    spp: int = 4, max_depth: int = 8, extent: float = 4.0) -> dict:
+    """Many instances of two small grids (a 32^3 fibre volume and a 16^3 value-noise
+    volume) on a jittered lattice over [-extent, extent]^2, standing on the forest
+    ground: random scales and axis-aligned orientations (getRotation keeps them
+    exact), a few neighbouring boxes overlapping, seen from above the canopy under
+    the sky + sun map.  (A path whose box meets another box ends, as in the
+    reference's Li, so overlaps are kept rare.)"""
+    rng = np.random.default_rng(seed)
+    fib = make_workload("c2", n=32, width=width, height=height, spp=spp, max_depth=max_depth)
+    noise = make_workload("c1", n=16)
+    grids = [fib["grids"][0], noise["grids"][0]]
+    side = int(np.ceil(np.sqrt(n_trees)))
+    cell = 2.0 * extent / side
+    orients = [(0, 1, 0), (1, 0, 0), (0, 0, 1), (-1, 0, 0), (0, 0, -1)]
+    instances = []
+    for k in range(n_trees):
+        i, j = k % side, k // side
+        sc = float(rng.uniform(0.45, 0.8)) * cell
+        sy = sc * float(rng.uniform(0.8, 1.6))
+        x = -extent + (i + 0.5) * cell + float(rng.uniform(-0.12, 0.12)) * cell
+        z = -extent + (j + 0.5) * cell + float(rng.uniform(-0.12, 0.12)) * cell
+        instances.append(dict(grid=int(rng.integers(0, 2)), pos=(x, 0.5 * sy, z),
+                              orientation=orients[int(rng.integers(0, len(orients)))], scaling=(sc, sy, sc)))
+    render = dict(fib["render"], cam_pos=(0.0, 2.2, extent + 2.5), cam_look_at=(0.0, 0.2, 0.0), fov_deg=55.0)
+    if ground:
+        render["surfaces"] = [dict(GROUND)]
+    return dict(name=f"forest{n_trees}", grids=grids, instances=instances, env=synth_env("sky", 64, 32),
+                render=render)
+
+
 def filtered_grid(fgrid, voxel_size: float, albedo: float = 0.9, records=None, records_dev: int | None = None,
                   precision: str = "fp32") -> dict:
     """Grid description of one filtered LOD (r2_filter_grid): the index -> model map

@@ -162,9 +162,14 @@ def test_argument_errors(ctx, c1):
     with pytest.raises(r2.R2Error) as e:
         ctx.render(c1["render"], [grids[0], packed], two, env)
     assert e.value.code == -4  # R2_ERR_UNSUPPORTED: one record precision per frame
+    # any number of instances, of at most 16 distinct grids
+    five = ctx.render(dict(c1["render"], spp=1), grids * 5, [dict(grid=k, pos=(0.2 * k, 0, 0)) for k in range(5)], env)
+    assert np.isfinite(five).all()
+    g16 = scenes.make_workload("c1", n=16)["grids"][0]
+    many = [ctx.upload_grid(g16) for _ in range(17)]
     with pytest.raises(r2.R2Error) as e:
-        ctx.render(c1["render"], grids * 5, [dict(grid=k) for k in range(5)], env)
-    assert e.value.code == -4  # at most 4 medium instances
+        ctx.render(dict(c1["render"], spp=1), many, [dict(grid=k) for k in range(17)], env)
+    assert e.value.code == -4  # R2_ERR_UNSUPPORTED: at most 16 distinct grids
 
 
 def test_progressive_render_matches_and_stops(ctx):

@@ -148,6 +148,13 @@ def test_server_deadlines(ctx, world):
             srv.submit(dict(cfg, spp=8), g, inst, env, deadline_ms=-1.0)
         assert e.value.code == -1  # R2_ERR_INVALID
         a = srv.submit(dict(cfg, spp=256), g, inst, env)  # resident first, no deadline
+        # b and c must queue behind a: wait until the scheduler has admitted a
+        # (else b, having a deadline, would be admitted before a and ahead of c)
+        import time
+        t_end = time.monotonic() + 10.0
+        while srv.stats()["peak_bytes"] == 0 and time.monotonic() < t_end:
+            time.sleep(1e-4)
+        assert srv.stats()["peak_bytes"] > 0
         b = srv.submit(cb, g, inst, env, deadline_ms=60000.0)
         c = srv.submit(cc, g, inst, env, deadline_ms=30000.0)  # earlier deadline: admitted before b
         d = srv.submit(dict(cfg, spp=8), g, inst, env, deadline_ms=1e-3)  # expires while queued
