@@ -74,6 +74,16 @@ typedef struct {
   float sigma_a[3], sigma_s[3];
   float voxel_size_filtering;  /* filter voxel size of this LOD (nvdb_medium.cu:180-188) */
   int32_t precision;           /* R2_PREC_* */
+  /* Sparse form (n_leaves > 0; since version 3), as a NanoVDB tree holds a medium: records
+   * then holds n_leaves leaves of 8^3 records each (voxel x + 8 y + 64 z within the leaf, host
+   * or device memory per records_on_device), leaf_origins their index-space origins (3 ints per
+   * leaf, host memory: multiples of 8 inside [0, nx) x [0, ny) x [0, nz), no two alike) and
+   * leaf_masks their active voxels (64 bytes per leaf, bit v % 8 of byte v / 8; host memory),
+   * or NULL for the dense rule (density > 0 or a corner of the index box).  nx, ny, nz are
+   * the index extent; voxels outside every leaf are background.  n_leaves = 0: dense records. */
+  int64_t n_leaves;
+  const int32_t* leaf_origins;
+  const uint8_t* leaf_masks;
 } r2_grid_desc;
 
 typedef struct {
@@ -422,6 +432,16 @@ int r2_nvdb_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const vo
                   const float* map_vec, char* err, int32_t err_len);
 int r2_nvdb_read(const char* path, r2_nvdb_info* info, void* records, int64_t records_bytes, char* err,
                  int32_t err_len);
+/* The same file as sparse leaves, for the sparse form of r2_grid_desc (the grid load of
+ * CNVDBMedium without densifying the tree, raytracer/src/medium/nvdb_medium.cu:130-173):
+ * the file's leaves with their value masks, and the active tiles of its internal nodes as
+ * full leaves, in an index space shifted by origin_shift (the index box minimum rounded down
+ * to a multiple of 8, so origins are non-negative multiples of 8; fold map_mat * shift into
+ * map_vec).  In: *n_leaves = capacity of the buffers (leaf_origins 3 ints, records 512 x 32 B,
+ * leaf_masks 64 B per leaf, voxels x fastest).  Out: *n_leaves = the leaves of the file; with
+ * records == NULL only that count (and info, origin_shift) is returned.  Host only. */
+int r2_nvdb_read_leaves(const char* path, r2_nvdb_info* info, int32_t* origin_shift, int64_t* n_leaves,
+                        int32_t* leaf_origins, void* records, uint8_t* leaf_masks, char* err, int32_t err_len);
 
 /* Parse a config (and its scene description) without a device: the fields of SConfig
  * (raytracer/include/backend/config_loader.hpp:9-33) that drive this library. */

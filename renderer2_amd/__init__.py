@@ -76,6 +76,30 @@ def nvdb_read(path, with_records: bool = True):
     return d, rec
 
 
+def nvdb_read_leaves(path):
+    """Sparse read of a NanoVDB file: (info dict with origin_shift, leaf_origins (n, 3) int32,
+    records (n, 512, 32) u8 x fastest, leaf_masks (n, 64) u8).  Host only."""
+    info = capi.NvdbInfo()
+    err = C.create_string_buffer(512)
+    shift = (C.c_int32 * 3)()
+    n = C.c_int64(0)
+    rc = capi.lib().r2_nvdb_read_leaves(str(path).encode(), C.byref(info), shift, C.byref(n), None, None, None, err,
+                                        512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
+    org = np.empty((n.value, 3), np.int32)
+    rec = np.empty((n.value, 512, 32), np.uint8)
+    msk = np.empty((n.value, 64), np.uint8)
+    rc = capi.lib().r2_nvdb_read_leaves(str(path).encode(), C.byref(info), shift, C.byref(n), org.ctypes.data,
+                                        rec.ctypes.data, msk.ctypes.data, err, 512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
+    d = {k: (np.array(getattr(info, k)) if not isinstance(getattr(info, k), (int, float)) else getattr(info, k))
+         for k, _ in info._fields_}
+    d["origin_shift"] = np.array(shift[:], np.int32)
+    return d, org, rec, msk
+
+
 class Context:
     """Owns an r2_ctx (one device, one HIP stream)."""
 
@@ -100,16 +124,30 @@ class Context:
 
     # ---------------------------------------------------------------- grid
     def upload_grid(self, g: dict) -> "Grid":
-        """g: records (host array) or records_dev (device address), shape (nz, ny, nx), map, optics."""
+        """g: records (host array) or records_dev (device address), shape (nz, ny, nx), map, optics.
+        Sparse form: leaf_origins (n, 3) int32 (x, y, z, multiples of 8) with records (n, 512, 32)
+        u8 and optionally leaf_masks (n, 64) u8 of active voxels; shape is then the index extent."""
         d = capi.GridDesc()
         d.nz, d.ny, d.nx = g["shape"]
+        keep = []
         if g.get("records_dev"):
             d.records = g["records_dev"]
             d.records_on_device = 1
         else:
             rec = np.ascontiguousarray(g["records"])
+            keep.append(rec)
             d.records = rec.ctypes.data
             d.records_on_device = 0
+        if g.get("leaf_origins") is not None:
+            org = np.ascontiguousarray(g["leaf_origins"], np.int32).reshape(-1, 3)
+            keep.append(org)
+            d.n_leaves = len(org)
+            d.leaf_origins = org.ctypes.data
+            if g.get("leaf_masks") is not None:
+                msk = np.ascontiguousarray(g["leaf_masks"], np.uint8).reshape(-1, 64)
+                assert len(msk) == len(org)
+                keep.append(msk)
+                d.leaf_masks = msk.ctypes.data
         d.map_mat[:] = list(g["map_mat"])
         d.map_vec[:] = list(g["map_vec"])
         d.sigma_a[:] = list(g["sigma_a"])
@@ -339,15 +377,17 @@ class Grid:
         if rc != 0:
             raise capi.R2Error(rc, err.value.decode(errors="replace"))
 
-    def download_bricks(self):
+    def download_bricks(self, with_atlas: bool = True):
+        """(level-0 ranges, the atlas as dense u8 densities (None without with_atlas), mips)."""
         inf = self.info()
         nbx, nby, nbz = [int(x) for x in inf["n_bricks"]]
         rng = np.zeros(nbx * nby * nbz, np.uint32)
-        u8 = np.zeros(nbx * nby * nbz * 512, np.uint8)
+        u8 = np.zeros(nbx * nby * nbz * 512, np.uint8) if with_atlas else None
         nm = sum((nbx >> (l + 1)) * (nby >> (l + 1)) * (nbz >> (l + 1)) for l in range(3))
         mips = np.zeros(nm, np.uint32)
-        capi.check(capi.lib().r2_grid_download_bricks(self.ctx.h, self.h, rng.ctypes.data, u8.ctypes.data,
-                                                      mips.ctypes.data), self.ctx.h)
+        capi.check(capi.lib().r2_grid_download_bricks(self.ctx.h, self.h, rng.ctypes.data,
+                                                      u8.ctypes.data if with_atlas else None, mips.ctypes.data),
+                   self.ctx.h)
         out_m, off = [], 0
         for l in range(3):
             c = (nbx >> (l + 1)) * (nby >> (l + 1)) * (nbz >> (l + 1))

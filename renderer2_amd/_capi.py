@@ -47,6 +47,7 @@ class GridDesc(C.Structure):
         ("sigma_a", f3), ("sigma_s", f3),
         ("voxel_size_filtering", C.c_float),
         ("precision", C.c_int32),
+        ("n_leaves", C.c_int64), ("leaf_origins", C.c_void_p), ("leaf_masks", C.c_void_p),
     ]
 
 
@@ -202,6 +203,8 @@ SIGNATURES = [
     ("r2_nvdb_write", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_char_p, C.c_int32]),
     ("r2_nvdb_read", C.c_int, [C.c_char_p, C.POINTER(NvdbInfo), C.c_void_p, C.c_int64, C.c_char_p, C.c_int32]),
+    ("r2_nvdb_read_leaves", C.c_int, [C.c_char_p, C.POINTER(NvdbInfo), C.c_void_p, C.POINTER(C.c_int64), C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
     ("r2_dda_cells", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                                C.c_void_p]),
     ("r2_env_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

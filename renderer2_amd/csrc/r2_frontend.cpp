@@ -781,25 +781,32 @@ void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeli
       // same path (CNVDBMedium loads "<Path>.nvdb", nvdb_medium.cu:130-173)
       const fs::path own = resolve(c, o.path + ".r2grid"), nano = resolve(c, o.path + ".nvdb");
       GridFile gf;
+      std::vector<int32_t> leaf_org;
+      std::vector<uint8_t> leaf_mask;
+      int64_t n_leaves = 0;
       if (fs::exists(own) || !fs::exists(nano)) {
         gf = read_grid(own);
       } else {
+        // the tree's leaves as they are (no densified copy): the sparse form of the grid
         r2_nvdb_info ni{};
-        r2host::nvdb_read(nano.string(), &ni, nullptr, 0);
-        if (ni.nx > 8192 || ni.ny > 8192 || ni.nz > 8192)
+        int32_t shift[3];
+        n_leaves = r2host::nvdb_read_leaves(nano.string(), &ni, shift, 0, nullptr, nullptr, nullptr);
+        for (int a = 0; a < 3; ++a)
+          gf.n[a] = (a == 0 ? ni.nx : a == 1 ? ni.ny : ni.nz) + ni.ibb_min[a] - shift[a];
+        if (gf.n[0] > 8192 || gf.n[1] > 8192 || gf.n[2] > 8192)
           throw std::runtime_error("grid file larger than 8192 voxels per axis: " + nano.string());
-        const size_t bytes = (size_t)ni.nx * ni.ny * ni.nz * 32;
-        gf.rec.resize(bytes);
-        r2host::nvdb_read(nano.string(), &ni, gf.rec.data(), (int64_t)bytes);
-        gf.n[0] = ni.nx, gf.n[1] = ni.ny, gf.n[2] = ni.nz;
-        // an index box starting at ibb_min != 0: fold the offset into the translation
-        const bool shifted = ni.ibb_min[0] != 0 || ni.ibb_min[1] != 0 || ni.ibb_min[2] != 0;
+        leaf_org.resize((size_t)n_leaves * 3);
+        leaf_mask.resize((size_t)n_leaves * 64);
+        gf.rec.resize((size_t)n_leaves * 512 * 32);
+        r2host::nvdb_read_leaves(nano.string(), &ni, shift, n_leaves, leaf_org.data(), gf.rec.data(), leaf_mask.data());
+        // an index space shifted to a multiple of 8 below the index box: fold the shift into the translation
+        const bool shifted = shift[0] != 0 || shift[1] != 0 || shift[2] != 0;
         for (int r = 0; r < 3; ++r) {
           for (int col = 0; col < 3; ++col) gf.i2m[col * 3 + r] = ni.map_mat[r * 3 + col];
           gf.i2m[9 + r] = !shifted ? ni.map_vec[r]
-                                   : ni.map_vec[r] + ni.map_mat[r * 3] * (float)ni.ibb_min[0] +
-                                         ni.map_mat[r * 3 + 1] * (float)ni.ibb_min[1] +
-                                         ni.map_mat[r * 3 + 2] * (float)ni.ibb_min[2];
+                                   : ni.map_vec[r] + ni.map_mat[r * 3] * (float)shift[0] +
+                                         ni.map_mat[r * 3 + 1] * (float)shift[1] +
+                                         ni.map_mat[r * 3 + 2] * (float)shift[2];
         }
       }
       r2_grid_desc d{};
@@ -808,6 +815,9 @@ void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeli
       d.nz = gf.n[2];
       d.records = gf.rec.data();
       d.records_on_device = 0;
+      d.n_leaves = n_leaves;
+      d.leaf_origins = n_leaves ? leaf_org.data() : nullptr;
+      d.leaf_masks = n_leaves ? leaf_mask.data() : nullptr;
       for (int r = 0; r < 3; ++r) {
         for (int col = 0; col < 3; ++col) d.map_mat[r * 3 + col] = gf.i2m[col * 3 + r];
         d.map_vec[r] = gf.i2m[9 + r];

@@ -26,38 +26,12 @@ __device__ __forceinline__ uint32_t float_key(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-// max_bits[0]: max active density (float bits); max_bits[1]: order key of the
-// minimum over all voxels, inactive ones reading as 0 (the brick grid's minorant)
-__global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, float* __restrict__ dens,
-                         uint8_t* __restrict__ active, uint32_t* __restrict__ leaf_flag, int nlx, int nly,
-                         int* __restrict__ bbox, uint32_t* __restrict__ max_bits) {
-  const size_t n = (size_t)nx * ny * nz;
-  const bool small = n <= 0xFFFFFFFFull;  // 32-bit index arithmetic (the common case)
-  float lmax = 0.f, lmin = FLT_MAX;
-  int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    int x, y, z;
-    if (small) {
-      const uint32_t i32 = (uint32_t)i, r = i32 / (uint32_t)nx;
-      x = (int)(i32 - r * (uint32_t)nx);
-      y = (int)(r % (uint32_t)ny);
-      z = (int)(r / (uint32_t)ny);
-    } else {
-      x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
-    }
-    const float d = __uint_as_float(rec[2 * i].x);
-    const bool a = (d > 0.f) || is_corner(x, y, z, nx, ny, nz);
-    active[i] = a ? 1 : 0;
-    dens[i] = a ? d : 0.f;
-    lmin = fminf(lmin, a ? d : 0.f);
-    if (a) {
-      leaf_flag[((size_t)(z >> 3) * nly + (y >> 3)) * nlx + (x >> 3)] = 1u;
-      lmax = fmaxf(lmax, d);
-      lo[0] = min(lo[0], x); lo[1] = min(lo[1], y); lo[2] = min(lo[2], z);
-      hi[0] = max(hi[0], x); hi[1] = max(hi[1], y); hi[2] = max(hi[2], z);
-    }
-  }
-  // wave-level reduction, one atomic per wave
+// wave, then block reduction of a thread's extrema and active box, published
+// with one set of atomics per block (same-address atomics from every wavefront
+// serialise in L2): max_bits[0] = max active density (float bits), max_bits[1] =
+// order key of the minimum, bbox = active index box
+__device__ __forceinline__ void publish_extrema(float lmax, float lmin, int lo[3], int hi[3], int* __restrict__ bbox,
+                                                uint32_t* __restrict__ max_bits) {
   for (int off = 32; off > 0; off >>= 1) {
     lmax = fmaxf(lmax, __shfl_xor(lmax, off));
     lmin = fminf(lmin, __shfl_xor(lmin, off));
@@ -66,8 +40,6 @@ __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, 
       hi[a] = max(hi[a], __shfl_xor(hi[a], off));
     }
   }
-  // then across the block's wavefronts in LDS, and one set of atomics per block
-  // (same-address atomics from every wavefront serialise in L2)
   __shared__ float s_max[4], s_min[4];
   __shared__ int s_lo[4][3], s_hi[4][3];
   const int wave = threadIdx.x >> 6;
@@ -91,6 +63,75 @@ __global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, 
       atomicMax(&bbox[3 + a], hi[a]);
     }
   }
+}
+
+// dense records: the active rule (density > 0, or a corner of the index box),
+// leaf occupancy, and the extrema and active box over all voxels (inactive ones
+// reading 0 for the minimum, the brick grid's minorant)
+__global__ void k_active(const uint4* __restrict__ rec, int nx, int ny, int nz, uint8_t* __restrict__ active,
+                         uint32_t* __restrict__ leaf_flag, int nlx, int nly, int* __restrict__ bbox,
+                         uint32_t* __restrict__ max_bits) {
+  const size_t n = (size_t)nx * ny * nz;
+  const bool small = n <= 0xFFFFFFFFull;  // 32-bit index arithmetic (the common case)
+  float lmax = 0.f, lmin = FLT_MAX;
+  int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    int x, y, z;
+    if (small) {
+      const uint32_t i32 = (uint32_t)i, r = i32 / (uint32_t)nx;
+      x = (int)(i32 - r * (uint32_t)nx);
+      y = (int)(r % (uint32_t)ny);
+      z = (int)(r / (uint32_t)ny);
+    } else {
+      x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    }
+    const float d = __uint_as_float(rec[2 * i].x);
+    const bool a = (d > 0.f) || is_corner(x, y, z, nx, ny, nz);
+    active[i] = a ? 1 : 0;
+    lmin = fminf(lmin, a ? d : 0.f);
+    if (a) {
+      leaf_flag[((size_t)(z >> 3) * nly + (y >> 3)) * nlx + (x >> 3)] = 1u;
+      lmax = fmaxf(lmax, d);
+      lo[0] = min(lo[0], x); lo[1] = min(lo[1], y); lo[2] = min(lo[2], z);
+      hi[0] = max(hi[0], x); hi[1] = max(hi[1], y); hi[2] = max(hi[2], z);
+    }
+  }
+  publish_extrema(lmax, lmin, lo, hi, bbox, max_bits);
+}
+
+// sparse records: leaf k of the caller's list becomes pool slot k; a voxel is
+// active by its mask bit (or, without masks, by the dense rule), inactive voxels
+// and voxels beyond the index extent hold the background record; extrema and
+// active box over the voxels of the leaves
+__global__ void k_sparse_leaves(const uint4* __restrict__ rec, const uint8_t* __restrict__ masks,
+                                const int32_t* __restrict__ origins, int64_t n_leaves, int nx, int ny, int nz,
+                                uint4* __restrict__ leaves, int* __restrict__ bbox, uint32_t* __restrict__ max_bits) {
+  float lmax = 0.f, lmin = FLT_MAX;
+  int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  for (int64_t li = blockIdx.x; li < n_leaves; li += gridDim.x) {
+    const int ox = origins[3 * li], oy = origins[3 * li + 1], oz = origins[3 * li + 2];
+    for (int v = threadIdx.x; v < kLeafVoxels; v += blockDim.x) {
+      const int x = ox + (v & 7), y = oy + ((v >> 3) & 7), z = oz + (v >> 6);
+      const size_t i = (size_t)li * kLeafVoxels + v;
+      uint4 a = rec[2 * i], b = rec[2 * i + 1];
+      const float d = __uint_as_float(a.x);
+      const bool inside = x < nx && y < ny && z < nz;
+      const bool act = inside && (masks ? ((masks[(size_t)li * 64 + (v >> 3)] >> (v & 7)) & 1u) != 0u
+                                        : (d > 0.f || is_corner(x, y, z, nx, ny, nz)));
+      if (inside) lmin = fminf(lmin, act ? d : 0.f);
+      if (act) {
+        lmax = fmaxf(lmax, d);
+        lo[0] = min(lo[0], x); lo[1] = min(lo[1], y); lo[2] = min(lo[2], z);
+        hi[0] = max(hi[0], x); hi[1] = max(hi[1], y); hi[2] = max(hi[2], z);
+      } else {
+        a = make_uint4(0u, 0u, 0u, 0u);
+        b = make_uint4(0u, 0u, 0u, 0x3C00u);
+      }
+      leaves[2 * i] = a;
+      leaves[2 * i + 1] = b;
+    }
+  }
+  publish_extrema(lmax, lmin, lo, hi, bbox, max_bits);
 }
 
 // scatter records into the leaf pool; inactive voxels of an allocated leaf hold the background record
@@ -182,23 +223,48 @@ __global__ void k_sample(DGrid g, int n, const float* __restrict__ pos, float* _
   o[16] = r.ior;
 }
 
-__device__ __forceinline__ float dens_at(const float* __restrict__ dens, int nx, int ny, int nz, int x, int y, int z) {
-  if ((unsigned)x >= (unsigned)nx || (unsigned)y >= (unsigned)ny || (unsigned)z >= (unsigned)nz) return 0.f;
-  return dens[((size_t)z * ny + y) * nx + x];
-}
+// density of a voxel as the brick grid sees it: the active record's density,
+// 0 for inactive voxels (they hold the background record) and outside the grid
+struct PoolDens {
+  const int32_t* table;
+  const uint4* leaves;
+  int nx, ny, nz, nlx, nly;
+  __device__ __forceinline__ float operator()(int x, int y, int z) const {
+    if ((unsigned)x >= (unsigned)nx || (unsigned)y >= (unsigned)ny || (unsigned)z >= (unsigned)nz) return 0.f;
+    const int32_t leaf = table[((size_t)(z >> 3) * nly + (y >> 3)) * nlx + (x >> 3)];
+    if (leaf < 0) return 0.f;
+    return __uint_as_float(leaves[((size_t)leaf * kLeafVoxels + (((z & 7) << 6) | ((y & 7) << 3) | (x & 7))) * 2].x);
+  }
+};
 
 // one wavefront per brick: min/max over the dilated 10^3 neighbourhood
-__global__ void k_brick_range(const float* __restrict__ dens, int nx, int ny, int nz, int nbx, int nby, int nbz,
-                              uint32_t* __restrict__ range, uint32_t* __restrict__ nonempty) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+__global__ void k_brick_range(PoolDens dens, int nbx, int nby, int nbz, uint32_t* __restrict__ range,
+                              uint32_t* __restrict__ nonempty) {
+  const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const int nbt = nbx * nby * nbz;
+  const size_t nbt = (size_t)nbx * nby * nbz;
   if (wave >= nbt) return;
-  const int bx = wave % nbx, by = (wave / nbx) % nby, bz = wave / (nbx * nby);
+  const int bx = (int)(wave % nbx), by = (int)((wave / nbx) % nby), bz = (int)(wave / ((size_t)nbx * nby));
+  // a brick with no leaf in its 3x3x3 leaf neighbourhood reads density 0 at every
+  // voxel of its dilated range (the common case of a sparse grid)
+  bool leaf = false;
+  if (lane < 27) {
+    const int lx = bx + lane % 3 - 1, ly = by + (lane / 3) % 3 - 1, lz = bz + lane / 9 - 1;
+    const int nlx = (dens.nx + 7) >> 3, nly = (dens.ny + 7) >> 3, nlz = (dens.nz + 7) >> 3;
+    if ((unsigned)lx < (unsigned)nlx && (unsigned)ly < (unsigned)nly && (unsigned)lz < (unsigned)nlz)
+      leaf = dens.table[((size_t)lz * nly + ly) * nlx + lx] >= 0;
+  }
+  if (__ballot(leaf) == 0ull) {
+    if (lane == 0) {
+      range[wave] = encode_range(0.f, 0.f);
+      nonempty[wave] = 0u;
+    }
+    return;
+  }
   float lo = FLT_MAX, hi = -FLT_MAX;
   for (int k = lane; k < 1000; k += 64) {
     const int x = k % 10 - 1, y = (k / 10) % 10 - 1, z = k / 100 - 1;
-    const float d = dens_at(dens, nx, ny, nz, bx * 8 + x, by * 8 + y, bz * 8 + z);
+    const float d = dens(bx * 8 + x, by * 8 + y, bz * 8 + z);
     lo = fminf(lo, d);
     hi = fmaxf(hi, d);
   }
@@ -213,15 +279,14 @@ __global__ void k_brick_range(const float* __restrict__ dens, int nx, int ny, in
 }
 
 // one wavefront per brick: quantise the 512 voxels of every non-empty brick
-__global__ void k_brick_atlas(const float* __restrict__ dens, int nx, int ny, int nz, int nbx, int nby, int nbz,
-                              const uint32_t* __restrict__ range, const uint32_t* __restrict__ slot,
+__global__ void k_brick_atlas(PoolDens dens, int nbx, int nby, int nbz, const uint32_t* __restrict__ range, const uint32_t* __restrict__ slot,
                               const uint32_t* __restrict__ nonempty, uint2* __restrict__ bricks,
                               uint8_t* __restrict__ atlas, float* __restrict__ maj0) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const int nbt = nbx * nby * nbz;
+  const size_t nbt = (size_t)nbx * nby * nbz;
   if (wave >= nbt) return;
-  const int bx = wave % nbx, by = (wave / nbx) % nby, bz = wave / (nbx * nby);
+  const int bx = (int)(wave % nbx), by = (int)((wave / nbx) % nby), bz = (int)(wave / ((size_t)nbx * nby));
   const uint32_t r = range[wave];
   const bool ne = nonempty[wave] != 0u;
   if (lane == 0) {
@@ -233,7 +298,7 @@ __global__ void k_brick_atlas(const float* __restrict__ dens, int nx, int ny, in
   uint8_t* dst = atlas + (size_t)slot[wave] * 512;
   for (int v = lane; v < 512; v += 64) {
     const int x = v & 7, y = (v >> 3) & 7, z = v >> 6;
-    dst[v] = encode_voxel(dens_at(dens, nx, ny, nz, bx * 8 + x, by * 8 + y, bz * 8 + z), lo, hi);
+    dst[v] = encode_voxel(dens(bx * 8 + x, by * 8 + y, bz * 8 + z), lo, hi);
   }
 }
 
@@ -291,25 +356,13 @@ namespace r2host {
 int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   using namespace r2;
   const int nx = desc->nx, ny = desc->ny, nz = desc->nz;
-  const size_t nv = (size_t)nx * ny * nz;
+  const bool sparse = desc->n_leaves > 0;
   hipStream_t st = ctx->stream;
   int rc;
-  // records staged in device memory
-  const uint4* d_rec = nullptr;
-  void* staged = nullptr;
-  if (desc->records_on_device) {
-    d_rec = (const uint4*)desc->records;
-  } else {
-    R2_HIP(ctx, hipMalloc(&staged, nv * 32));
-    hipError_t e = hipMemcpyAsync(staged, desc->records, nv * 32, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) { (void)hipFree(staged); return hip_fail(ctx, e, "hipMemcpyAsync(records)"); }
-    d_rec = (const uint4*)staged;
-  }
   struct Tmp {
     std::vector<void*> p;
     ~Tmp() { for (void* q : p) (void)hipFree(q); }
   } tmp;
-  if (staged) tmp.p.push_back(staged);
   auto tmalloc = [&](void** q, size_t bytes) -> hipError_t {
     hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 1));
     if (e == hipSuccess) tmp.p.push_back(*q);
@@ -317,43 +370,96 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   };
   const int nlx = (nx + 7) / 8, nly = (ny + 7) / 8, nlz = (nz + 7) / 8;
   const size_t nleaf_total = (size_t)nlx * nly * nlz;
-  float* d_dens = nullptr;
-  uint8_t* d_active = nullptr;
-  uint32_t* d_leaf_flag = nullptr;
+  // records staged in device memory: nx*ny*nz dense records, or n_leaves leaves of 512
+  const size_t nrec = sparse ? (size_t)desc->n_leaves * kLeafVoxels : (size_t)nx * ny * nz;
+  const uint4* d_rec = nullptr;
+  if (desc->records_on_device) {
+    d_rec = (const uint4*)desc->records;
+  } else {
+    void* staged = nullptr;
+    R2_HIP(ctx, tmalloc(&staged, nrec * 32));
+    R2_HIP(ctx, hipMemcpyAsync(staged, desc->records, nrec * 32, hipMemcpyHostToDevice, st));
+    d_rec = (const uint4*)staged;
+  }
   int* d_bbox = nullptr;
   uint32_t* d_max = nullptr;
-  R2_HIP(ctx, tmalloc((void**)&d_dens, nv * 4));
-  R2_HIP(ctx, tmalloc((void**)&d_active, nv));
-  R2_HIP(ctx, tmalloc((void**)&d_leaf_flag, nleaf_total * 4));
   R2_HIP(ctx, tmalloc((void**)&d_bbox, 6 * 4));
   R2_HIP(ctx, tmalloc((void**)&d_max, 8));
   const int bbox_init[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
   R2_HIP(ctx, hipMemcpyAsync(d_bbox, bbox_init, sizeof(bbox_init), hipMemcpyHostToDevice, st));
   const uint32_t mm_init[2] = {0u, 0xFFFFFFFFu};
   R2_HIP(ctx, hipMemcpyAsync(d_max, mm_init, 8, hipMemcpyHostToDevice, st));
-  R2_HIP(ctx, hipMemsetAsync(d_leaf_flag, 0, nleaf_total * 4, st));
-  {
-    // grid-stride over the voxels: 8 blocks per CU are enough to stream the records
-    const int blocks = (int)std::min<size_t>((nv + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_active, dim3(blocks), dim3(256), 0, st, d_rec, nx, ny, nz, d_dens, d_active, d_leaf_flag, nlx,
-                       nly, d_bbox, d_max);
-    R2_HIP(ctx, hipGetLastError());
-  }
-  // leaf allocation: deterministic prefix order over the leaf table
-  std::vector<uint32_t> flags(nleaf_total);
+
+  std::vector<int32_t> table(nleaf_total, -1);
+  int64_t nleaves = 0;
+  const uint8_t* d_dense_active = nullptr;  // dense path: the active flags for the leaf scatter
+  uint4* d_leaves = nullptr;
+  bool covers_all = true;  // every voxel of the index box lies in a leaf (else some voxel reads 0)
   int bbox[6];
   uint32_t mm[2] = {0u, 0u};
-  R2_HIP(ctx, hipMemcpyAsync(flags.data(), d_leaf_flag, nleaf_total * 4, hipMemcpyDeviceToHost, st));
-  R2_HIP(ctx, hipMemcpyAsync(bbox, d_bbox, sizeof(bbox), hipMemcpyDeviceToHost, st));
-  R2_HIP(ctx, hipMemcpyAsync(mm, d_max, 8, hipMemcpyDeviceToHost, st));
-  R2_HIP(ctx, hipStreamSynchronize(st));
+  if (!sparse) {
+    // dense: the active rule per voxel, leaves allocated in table order
+    const size_t nv = nrec;
+    uint8_t* d_active = nullptr;
+    uint32_t* d_leaf_flag = nullptr;
+    R2_HIP(ctx, tmalloc((void**)&d_active, nv));
+    R2_HIP(ctx, tmalloc((void**)&d_leaf_flag, nleaf_total * 4));
+    R2_HIP(ctx, hipMemsetAsync(d_leaf_flag, 0, nleaf_total * 4, st));
+    // grid-stride over the voxels: 8 blocks per CU are enough to stream the records
+    const int blocks = (int)std::min<size_t>((nv + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_active, dim3(blocks), dim3(256), 0, st, d_rec, nx, ny, nz, d_active, d_leaf_flag, nlx, nly,
+                       d_bbox, d_max);
+    R2_HIP(ctx, hipGetLastError());
+    std::vector<uint32_t> flags(nleaf_total);
+    R2_HIP(ctx, hipMemcpyAsync(flags.data(), d_leaf_flag, nleaf_total * 4, hipMemcpyDeviceToHost, st));
+    R2_HIP(ctx, hipMemcpyAsync(bbox, d_bbox, sizeof(bbox), hipMemcpyDeviceToHost, st));
+    R2_HIP(ctx, hipMemcpyAsync(mm, d_max, 8, hipMemcpyDeviceToHost, st));
+    R2_HIP(ctx, hipStreamSynchronize(st));
+    for (size_t i = 0; i < nleaf_total; ++i) table[i] = flags[i] ? (int32_t)(nleaves++) : -1;
+    if (nleaves == 0) return fail(ctx, R2_ERR_INVALID, "grid has no active voxel");
+    if ((rc = dmalloc(ctx, g, &d_leaves, (size_t)nleaves * kLeafVoxels * 2))) return rc;
+    d_dense_active = d_active;
+  } else {
+    // sparse: the caller's leaves in their order
+    nleaves = desc->n_leaves;
+    if (!desc->leaf_origins) return fail(ctx, R2_ERR_INVALID, "grid: sparse records without leaf origins");
+    uint64_t covered = 0;
+    for (int64_t k = 0; k < nleaves; ++k) {
+      const int32_t* o = desc->leaf_origins + 3 * k;
+      if ((o[0] | o[1] | o[2]) & 7 || o[0] < 0 || o[1] < 0 || o[2] < 0 || o[0] >= nx || o[1] >= ny || o[2] >= nz)
+        return fail(ctx, R2_ERR_INVALID, "grid: leaf origin " + std::to_string(k) +
+                                             " is not a multiple of 8 inside the index extent");
+      int32_t& t = table[((size_t)(o[2] >> 3) * nly + (o[1] >> 3)) * nlx + (o[0] >> 3)];
+      if (t >= 0) return fail(ctx, R2_ERR_INVALID, "grid: two leaves at one origin");
+      t = (int32_t)k;
+      covered += (uint64_t)std::min(8, nx - o[0]) * std::min(8, ny - o[1]) * std::min(8, nz - o[2]);
+    }
+    covers_all = covered == (uint64_t)nx * ny * nz;
+    int32_t* d_orig = nullptr;
+    uint8_t* d_mask = nullptr;
+    R2_HIP(ctx, tmalloc((void**)&d_orig, (size_t)nleaves * 12));
+    R2_HIP(ctx, hipMemcpyAsync(d_orig, desc->leaf_origins, (size_t)nleaves * 12, hipMemcpyHostToDevice, st));
+    if (desc->leaf_masks) {
+      R2_HIP(ctx, tmalloc((void**)&d_mask, (size_t)nleaves * 64));
+      R2_HIP(ctx, hipMemcpyAsync(d_mask, desc->leaf_masks, (size_t)nleaves * 64, hipMemcpyHostToDevice, st));
+    }
+    if ((rc = dmalloc(ctx, g, &d_leaves, (size_t)nleaves * kLeafVoxels * 2))) return rc;
+    hipLaunchKernelGGL(k_sparse_leaves, dim3((unsigned)std::min<int64_t>(nleaves, 65536)), dim3(256), 0, st, d_rec,
+                       d_mask, d_orig, nleaves, nx, ny, nz, d_leaves, d_bbox, d_max);
+    R2_HIP(ctx, hipGetLastError());
+    R2_HIP(ctx, hipMemcpyAsync(bbox, d_bbox, sizeof(bbox), hipMemcpyDeviceToHost, st));
+    R2_HIP(ctx, hipMemcpyAsync(mm, d_max, 8, hipMemcpyDeviceToHost, st));
+    R2_HIP(ctx, hipStreamSynchronize(st));
+    if (bbox[0] > bbox[3]) return fail(ctx, R2_ERR_INVALID, "grid has no active voxel");
+  }
   // read the copied extrema only after the stream has finished the copies
   const uint32_t maxbits = mm[0];
-  const uint32_t minbits = (mm[1] & 0x80000000u) ? (mm[1] & 0x7FFFFFFFu) : ~mm[1];
-  std::vector<int32_t> table(nleaf_total);
-  int64_t nleaves = 0;
-  for (size_t i = 0; i < nleaf_total; ++i) table[i] = flags[i] ? (int32_t)(nleaves++) : -1;
-  if (nleaves == 0) return fail(ctx, R2_ERR_INVALID, "grid has no active voxel");
+  uint32_t minbits = (mm[1] & 0x80000000u) ? (mm[1] & 0x7FFFFFFFu) : ~mm[1];
+  if (!covers_all) {
+    float mn;
+    memcpy(&mn, &minbits, 4);
+    if (!(mn <= 0.f)) minbits = 0u;  // the voxels outside every leaf read 0
+  }
 
   DGrid& d = g->dev;
   d.nx = nx; d.ny = ny; d.nz = nz;
@@ -362,12 +468,12 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   int32_t* d_table = nullptr;
   if ((rc = dmalloc(ctx, g, &d_table, nleaf_total))) return rc;
   R2_HIP(ctx, hipMemcpyAsync(d_table, table.data(), nleaf_total * 4, hipMemcpyHostToDevice, st));
+  if (!sparse) {
+    hipLaunchKernelGGL(k_leaves, dim3((unsigned)std::min<size_t>(nleaf_total, 65536)), dim3(256), 0, st, d_rec,
+                       d_dense_active, nx, ny, nz, d_table, nlx, nly, nlz, d_leaves);
+    R2_HIP(ctx, hipGetLastError());
+  }
   d.leaf_table = d_table;
-  uint4* d_leaves = nullptr;
-  if ((rc = dmalloc(ctx, g, &d_leaves, (size_t)nleaves * kLeafVoxels * 2))) return rc;
-  hipLaunchKernelGGL(k_leaves, dim3((unsigned)std::min<size_t>(nleaf_total, 65536)), dim3(256), 0, st, d_rec, d_active,
-                     nx, ny, nz, d_table, nlx, nly, nlz, d_leaves);
-  R2_HIP(ctx, hipGetLastError());
   d.leaves = d_leaves;
   d.leaves_packed = nullptr;
   d.leaves_cold = nullptr;
@@ -383,6 +489,7 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
     d.leaves_packed = d_hot;
     d.leaves_cold = d_cold;
   }
+  const PoolDens pool{d_table, d_leaves, nx, ny, nz, nlx, nly};
 
   // brick grid
   int nb[3];
@@ -399,8 +506,8 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   R2_HIP(ctx, tmalloc((void**)&d_slot, nbt * 4));
   {
     const size_t threads = nbt * 64;
-    hipLaunchKernelGGL(k_brick_range, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, d_dens, nx, ny, nz,
-                       nb[0], nb[1], nb[2], d_range, d_nonempty);
+    hipLaunchKernelGGL(k_brick_range, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, pool, nb[0], nb[1],
+                       nb[2], d_range, d_nonempty);
     R2_HIP(ctx, hipGetLastError());
   }
   std::vector<uint32_t> ne(nbt), slot(nbt);
@@ -418,8 +525,8 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   if ((rc = dmalloc(ctx, g, &d_maj[0], nbt))) return rc;
   {
     const size_t threads = nbt * 64;
-    hipLaunchKernelGGL(k_brick_atlas, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, d_dens, nx, ny, nz,
-                       nb[0], nb[1], nb[2], d_range, d_slot, d_nonempty, d_bricks, d_atlas, d_maj[0]);
+    hipLaunchKernelGGL(k_brick_atlas, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, pool, nb[0], nb[1],
+                       nb[2], d_range, d_slot, d_nonempty, d_bricks, d_atlas, d_maj[0]);
     R2_HIP(ctx, hipGetLastError());
   }
   for (int l = 0; l < 3; ++l) {

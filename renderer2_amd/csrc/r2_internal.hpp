@@ -155,6 +155,9 @@ r2::m44 get_rotation(r2::v3 n);
 void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* records, const float* map_mat,
                 const float* map_vec);
 void nvdb_read(const std::string& path, r2_nvdb_info* info, uint8_t* records, int64_t records_bytes);
+// the file's active values as sparse 8^3 leaves (records NULL: count only); returns the leaf count
+int64_t nvdb_read_leaves(const std::string& path, r2_nvdb_info* info, int32_t shift[3], int64_t max_leaves,
+                         int32_t* origins, uint8_t* records, uint8_t* masks);
 // image files of 8-bit RGB frames, rows bottom to top, written flipped (r2_image.cpp)
 void write_png(const std::string& path, int w, int h, const uint8_t* rgb);
 void write_jpg(const std::string& path, int w, int h, const uint8_t* rgb, int quality);

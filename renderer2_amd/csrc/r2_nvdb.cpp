@@ -366,8 +366,29 @@ void nvdb_write(const std::string& path, int nx, int ny, int nz, const uint8_t* 
 // Header, map and index box of the first grid of a NanoVDB file; with `records`,
 // also the dense records over the index box (inactive voxels hold the background),
 // x fastest.
-void nvdb_read(const std::string& path, r2_nvdb_info* info, uint8_t* records, int64_t records_bytes) {
-  std::ifstream f(path, std::ios::binary | std::ios::ate);
+// an opened NanoVDB file: its grid's header fields, node offsets and index box
+struct NvdbFile {
+  std::ifstream f;
+  std::string path;
+  uint64_t base = 0, grid_size = 0;
+  uint64_t leaf_off = 0, lower_off = 0, upper_off = 0, root_off = 0;
+  uint32_t n_leaf = 0, n_lower = 0, n_upper = 0;
+  std::vector<uint8_t> root;
+  int32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  std::vector<uint8_t> read_at(uint64_t off, size_t n) {
+    if (off > grid_size || n > grid_size - off) throw std::runtime_error("nvdb: node outside the grid buffer");
+    std::vector<uint8_t> b(n);
+    f.seekg((std::streamoff)(base + off));
+    f.read((char*)b.data(), (std::streamsize)n);
+    if (!f) throw std::runtime_error("nvdb: read failed: " + path);
+    return b;
+  }
+};
+
+void nvdb_open(const std::string& path, r2_nvdb_info* info, NvdbFile& F) {
+  F.path = path;
+  F.f.open(path, std::ios::binary | std::ios::ate);
+  std::ifstream& f = F.f;
   if (!f) throw std::runtime_error("cannot open " + path);
   const uint64_t file_bytes = (uint64_t)f.tellg();
   f.seekg(0);
@@ -377,86 +398,142 @@ void nvdb_read(const std::string& path, r2_nvdb_info* info, uint8_t* records, in
   if ((get<uint32_t>(h, 8) >> 21) != kMajor) throw std::runtime_error("nvdb: unsupported NanoVDB major version");
   if (get<uint16_t>(h, 12) < 1) throw std::runtime_error("nvdb: the file holds no grid");
   if (get<uint16_t>(h, 14) != 0) throw std::runtime_error("nvdb: compressed grids are not supported");
-  const uint64_t grid_size = get<uint64_t>(h, kFileHeader + 0);
+  F.grid_size = get<uint64_t>(h, kFileHeader + 0);
   const uint32_t name_size = get<uint32_t>(h, kFileHeader + 136);
-  const uint64_t base = kFileHeader + kMetaData + (uint64_t)name_size;
-  if (name_size > (1u << 20) || grid_size < kGridData + kTreeData + kRootData || base > file_bytes ||
-      grid_size > file_bytes - base)
+  F.base = kFileHeader + kMetaData + (uint64_t)name_size;
+  if (name_size > (1u << 20) || F.grid_size < kGridData + kTreeData + kRootData || F.base > file_bytes ||
+      F.grid_size > file_bytes - F.base)
     throw std::runtime_error("nvdb: truncated file or bad grid size: " + path);
-  auto read_at = [&](uint64_t off, size_t n) {
-    if (off > grid_size || n > grid_size - off) throw std::runtime_error("nvdb: node outside the grid buffer");
-    std::vector<uint8_t> b(n);
-    f.seekg((std::streamoff)(base + off));
-    f.read((char*)b.data(), (std::streamsize)n);
-    if (!f) throw std::runtime_error("nvdb: read failed: " + path);
-    return b;
-  };
-  const std::vector<uint8_t> g = read_at(0, kGridData + kTreeData);
+  const std::vector<uint8_t> g = F.read_at(0, kGridData + kTreeData);
   if (get<uint64_t>(g, 0) != kMagic || (get<uint32_t>(g, 16) >> 21) != kMajor)
     throw std::runtime_error("nvdb: bad grid header");
   if (get<uint32_t>(g, kGdType) != kGridTypeVec4d) throw std::runtime_error("nvdb: not a Vec4d grid");
   for (int i = 0; i < 9; ++i) info->map_mat[i] = get<float>(g, kGdMap + 4 * i);
   for (int i = 0; i < 3; ++i) info->map_vec[i] = get<float>(g, kGdMap + 72 + 4 * i);
-  const uint64_t leaf_off = get<uint64_t>(g, kGridData + 0), lower_off = get<uint64_t>(g, kGridData + 8),
-                 upper_off = get<uint64_t>(g, kGridData + 16), root_off = get<uint64_t>(g, kGridData + 24);
-  const uint32_t n_leaf = get<uint32_t>(g, kGridData + 32), n_lower = get<uint32_t>(g, kGridData + 36),
-                 n_upper = get<uint32_t>(g, kGridData + 40);
-  const std::vector<uint8_t> root = read_at(kGridData + root_off, kRootData);
-  int32_t lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) lo[a] = get<int32_t>(root, 4 * a), hi[a] = get<int32_t>(root, 12 + 4 * a);
+  F.leaf_off = get<uint64_t>(g, kGridData + 0), F.lower_off = get<uint64_t>(g, kGridData + 8);
+  F.upper_off = get<uint64_t>(g, kGridData + 16), F.root_off = get<uint64_t>(g, kGridData + 24);
+  F.n_leaf = get<uint32_t>(g, kGridData + 32), F.n_lower = get<uint32_t>(g, kGridData + 36);
+  F.n_upper = get<uint32_t>(g, kGridData + 40);
+  F.root = F.read_at(kGridData + F.root_off, kRootData);
+  for (int a = 0; a < 3; ++a) F.lo[a] = get<int32_t>(F.root, 4 * a), F.hi[a] = get<int32_t>(F.root, 12 + 4 * a);
   for (int a = 0; a < 3; ++a)
-    if (hi[a] < lo[a] || (int64_t)hi[a] - lo[a] >= kMaxExtent)
+    if (F.hi[a] < F.lo[a] || (int64_t)F.hi[a] - F.lo[a] >= kMaxExtent)
       throw std::runtime_error("nvdb: empty or too large index bounding box");
-  const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-  for (int a = 0; a < 3; ++a) info->ibb_min[a] = lo[a];
-  info->nx = nx, info->ny = ny, info->nz = nz;
+  for (int a = 0; a < 3; ++a) info->ibb_min[a] = F.lo[a];
+  info->nx = F.hi[0] - F.lo[0] + 1, info->ny = F.hi[1] - F.lo[1] + 1, info->nz = F.hi[2] - F.lo[2] + 1;
   info->active_voxels = (int64_t)get<uint64_t>(g, kGridData + 56);
-  info->leaf_count = (int32_t)n_leaf, info->lower_count = (int32_t)n_lower, info->upper_count = (int32_t)n_upper;
-  if (!records) return;
-  const uint64_t need = (uint64_t)nx * ny * nz * 32;
-  if (records_bytes < 0 || (uint64_t)records_bytes < need)
-    throw std::invalid_argument("nvdb: record buffer smaller than the grid's index box");
-  Rec bg;
-  memcpy(bg.data(), root.data() + 32, 32);
-  fill_records(records, (size_t)nx * ny * nz, bg);
-  auto store = [&](int x, int y, int z, const uint8_t* v) {
-    if (x < lo[0] || y < lo[1] || z < lo[2] || x > hi[0] || y > hi[1] || z > hi[2]) return;
-    memcpy(records + ((((size_t)(z - lo[2]) * ny) + (y - lo[1])) * nx + (x - lo[0])) * 32, v, 32);
-  };
-  const uint32_t table = get<uint32_t>(root, 24);
-  const std::vector<uint8_t> tiles = read_at(kGridData + root_off + kRootData, (size_t)table * kRootTile);
+  info->leaf_count = (int32_t)F.n_leaf, info->lower_count = (int32_t)F.n_lower, info->upper_count = (int32_t)F.n_upper;
+  const uint32_t table = get<uint32_t>(F.root, 24);
+  const std::vector<uint8_t> tiles = F.read_at(kGridData + F.root_off + kRootData, (size_t)table * kRootTile);
   for (uint32_t k = 0; k < table; ++k)
     if (get<int64_t>(tiles, k * kRootTile + 8) == 0 && get<uint32_t>(tiles, k * kRootTile + 16) != 0)
       throw std::runtime_error("nvdb: active root tiles are not supported");
-  // active tiles of the internal nodes (value mask set, no child) cover their child's region
+}
+
+// every active value of the tree: fn_tile(origin xyz, child dim, value) for the
+// active tiles of the internal nodes (value mask set, no child), which cover their
+// child's region, and fn_leaf(origin xyz, leaf node bytes) for the leaves
+template <typename Tile, typename Leaf>
+void nvdb_walk(NvdbFile& F, Tile fn_tile, Leaf fn_leaf) {
   auto tiles_of = [&](uint64_t first, uint32_t count, size_t size, size_t header, size_t mask_bytes, int log2dim,
                       int child_dim) {
     const uint32_t dim = 1u << log2dim;
     for (uint32_t k = 0; k < count; ++k) {
-      const std::vector<uint8_t> nd = read_at(kGridData + first + (uint64_t)k * size, size);
+      const std::vector<uint8_t> nd = F.read_at(kGridData + first + (uint64_t)k * size, size);
       const int32_t m = ~(int32_t)(dim * child_dim - 1);
       const int ox = get<int32_t>(nd, 0) & m, oy = get<int32_t>(nd, 4) & m, oz = get<int32_t>(nd, 8) & m;
       for (uint32_t n = 0; n < dim * dim * dim; ++n) {
         const bool on = (nd[32 + n / 8] >> (n % 8)) & 1u, child = (nd[32 + mask_bytes + n / 8] >> (n % 8)) & 1u;
         if (!on || child) continue;
-        const int cx = ox + (int)(n >> (2 * log2dim)) * child_dim, cy = oy + (int)((n >> log2dim) & (dim - 1)) * child_dim,
-                  cz = oz + (int)(n & (dim - 1)) * child_dim;
-        for (int z = std::max(cz, lo[2]); z < std::min(cz + child_dim, hi[2] + 1); ++z)
-          for (int y = std::max(cy, lo[1]); y < std::min(cy + child_dim, hi[1] + 1); ++y)
-            for (int x = std::max(cx, lo[0]); x < std::min(cx + child_dim, hi[0] + 1); ++x)
-              store(x, y, z, nd.data() + header + 32 * n);
+        fn_tile(ox + (int)(n >> (2 * log2dim)) * child_dim, oy + (int)((n >> log2dim) & (dim - 1)) * child_dim,
+                oz + (int)(n & (dim - 1)) * child_dim, child_dim, nd.data() + header + 32 * n);
       }
     }
   };
-  tiles_of(upper_off, n_upper, kUpperSize, kUpperHeader, kUpperMask, 5, 128);
-  tiles_of(lower_off, n_lower, kLowerSize, kLowerHeader, kLowerMask, 4, 8);
-  for (uint32_t k = 0; k < n_leaf; ++k) {
-    const std::vector<uint8_t> nd = read_at(kGridData + leaf_off + (uint64_t)k * kLeafSize, kLeafSize);
-    const int ox = get<int32_t>(nd, 0) & ~7, oy = get<int32_t>(nd, 4) & ~7, oz = get<int32_t>(nd, 8) & ~7;
-    for (uint32_t n = 0; n < 512; ++n)
-      if ((nd[16 + n / 8] >> (n % 8)) & 1u)
-        store(ox + (int)(n >> 6), oy + (int)((n >> 3) & 7), oz + (int)(n & 7), nd.data() + kLeafHeader + 32 * n);
+  tiles_of(F.upper_off, F.n_upper, kUpperSize, kUpperHeader, kUpperMask, 5, 128);
+  tiles_of(F.lower_off, F.n_lower, kLowerSize, kLowerHeader, kLowerMask, 4, 8);
+  for (uint32_t k = 0; k < F.n_leaf; ++k) {
+    const std::vector<uint8_t> nd = F.read_at(kGridData + F.leaf_off + (uint64_t)k * kLeafSize, kLeafSize);
+    fn_leaf(get<int32_t>(nd, 0) & ~7, get<int32_t>(nd, 4) & ~7, get<int32_t>(nd, 8) & ~7, nd);
   }
+}
+
+void nvdb_read(const std::string& path, r2_nvdb_info* info, uint8_t* records, int64_t records_bytes) {
+  NvdbFile F;
+  nvdb_open(path, info, F);
+  if (!records) return;
+  const int32_t* lo = F.lo;
+  const int32_t* hi = F.hi;
+  const int nx = info->nx, ny = info->ny, nz = info->nz;
+  const uint64_t need = (uint64_t)nx * ny * nz * 32;
+  if (records_bytes < 0 || (uint64_t)records_bytes < need)
+    throw std::invalid_argument("nvdb: record buffer smaller than the grid's index box");
+  Rec bg;
+  memcpy(bg.data(), F.root.data() + 32, 32);
+  fill_records(records, (size_t)nx * ny * nz, bg);
+  auto store = [&](int x, int y, int z, const uint8_t* v) {
+    if (x < lo[0] || y < lo[1] || z < lo[2] || x > hi[0] || y > hi[1] || z > hi[2]) return;
+    memcpy(records + ((((size_t)(z - lo[2]) * ny) + (y - lo[1])) * nx + (x - lo[0])) * 32, v, 32);
+  };
+  nvdb_walk(
+      F,
+      [&](int cx, int cy, int cz, int cd, const uint8_t* v) {
+        for (int z = std::max(cz, lo[2]); z < std::min(cz + cd, hi[2] + 1); ++z)
+          for (int y = std::max(cy, lo[1]); y < std::min(cy + cd, hi[1] + 1); ++y)
+            for (int x = std::max(cx, lo[0]); x < std::min(cx + cd, hi[0] + 1); ++x) store(x, y, z, v);
+      },
+      [&](int ox, int oy, int oz, const std::vector<uint8_t>& nd) {
+        for (uint32_t n = 0; n < 512; ++n)
+          if ((nd[16 + n / 8] >> (n % 8)) & 1u)
+            store(ox + (int)(n >> 6), oy + (int)((n >> 3) & 7), oz + (int)(n & 7), nd.data() + kLeafHeader + 32 * n);
+      });
+}
+
+// The active values as 8^3 leaves in an index space shifted by `shift` (the
+// index box minimum rounded down to a multiple of 8), so that leaf origins are
+// non-negative multiples of 8: the file's leaves with their value masks, and
+// the active tiles of the internal nodes as full leaves.  Voxels of a leaf are
+// stored x fastest (NanoVDB's leaves store z fastest).  With records == NULL only
+// the leaf count is returned.
+int64_t nvdb_read_leaves(const std::string& path, r2_nvdb_info* info, int32_t shift[3], int64_t max_leaves,
+                         int32_t* origins, uint8_t* records, uint8_t* masks) {
+  NvdbFile F;
+  nvdb_open(path, info, F);
+  for (int a = 0; a < 3; ++a) shift[a] = (int32_t)(((int64_t)F.lo[a] >> 3) << 3);
+  int64_t n = 0;
+  Rec bg;
+  memcpy(bg.data(), F.root.data() + 32, 32);
+  auto leaf_at = [&](int ox, int oy, int oz) -> int64_t {
+    if (!records) return n++;
+    if (n >= max_leaves) throw std::invalid_argument("nvdb: leaf buffers smaller than the grid's leaves");
+    origins[3 * n] = ox - shift[0], origins[3 * n + 1] = oy - shift[1], origins[3 * n + 2] = oz - shift[2];
+    fill_records(records + (size_t)n * 512 * 32, 512, bg);
+    memset(masks + (size_t)n * 64, 0, 64);
+    return n++;
+  };
+  nvdb_walk(
+      F,
+      [&](int cx, int cy, int cz, int cd, const uint8_t* v) {
+        for (int z = cz; z < cz + cd; z += 8)
+          for (int y = cy; y < cy + cd; y += 8)
+            for (int x = cx; x < cx + cd; x += 8) {
+              const int64_t k = leaf_at(x, y, z);
+              if (!records) continue;
+              for (int i = 0; i < 512; ++i) memcpy(records + ((size_t)k * 512 + i) * 32, v, 32);
+              memset(masks + (size_t)k * 64, 0xff, 64);
+            }
+      },
+      [&](int ox, int oy, int oz, const std::vector<uint8_t>& nd) {
+        const int64_t k = leaf_at(ox, oy, oz);
+        if (!records) return;
+        for (uint32_t m = 0; m < 512; ++m) {
+          if (!((nd[16 + m / 8] >> (m % 8)) & 1u)) continue;
+          const uint32_t v = (m >> 6) | (((m >> 3) & 7) << 3) | ((m & 7) << 6);
+          memcpy(records + ((size_t)k * 512 + v) * 32, nd.data() + kLeafHeader + 32 * m, 32);
+          masks[(size_t)k * 64 + v / 8] |= (uint8_t)(1u << (v % 8));
+        }
+      });
+  return n;
 }
 
 }  // namespace r2host
@@ -466,6 +543,28 @@ extern "C" int r2_nvdb_write(const char* path, int32_t nx, int32_t ny, int32_t n
   if (!path || !records || nx <= 0 || ny <= 0 || nz <= 0) return R2_ERR_INVALID;
   try {
     r2host::nvdb_write(path, nx, ny, nz, (const uint8_t*)records, map_mat, map_vec);
+  } catch (const std::invalid_argument& e) {
+    copy_err(err, err_len, e.what());
+    return R2_ERR_INVALID;
+  } catch (const std::bad_alloc&) {
+    copy_err(err, err_len, "nvdb: out of host memory");
+    return R2_ERR_OOM;
+  } catch (const std::exception& e) {
+    copy_err(err, err_len, e.what());
+    return R2_ERR_IO;
+  }
+  return R2_OK;
+}
+
+extern "C" int r2_nvdb_read_leaves(const char* path, r2_nvdb_info* info, int32_t* origin_shift, int64_t* n_leaves,
+                                   int32_t* leaf_origins, void* records, uint8_t* leaf_masks, char* err,
+                                   int32_t err_len) {
+  if (!path || !info || !origin_shift || !n_leaves) return R2_ERR_INVALID;
+  if (records && (!leaf_origins || !leaf_masks)) return R2_ERR_INVALID;
+  try {
+    memset(info, 0, sizeof(*info));
+    const int64_t cap = *n_leaves;
+    *n_leaves = r2host::nvdb_read_leaves(path, info, origin_shift, cap, leaf_origins, (uint8_t*)records, leaf_masks);
   } catch (const std::invalid_argument& e) {
     copy_err(err, err_len, e.what());
     return R2_ERR_INVALID;
