@@ -115,6 +115,32 @@ def test_round_trip_and_active_set(tmp_path, shape, occ):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("shape,occ", [((5, 9, 13), 0.3), ((24, 16, 8), 0.05), ((20, 140, 9), 0.02)])
+def test_read_leaves_matches_dense_read(tmp_path, shape, occ):
+    """The sparse read: one leaf per leaf node of the file, its active voxels and
+    their records (x fastest within the leaf), background elsewhere."""
+    rec, dens = records(shape, 11, occ)
+    p = tmp_path / "g.nvdb"
+    r2.nvdb_write(p, rec)
+    info, dense = r2.nvdb_read(p)
+    li, org, lrec, msk = r2.nvdb_read_leaves(p)
+    nz, ny, nx = shape
+    act = active_mask(dens)
+    assert tuple(li["origin_shift"]) == (0, 0, 0) and len(org) == info["leaf_count"]
+    assert (org % 8 == 0).all() and len({tuple(o) for o in org}) == len(org)
+    seen = np.zeros(shape, bool)
+    for (ox, oy, oz), lr, m in zip(org, lrec, msk):
+        bits = np.unpackbits(m, bitorder="little").astype(bool)
+        for v in range(512):
+            x, y, z = ox + (v & 7), oy + ((v >> 3) & 7), oz + (v >> 6)
+            inside = x < nx and y < ny and z < nz
+            assert bits[v] == (inside and act[z, y, x])
+            if inside:
+                assert np.array_equal(lr[v], dense[z, y, x])
+                seen[z, y, x] |= bits[v]
+    assert np.array_equal(seen, act)
+
+
 def test_node_counts_follow_from_the_active_set(tmp_path):
     rec, dens = records((40, 150, 70), 3, 0.01)
     p = tmp_path / "g.nvdb"
