@@ -383,6 +383,9 @@ typedef struct {
   double render_ms, mpaths_per_s;
   int32_t media_rendered, objects_skipped;
   int32_t surfaces_rendered;    /* Circle / Sphere objects rendered (since version 3) */
+  /* media whose .brickgrid file (beside the grid file) equals / differs from the brick grid
+   * built from their records (since version 3; a differing file is reported and not used) */
+  int32_t brickgrids_matched, brickgrids_differing;
 } r2_pipeline_report;
 int r2_pipeline_run(const char* config_path, const char* out_dir, int32_t device, r2_pipeline_report* report,
                     char* err, int32_t err_len);
@@ -405,6 +408,20 @@ int r2_image_write(const char* path, int32_t w, int32_t h, const uint8_t* rgb, i
  * raytracer/src/grid_brick/serialization.cpp:37-57; fields host_grid_brick.hpp:47-72).
  * r2_pipeline_run writes one beside each filtered LOD. */
 int r2_grid_write_brickgrid(r2_ctx* ctx, const r2_grid* grid, const char* path, char* err, int32_t err_len);
+/* Read a .brickgrid file (rt::read_grid, raytracer/src/grid_brick/serialization.cpp:63-70, which
+ * CNVDBMedium uses in place of building the brick grid): brick counts, density extrema and the
+ * non-empty brick count into info, and, where the pointers are not NULL, the level-0 ranges (one
+ * encoded half pair per brick), the atlas densities put back in brick layout (8 nbx x 8 nby x
+ * 8 nbz bytes, x fastest: the layout of r2_grid_download_bricks) and the 3 mip levels.  The
+ * front end checks the file beside a medium against the brick grid built from its records.
+ * R2_ERR_IO for a missing or malformed file.  Host only. */
+typedef struct {
+  int32_t n_bricks[3];
+  float min_density, max_density;
+  int64_t n_nonempty_bricks;
+} r2_brickgrid_info;
+int r2_brickgrid_read(const char* path, r2_brickgrid_info* info, uint32_t* ranges, uint8_t* u8, uint32_t* mips,
+                      char* err, int32_t err_len);
 
 /* NanoVDB grid files (.nvdb, NanoVDB 32.3 layout, one Vec4d grid, uncompressed), host
  * only.  r2_nvdb_write replaces COpenvdbBackend::setValues + writeToFile

@@ -76,6 +76,28 @@ def nvdb_read(path, with_records: bool = True):
     return d, rec
 
 
+def brickgrid_read(path):
+    """(info dict, level-0 ranges, atlas densities in brick layout, [mip 1, 2, 3]) of a .brickgrid
+    file, in the layout of Grid.download_bricks.  Host only."""
+    info = capi.BrickgridInfo()
+    err = C.create_string_buffer(512)
+    rc = capi.lib().r2_brickgrid_read(str(path).encode(), C.byref(info), None, None, None, err, 512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
+    nbx, nby, nbz = info.n_bricks
+    rng = np.zeros(nbx * nby * nbz, np.uint32)
+    u8 = np.zeros(nbx * nby * nbz * 512, np.uint8)
+    cnt = [(nbx >> (l + 1)) * (nby >> (l + 1)) * (nbz >> (l + 1)) for l in range(3)]
+    mips = np.zeros(sum(cnt), np.uint32)
+    rc = capi.lib().r2_brickgrid_read(str(path).encode(), C.byref(info), rng.ctypes.data, u8.ctypes.data,
+                                      mips.ctypes.data, err, 512)
+    if rc != 0:
+        raise R2Error(rc, err.value.decode(errors="replace"))
+    d = dict(n_bricks=tuple(info.n_bricks), min_density=info.min_density, max_density=info.max_density,
+             n_nonempty_bricks=info.n_nonempty_bricks)
+    return d, rng, u8, [mips[sum(cnt[:l]):sum(cnt[:l + 1])] for l in range(3)]
+
+
 def nvdb_read_leaves(path):
     """Sparse read of a NanoVDB file: (info dict with origin_shift, leaf_origins (n, 3) int32,
     records (n, 512, 32) u8 x fastest, leaf_masks (n, 64) u8).  Host only."""

@@ -148,6 +148,11 @@ class FilterStats(C.Structure):
     ]
 
 
+class BrickgridInfo(C.Structure):
+    _fields_ = [("n_bricks", C.c_int32 * 3), ("min_density", C.c_float), ("max_density", C.c_float),
+                ("n_nonempty_bricks", C.c_int64)]
+
+
 class PipelineReport(C.Structure):
     _fields_ = [
         ("lods_filtered", C.c_int32), ("filter_ms", C.c_double),
@@ -155,6 +160,7 @@ class PipelineReport(C.Structure):
         ("render_ms", C.c_double), ("mpaths_per_s", C.c_double),
         ("media_rendered", C.c_int32), ("objects_skipped", C.c_int32),
         ("surfaces_rendered", C.c_int32),
+        ("brickgrids_matched", C.c_int32), ("brickgrids_differing", C.c_int32),
     ]
 
 
@@ -197,6 +203,8 @@ SIGNATURES = [
     ("r2_grid_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("r2_grid_get_info", C.c_int, [C.c_void_p, C.POINTER(GridInfo)]),
     ("r2_grid_download_bricks", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("r2_brickgrid_read", C.c_int, [C.c_char_p, C.POINTER(BrickgridInfo), C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_char_p, C.c_int32]),
     ("r2_grid_lookup", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("r2_grid_write_brickgrid", C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p, C.c_int32]),

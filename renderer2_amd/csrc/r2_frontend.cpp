@@ -543,6 +543,122 @@ struct LeWriter {
   void uvec3(uint32_t x, uint32_t y, uint32_t z) { put(x), put(y), put(z); }
 };
 
+// reader of the same archive: little-endian fields in declaration order, a
+// std::vector as a uint64 length and its elements
+struct LeReader {
+  std::vector<uint8_t> b;
+  size_t pos = 0;
+  template <typename T>
+  T get() {
+    if (b.size() - pos < sizeof(T)) throw std::runtime_error("brickgrid: truncated file");
+    T v;
+    memcpy(&v, b.data() + pos, sizeof(T));
+    pos += sizeof(T);
+    return v;
+  }
+  template <typename T>
+  void vec(std::vector<T>& v, uint64_t want) {
+    const uint64_t n = get<uint64_t>();
+    if (n != want) throw std::runtime_error("brickgrid: a buffer's length does not match its dimensions");
+    if ((b.size() - pos) / sizeof(T) < n) throw std::runtime_error("brickgrid: truncated file");
+    v.resize(n);
+    memcpy(v.data(), b.data() + pos, n * sizeof(T));
+    pos += n * sizeof(T);
+  }
+  void uvec3(uint32_t* d) { d[0] = get<uint32_t>(), d[1] = get<uint32_t>(), d[2] = get<uint32_t>(); }
+};
+
+// rt::read_grid (serialization.cpp:63-70) of a CHostBrickGrid archive: brick
+// counts, extrema, non-empty count; level-0 ranges, the atlas densities put
+// back in brick layout through the indirection, and the mips (each optional)
+void read_brickgrid(const std::string& path, r2_brickgrid_info* info, uint32_t* ranges, uint8_t* u8, uint32_t* mips) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  LeReader r;
+  r.b.resize((size_t)f.tellg());
+  f.seekg(0);
+  f.read((char*)r.b.data(), (std::streamsize)r.b.size());
+  if (!f) throw std::runtime_error("brickgrid: read failed: " + path);
+  if (r.get<uint8_t>() != 1) throw std::runtime_error("brickgrid: not a little-endian portable archive");
+  for (int k = 0; k < 16; ++k) (void)r.get<float>();  // the grid transform (identity when written here)
+  uint32_t nb[3], d[3];
+  r.uvec3(nb);
+  for (int a = 0; a < 3; ++a)
+    if (nb[a] == 0 || nb[a] > 1024 || nb[a] % 8) throw std::runtime_error("brickgrid: bad brick counts");
+  info->n_bricks[0] = (int32_t)nb[0], info->n_bricks[1] = (int32_t)nb[1], info->n_bricks[2] = (int32_t)nb[2];
+  info->min_density = r.get<float>();
+  info->max_density = r.get<float>();
+  info->n_nonempty_bricks = (int64_t)r.get<uint64_t>();
+  const uint64_t n = (uint64_t)nb[0] * nb[1] * nb[2];
+  std::vector<uint32_t> ind, rng;
+  std::vector<uint8_t> atlas;
+  r.uvec3(d);
+  if (d[0] != nb[0] || d[1] != nb[1] || d[2] != nb[2]) throw std::runtime_error("brickgrid: bad indirection size");
+  r.vec(ind, n);
+  r.uvec3(d);
+  if (d[0] != nb[0] || d[1] != nb[1] || d[2] != nb[2]) throw std::runtime_error("brickgrid: bad range size");
+  r.vec(rng, n);
+  r.uvec3(d);
+  if (d[0] != 8 * nb[0] || d[1] != 8 * nb[1] || d[2] % 8) throw std::runtime_error("brickgrid: bad atlas size");
+  r.vec(atlas, (uint64_t)d[0] * d[1] * d[2]);
+  const uint32_t az = d[2];
+  if (r.get<uint64_t>() != 3) throw std::runtime_error("brickgrid: expected 3 mip levels");
+  std::vector<uint32_t> mip;
+  size_t off = 0;
+  for (int l = 0; l < 3; ++l) {
+    r.uvec3(d);
+    if (d[0] != nb[0] >> (l + 1) || d[1] != nb[1] >> (l + 1) || d[2] != nb[2] >> (l + 1))
+      throw std::runtime_error("brickgrid: bad mip size");
+    const uint64_t m = (uint64_t)d[0] * d[1] * d[2];
+    r.vec(mip, m);
+    if (mips) memcpy(mips + off, mip.data(), m * 4);
+    off += m;
+  }
+  if (r.pos != r.b.size()) throw std::runtime_error("brickgrid: trailing bytes");
+  if (ranges) memcpy(ranges, rng.data(), n * 4);
+  if (u8) {
+    const size_t ex = (size_t)nb[0] * 8, ey = (size_t)nb[1] * 8, ez = (size_t)nb[2] * 8;
+    memset(u8, 0, ex * ey * ez);
+    for (uint64_t bi = 0; bi < n; ++bi) {
+      if ((rng[bi] & 0xFFFFu) == (rng[bi] >> 16)) continue;  // empty brick: no atlas slot
+      const uint32_t p = ind[bi], px = (p >> 22) & 1023u, py = (p >> 12) & 1023u, pz = (p >> 2) & 1023u;
+      if (px >= nb[0] || py >= nb[1] || 8 * pz + 8 > az) throw std::runtime_error("brickgrid: slot outside the atlas");
+      const size_t bx = bi % nb[0], by = (bi / nb[0]) % nb[1], bz = bi / ((size_t)nb[0] * nb[1]);
+      for (int v = 0; v < 512; ++v) {
+        const size_t x = v & 7, y = (v >> 3) & 7, z = v >> 6;
+        u8[((bz * 8 + z) * ey + by * 8 + y) * ex + bx * 8 + x] =
+            atlas[(((size_t)pz * 8 + z) * ey + py * 8 + y) * ex + px * 8 + x];
+      }
+    }
+  }
+}
+
+// a .brickgrid file against the brick grid the device built for g
+bool brickgrid_matches(r2_ctx* ctx, const r2_grid* g, const fs::path& path) {
+  r2_grid_info inf{};
+  if (r2_grid_get_info(g, &inf) != R2_OK) return false;
+  r2_brickgrid_info bi{};
+  try {
+    read_brickgrid(path.string(), &bi, nullptr, nullptr, nullptr);
+  } catch (const std::exception&) {
+    return false;
+  }
+  for (int a = 0; a < 3; ++a)
+    if (bi.n_bricks[a] != inf.n_bricks[a]) return false;
+  if (bi.n_nonempty_bricks != inf.n_nonempty_bricks || bi.max_density != inf.max_density ||
+      bi.min_density != inf.min_density)
+    return false;
+  const size_t nb = (size_t)inf.n_bricks[0] * inf.n_bricks[1] * inf.n_bricks[2];
+  size_t nmip = 0;
+  for (int l = 0; l < 3; ++l)
+    nmip += (size_t)(inf.n_bricks[0] >> (l + 1)) * (inf.n_bricks[1] >> (l + 1)) * (inf.n_bricks[2] >> (l + 1));
+  std::vector<uint32_t> fr(nb), dr(nb), fm(nmip), dm(nmip);
+  std::vector<uint8_t> fu(nb * 512), du(nb * 512);
+  read_brickgrid(path.string(), &bi, fr.data(), fu.data(), fm.data());
+  if (r2_grid_download_bricks(ctx, g, dr.data(), du.data(), dm.data()) != R2_OK) return false;
+  return fr == dr && fm == dm && fu == du;
+}
+
 void write_brickgrid(r2_ctx* ctx, const r2_grid* g, const fs::path& path) {
   r2_grid_info inf{};
   if (r2_grid_get_info(g, &inf) != R2_OK) throw std::runtime_error("brickgrid: grid info");
@@ -829,6 +945,19 @@ void run_render(r2_ctx* ctx, const Config& c, const fs::path& out_dir, r2_pipeli
       check(ctx, r2_grid_upload(ctx, &d, &g), "grid upload");
       grids.push_back(g);
       cache[o.path] = g;
+      // the reference loads the medium's .brickgrid instead of building it
+      // (rt::read_grid, serialization.cpp:63-70): here the grid is built from the
+      // records, and a file beside them is checked against that build
+      const fs::path bgf = resolve(c, o.path + ".brickgrid");
+      if (fs::exists(bgf)) {
+        if (brickgrid_matches(ctx, g, bgf)) {
+          ++rep->brickgrids_matched;
+        } else {
+          ++rep->brickgrids_differing;
+          fprintf(stderr, "[WARNING]: %s differs from the brick grid of its medium's records; using the latter\n",
+                  bgf.string().c_str());
+        }
+      }
     }
     if ((int)inst.size() >= r2::kMaxInstances) throw std::runtime_error("too many media in the scene");
     r2_instance in{};
@@ -918,6 +1047,28 @@ extern "C" int r2_grid_write_brickgrid(r2_ctx* ctx, const r2_grid* grid, const c
   if (!ctx || !grid || !path) return R2_ERR_INVALID;
   try {
     write_brickgrid(ctx, grid, fs::path(path));
+  } catch (const std::exception& e) {
+    if (err && err_len > 0) {
+      strncpy(err, e.what(), (size_t)err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return R2_ERR_IO;
+  }
+  return R2_OK;
+}
+
+extern "C" int r2_brickgrid_read(const char* path, r2_brickgrid_info* info, uint32_t* ranges, uint8_t* u8,
+                                 uint32_t* mips, char* err, int32_t err_len) {
+  if (!path || !info) return R2_ERR_INVALID;
+  try {
+    memset(info, 0, sizeof(*info));
+    read_brickgrid(path, info, ranges, u8, mips);
+  } catch (const std::bad_alloc&) {
+    if (err && err_len > 0) {
+      strncpy(err, "brickgrid: out of host memory", (size_t)err_len - 1);
+      err[err_len - 1] = 0;
+    }
+    return R2_ERR_OOM;
   } catch (const std::exception& e) {
     if (err && err_len > 0) {
       strncpy(err, e.what(), (size_t)err_len - 1);

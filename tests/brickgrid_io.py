@@ -77,3 +77,38 @@ def check_against_device(grid, path, records):
         assert ms == (nbx >> (lvl + 1), nby >> (lvl + 1), nbz >> (lvl + 1))
         assert np.array_equal(md, mips_dev[lvl])
     return g
+
+
+def write_brickgrid(path, nb, rng, atlas_dense, mips, vmin=0.0, vmax=1.0):
+    """A .brickgrid file by the same rules (the test side of r2_brickgrid_read):
+    non-empty bricks get atlas slots in z, y, x brick order."""
+    nbx, nby, nbz = nb
+    rng = np.asarray(rng, np.uint32).reshape(-1)
+    empty = (rng & 0xFFFF) == (rng >> 16)
+    count = int((~empty).sum())
+    zs = 8 * int(np.ceil(count / (nbx * nby)))
+    dense = np.asarray(atlas_dense, np.uint8).reshape(8 * nbz, 8 * nby, 8 * nbx)
+    atlas = np.zeros((zs, 8 * nby, 8 * nbx), np.uint8)
+    ind = np.zeros(rng.size, np.uint32)
+    slot = 0
+    for bi in np.flatnonzero(~empty):
+        px, py, pz = slot % nbx, (slot // nbx) % nby, slot // (nbx * nby)
+        slot += 1
+        ind[bi] = (px << 22) | (py << 12) | (pz << 2)
+        bz, by, bx = bi // (nbx * nby), (bi // nbx) % nby, bi % nbx
+        atlas[8 * pz:8 * pz + 8, 8 * py:8 * py + 8, 8 * px:8 * px + 8] = \
+            dense[8 * bz:8 * bz + 8, 8 * by:8 * by + 8, 8 * bx:8 * bx + 8]
+    out = [struct.pack("<B", 1), np.eye(4, dtype=np.float32).tobytes(), struct.pack("<3I", *nb),
+           struct.pack("<2f", vmin, vmax), struct.pack("<Q", count)]
+
+    def buf(stride, a):
+        out.append(struct.pack("<3I", *stride) + struct.pack("<Q", a.size) + a.tobytes())
+
+    buf(nb, ind)
+    buf(nb, rng)
+    buf((8 * nbx, 8 * nby, zs), atlas.reshape(-1))
+    out.append(struct.pack("<Q", 3))
+    for lvl in range(3):
+        buf((nbx >> (lvl + 1), nby >> (lvl + 1), nbz >> (lvl + 1)), np.asarray(mips[lvl], np.uint32))
+    with open(path, "wb") as f:
+        f.write(b"".join(out))

@@ -124,6 +124,8 @@ def test_pipeline_filters_and_renders(ctx, tmp_path):
                                     err, 1024)
     assert rc == 0, err.value.decode()
     assert rep.lods_filtered == 2 and rep.media_rendered == 1 and rep.objects_skipped == 0
+    # the rendered LOD's .brickgrid (written by the filter step) equals its rebuilt brick grid
+    assert rep.brickgrids_matched == 1 and rep.brickgrids_differing == 0
     assert (rep.width, rep.height, rep.spp) == (96, 64, 8)
     # grid files equal the library's own filter of the same mesh and parameters
     gm = ctx.upload_mesh(dict(mesh, pos=(0, 0, 0)))

@@ -249,3 +249,40 @@ def test_errors(tmp_path):
     assert rc == -1  # R2_ERR_INVALID: buffer smaller than 8^3 records
     rc = capi.lib().r2_nvdb_write(str(p).encode(), 0, 1, 1, buf.ctypes.data, None, None, None, 0)
     assert rc == -1
+
+
+def test_brickgrid_reader_decodes_the_archive(tmp_path):
+    """r2_brickgrid_read on a file written by the test-side writer of cereal's
+    portable binary rules: ranges, atlas blocks put back in brick layout through the
+    indirection, mips and header fields; malformed files are R2_ERR_IO."""
+    from brickgrid_io import write_brickgrid
+
+    rng = np.random.default_rng(9)
+    nb = (8, 16, 8)
+    n = nb[0] * nb[1] * nb[2]
+    lo = rng.integers(0, 30000, n).astype(np.uint32)
+    hi = lo + rng.integers(0, 2, n).astype(np.uint32) * rng.integers(1, 3000, n).astype(np.uint32)
+    ranges = (hi << 16) | lo
+    dense = rng.integers(0, 256, (8 * nb[2], 8 * nb[1], 8 * nb[0])).astype(np.uint8)
+    empty = ((ranges & 0xFFFF) == (ranges >> 16)).reshape(nb[2], nb[1], nb[0])
+    want = dense.copy()
+    for bz, by, bx in zip(*np.nonzero(empty)):
+        want[8 * bz:8 * bz + 8, 8 * by:8 * by + 8, 8 * bx:8 * bx + 8] = 0
+    mips = [rng.integers(0, 2 ** 32, (nb[0] >> (l + 1)) * (nb[1] >> (l + 1)) * (nb[2] >> (l + 1)),
+                         dtype=np.uint64).astype(np.uint32) for l in range(3)]
+    p = tmp_path / "g.brickgrid"
+    write_brickgrid(p, nb, ranges, dense, mips, 0.25, 7.5)
+    info, r, u8, m = r2.brickgrid_read(p)
+    assert info["n_bricks"] == nb and info["n_nonempty_bricks"] == int((~empty).sum())
+    assert info["min_density"] == 0.25 and info["max_density"] == 7.5
+    assert np.array_equal(r, ranges) and np.array_equal(u8, want.reshape(-1))
+    for a, b in zip(m, mips):
+        assert np.array_equal(a, b)
+    data = p.read_bytes()
+    for bad in (data[:-3], data + b"\0", b"\2" + data[1:]):
+        (tmp_path / "bad.brickgrid").write_bytes(bad)
+        with pytest.raises(r2.R2Error) as e:
+            r2.brickgrid_read(tmp_path / "bad.brickgrid")
+        assert e.value.code == -5
+    with pytest.raises(r2.R2Error):
+        r2.brickgrid_read(tmp_path / "missing.brickgrid")
