@@ -45,8 +45,8 @@ enum : int { RL_VERTEX = 0, RL_LIGHT = 1, RL_PHASE = 2, RL_CONT = 3 };
 // how the ray that arrived at a vertex ended (set by the continuation walk)
 enum : int { CR_START = 0, CR_NONE, CR_END, CR_ESCAPED, CR_COLLIDED, CR_SURFACE };
 enum : int { TK_FLIGHT_DDA = 1, TK_TR_DDA = 2, TK_FLIGHT_DELTA = 3, TK_TR_RATIO = 4 };
-// ColdLane::flags (FL_PREMUL: the direct light already carries the throughput)
-enum : uint32_t { FL_LIGHT = 1u, FL_PHASE = 2u, FL_CONT = 4u, FL_NEE = 8u, FL_RR = 16u, FL_PREMUL = 32u };
+// ColdLane::flags: the rays of the vertex that wait for their walks
+enum : uint32_t { FL_LIGHT = 1u, FL_PHASE = 2u, FL_CONT = 4u };
 // caps on the segments of one intersectTr walk and on the bounce loop's
 // iterations, which the reference loops until a miss and does not count box
 // pass-throughs: 64 (beyond 4 max_depth iterations) plus 4 per instance beyond
@@ -56,13 +56,12 @@ __device__ __forceinline__ int inst_extra(const DScene& s) { return s.n_inst > 4
 
 // Path state that is touched only in shading passes and at walk boundaries
 // lives in LDS, one slot per lane; the state the walk steps touch stays in
-// registers.  47 dwords: an odd stride, so the 64 lanes of a wavefront fall on
-// distinct banks, and three blocks of 256 lanes (141 KB) fit a CU's LDS.
+// registers.  39 dwords: an odd stride, so the 64 lanes of a wavefront fall on
+// distinct banks, and four blocks of 256 lanes (156 KB) fit a CU's 160 KB of
+// LDS.  The lane's pixel and sample are the counter words 0 and 1 of its random
+// stream (registers); the item's chunk follows from the sample.
 struct ColdLane {
-  uint32_t pix;
-  int32_t s_cur, s_end;
-  uint32_t item;
-  float ax, ay, az;  // FP32 item sum (the reference's framebuffer += order)
+  float ax, ay, az;  // FP32 item sum (the reference's framebuffer += order); a probe keeps its item in ax
   uint32_t ring[8];  // random words of the current stream (PathRng)
   // rays of the current vertex waiting for their walk (the light-sample shadow
   // ray, walked first, goes straight to the walk registers): the phase-sample
@@ -73,18 +72,13 @@ struct ColdLane {
   v3 c_o, c_d;
   float c_tmax;
   int32_t c_med;
-  // direct light of the vertex per strategy: Le * (mis * p), then times the
-  // shadow walk's transmittance over the strategy's pdf once the walk ends
-  v3 nee_a;
-  float pdf_a;
-  v3 nee_b;
-  float pdf_b;
-  float rr_div;    // 1 - q of the vertex's Russian roulette (FL_RR)
-  uint32_t flags;  // FL_*: rays of the vertex, direct light pending, roulette applied
-  v3 L, thr;       // radiance and throughput of the path (touched only when shading)
-  uint32_t pad;
+  // direct light of the vertex per strategy, throughput * (Le * (mis * p)) / pdf,
+  // added to L times the shadow walk's transmittance when that walk ends
+  v3 nee_a, nee_b;
+  uint32_t flags;  // FL_*: rays of the vertex waiting for their walks
+  v3 L, thr;       // radiance and throughput of the path
 };
-static_assert(sizeof(ColdLane) == 47 * 4, "ColdLane must keep an odd dword stride");
+static_assert(sizeof(ColdLane) == 39 * 4, "ColdLane must keep an odd dword stride");
 
 // The random stream of one (path, vertex, role): Philox4x32-10 keyed (seed,
 // stream), counter (pixel, sample, block, 4 k + role); draw j is word j & 3 of
@@ -142,10 +136,8 @@ struct PathRng {
 
 struct PathLane {
   ColdLane& c;
-  // work item (LDS)
-  uint32_t& pix;
-  int32_t &s_cur, &s_end;
-  uint32_t& item;
+  // work item: the pixel is rng.c0, the sample rng.c1 (+ the launch's first);
+  // its FP32 sum in LDS
   float &ax, &ay, &az;
   double dax, day, daz;  // FP64 item sum (compensated framebuffer form)
   PathRng rng;
@@ -167,8 +159,7 @@ struct PathLane {
   float tt, ttau, tmip, ttmax, ttr;
 
   __device__ PathLane(ColdLane& cl, const DScene& s)
-      : c(cl), pix(cl.pix), s_cur(cl.s_cur), s_end(cl.s_end), item(cl.item), ax(cl.ax), ay(cl.ay), az(cl.az),
-        rng(cl.ring, s.seed, s.stream), L(cl.L), thr(cl.thr) {}
+      : c(cl), ax(cl.ax), ay(cl.ay), az(cl.az), rng(cl.ring, s.seed, s.stream), L(cl.L), thr(cl.thr) {}
 };
 
 // scatter record of a collision (CMediumInstance::sample's SMediumInteraction)
@@ -180,6 +171,26 @@ struct Scatter {
 };
 
 __device__ __forceinline__ bool any_zero(v3 a) { return a.x == 0.f || a.y == 0.f || a.z == 0.f; }
+
+// The lane's fields are carried around the scheduling loop, and the compiler
+// cannot see which of them a block's successors read before writing, so it
+// keeps every field in a register through every block.  A block marks the fields
+// that are always written again before their next read as dead (an undefined
+// value: no instruction), which frees their registers for its temporaries.
+template <typename T>
+__device__ __forceinline__ void dead(T& x) { x = __builtin_nondeterministic_value(x); }
+__device__ __forceinline__ void dead(v3& a) { dead(a.x); dead(a.y); dead(a.z); }
+// the traversal fields: start_traversal writes all that a traversal reads
+__device__ __forceinline__ void dead_traversal(PathLane& p) {
+  dead(p.tk); dead(p.tinst); dead(p.tsteps); dead(p.tres);
+  dead(p.to); dead(p.td); dead(p.tri);
+  dead(p.tt); dead(p.ttau); dead(p.tmip); dead(p.ttmax); dead(p.ttr);
+}
+// the walk fields: a segment writes the intersection fields before reading them,
+// walk_begin the shadow walk's counters
+__device__ __forceinline__ void dead_walk(PathLane& p) {
+  dead(p.si_pos); dead(p.si_medium); dead(p.si_hit); dead(p.sh_seg); dead(p.sh_Tr);
+}
 
 // --------------------------------------------------------------- traversal --
 // Prologue of sampleDDA / sampleInternal / trDDA / trInternal on instance k.
@@ -362,12 +373,11 @@ __device__ __forceinline__ void walk_next(PathLane& p, int after) {
   else p.stage = ST_SHADE;
 }
 
-// end of a shadow walk: the strategy's share of direct(),
-// ((Le * (mis * p)) * Tr) / pdf, with Tr = 0 when the walk hit a surface
+// end of a shadow walk: the strategy's share of direct() times the throughput
+// is added to the path's radiance, with Tr = 0 when the walk hit a surface
 __device__ __forceinline__ void shadow_done(PathLane& p) {
   const float tr = p.si_hit ? 0.f : p.sh_Tr;
-  if (p.role == RL_LIGHT) p.c.nee_a = (p.c.nee_a * tr) / p.c.pdf_a;
-  else p.c.nee_b = (p.c.nee_b * tr) / p.c.pdf_b;
+  p.L = p.L + (p.role == RL_LIGHT ? p.c.nee_a : p.c.nee_b) * tr;
   walk_next(p, p.role);
 }
 
@@ -400,6 +410,7 @@ __device__ __forceinline__ void cont_nohit(PathLane& p) {
 // whose lanes hold both kinds of ray runs each once.
 template <bool kSingle, bool kSurf>
 __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
+  dead_traversal(p);  // read by the step that ended the last traversal
   const bool dda = (s.flags & 1u) != 0u;
   const bool cont = p.role == RL_CONT;
   if (cont ? (!(p.b < s.max_depth) || ++p.iters > 4 * s.max_depth + 64 + inst_extra(s))
@@ -493,8 +504,10 @@ __device__ __forceinline__ void shade_scatter(const DScene& s, PathLane& p, cons
   ++p.k;
   p.rng.init(p.rng.c0, p.rng.c1, 4u * (uint32_t)p.k);
   p.rng.top_up();
-  uint32_t flags = FL_NEE;
+  uint32_t flags = 0u;
   const v3 wo = -p.ray.d;
+  dead_walk(p);
+  dead(p.ray.o); dead(p.ray.d); dead(p.ray.tmax); dead(p.ray.medium);
   // direct(): light sample, its shadow ray and the phase function toward it
   v3 ldir;
   float lpdf = 0.f;
@@ -507,8 +520,7 @@ __device__ __forceinline__ void shade_scatter(const DScene& s, PathLane& p, cons
     p.ray.medium = r.medium;
     const float ph = phase_p(sc.S, sc.n, sc.ior, wo, dl, p.rng);
     const float mis = (1.f * lpdf) / (1.f * lpdf + 1.f * ph);
-    p.c.nee_a = Le * (mis * ph);
-    p.c.pdf_a = lpdf;
+    p.c.nee_a = p.thr * ((Le * (mis * ph)) / lpdf);
     flags |= FL_LIGHT;
   }
   p.rng.top_up();
@@ -525,8 +537,7 @@ __device__ __forceinline__ void shade_scatter(const DScene& s, PathLane& p, cons
       float epdf = 0.f;
       const v3 Le2 = env_le<kCount>(s.env, dw, epdf, cnt);
       const float mis = (1.f * ph) / (1.f * ph + 1.f * epdf);
-      p.c.nee_b = Le2 * (mis * ph);
-      p.c.pdf_b = ph;
+      p.c.nee_b = p.thr * ((Le2 * (mis * ph)) / ph);
       flags |= FL_PHASE;
     }
   }
@@ -542,16 +553,12 @@ __device__ __forceinline__ void shade_scatter(const DScene& s, PathLane& p, cons
     p.c.c_med = r.medium;
     flags |= FL_CONT;
   }
-  // Russian roulette (the throughput's division waits until the vertex's
-  // direct light, which uses the throughput before it, has been added)
+  // Russian roulette (the vertex's direct light above took the throughput
+  // before it)
   if (p.b > 3) {
     const float q = 1.f - (p.thr.x + p.thr.y + p.thr.z) / 3.f;
-    if (p.rng.uni() <= q) {
-      flags &= ~FL_CONT;
-    } else {
-      p.c.rr_div = 1.f - q;
-      flags |= FL_RR;
-    }
+    if (p.rng.uni() <= q) flags &= ~FL_CONT;
+    else p.thr = p.thr / (1.f - q);
   }
   ++p.b;
   // the bounce loop's depth test: a continuation past max_depth is never walked
@@ -566,18 +573,20 @@ __device__ __forceinline__ void shade_scatter(const DScene& s, PathLane& p, cons
 // sample in the tangent frame of the normal, their shadow rays starting in the
 // medium the incoming ray travelled in, Li's BSDF continuation, its throughput
 // and Russian roulette, in the reference's order of draws from the vertex's own
-// stream.  The throughput is updated here, so the direct light of the vertex
-// is stored already multiplied by the throughput it arrived with (FL_PREMUL).
+// stream.  As at a scatter vertex, the direct light of the vertex is stored
+// multiplied by the throughput it arrived with, before the throughput update.
 template <bool kCount>
 __device__ __forceinline__ void shade_surface(const DScene& s, PathLane& p, Cnt& cnt) {
   const DSurface& sf = s.surf[p.si_medium];
   const v3 n = p.ray.o, pos = p.si_pos, rd = p.ray.d;
   const int cur = p.ray.medium;
+  dead_walk(p);
+  dead(p.ray.o); dead(p.ray.d); dead(p.ray.tmax); dead(p.ray.medium);
   const v3 thr0 = p.thr;
   ++p.k;
   p.rng.init(p.rng.c0, p.rng.c1, 4u * (uint32_t)p.k);
   p.rng.top_up();
-  uint32_t flags = FL_NEE | FL_PREMUL;
+  uint32_t flags = 0u;
   const SFrame fr = surface_frame(n);
   const v3 wot = mat3_mul(fr.w2t, -rd);
   {
@@ -595,8 +604,7 @@ __device__ __forceinline__ void shade_surface(const DScene& s, PathLane& p, Cnt&
       const v3 f = mat_f(sf, wot, lt) * fabsf(dot(n, r.d));
       const float sp = mat_pdf(sf, wot, lt);
       const float mis = (1.f * lpdf) / (1.f * lpdf + 1.f * sp);
-      p.c.nee_a = thr0 * ((f * mis) * Le);
-      p.c.pdf_a = lpdf;
+      p.c.nee_a = (thr0 * ((f * mis) * Le)) / lpdf;
       flags |= FL_LIGHT;
     }
   }
@@ -617,8 +625,7 @@ __device__ __forceinline__ void shade_surface(const DScene& s, PathLane& p, Cnt&
       float epdf = 0.f;
       const v3 Le2 = env_le<kCount>(s.env, wi, epdf, cnt);
       const float mis = (1.f * sp) / (1.f * sp + 1.f * epdf);
-      p.c.nee_b = thr0 * ((f * mis) * Le2);
-      p.c.pdf_b = sp;
+      p.c.nee_b = (thr0 * ((f * mis) * Le2)) / sp;
       flags |= FL_PHASE;
     }
   }

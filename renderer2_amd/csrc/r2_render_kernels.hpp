@@ -66,7 +66,7 @@ __device__ __forceinline__ void stats_end() {}
 // end of one path: NaN guard, radiance / spp, accumulation into the item (or the
 // probe output); the lane starts the item's next sample, or fetches a new item
 template <typename Acc>
-__device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix,
+__device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, int chunk,
                                             Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
                                             float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
   v3 L = p.L;
@@ -74,12 +74,13 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
     L = mk(0.f, 0.f, 0.f);
   else
     L = L / (float)s.spp;
-  if (probe_pix) {
-    probe_out[3 * p.item] = L.x;
-    probe_out[3 * p.item + 1] = L.y;
-    probe_out[3 * p.item + 2] = L.z;
+  if (probe_pix) {  // one path per item
+    const uint32_t item = __float_as_uint(p.ax);
+    probe_out[3 * item] = L.x;
+    probe_out[3 * item + 1] = L.y;
+    probe_out[3 * item + 2] = L.z;
     if (probe_cnt)
-      for (int c = 0; c < kNumCnt; ++c) probe_cnt[p.item * kNumCnt + c] = cnt.c[c];
+      for (int c = 0; c < kNumCnt; ++c) probe_cnt[item * kNumCnt + c] = cnt.c[c];
 #pragma unroll
     for (int c = 0; c < kNumCnt; ++c) cnt.c[c] = 0;
     p.stage = ST_FETCH;
@@ -94,12 +95,13 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
     p.ay += L.y;
     p.az += L.z;
   }
-  ++p.s_cur;
-  if (p.s_cur < p.s_end) {
+  // the item is sample chunk c of the pixel: samples [c chunk, min(spp, (c + 1) chunk))
+  const uint32_t smp = p.rng.c1 - (uint32_t)s.sample_begin, c = smp / (uint32_t)chunk;
+  if (smp + 1u < min((uint32_t)s.spp, (c + 1u) * (uint32_t)chunk)) {
+    ++p.rng.c1;
     p.cres = CR_START;
   } else {
-    const uint32_t c = (uint32_t)(p.item / npix);
-    Acc* o = partial + ((size_t)c * npix + p.pix) * 3;
+    Acc* o = partial + ((size_t)c * npix + p.rng.c0) * 3;
     if constexpr (sizeof(Acc) == 8) {
       o[0] = p.dax;
       o[1] = p.day;
@@ -119,18 +121,9 @@ __device__ __forceinline__ void finish_path(const DScene& s, PathLane& p, Cnt& c
 // ray sees the environment; anything else ends the path), and for a path that
 // ends the start of the item's next sample (vertex 0: the camera ray).
 template <bool kSingle, bool kSurf, bool kCount, typename Acc>
-__device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, Acc* __restrict__ partial,
-                                      const int32_t* __restrict__ probe_pix, float* __restrict__ probe_out,
-                                      uint32_t* __restrict__ probe_cnt) {
-  const uint32_t fl = p.c.flags;
-  if (fl & FL_NEE) {
-    v3 Ld = mk(0.f, 0.f, 0.f);
-    if (fl & FL_LIGHT) Ld = Ld + p.c.nee_a;
-    if (fl & FL_PHASE) Ld = Ld + p.c.nee_b;
-    p.L = p.L + ((kSurf && (fl & FL_PREMUL)) ? Ld : p.thr * Ld);
-    if (fl & FL_RR) p.thr = p.thr / p.c.rr_div;
-    p.c.flags = 0u;
-  }
+__device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, uint32_t npix, int chunk,
+                                      Acc* __restrict__ partial, const int32_t* __restrict__ probe_pix,
+                                      float* __restrict__ probe_out, uint32_t* __restrict__ probe_cnt) {
   bool scatter = false;
   Scatter sc;
   if (p.cres == CR_COLLIDED) {
@@ -141,6 +134,11 @@ __device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, ui
     const v3 Le = env_le<kCount>(s.env, p.ray.d, lpdf, cnt);
     p.L = p.L + Le * p.thr;
   }
+  // every path out of a shading pass starts its walks with a segment
+  dead_traversal(p);
+  dead(p.sh_seg);
+  dead(p.sh_Tr);
+  dead(p.si_hit);
   if (scatter) {
     shade_scatter<kSingle, kCount>(s, p, sc, cnt);
     return;
@@ -150,12 +148,13 @@ __device__ __forceinline__ void shade(const DScene& s, PathLane& p, Cnt& cnt, ui
     return;
   }
   if (p.cres != CR_START) {
-    finish_path<Acc>(s, p, cnt, npix, partial, probe_pix, probe_out, probe_cnt);
+    finish_path<Acc>(s, p, cnt, npix, chunk, partial, probe_pix, probe_out, probe_cnt);
     if (p.stage == ST_FETCH) return;
   }
   // vertex 0 of the item's next sample: the camera ray
-  p.rng.init(p.pix, (uint32_t)(s.sample_begin + p.s_cur), 0u);
-  const Ray r = camera_ray(s, (int)(p.pix % (uint32_t)s.W), (int)(p.pix / (uint32_t)s.W), p.rng);
+  dead_walk(p);
+  p.rng.init(p.rng.c0, p.rng.c1, 0u);
+  const Ray r = camera_ray(s, (int)(p.rng.c0 % (uint32_t)s.W), (int)(p.rng.c0 / (uint32_t)s.W), p.rng);
   p.c.c_o = r.o;
   p.c.c_d = r.d;
   p.c.c_tmax = r.tmax;
@@ -179,6 +178,7 @@ struct FrameFetch {
   int chunk;
   const int32_t* probe_pix;
   const int32_t* probe_smp;
+  __device__ __forceinline__ int chunk_len() const { return chunk; }
   __device__ __forceinline__ void operator()(const DScene& s, PathLane& p, int lane, uint32_t npix) const {
     const uint64_t need = __ballot(p.stage == ST_FETCH);
     if (!need) return;
@@ -193,21 +193,19 @@ struct FrameFetch {
       p.stage = ST_IDLE;
       return;
     }
-    p.item = item;
     p.ax = p.ay = p.az = 0.f;
     p.dax = p.day = p.daz = 0.0;
     if (probe_pix) {
-      p.pix = (uint32_t)probe_pix[item];
-      p.s_cur = probe_smp[item];
-      p.s_end = p.s_cur + 1;
+      p.rng.c0 = (uint32_t)probe_pix[item];
+      p.rng.c1 = (uint32_t)(s.sample_begin + probe_smp[item]);
+      p.ax = __uint_as_float((uint32_t)item);
     } else {
       // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
       const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
       int x, y;
       item_pixel(q, s.W, s.H, x, y);
-      p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-      p.s_cur = (int)c * chunk;
-      p.s_end = min(s.spp, p.s_cur + chunk);
+      p.rng.c0 = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+      p.rng.c1 = (uint32_t)s.sample_begin + c * (uint32_t)chunk;
     }
     p.c.flags = 0u;
     p.cres = CR_START;
@@ -250,7 +248,7 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
     SM_STAT(1, 1);
     SM_STAT(2, __popcll(__ballot(p.stage == ST_SHADE)));
     if (p.stage == ST_SHADE)
-      shade<kSingle, kSurf, kCount, Acc>(s, p, cnt, npix, partial, probe_pix, probe_out, probe_cnt);
+      shade<kSingle, kSurf, kCount, Acc>(s, p, cnt, npix, fetch.chunk_len(), partial, probe_pix, probe_out, probe_cnt);
 #ifdef R2_SM_STATS
     SM_STAT(5, SM_CLOCK() - t0);
 #endif
@@ -279,7 +277,7 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
 }
 
 #ifndef R2_SM_WAVES
-#define R2_SM_WAVES 3  // wavefronts per SIMD the register budget is sized for
+#define R2_SM_WAVES 4  // wavefronts per SIMD the register budget is sized for
 #endif
 template <bool kSingle, bool kSurf, bool kCount, typename Acc>
 __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __restrict__ sp, uint64_t* __restrict__ counter,
@@ -293,7 +291,6 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
   __shared__ ColdLane cold[256];
   PathLane p(cold[threadIdx.x], s);
   p.stage = ST_FETCH;
-  p.item = 0;
   p.cres = CR_START;
   Cnt cnt;
 #pragma unroll
@@ -316,6 +313,7 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_sm(const DScene* __r
 struct BatchFetch {
   const BatchEntry* e;
   uint64_t* counter;
+  __device__ __forceinline__ int chunk_len() const { return e->chunk; }
   __device__ __forceinline__ void operator()(const DScene& s, PathLane& p, int lane, uint32_t npix) const {
     const uint64_t need = __ballot(p.stage == ST_FETCH);
     if (!need) return;
@@ -331,16 +329,14 @@ struct BatchFetch {
       return;
     }
     const uint64_t item = e->first + local;
-    p.item = item;
     p.ax = p.ay = p.az = 0.f;
     p.dax = p.day = p.daz = 0.0;
     // items of a launch fit 32 bits (the lane keeps them as 32-bit values)
     const uint32_t it = (uint32_t)item, c = it / npix, q = it - c * npix;
     int x, y;
     item_pixel(q, s.W, s.H, x, y);
-    p.pix = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
-    p.s_cur = (int)c * e->chunk;
-    p.s_end = min(s.spp, p.s_cur + e->chunk);
+    p.rng.c0 = (uint32_t)y * (uint32_t)s.W + (uint32_t)x;
+    p.rng.c1 = (uint32_t)s.sample_begin + c * (uint32_t)e->chunk;
     p.c.flags = 0u;
     p.cres = CR_START;
     p.stage = ST_SHADE;
@@ -368,7 +364,6 @@ __global__ void __launch_bounds__(256, R2_SM_WAVES) k_paths_batch(const BatchEnt
     Acc* partial = (Acc*)E.partial;
     PathLane p(cold[threadIdx.x], s);
     p.stage = ST_FETCH;
-    p.item = 0;
     p.cres = CR_START;
     const BatchFetch fetch{&E, &counters[e]};
     for (uint32_t guard = 0; guard < (1u << 30); ++guard) {

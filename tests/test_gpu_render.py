@@ -95,7 +95,7 @@ def test_paths_match_oracle_fp32(ctx, wl, use_brick, bar, bar_cnt, request):
     oracle rounds every operation correctly, so such decisions occur.  The
     anisotropic scene under a 1e4:1 sun is the most sensitive: the FP32 and FP64
     oracles themselves agree on only ~96.6% of its paths, and the GPU agrees with
-    the FP32 oracle on ~94.9% (lookup counters, the route a path took, on ~97.2%),
+    the FP32 oracle on ~94.5% (lookup counters, the route a path took, on ~97.0%),
     hence its bars.  The smooth scene keeps 99% bars for both."""
     w = request.getfixturevalue(wl)
     grids, env = _upload(ctx, w)
