@@ -121,6 +121,22 @@ def build_cli(verbose: bool = True) -> Path:
         raise RuntimeError(f"building the command-line program failed:\n{res.stderr[-4000:]}")
     if verbose:
         print(f"[build] linked {exe}", flush=True)
+    build_valu_peak(verbose)
+    return exe
+
+
+def build_valu_peak(verbose: bool = True) -> Path:
+    """valu_peak [iters]: the device's FP32 VALU issue rate (the ceiling the
+    kernels' VALU issue rates are read against)."""
+    BIN.mkdir(parents=True, exist_ok=True)
+    exe = BIN / "valu_peak"
+    cmd = [HIPCC, "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-Wno-unused-command-line-argument", str(CSRC / "r2_valu_peak.hip"), "-o", str(exe)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"building valu_peak failed:\n{res.stderr[-4000:]}")
+    if verbose:
+        print(f"[build] linked {exe}", flush=True)
     return exe
 
 
