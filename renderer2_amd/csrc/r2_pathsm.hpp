@@ -89,7 +89,7 @@ static_assert(sizeof(ColdLane) == 39 * 4, "ColdLane must keep an odd dword strid
 // only when its lane has run out.  init() computes nothing: the first top_up()
 // or draw does.
 #ifndef R2_RNG_AHEAD
-#define R2_RNG_AHEAD 5
+#define R2_RNG_AHEAD 4
 #endif
 static_assert(R2_RNG_AHEAD >= 1 && R2_RNG_AHEAD <= 5, "the ring holds two blocks");
 struct PathRng {
