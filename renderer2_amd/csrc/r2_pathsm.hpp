@@ -148,6 +148,8 @@ struct PathLane {
   // intersection (a surface hit keeps the surface in si_medium and its normal in
   // ray.o until it is shaded).  Shadow walks keep intersectTr's p0 in ray.o, p1
   // in ray.d and currentMedium in ray.medium, the segment's hit flag in si_hit.
+  // During a shadow traversal si_hit = 2 says that the segment to follow it is
+  // known to meet nothing.
   Ray ray;
   v3 si_pos;
   int si_medium, si_hit;
@@ -476,6 +478,14 @@ __device__ __forceinline__ void seg_step(const DScene& s, PathLane& p) {
       return;
     }
     kind = dda ? TK_TR_DDA : TK_TR_RATIO;
+    // The segment after this one starts at the box face, outside every medium,
+    // toward p1.  Its scene intersection is made now (the same one, so the
+    // same result); when it meets nothing (si_hit = 2 through the traversal),
+    // the traversal's last step ends the walk without another segment pass.
+    if (p.sh_seg < 64 + inst_extra(s)) {
+      Ray er = spawn_ray(p.ray.o, p.ray.d, p.ray.medium);
+      if (!scene_intersect<kSurf, kSingle>(s, er).hit) p.si_hit = 2;
+    }
   }
   start_traversal<kSingle>(s, p, tr.medium, tr, kind);
 }
@@ -491,7 +501,13 @@ __device__ __forceinline__ void trav_step(const DScene& s, PathLane& p, Cnt& cnt
     else cont_nohit(p);
   } else {
     p.sh_Tr *= p.ttr;  // intersectTr continues from the box face
-    p.stage = ST_SEG;
+    if (p.si_hit == 2) {  // the next segment, intersected already, meets nothing
+      ++p.sh_seg;
+      p.si_hit = 0;
+      shadow_done(p);
+    } else {
+      p.stage = ST_SEG;
+    }
   }
 }
 
