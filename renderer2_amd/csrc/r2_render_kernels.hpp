@@ -252,6 +252,16 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
 #ifdef R2_SM_STATS
     SM_STAT(5, SM_CLOCK() - t0);
 #endif
+#ifndef R2_NO_FUSED_SEG
+    // most shaded lanes now start a walk: their first segment runs in the same
+    // step, together with the lanes that were already waiting at a segment
+    SM_STAT(3, 1);
+    SM_STAT(4, __popcll(__ballot(p.stage == ST_SEG)));
+    if (p.stage == ST_SEG) {
+      p.rng.top_up();
+      seg_step<kSingle, kSurf>(s, p);
+    }
+#endif
   } else if (ns >= R2_SEG_MIN || ns >= R2_SEG_RATIO * nt) {
     SM_STAT(3, 1);
     SM_STAT(4, ns);
