@@ -238,7 +238,7 @@ __device__ __forceinline__ bool traversal_step(const DScene& s, PathLane& p, Cnt
     ++p.tsteps;
     count<kCount>(cnt, kCntDda);
     const v3 cur = p.to + p.tt * p.td;
-    const int imip = (int)roundf(p.tmip);
+    const int imip = (int)(p.tmip + 0.5f);  // roundf: tmip is a multiple of 0.25 in [0, 3]
     const float maj = ratio * lookup_majorant<kCount>(g, cur, imip, cnt);
     const float dt = step_dda(cur, p.tri, imip);
     const float prev = p.tt;

@@ -349,11 +349,13 @@ __device__ __forceinline__ float lookup_density(const DGrid& g, v3 ipos, v3 u, C
 
 // CNVDBMedium::stepDDA (nvdb_medium_impl.hpp:241-246)
 __device__ __forceinline__ float step_dda(v3 pos, v3 inv_dir, int mip) {
-  const float dim = (float)(8 << mip);
+  // dim = 8 << mip and its reciprocal, both powers of two: exact by ldexp, without
+  // a conversion and a quarter-rate reciprocal
+  const float dim = ldexpf(1.f, 3 + mip);
   const float ox = inv_dir.x >= 0.f ? dim + 0.5f : -0.5f;
   const float oy = inv_dir.y >= 0.f ? dim + 0.5f : -0.5f;
   const float oz = inv_dir.z >= 0.f ? dim + 0.5f : -0.5f;
-  const float inv_dim = 1.f / dim;
+  const float inv_dim = ldexpf(1.f, -3 - mip);
   const float tx = (floorf(pos.x * inv_dim) * dim + ox - pos.x) * inv_dir.x;
   const float ty = (floorf(pos.y * inv_dim) * dim + oy - pos.y) * inv_dir.y;
   const float tz = (floorf(pos.z * inv_dim) * dim + oz - pos.z) * inv_dir.z;
