@@ -229,45 +229,40 @@ __device__ __forceinline__ Rec filtered_data(const DGrid& g, v3 p, Cnt& cnt) {
       o.ior += w * r.ior;
     };
     auto nlerp = [](float t, v3 u, v3 v) { return normalize((1.f - t) * normalize(u) + t * normalize(v)); };
-    // all corner loads are issued before any is decoded (two round trips)
-    uint4 hw[8];
-    uint2 cw[8];
-    {
-      int32_t lf[8];
+    // the eight leaf-table entries are loaded at once; the records of each
+    // pair of corners are loaded when the pair is decoded (four more round
+    // trips, with two corners' words live instead of eight)
+    int32_t lf[8];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) lf[c] = leaf_of(g, x + (c & 1), y + ((c >> 1) & 1), z + (c >> 2));
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        // background: the packed form of the FP32 background record (ior 1,
-        // normal decoded from the zero code, normalize(-1,-1,-1): octahedral 0xABAB)
-        hw[c] = make_uint4(0u, 0u, 0u, 0x3C00u << 16);
-        cw[c] = make_uint2(0u, 0xABABu << 16);
-        if (lf[c] >= 0) {
-          const int vx = x + (c & 1), vy = y + ((c >> 1) & 1), vz = z + (c >> 2);
-          const size_t v = (size_t)lf[c] * kLeafVoxels + (((vz & 7) << 6) | ((vy & 7) << 3) | (vx & 7));
-          hw[c] = g.leaves_packed[v];
-          cw[c] = g.leaves_cold[v];
-        }
-      }
-    }
+    for (int c = 0; c < 8; ++c) lf[c] = leaf_of(g, x + (c & 1), y + ((c >> 1) & 1), z + (c >> 2));
 #pragma unroll
     for (int c = 0; c < 8; ++c) count<kCount>(cnt, kCntVox);
     v3 e = mk(0.f, 0.f, 0.f), d0 = e, d1 = e;
 #pragma unroll 1
     for (int pr = 0; pr < 4; ++pr) {
       const int cy = pr & 1, cz = pr >> 1;
-      // the pair is always in slots 0 and 1: the words rotate down by two each
-      // iteration, so every access has a constant index and stays in registers
-      const uint4 ha = hw[0], hb = hw[1];
-      const uint2 ca = cw[0], cb = cw[1];
+      const int vy = y + cy, vz = z + cz;
+      uint4 hw[2];
+      uint2 cw[2];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        hw[c] = hw[c + 2];
-        cw[c] = cw[c + 2];
+      for (int c = 0; c < 2; ++c) {
+        // background: the packed form of the FP32 background record (ior 1,
+        // normal decoded from the zero code, normalize(-1,-1,-1): octahedral 0xABAB)
+        hw[c] = make_uint4(0u, 0u, 0u, 0x3C00u << 16);
+        cw[c] = make_uint2(0u, 0xABABu << 16);
+        if (lf[c] >= 0) {
+          const int vx = x + c;
+          const size_t v = (size_t)lf[c] * kLeafVoxels + (((vz & 7) << 6) | ((vy & 7) << 3) | (vx & 7));
+          hw[c] = g.leaves_packed[v];
+          cw[c] = g.leaves_cold[v];
+        }
       }
-      const Rec ra = decode_packed(ha, ca);
+      // the next pair's leaf entries move down (constant indices keep them in registers)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) lf[c] = lf[c + 2];
+      const Rec ra = decode_packed(hw[0], cw[0]);
       acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * (1.f - dx), ra);
-      const Rec rb = decode_packed(hb, cb);
+      const Rec rb = decode_packed(hw[1], cw[1]);
       acc((cy ? dy : 1.f - dy) * (cz ? dz : 1.f - dz) * dx, rb);
       const v3 f = nlerp(dx, ra.normal, rb.normal);
       if (cy == 0) e = f;
