@@ -218,9 +218,10 @@ struct FrameFetch {
 // or a traversal step.  The wavefront runs one block per step for the lanes
 // waiting at it, chosen from ballots so that the costly blocks run with many
 // lanes: shading once at least R2_SHADE_MIN lanes wait for it (or nothing else
-// is left), segment starts once at least R2_SEG_MIN lanes, or R2_SEG_RATIO times
-// as many lanes as are in traversals, wait for them, else a traversal step.  Returns false when every
-// lane is idle.
+// is left), followed by segment starts for the lanes at them; else segment
+// starts once at least R2_SEG_MIN lanes, or R2_SEG_RATIO times as many lanes as
+// are in traversals, wait for them; else a traversal step.  Returns false when
+// every lane is idle.
 #ifndef R2_SHADE_MIN
 #define R2_SHADE_MIN 48
 #endif
@@ -252,7 +253,6 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
 #ifdef R2_SM_STATS
     SM_STAT(5, SM_CLOCK() - t0);
 #endif
-#ifndef R2_NO_FUSED_SEG
     // most shaded lanes now start a walk: their first segment runs in the same
     // step, together with the lanes that were already waiting at a segment
     SM_STAT(3, 1);
@@ -261,7 +261,6 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
       p.rng.top_up();
       seg_step<kSingle, kSurf>(s, p);
     }
-#endif
   } else if (ns >= R2_SEG_MIN || ns >= R2_SEG_RATIO * nt) {
     SM_STAT(3, 1);
     SM_STAT(4, ns);
