@@ -522,7 +522,13 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
   uint32_t* d_mip[3] = {nullptr, nullptr, nullptr};
   if ((rc = dmalloc(ctx, g, &d_bricks, nbt))) return rc;
   if ((rc = dmalloc(ctx, g, &d_atlas, (size_t)std::max<uint32_t>(nne, 1) * 512))) return rc;
-  if ((rc = dmalloc(ctx, g, &d_maj[0], nbt))) return rc;
+  // the four majorant levels in one buffer: level l (nbt / 8^l texels, nbt a
+  // multiple of 512) starts at (nbt / 64) x {0, 64, 72, 73}[l], so that a lookup
+  // finds its level's table by arithmetic
+  if ((rc = dmalloc(ctx, g, &d_maj[0], nbt + nbt / 8 + nbt / 64 + nbt / 512))) return rc;
+  d_maj[1] = d_maj[0] + nbt;
+  d_maj[2] = d_maj[1] + nbt / 8;
+  d_maj[3] = d_maj[2] + nbt / 64;
   {
     const size_t threads = nbt * 64;
     hipLaunchKernelGGL(k_brick_atlas, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, pool, nb[0], nb[1],
@@ -534,7 +540,6 @@ int grid_build(r2_ctx* ctx, const r2_grid_desc* desc, r2_grid* g) {
     const int px = nb[0] >> l, py = nb[1] >> l;
     const size_t cnt = (size_t)sx * sy * sz;
     if ((rc = dmalloc(ctx, g, &d_mip[l], cnt))) return rc;
-    if ((rc = dmalloc(ctx, g, &d_maj[l + 1], cnt))) return rc;
     const uint32_t* src = l == 0 ? d_range : d_mip[l - 1];
     hipLaunchKernelGGL(k_brick_mip, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, src, px, py, d_mip[l], sx, sy,
                        sz, d_maj[l + 1]);

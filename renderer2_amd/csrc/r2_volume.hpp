@@ -324,7 +324,14 @@ __device__ __forceinline__ float lookup_majorant(const DGrid& g, v3 ipos, int mi
   const int bx = ((int)floorf(ipos.x)) >> 3, by = ((int)floorf(ipos.y)) >> 3, bz = ((int)floorf(ipos.z)) >> 3;
   if ((unsigned)bx >= (unsigned)g.nb[0] || (unsigned)by >= (unsigned)g.nb[1] || (unsigned)bz >= (unsigned)g.nb[2])
     return 0.f;
-  return g.maj[mip][mip_texel(g, bx, by, bz, mip)];
+  // The grid build keeps the four levels in one buffer, level l at (texels of
+  // level 0) / 64 x {0, 64, 72, 73}[l]: the lane's level finds its table by
+  // arithmetic (indexing g.maj with it would load the pointer from memory, one
+  // more dependent load on every DDA step).  Level-0 texels are fewer than 2^30
+  // (fewer than 1024 bricks per axis), so the 24-bit products are exact.
+  const uint32_t n64 = (uint32_t)(g.nb[0] * g.nb[1] * g.nb[2]) >> 6;
+  const uint32_t k = (0x49484000u >> (8 * mip)) & 0xffu;
+  return g.maj[0][__umul24(n64, k) + mip_texel(g, bx, by, bz, mip)];
 }
 // CDeviceBrickGrid::lookupDensity (device_grid_brick_impl.hpp:9-27)
 template <bool kCount>
