@@ -225,6 +225,11 @@ struct FrameFetch {
 #ifndef R2_SHADE_MIN
 #define R2_SHADE_MIN 48
 #endif
+// Issue priority of a wavefront (s_setprio) while it runs a block: the short,
+// load-bound traversal steps above segment starts above shading, so that a
+// SIMD's shading wavefronts fill the cycles in which the others wait on memory
+// (+1.7 % at C2 over equal priorities; shading first measured 1 % slower).
+constexpr short kPrioTrav = 2, kPrioSeg = 1;
 #ifndef R2_SEG_MIN
 #define R2_SEG_MIN 32
 #endif
@@ -257,27 +262,33 @@ __device__ __forceinline__ bool sched_step(const DScene& s, PathLane& p, Cnt& cn
     // step, together with the lanes that were already waiting at a segment
     SM_STAT(3, 1);
     SM_STAT(4, __popcll(__ballot(p.stage == ST_SEG)));
+    __builtin_amdgcn_s_setprio(kPrioSeg);
     if (p.stage == ST_SEG) {
       p.rng.top_up();
       seg_step<kSingle, kSurf>(s, p);
     }
+    __builtin_amdgcn_s_setprio(0);
   } else if (ns >= R2_SEG_MIN || ns >= R2_SEG_RATIO * nt) {
     SM_STAT(3, 1);
     SM_STAT(4, ns);
+    __builtin_amdgcn_s_setprio(kPrioSeg);
     if (at_seg) {
       p.rng.top_up();
       seg_step<kSingle, kSurf>(s, p);
     }
+    __builtin_amdgcn_s_setprio(0);
 #ifdef R2_SM_STATS
     SM_STAT(6, SM_CLOCK() - t0);
 #endif
   } else {
     SM_STAT(7, 1);
     SM_STAT(8, nt);
+    __builtin_amdgcn_s_setprio(kPrioTrav);
     if (at_trav) {
       p.rng.top_up();
       trav_step<kSingle, kCount>(s, p, cnt);
     }
+    __builtin_amdgcn_s_setprio(0);
 #ifdef R2_SM_STATS
     SM_STAT(9, SM_CLOCK() - t0);
 #endif
