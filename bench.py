@@ -221,6 +221,7 @@ def main(argv=None) -> int:
                 "bytes_per_path": round(b_path, 1), "kernel": "k_paths_sm", "kernel_ms": round(avg_kernel_ms, 3)}
         if a.workload == "c2" and w["grids"][0].get("precision", "fp32") == "fp32" and not a.surfaces:
             roof["traffic"], roof["traffic_source"] = profile_traffic("k_paths_sm_c2_256cube_1024sq_64spp")
+            roof["valu_issue"] = profile_valu_issue("k_paths_sm_c2_256cube_1024sq_64spp")
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -267,6 +268,25 @@ def profile_traffic(name: str):
         return None, None
     m = re.search(r"\*\*([0-9.]+) MB per launch\*\*", f.read_text())
     return (int(float(m.group(1)) * 1e6), f"profiles/{f.name}") if m else (None, None)
+
+
+def profile_valu_issue(name: str):
+    """VALU issue rate of the profiled kernel (SQ_INSTS_VALU per CU per cycle, from
+    the rocprofv3 summary of the same workload) against the rate a CU sustains on
+    independent FP32 FMAs (renderer2_amd/bin/valu_peak), or None."""
+    import re
+
+    f = PROFILES / f"{name}.md"
+    if not f.exists():
+        return None
+    m = re.search(r"VALU issue: .*?= ([0-9.]+) wavefront instructions per CU per cycle, [0-9.]+ % of the ([0-9.]+)",
+                  f.read_text())
+    if not m:
+        return None
+    achieved, ceiling = float(m.group(1)), float(m.group(2))
+    return {"achieved": achieved, "ceiling": ceiling, "frac": round(achieved / ceiling, 3),
+            "unit": "wavefront instructions per CU per cycle",
+            "source": f"profiles/{f.name}, profiles/valu_peak_fma_exp2.md"}
 
 
 def copy_calibration(mib: int) -> None:
